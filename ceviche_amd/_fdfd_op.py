@@ -1,0 +1,307 @@
+"""Device-side FDFD operators and their differentiable entry points.
+
+``FdfdOperator`` holds a batch of 2D FDFD instances of one polarisation:
+their descriptors (cv_fdfd_desc), the tile table that maps workgroups to
+cells, and the 1D SC-PML derivative profiles, all resident on the GPU.  It
+wraps the C-ABI calls of include/ceviche_amd.h.
+
+The autograd Functions below are the reference's sparse primitives
+specialised to the FDFD operator (ceviche/primitives.py:103-148 for the
+solve, :58-98 for the derivative matrices), in torch's complex convention:
+the cotangent of a complex z is dL/dRe z + i dL/dIm z, so reverse mode uses
+A^H = conj(A)^T where the reference (autograd convention) uses A^T.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_ptr
+from .constants import EPSILON_0, MU_0
+from .derivatives import axis_profiles
+
+_REAL_OF = {torch.complex128: torch.float64, torch.complex64: torch.float32}
+_CODE_OF = {torch.complex128: _lib.CV_C128, torch.complex64: _lib.CV_C64}
+
+
+class SolverOptions:
+    """Krylov settings.  tol is the relative FP64 residual ||b - A x|| / ||b||."""
+
+    def __init__(self, tol=None, max_iter=200000, check_every=32, inner_tol=1e-4, max_sweeps=12):
+        self.tol = tol
+        self.max_iter = max_iter
+        self.check_every = check_every
+        self.inner_tol = inner_tol
+        self.max_sweeps = max_sweeps
+
+
+DEFAULT_TOL = {torch.complex128: 1e-10, torch.complex64: 1e-6}
+
+
+class FdfdOperator:
+    """A batch of FDFD instances (``specs``: list of dicts with keys omega, dL,
+    shape, npml, bloch) of polarisation 'ez' or 'hz' in precision ``dtype``
+    (torch.complex128 or torch.complex64).  Vectors of the batch are packed
+    instance after instance, each in C order over (nx, ny)."""
+
+    def __init__(self, pol, specs, dtype=torch.complex128, device=None):
+        if pol not in ('ez', 'hz'):
+            raise ValueError("pol must be 'ez' or 'hz'")
+        if dtype not in _CODE_OF:
+            raise ValueError('dtype must be torch.complex128 or torch.complex64')
+        self.pol = pol
+        self.dtype = dtype
+        self.rdtype = _REAL_OF[dtype]
+        self.code = _CODE_OF[dtype]
+        self.device = device if device is not None else _lib.device()
+        self.specs = list(specs)
+        n_inst = len(self.specs)
+        desc = np.zeros(n_inst, dtype=_lib.DESC_DTYPE)
+        tiles = []
+        profs = []
+        cell_off = 0
+        prof_off = 0
+        self.sys_scale = np.zeros(n_inst)
+        self.shapes = []
+        self.offsets = []
+        for k, sp in enumerate(self.specs):
+            nx, ny = (int(v) for v in sp['shape'])
+            omega, dL = float(sp['omega']), float(sp['dL'])
+            npml = sp['npml']
+            bx, by = sp.get('bloch', (0.0, 0.0))
+            kxf, kxb = axis_profiles(omega, dL, nx, int(npml[0]))
+            kyf, kyb = axis_profiles(omega, dL, ny, int(npml[1]))
+            profs += [kxf, kxb, kyf, kyb]
+            n = nx * ny
+            ntile = (n + _lib.CV_TILE_CELLS - 1) // _lib.CV_TILE_CELLS
+            first_tile = len(tiles)
+            tiles += [(k, t * _lib.CV_TILE_CELLS) for t in range(ntile)]
+            # The system is scaled by c = mu0 dL^2 (Ez) or eps0 dL^2 (Hz) so the
+            # stencil coefficients are O(1) in either precision; the solution is
+            # unchanged and the caller's right-hand side is scaled by c as well.
+            if pol == 'ez':
+                c = MU_0 * dL ** 2
+                scale, dshift = -1 / MU_0 * c, -EPSILON_0 * omega ** 2 * c
+            else:
+                c = EPSILON_0 * dL ** 2
+                scale, dshift = 1 / EPSILON_0 * c, MU_0 * omega ** 2 * c
+            self.sys_scale[k] = c
+            phx, phy = np.exp(1j * bx), np.exp(1j * by)
+            flags = (_lib.CV_FLAG_X_IDENTITY if nx == 1 else 0) | (_lib.CV_FLAG_Y_IDENTITY if ny == 1 else 0)
+            desc[k] = (nx, ny, _lib.CV_POL_EZ if pol == 'ez' else _lib.CV_POL_HZ, flags, cell_off, prof_off,
+                       first_tile, ntile, scale, dshift, phx.real, phx.imag, phy.real, phy.imag, omega)
+            self.shapes.append((nx, ny))
+            self.offsets.append(cell_off)
+            cell_off += n
+            prof_off += 2 * (nx + ny)
+        self.total_cells = cell_off
+        self.n_inst = n_inst
+        self.desc_host = desc
+        self.tiles_host = np.array(tiles, dtype=_lib.TILE_DTYPE)
+        self.n_tiles = len(tiles)
+        dev = self.device
+        self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
+        self.tiles = torch.from_numpy(self.tiles_host.view(np.uint8).copy()).to(dev)
+        prof = np.concatenate(profs)
+        self.prof = torch.from_numpy(prof).to(dev).to(dtype)
+        self.prof64 = self.prof if dtype == torch.complex128 else torch.from_numpy(prof).to(dev)
+        self._ws = None
+        self._ws_mixed = None
+        self._scale_vec = None
+        self._desc64 = None
+        self.last_info = None
+
+    # ------------------------------------------------------------- helpers
+    def _args(self):
+        return (ptr(self.desc), ptr(self.tiles), self.n_tiles)
+
+    def scale_vector(self, dtype=None):
+        """Per-cell system scale c (real), for packing right-hand sides."""
+        if self._scale_vec is None:
+            v = np.concatenate([np.full(s[0] * s[1], c) for s, c in zip(self.shapes, self.sys_scale)])
+            self._scale_vec = torch.from_numpy(v).to(self.device)
+        return self._scale_vec if dtype is None else self._scale_vec.to(dtype)
+
+    def materials(self, eps):
+        """(m0, m1) for a packed eps (real or complex) of total_cells entries."""
+        eps = eps.to(self.device)
+        if self.pol == 'ez':
+            if torch.is_complex(eps):
+                return eps.real.to(self.rdtype).contiguous(), eps.imag.to(self.rdtype).contiguous()
+            return eps.to(self.rdtype).contiguous(), None
+        if torch.is_complex(eps):
+            raise NotImplementedError('fdfd_hz with complex eps_r is not supported')
+        e = eps.to(self.rdtype).contiguous()
+        m0 = torch.empty_like(e)
+        m1 = torch.empty_like(e)
+        check(lib().cv_fdfd_hz_materials(self.code, *self._args(), ptr(e), ptr(m0), ptr(m1), stream_ptr()),
+              'hz_materials')
+        return m0, m1
+
+    def materials_grad(self, m0, m1, g0, g1):
+        """Reverse of materials(): d eps for cotangents on (m0, m1)."""
+        out = torch.zeros_like(m0)
+        check(lib().cv_fdfd_hz_materials_grad(self.code, *self._args(), ptr(m0), ptr(m1), ptr(g0), ptr(g1),
+                                              ptr(out), stream_ptr()), 'hz_materials_grad')
+        return out
+
+    def apply(self, x, m0, m1, transpose=False):
+        """c * A x (or c * A^T x) with the per-instance system scale c."""
+        x = x.to(self.dtype).contiguous()
+        y = torch.empty_like(x)
+        check(lib().cv_fdfd_apply(self.code, *self._args(), ptr(self.prof), ptr(m0), ptr(m1), ptr(x), ptr(y),
+                                  int(bool(transpose)), stream_ptr()), 'apply')
+        return y
+
+    def deriv(self, u, which, coef, mat=None, mode=0, out=None):
+        """coef * mat .* (D u) (mode 0), coef * D^T(mat .* u) (1), conj(coef) * D^H(mat .* u) (2)."""
+        u = u.to(self.dtype).contiguous()
+        acc = out is not None
+        if out is None:
+            out = torch.empty_like(u)
+        check(lib().cv_fdfd_deriv(self.code, *self._args(), ptr(self.prof), ptr(mat), ptr(u), ptr(out),
+                                  which, mode, float(np.real(coef)), float(np.imag(coef)), int(acc),
+                                  stream_ptr()), 'deriv')
+        return out
+
+    def deriv_matgrad(self, u, g, which, coef, gmat):
+        check(lib().cv_fdfd_deriv_matgrad(self.code, *self._args(), ptr(self.prof), ptr(u), ptr(g), ptr(gmat),
+                                          which, float(np.real(coef)), float(np.imag(coef)), stream_ptr()),
+              'deriv_matgrad')
+        return gmat
+
+    def material_grad(self, lam, x, g0, g1, scale):
+        check(lib().cv_fdfd_material_grad(self.code, *self._args(), ptr(self.prof), ptr(lam), ptr(x), ptr(g0),
+                                          ptr(g1), float(scale), stream_ptr()), 'material_grad')
+
+    # --------------------------------------------------------------- solve
+    def _workspace(self, mixed):
+        L = lib()
+        if mixed:
+            need = L.cv_fdfd_solve_mixed_workspace_bytes(self.total_cells, self.n_inst, self.n_tiles)
+            if self._ws_mixed is None or self._ws_mixed.numel() < need:
+                self._ws_mixed = torch.empty(need, dtype=torch.uint8, device=self.device)
+            return self._ws_mixed
+        need = L.cv_fdfd_solve_workspace_bytes(self.code, self.total_cells, self.n_inst, self.n_tiles)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def solve_scaled(self, bs, m0, m1, transpose=False, x0=None, opts=None, mixed_ops=None):
+        """Solve (c A) x = bs (bs already multiplied by c).  Returns x.
+
+        For complex64 operators, ``mixed_ops`` = (op64, m0_64, m1_64) selects
+        mixed-precision iterative refinement: x and the residual in complex128,
+        complex64 BiCGSTAB corrections."""
+        opts = opts or SolverOptions()
+        info = (_lib.SolveInfo * self.n_inst)()
+        if mixed_ops is not None:
+            op64, m0_64, m1_64 = mixed_ops
+            tol = opts.tol if opts.tol is not None else DEFAULT_TOL[torch.complex64]
+            b64 = bs.to(torch.complex128).contiguous()
+            x = torch.zeros_like(b64) if x0 is None else x0.to(torch.complex128).clone()
+            ws = self._workspace(True)
+            rc = lib().cv_fdfd_solve_mixed(
+                ptr(op64.desc), self.desc_host.ctypes.data_as(ctypes.c_void_p), self.n_inst, ptr(self.tiles),
+                self.n_tiles, ptr(op64.prof), ptr(m0_64), ptr(m1_64), ptr(self.prof), ptr(m0), ptr(m1),
+                ptr(b64), ptr(x), int(bool(transpose)), float(tol), float(opts.inner_tol), int(opts.max_iter),
+                int(opts.max_sweeps), int(opts.check_every), ptr(ws), ws.numel(), info, stream_ptr())
+        else:
+            tol = opts.tol if opts.tol is not None else DEFAULT_TOL[self.dtype]
+            b = bs.to(self.dtype).contiguous()
+            x = torch.zeros_like(b) if x0 is None else x0.to(self.dtype).clone().contiguous()
+            ws = self._workspace(False)
+            rc = lib().cv_fdfd_solve(
+                self.code, ptr(self.desc), self.desc_host.ctypes.data_as(ctypes.c_void_p), self.n_inst,
+                ptr(self.tiles), self.n_tiles, ptr(self.prof), ptr(m0), ptr(m1), ptr(b), ptr(x),
+                int(bool(transpose)), float(tol), int(opts.max_iter), int(opts.check_every), ptr(ws), ws.numel(),
+                info, stream_ptr())
+        self.last_info = [dict(iterations=i.iterations, status=i.status, restarts=i.restarts,
+                               rel_residual=i.rel_residual) for i in info]
+        check(rc, 'fdfd solve')
+        return x
+
+
+# ---------------------------------------------------------------- autograd
+class FdfdSolve(torch.autograd.Function):
+    """x = A(m)^-1 b for the FDFD operator (sp_solve, primitives.py:103-137).
+
+    ``op`` is the complex128 operator; when ``op32`` (complex64, same specs) is
+    given, the solve runs complex64 BiCGSTAB corrections inside complex128
+    iterative refinement.  Backward: A^H lam = g is solved as A^T mu = conj(g),
+    lam = conj(mu) (one adjoint solve shared by both cotangents, unlike the two
+    of primitives.py:121-135), grad_b = lam, grad_m = -Re(mu^T dA/dm x)."""
+
+    @staticmethod
+    def _solve(op, op32, opts, m0, m1, bs, transpose):
+        if op32 is None:
+            return op.solve_scaled(bs, m0, m1, transpose=transpose, opts=opts)
+        m0s = m0.to(torch.float32).contiguous()
+        m1s = None if m1 is None else m1.to(torch.float32).contiguous()
+        return op32.solve_scaled(bs, m0s, m1s, transpose=transpose, opts=opts, mixed_ops=(op, m0, m1))
+
+    @staticmethod
+    def forward(ctx, m0, m1, b, op, op32, opts):
+        c = op.scale_vector(torch.float64)
+        x = FdfdSolve._solve(op, op32, opts, m0, m1, b * c, False)
+        ctx.op, ctx.op32, ctx.opts = op, op32, opts
+        ctx.save_for_backward(m0, m1, x)
+        ctx.has_m1 = m1 is not None
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        m0, m1, x = ctx.saved_tensors
+        op = ctx.op
+        mu = FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, torch.conj(gx).resolve_conj().contiguous(), True)
+        c = op.scale_vector(torch.float64)
+        gb = torch.conj(mu).resolve_conj() * c if ctx.needs_input_grad[2] else None
+        g0 = g1 = None
+        if ctx.needs_input_grad[0] or (ctx.has_m1 and ctx.needs_input_grad[1]):
+            g0 = torch.zeros_like(m0)
+            g1 = torch.zeros_like(m0) if ctx.has_m1 else None
+            op.material_grad(mu.contiguous(), x.contiguous(), g0, g1, -1.0)
+        return g0, g1, gb, None, None, None
+
+
+class FdfdDeriv(torch.autograd.Function):
+    """out = coef * mat .* (D u): field recovery (fdfd.py:135-162) through the
+    sp_mult of a derivative matrix (primitives.py:58-98)."""
+
+    @staticmethod
+    def forward(ctx, u, mat, op, which, coef):
+        ctx.op, ctx.which, ctx.coef = op, which, coef
+        ctx.save_for_backward(u, mat)
+        ctx.has_mat = mat is not None
+        return op.deriv(u, which, coef, mat=mat, mode=0)
+
+    @staticmethod
+    def backward(ctx, g):
+        u, mat = ctx.saved_tensors
+        op = ctx.op
+        g = g.to(op.dtype).contiguous()
+        gu = op.deriv(g, ctx.which, ctx.coef, mat=mat, mode=2) if ctx.needs_input_grad[0] else None
+        gm = None
+        if ctx.has_mat and ctx.needs_input_grad[1]:
+            gm = torch.zeros_like(mat)
+            op.deriv_matgrad(u.to(op.dtype).contiguous(), g, ctx.which, ctx.coef, gm)
+        return gu, gm, None, None, None
+
+
+class HzMaterials(torch.autograd.Function):
+    """(m0, m1) = (1/(eps_xx + 1e-5), 1/(eps_yy + 1e-5)); fdfd.py:200-215."""
+
+    @staticmethod
+    def forward(ctx, eps, op):
+        m0, m1 = op.materials(eps)
+        ctx.op = op
+        ctx.save_for_backward(m0, m1)
+        return m0, m1
+
+    @staticmethod
+    def backward(ctx, g0, g1):
+        m0, m1 = ctx.saved_tensors
+        g0 = None if g0 is None else g0.contiguous()
+        g1 = None if g1 is None else g1.contiguous()
+        return ctx.op.materials_grad(m0, m1, g0, g1), None

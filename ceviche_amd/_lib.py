@@ -1,0 +1,147 @@
+"""ctypes binding of the C-ABI in include/ceviche_amd.h.
+
+The shared library is loaded after ``import torch`` so that it binds to the
+HIP runtime torch already loaded (both are SONAME libamdhip64.so.7).  There is
+no CPU fallback: if the library is missing or no GPU is present, the product
+entry points raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libceviche_amd.so')
+
+CV_OK = 0
+CV_ERR_INVALID = 1
+CV_ERR_HIP = 2
+CV_ERR_NOT_CONVERGED = 3
+CV_ERR_NONFINITE = 4
+CV_ERR_BREAKDOWN = 5
+CV_ERR_SINGULAR = 6
+
+CV_C64 = 0
+CV_C128 = 1
+CV_POL_EZ = 0
+CV_POL_HZ = 1
+CV_FLAG_X_IDENTITY = 1
+CV_FLAG_Y_IDENTITY = 2
+CV_TILE_CELLS = 1024
+
+_STATUS_NAMES = {
+    CV_ERR_INVALID: 'invalid argument',
+    CV_ERR_HIP: 'HIP runtime error',
+    CV_ERR_NOT_CONVERGED: 'solver did not converge',
+    CV_ERR_NONFINITE: 'non-finite value in solver',
+    CV_ERR_BREAKDOWN: 'Krylov breakdown',
+    CV_ERR_SINGULAR: 'singular matrix',
+}
+
+DESC_DTYPE = np.dtype([
+    ('nx', '<i4'), ('ny', '<i4'), ('pol', '<i4'), ('flags', '<i4'),
+    ('cell_off', '<i8'), ('prof_off', '<i8'),
+    ('tile_first', '<i4'), ('tile_count', '<i4'),
+    ('scale', '<f8'), ('dshift', '<f8'),
+    ('phx_re', '<f8'), ('phx_im', '<f8'), ('phy_re', '<f8'), ('phy_im', '<f8'),
+    ('omega', '<f8'),
+])
+assert DESC_DTYPE.itemsize == 96
+TILE_DTYPE = np.dtype([('inst', '<i4'), ('first', '<i4')])
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [('iterations', ctypes.c_int32), ('status', ctypes.c_int32),
+                ('restarts', ctypes.c_int32), ('pad', ctypes.c_int32),
+                ('rel_residual', ctypes.c_double)]
+
+
+class FdtdDesc(ctypes.Structure):
+    _fields_ = [('nx', ctypes.c_int32), ('ny', ctypes.c_int32), ('nz', ctypes.c_int32),
+                ('tm2d', ctypes.c_int32), ('dt', ctypes.c_double), ('dL', ctypes.c_double),
+                ('c0', ctypes.c_double), ('eps0', ctypes.c_double)]
+
+
+class CevicheError(RuntimeError):
+    """Raised for a non-zero status from the HIP library."""
+
+    def __init__(self, status, msg):
+        super().__init__('%s (status %d): %s' % (_STATUS_NAMES.get(status, 'error'), status, msg))
+        self.status = status
+
+
+class NotConvergedError(CevicheError):
+    pass
+
+
+_lib = None
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+I64 = ctypes.c_int64
+D = ctypes.c_double
+
+_SIGS = {
+    'cv_version': (ctypes.c_char_p, []),
+    'cv_last_error': (ctypes.c_char_p, []),
+    'cv_fdfd_apply': (I, [I, P, P, I, P, P, P, P, P, I, P]),
+    'cv_fdfd_hz_materials': (I, [I, P, P, I, P, P, P, P]),
+    'cv_fdfd_hz_materials_grad': (I, [I, P, P, I, P, P, P, P, P, P]),
+    'cv_fdfd_deriv': (I, [I, P, P, I, P, P, P, P, I, I, D, D, I, P]),
+    'cv_fdfd_deriv_matgrad': (I, [I, P, P, I, P, P, P, P, I, D, D, P]),
+    'cv_fdfd_material_grad': (I, [I, P, P, I, P, P, P, P, P, D, P]),
+    'cv_fdfd_solve_workspace_bytes': (I64, [I, I64, I, I]),
+    'cv_fdfd_solve': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
+    'cv_fdfd_solve_mixed_workspace_bytes': (I64, [I64, I, I]),
+    'cv_fdfd_solve_mixed': (I, [P, P, I, P, I, P, P, P, P, P, P, P, P, I, D, D, I, I, I, P, I64, P, P]),
+}
+
+
+def lib():
+    """The loaded library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError('ceviche_amd HIP library not built: %s is missing '
+                              '(run __graft_entry__.build() or python -m ceviche_amd._build)' % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc, what=''):
+    if rc != CV_OK:
+        msg = lib().cv_last_error().decode(errors='replace')
+        if rc == CV_ERR_NOT_CONVERGED:
+            raise NotConvergedError(rc, (what + ': ' + msg) if what else msg)
+        raise CevicheError(rc, (what + ': ' + msg) if what else msg)
+
+
+def device():
+    """The GPU every product path runs on; raises when none is present."""
+    if not torch.cuda.is_available():
+        raise RuntimeError('ceviche_amd needs a ROCm GPU (torch.cuda.is_available() is False); '
+                           'there is no CPU fallback')
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return ctypes.c_void_p(0)
+    return ctypes.c_void_p(t.data_ptr())

@@ -1,0 +1,294 @@
+// FDFD operator kernels: stencil apply (A, A^T), Hz material averaging,
+// single derivative operators (field recovery and its adjoints) and the
+// matrix-free material gradient.  See fdfd_stencil.h for the stencil.
+#include "fdfd_stencil.h"
+
+namespace cv {
+
+constexpr int kCellsPerThread = CV_TILE_CELLS / kBlock;  // 4
+
+// ---------------------------------------------------------------- y = A x
+template <typename R, bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_apply(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+        const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
+        const cx<R>* __restrict__ x, cx<R>* __restrict__ y) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+    auto fx = [&](int64_t k) { return x[k]; };
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        y[k] = apply_any<R, TRANS>(d, prof, m0, m1, i, j, fx, x[k]);
+    }
+}
+
+// ------------------------------------------------------- Hz materials
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_hz_materials(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+               const R* __restrict__ eps, R* __restrict__ m0, R* __restrict__ m1) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int im = (i == 0) ? d.nx - 1 : i - 1;
+        const int jm = (j == 0) ? d.ny - 1 : j - 1;
+        const int64_t k = d.cell_off + loc;
+        const R e = eps[k];
+        // fdfd.py:203-204: eps_xx rolls along axis 1 (y), eps_yy along axis 0 (x)
+        const R exx = R(0.5) * (e + eps[d.cell_off + (int64_t)i * d.ny + jm]);
+        const R eyy = R(0.5) * (e + eps[d.cell_off + (int64_t)im * d.ny + j]);
+        m0[k] = R(1) / (exx + R(1e-5));
+        m1[k] = R(1) / (eyy + R(1e-5));
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_deriv(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+        const cx<R>* __restrict__ prof, const R* __restrict__ mat, const cx<R>* __restrict__ u,
+        cx<R>* __restrict__ out, int which, int mode, double cre, double cim, int accumulate) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+    cx<R> coef = mk<R>(R(cre), R(cim));
+    if (mode == 2) coef = conj(coef);
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        cx<R> v = coef * deriv_at<R>(d, prof, u, mat, which, mode, i, j);
+        if (accumulate) v = v + out[k];
+        out[k] = v;
+    }
+}
+
+// -------------------------------------------------- material gradients
+// out_m0[k] += scale * Re( lam^T (dA/dm0[k]) x ),  out_m1 likewise.
+//   Ez: dA/dm0[k] = dshift e_k e_k^T                   -> dshift lam_k x_k
+//   Hz: dA/dm1[k] = s Dxb e_k e_k^T Dxf                -> s (Dxb^T lam)_k (Dxf x)_k
+//       dA/dm0[k] = s Dyb e_k e_k^T Dyf                -> s (Dyb^T lam)_k (Dyf x)_k
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_material_grad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+                const cx<R>* __restrict__ prof, const cx<R>* __restrict__ lam,
+                const cx<R>* __restrict__ x, R* __restrict__ g0, R* __restrict__ g1, double scale) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        if (d.pol == CV_POL_EZ) {
+            // diag = dshift (m0 + i m1): d/dm0 -> dshift, d/dm1 -> i dshift
+            const cx<R> p = lam[k] * x[k];
+            g0[k] += R(scale * d.dshift) * p.re;
+            if (g1) g1[k] -= R(scale * d.dshift) * p.im;
+        } else {
+            const cx<R> lx = deriv_at<R>(d, prof, lam, (const R*)nullptr, 1, 1, i, j);  // Dxb^T lam
+            const cx<R> dx = deriv_at<R>(d, prof, x, (const R*)nullptr, 0, 0, i, j);    // Dxf x
+            const cx<R> ly = deriv_at<R>(d, prof, lam, (const R*)nullptr, 3, 1, i, j);  // Dyb^T lam
+            const cx<R> dy = deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, j);    // Dyf x
+            const R s = R(scale * d.scale);
+            g1[k] += s * (lx * dx).re;
+            g0[k] += s * (ly * dy).re;
+        }
+    }
+}
+
+
+// ----------------------------------------- reverse of the Hz materials
+// m0 = 1/(exx + 1e-5), exx[i,j] = (eps[i,j] + eps[i,j-1])/2 (fdfd.py:203, :214)
+// m1 = 1/(eyy + 1e-5), eyy[i,j] = (eps[i,j] + eps[i-1,j])/2 (fdfd.py:204, :215)
+// g_eps[i,j] += 1/2 (gxx[i,j] + gxx[i,j+1]) + 1/2 (gyy[i,j] + gyy[i+1,j]),
+// gxx = -m0^2 g_m0, gyy = -m1^2 g_m1.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_hz_materials_grad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+                    const R* __restrict__ m0, const R* __restrict__ m1, const R* __restrict__ g0,
+                    const R* __restrict__ g1, R* __restrict__ geps) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int ip = (i == d.nx - 1) ? 0 : i + 1;
+        const int jp = (j == d.ny - 1) ? 0 : j + 1;
+        const int64_t k = d.cell_off + loc;
+        const int64_t kj = d.cell_off + (int64_t)i * d.ny + jp;
+        const int64_t ki = d.cell_off + (int64_t)ip * d.ny + j;
+        auto gxx = [&](int64_t c) { return g0 ? -m0[c] * m0[c] * g0[c] : R(0); };
+        auto gyy = [&](int64_t c) { return g1 ? -m1[c] * m1[c] * g1[c] : R(0); };
+        geps[k] += R(0.5) * (gxx(k) + gxx(kj)) + R(0.5) * (gyy(k) + gyy(ki));
+    }
+}
+
+// g_mat[k] += Re( conj(coef (D u)[k]) g[k] ): reverse mode of out = coef mat .* (D u)
+// with respect to the real material.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_deriv_matgrad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+                const cx<R>* __restrict__ prof, const cx<R>* __restrict__ u, const cx<R>* __restrict__ g,
+                R* __restrict__ gmat, int which, double cre, double cim) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+    const cx<R> coef = mk<R>(R(cre), R(cim));
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> w = coef * deriv_at<R>(d, prof, u, (const R*)nullptr, which, 0, i, j);
+        const cx<R> gk = g[k];
+        gmat[k] += w.re * gk.re + w.im * gk.im;
+    }
+}
+
+}  // namespace cv
+
+using namespace cv;
+
+extern "C" int cv_fdfd_apply(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                             const void* prof, const void* m0, const void* m1, const void* x, void* y,
+                             int transpose, void* stream) {
+    CV_CHECK(n_tiles >= 0, "n_tiles < 0");
+    CV_CHECK(desc && tiles && prof && m0 && x && y, "null pointer argument");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    dim3 g(n_tiles), b(kBlock);
+    if (dtype == CV_C128) {
+        if (transpose)
+            k_apply<double, true><<<g, b, 0, s>>>(desc, tiles, (const cx<double>*)prof, (const double*)m0,
+                                                  (const double*)m1, (const cx<double>*)x, (cx<double>*)y);
+        else
+            k_apply<double, false><<<g, b, 0, s>>>(desc, tiles, (const cx<double>*)prof, (const double*)m0,
+                                                   (const double*)m1, (const cx<double>*)x, (cx<double>*)y);
+    } else if (dtype == CV_C64) {
+        if (transpose)
+            k_apply<float, true><<<g, b, 0, s>>>(desc, tiles, (const cx<float>*)prof, (const float*)m0,
+                                                 (const float*)m1, (const cx<float>*)x, (cx<float>*)y);
+        else
+            k_apply<float, false><<<g, b, 0, s>>>(desc, tiles, (const cx<float>*)prof, (const float*)m0,
+                                                  (const float*)m1, (const cx<float>*)x, (cx<float>*)y);
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_hz_materials(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                                    const void* eps, void* m0, void* m1, void* stream) {
+    CV_CHECK(desc && tiles && eps && m0 && m1, "null pointer argument");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_hz_materials<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const double*)eps, (double*)m0, (double*)m1);
+    else if (dtype == CV_C64)
+        k_hz_materials<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const float*)eps, (float*)m0, (float*)m1);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_deriv(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                             const void* prof, const void* mat, const void* u, void* out, int which,
+                             int mode, double coef_re, double coef_im, int accumulate, void* stream) {
+    CV_CHECK(desc && tiles && prof && u && out, "null pointer argument");
+    CV_CHECK(which >= 0 && which < 4, "which must be 0..3 (xf, xb, yf, yb)");
+    CV_CHECK(mode >= 0 && mode < 3, "mode must be 0 (N), 1 (T) or 2 (H)");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_deriv<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof, (const double*)mat,
+                                                   (const cx<double>*)u, (cx<double>*)out, which, mode,
+                                                   coef_re, coef_im, accumulate);
+    else if (dtype == CV_C64)
+        k_deriv<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)prof, (const float*)mat,
+                                                  (const cx<float>*)u, (cx<float>*)out, which, mode,
+                                                  coef_re, coef_im, accumulate);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_material_grad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles,
+                                     int n_tiles, const void* prof, const void* lam, const void* x,
+                                     void* g0, void* g1, double scale, void* stream) {
+    CV_CHECK(desc && tiles && prof && lam && x && g0, "null pointer argument");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_material_grad<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof,
+                                                           (const cx<double>*)lam, (const cx<double>*)x,
+                                                           (double*)g0, (double*)g1, scale);
+    else if (dtype == CV_C64)
+        k_material_grad<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)prof,
+                                                          (const cx<float>*)lam, (const cx<float>*)x,
+                                                          (float*)g0, (float*)g1, scale);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_hz_materials_grad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                                         const void* m0, const void* m1, const void* g0, const void* g1,
+                                         void* geps, void* stream) {
+    CV_CHECK(desc && tiles && m0 && m1 && geps, "null pointer argument");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_hz_materials_grad<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const double*)m0, (const double*)m1,
+                                                               (const double*)g0, (const double*)g1,
+                                                               (double*)geps);
+    else if (dtype == CV_C64)
+        k_hz_materials_grad<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const float*)m0, (const float*)m1,
+                                                              (const float*)g0, (const float*)g1, (float*)geps);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_deriv_matgrad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                                     const void* prof, const void* u, const void* g, void* gmat, int which,
+                                     double coef_re, double coef_im, void* stream) {
+    CV_CHECK(desc && tiles && prof && u && g && gmat, "null pointer argument");
+    CV_CHECK(which >= 0 && which < 4, "which must be 0..3 (xf, xb, yf, yb)");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_deriv_matgrad<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof,
+                                                           (const cx<double>*)u, (const cx<double>*)g,
+                                                           (double*)gmat, which, coef_re, coef_im);
+    else if (dtype == CV_C64)
+        k_deriv_matgrad<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)prof,
+                                                          (const cx<float>*)u, (const cx<float>*)g, (float*)gmat,
+                                                          which, coef_re, coef_im);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}

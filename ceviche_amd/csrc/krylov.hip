@@ -1,0 +1,632 @@
+// Device-resident batched BiCGSTAB for the FDFD operators.
+//
+// Replaces the sparse direct solve behind ceviche's sp_solve
+// (ceviche/primitives.py:103-119 -> solvers.py:37-62).  One iteration is
+// three fused kernels over all instances of the batch:
+//   K1  p' = r + beta (p - omega v)  (or p' = r on restart), v' = A p',
+//       partial (rh, v')                                   -> alpha
+//   K2  s = r - alpha v' (on the fly), t = A s, partial (t, s), (t, t) -> omega
+//   K3  x += alpha p' + omega s, r = s - omega t, partial (rh, r), (r, r)
+//                                                          -> beta, convergence
+// Reductions are FP64 regardless of the vector precision; the last workgroup
+// of each instance turns the partial sums into the next scalar (no host round
+// trip).  p and v are double-buffered because K1 reads neighbours of the old
+// p / v while writing the new ones.
+#include "fdfd_stencil.h"
+
+#include <vector>
+#include <algorithm>
+
+namespace cv {
+
+constexpr int kCPT = CV_TILE_CELLS / kBlock;
+
+enum : int32_t { kRunning = 0, kConverged = 1, kMaxIter = 2, kNonFinite = 3, kBreakdown = 4 };
+
+struct KState {
+    cx<double> rho, alpha, omega, beta;
+    double rr, bb, tol2, rhrh;
+    double rr_true, scale;        // true residual norm^2 at the last check; rhs scale (mixed)
+    int32_t iter, done, restart, max_iter;
+    int32_t restarts, pad0, pad1, pad2;
+};
+
+__global__ void k_zero_u32(unsigned int* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// mode 0: fresh start (r = b - A x, state reset, iteration count reset)
+// mode 1: verification of instances flagged converged (true residual; restart
+//         the instance if it is above tolerance), report-only for the others
+template <typename R, bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_init(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+       const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
+       const cx<R>* __restrict__ b, const cx<R>* __restrict__ x, cx<R>* __restrict__ r,
+       cx<R>* __restrict__ rh, double* __restrict__ partials, unsigned int* __restrict__ counters,
+       KState* __restrict__ state, double tol, int max_iter, int mode) {
+    __shared__ double lds[kWaves * 2];
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    KState* st = state + t.inst;
+    const int done0 = (mode == 0) ? kRunning : st->done;
+    if (mode == 1 && done0 == kRunning) return;
+    const bool write_r = (mode == 0) || (done0 == kConverged);
+    const int n = d.nx * d.ny;
+    auto fx = [&](int64_t k) { return x[k]; };
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> bk = b[k];
+        const cx<R> rk = bk - apply_any<R, TRANS>(d, prof, m0, m1, i, j, fx, x[k]);
+        if (write_r) {
+            r[k] = rk;
+            rh[k] = rk;
+        }
+        acc[0] += norm2(ccast<double>(rk));
+        acc[1] += norm2(ccast<double>(bk));
+    }
+    block_sum<2>(acc, lds);
+    double tot[2];
+    if (!last_block_reduce<2>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+                              lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    const double rr = tot[0], bb = tot[1];
+    if (mode == 0) {
+        st->bb = bb;
+        st->tol2 = tol * tol * bb;
+        st->max_iter = max_iter;
+        st->iter = 0;
+        st->restarts = 0;
+    }
+    st->rr_true = rr;
+    if (mode == 1 && done0 != kConverged) return;  // report only
+    st->rr = rr;
+    st->rho = mk(rr, 0.0);
+    st->rhrh = rr;
+    st->restart = 1;
+    if (!isfinite(rr)) {
+        st->done = kNonFinite;
+    } else if (rr <= st->tol2) {
+        st->done = kConverged;
+    } else {
+        st->done = kRunning;
+        if (mode == 1) st->restarts += 1;
+        if (st->iter >= st->max_iter) st->done = kMaxIter;
+    }
+}
+
+// K1: p' = r + beta (p - omega v) (p' = r and rh = r on restart); v' = A p'.
+template <typename R, bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_bicg_p(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+         const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
+         const cx<R>* __restrict__ r, cx<R>* __restrict__ rh, const cx<R>* __restrict__ p_old,
+         const cx<R>* __restrict__ v_old, cx<R>* __restrict__ p_new, cx<R>* __restrict__ v_new,
+         double* __restrict__ partials, unsigned int* __restrict__ counters, KState* __restrict__ state) {
+    __shared__ double lds[kWaves * 2];
+    const cv_tile t = tiles[blockIdx.x];
+    KState* st = state + t.inst;
+    if (st->done != kRunning) return;
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const bool restart = st->restart != 0;
+    const cx<R> beta = ccast<R>(st->beta), omega = ccast<R>(st->omega);
+    const int n = d.nx * d.ny;
+    auto fp = [&](int64_t k) -> cx<R> {
+        if (restart) return r[k];
+        return r[k] + beta * (p_old[k] - omega * v_old[k]);
+    };
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> pk = fp(k);
+        p_new[k] = pk;
+        const cx<R> vk = apply_any<R, TRANS>(d, prof, m0, m1, i, j, fp, pk);
+        v_new[k] = vk;
+        cx<R> rhk;
+        if (restart) {
+            rhk = r[k];
+            rh[k] = rhk;
+        } else {
+            rhk = rh[k];
+        }
+        // (rh, v) = sum conj(rh) v
+        acc[0] += (double)rhk.re * (double)vk.re + (double)rhk.im * (double)vk.im;
+        acc[1] += (double)rhk.re * (double)vk.im - (double)rhk.im * (double)vk.re;
+    }
+    block_sum<2>(acc, lds);
+    double tot[2];
+    if (!last_block_reduce<2>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+                              lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    const cx<double> rhv = mk(tot[0], tot[1]);
+    st->restart = 0;
+    if (!cfinite(rhv)) {
+        st->done = kNonFinite;
+        return;
+    }
+    if (rhv.re == 0.0 && rhv.im == 0.0) {
+        st->done = kBreakdown;
+        return;
+    }
+    st->alpha = cdiv(st->rho, rhv);
+}
+
+// K2: s = r - alpha v'; t = A s; (t, s), (t, t) -> omega.
+template <typename R, bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_bicg_s(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+         const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
+         const cx<R>* __restrict__ r, const cx<R>* __restrict__ v, cx<R>* __restrict__ tv,
+         double* __restrict__ partials, unsigned int* __restrict__ counters, KState* __restrict__ state) {
+    __shared__ double lds[kWaves * 3];
+    const cv_tile t = tiles[blockIdx.x];
+    KState* st = state + t.inst;
+    if (st->done != kRunning) return;
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const cx<R> alpha = ccast<R>(st->alpha);
+    const int n = d.nx * d.ny;
+    auto fs = [&](int64_t k) -> cx<R> { return r[k] - alpha * v[k]; };
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> sk = fs(k);
+        const cx<R> tk = apply_any<R, TRANS>(d, prof, m0, m1, i, j, fs, sk);
+        tv[k] = tk;
+        acc[0] += (double)tk.re * (double)sk.re + (double)tk.im * (double)sk.im;   // conj(t) s
+        acc[1] += (double)tk.re * (double)sk.im - (double)tk.im * (double)sk.re;
+        acc[2] += norm2(ccast<double>(tk));
+    }
+    block_sum<3>(acc, lds);
+    double tot[3];
+    if (!last_block_reduce<3>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+                              lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    if (!(isfinite(tot[0]) && isfinite(tot[1]) && isfinite(tot[2]))) {
+        st->done = kNonFinite;
+        return;
+    }
+    st->omega = (tot[2] > 0.0) ? mk(tot[0] / tot[2], tot[1] / tot[2]) : mk(0.0, 0.0);
+}
+
+// K3: x += alpha p' + omega s; r = s - omega t; (rh, r), (r, r).
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_bicg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+         cx<R>* __restrict__ x, const cx<R>* __restrict__ p, cx<R>* __restrict__ r,
+         const cx<R>* __restrict__ v, const cx<R>* __restrict__ tv, const cx<R>* __restrict__ rh,
+         double* __restrict__ partials, unsigned int* __restrict__ counters, KState* __restrict__ state) {
+    __shared__ double lds[kWaves * 3];
+    const cv_tile t = tiles[blockIdx.x];
+    KState* st = state + t.inst;
+    if (st->done != kRunning) return;
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const cx<R> alpha = ccast<R>(st->alpha), omega = ccast<R>(st->omega);
+    const int n = d.nx * d.ny;
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> sk = r[k] - alpha * v[k];
+        x[k] = x[k] + alpha * p[k] + omega * sk;
+        const cx<R> rk = sk - omega * tv[k];
+        r[k] = rk;
+        const cx<R> hk = rh[k];
+        acc[0] += (double)hk.re * (double)rk.re + (double)hk.im * (double)rk.im;
+        acc[1] += (double)hk.re * (double)rk.im - (double)hk.im * (double)rk.re;
+        acc[2] += norm2(ccast<double>(rk));
+    }
+    block_sum<3>(acc, lds);
+    double tot[3];
+    if (!last_block_reduce<3>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+                              lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    const cx<double> rho_new = mk(tot[0], tot[1]);
+    const double rr = tot[2];
+    st->iter += 1;
+    st->rr = rr;
+    if (!(cfinite(rho_new) && isfinite(rr))) {
+        st->done = kNonFinite;
+    } else if (rr <= st->tol2) {
+        st->done = kConverged;
+    } else if (st->iter >= st->max_iter) {
+        st->done = kMaxIter;
+    } else {
+        const cx<double> om = st->omega;
+        const double rho_mag = sqrt(norm2(rho_new));
+        const bool om_zero = (om.re == 0.0 && om.im == 0.0);
+        if (om_zero || rho_mag <= 1e-13 * sqrt(st->rhrh * rr)) {
+            // breakdown / stagnation: restart with rh = r, p = r
+            st->restart = 1;
+            st->rho = mk(rr, 0.0);
+            st->rhrh = rr;
+            st->restarts += 1;
+        } else {
+            st->beta = cdiv(rho_new, st->rho) * cdiv(st->alpha, om);
+            st->rho = rho_new;
+        }
+    }
+}
+
+// ------------------------------------------------- mixed-precision helpers
+// rhs32 = r64 / sqrt(rr) (per instance), x32 = 0.  Instances that have already
+// converged get rhs32 = 0, so the inner solve marks them converged at once.
+__global__ void __launch_bounds__(kBlock)
+k_mixed_rhs(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+            const cx<double>* __restrict__ r64, cx<float>* __restrict__ b32, cx<float>* __restrict__ x32,
+            KState* __restrict__ outer) {
+    const cv_tile t = tiles[blockIdx.x];
+    KState* st = outer + t.inst;
+    const bool active = st->done == kRunning;
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+    const double sc = active ? 1.0 / sqrt(st->rr_true) : 0.0;
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int64_t k = d.cell_off + loc;
+        const cx<double> rv = active ? r64[k] : czero<double>();
+        b32[k] = mk<float>(float(rv.re * sc), float(rv.im * sc));
+        x32[k] = czero<float>();
+    }
+}
+
+// x64 += sqrt(rr) * x32 (per instance, only for instances still running)
+__global__ void __launch_bounds__(kBlock)
+k_mixed_update(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+               cx<double>* __restrict__ x64, const cx<float>* __restrict__ x32,
+               const KState* __restrict__ outer) {
+    const cv_tile t = tiles[blockIdx.x];
+    const KState* st = outer + t.inst;
+    if (st->done != kRunning) return;
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+    const double sc = sqrt(st->rr_true);
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int64_t k = d.cell_off + loc;
+        const cx<float> dv = x32[k];
+        const cx<double> xv = x64[k];
+        x64[k] = mk(xv.re + sc * (double)dv.re, xv.im + sc * (double)dv.im);
+    }
+}
+
+// outer residual r64 = b64 - A64 x64 with per-instance norms; sets outer
+// state: done = converged if ||r|| <= tol ||b||, else running.
+template <bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_mixed_resid(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+              const cx<double>* __restrict__ prof, const double* __restrict__ m0,
+              const double* __restrict__ m1, const cx<double>* __restrict__ b,
+              const cx<double>* __restrict__ x, cx<double>* __restrict__ r, double* __restrict__ partials,
+              unsigned int* __restrict__ counters, KState* __restrict__ outer, double tol, int first) {
+    __shared__ double lds[kWaves * 2];
+    const cv_tile t = tiles[blockIdx.x];
+    KState* st = outer + t.inst;
+    if (!first && st->done != kRunning) return;
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+    auto fx = [&](int64_t k) { return x[k]; };
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<double> bk = b[k];
+        const cx<double> rk = bk - apply_any<double, TRANS>(d, prof, m0, m1, i, j, fx, x[k]);
+        r[k] = rk;
+        acc[0] += norm2(rk);
+        acc[1] += norm2(bk);
+    }
+    block_sum<2>(acc, lds);
+    double tot[2];
+    if (!last_block_reduce<2>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+                              lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    if (first) {
+        st->bb = tot[1];
+        st->tol2 = tol * tol * tot[1];
+        st->iter = 0;
+        st->restarts = 0;
+    }
+    st->rr_true = tot[0];
+    st->rr = tot[0];
+    if (!isfinite(tot[0]))
+        st->done = kNonFinite;
+    else if (tot[0] <= st->tol2)
+        st->done = kConverged;
+    else
+        st->done = kRunning;
+}
+
+// --------------------------------------------------------------- host side
+struct Work {
+    void* r;
+    void* rh;
+    void* p[2];
+    void* v[2];
+    void* t;
+    KState* state;
+    unsigned int* counters;
+    double* partials;
+};
+
+static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+static int64_t layout(int64_t cs, int64_t cells, int n_inst, int n_tiles, char* base, Work* w) {
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) -> void* {
+        void* p = base ? (void*)(base + off) : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    Work tmp;
+    Work* o = w ? w : &tmp;
+    o->r = take(cs * cells);
+    o->rh = take(cs * cells);
+    o->p[0] = take(cs * cells);
+    o->p[1] = take(cs * cells);
+    o->v[0] = take(cs * cells);
+    o->v[1] = take(cs * cells);
+    o->t = take(cs * cells);
+    o->state = (KState*)take(sizeof(KState) * (int64_t)n_inst);
+    o->counters = (unsigned int*)take(sizeof(unsigned int) * (int64_t)n_inst);
+    o->partials = (double*)take(sizeof(double) * 4 * (int64_t)n_tiles);
+    return off;
+}
+
+template <typename R, bool TRANS>
+static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst, const cv_tile* tiles,
+                      int n_tiles, const void* prof_, const void* m0_, const void* m1_, const void* b_, void* x_,
+                      double tol, int max_iter, int check_every, Work& w, std::vector<KState>& hs,
+                      hipStream_t s) {
+    const cx<R>* prof = (const cx<R>*)prof_;
+    const R* m0 = (const R*)m0_;
+    const R* m1 = (const R*)m1_;
+    const cx<R>* b = (const cx<R>*)b_;
+    cx<R>* x = (cx<R>*)x_;
+    cx<R>* r = (cx<R>*)w.r;
+    cx<R>* rh = (cx<R>*)w.rh;
+    cx<R>* p[2] = {(cx<R>*)w.p[0], (cx<R>*)w.p[1]};
+    cx<R>* v[2] = {(cx<R>*)w.v[0], (cx<R>*)w.v[1]};
+    cx<R>* tv = (cx<R>*)w.t;
+    (void)desc_host;
+    k_zero_u32<<<(n_inst + 255) / 256, 256, 0, s>>>(w.counters, n_inst);
+    // p and v start at zero (they are only read when restart == 0, which never
+    // happens before the first K1), so no clear is needed.
+    k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.partials, w.counters,
+                                                w.state, tol, max_iter, 0);
+    CV_LAUNCH_CHECK();
+    int cur = 0;
+    const int max_checks = 8;
+    int checks = 0;
+    hs.resize(n_inst);
+    while (true) {
+        for (int c = 0; c < check_every; ++c) {
+            const int nxt = cur ^ 1;
+            k_bicg_p<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, r, rh, p[cur], v[cur], p[nxt],
+                                                         v[nxt], w.partials, w.counters, w.state);
+            k_bicg_s<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, r, v[nxt], tv, w.partials,
+                                                         w.counters, w.state);
+            k_bicg_x<R><<<n_tiles, kBlock, 0, s>>>(desc, tiles, x, p[nxt], r, v[nxt], tv, rh, w.partials,
+                                                   w.counters, w.state);
+            cur = nxt;
+        }
+        CV_LAUNCH_CHECK();
+        CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
+        CV_HIP(hipStreamSynchronize(s));
+        bool all_done = true;
+        for (int k = 0; k < n_inst; ++k) all_done &= (hs[k].done != kRunning);
+        if (!all_done) continue;
+        // verify converged instances with the true residual (restarts them if needed)
+        k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.partials, w.counters,
+                                                    w.state, tol, max_iter, 1);
+        CV_LAUNCH_CHECK();
+        CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
+        CV_HIP(hipStreamSynchronize(s));
+        bool again = false;
+        for (int k = 0; k < n_inst; ++k) again |= (hs[k].done == kRunning);
+        if (!again || ++checks >= max_checks) break;
+    }
+    return CV_OK;
+}
+
+static int status_of(const KState& k) {
+    switch (k.done) {
+        case kConverged: return CV_OK;
+        case kMaxIter: return CV_ERR_NOT_CONVERGED;
+        case kNonFinite: return CV_ERR_NONFINITE;
+        case kBreakdown: return CV_ERR_BREAKDOWN;
+        default: return CV_ERR_NOT_CONVERGED;
+    }
+}
+
+static void fill_info(const std::vector<KState>& hs, cv_solve_info* info, int n_inst, int* worst) {
+    *worst = CV_OK;
+    for (int k = 0; k < n_inst; ++k) {
+        info[k].iterations = hs[k].iter;
+        info[k].status = status_of(hs[k]);
+        info[k].restarts = hs[k].restarts;
+        info[k].pad = 0;
+        info[k].rel_residual = hs[k].bb > 0 ? sqrt(hs[k].rr_true / hs[k].bb) : 0.0;
+        if (info[k].status != CV_OK && *worst == CV_OK) *worst = info[k].status;
+    }
+}
+
+}  // namespace cv
+
+using namespace cv;
+
+extern "C" int64_t cv_fdfd_solve_workspace_bytes(int dtype, int64_t total_cells, int n_inst, int n_tiles) {
+    const int64_t cs = (dtype == CV_C128) ? 16 : 8;
+    return layout(cs, total_cells, n_inst, n_tiles, nullptr, nullptr);
+}
+
+extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                             const cv_tile* tiles, int n_tiles, const void* prof, const void* m0, const void* m1,
+                             const void* b, void* x, int transpose, double tol, int max_iter, int check_every,
+                             void* workspace, int64_t workspace_bytes, cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && tiles && prof && m0 && b && x && workspace && info, "null pointer argument");
+    CV_CHECK(n_inst > 0 && n_tiles > 0, "empty batch");
+    CV_CHECK(dtype == CV_C64 || dtype == CV_C128, "unknown dtype %d", dtype);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    int64_t cells = 0;
+    for (int k = 0; k < n_inst; ++k) {
+        CV_CHECK(desc_host[k].nx > 0 && desc_host[k].ny > 0, "instance %d has an empty grid", k);
+        cells = std::max<int64_t>(cells, desc_host[k].cell_off + (int64_t)desc_host[k].nx * desc_host[k].ny);
+        if (desc_host[k].pol == CV_POL_HZ) CV_CHECK(m1 != nullptr, "Hz operator needs m1");
+    }
+    const int64_t cs = (dtype == CV_C128) ? 16 : 8;
+    Work w;
+    const int64_t need = layout(cs, cells, n_inst, n_tiles, (char*)workspace, &w);
+    CV_CHECK(workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+             (long long)need);
+    if (check_every <= 0) check_every = 16;
+    hipStream_t s = as_stream(stream);
+    std::vector<KState> hs;
+    int rc;
+    if (dtype == CV_C128)
+        rc = transpose ? solve_impl<double, true>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
+                                                  max_iter, check_every, w, hs, s)
+                       : solve_impl<double, false>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x,
+                                                   tol, max_iter, check_every, w, hs, s);
+    else
+        rc = transpose ? solve_impl<float, true>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
+                                                 max_iter, check_every, w, hs, s)
+                       : solve_impl<float, false>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
+                                                  max_iter, check_every, w, hs, s);
+    if (rc != CV_OK) return rc;
+    int worst;
+    fill_info(hs, info, n_inst, &worst);
+    return worst;
+}
+
+// ------------------------------------------------------------ mixed solve
+extern "C" int64_t cv_fdfd_solve_mixed_workspace_bytes(int64_t total_cells, int n_inst, int n_tiles) {
+    int64_t inner = layout(8, total_cells, n_inst, n_tiles, nullptr, nullptr);
+    // r64, b32, x32, outer state, outer counters, outer partials
+    return inner + align256(16 * total_cells) + 2 * align256(8 * total_cells) +
+           align256(sizeof(KState) * (int64_t)n_inst) + align256(4 * (int64_t)n_inst) +
+           align256(sizeof(double) * 4 * (int64_t)n_tiles);
+}
+
+extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                                   const cv_tile* tiles, int n_tiles, const void* prof64, const void* m0_64,
+                                   const void* m1_64, const void* prof32, const void* m0_32, const void* m1_32,
+                                   const void* b64, void* x64, int transpose, double tol, double inner_tol,
+                                   int max_iter, int max_sweeps, int check_every, void* workspace,
+                                   int64_t workspace_bytes, cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && tiles && prof64 && m0_64 && prof32 && m0_32 && b64 && x64 && workspace && info,
+             "null pointer argument");
+    CV_CHECK(n_inst > 0 && n_tiles > 0, "empty batch");
+    int64_t cells = 0;
+    for (int k = 0; k < n_inst; ++k)
+        cells = std::max<int64_t>(cells, desc_host[k].cell_off + (int64_t)desc_host[k].nx * desc_host[k].ny);
+    const int64_t need = cv_fdfd_solve_mixed_workspace_bytes(cells, n_inst, n_tiles);
+    CV_CHECK(workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+             (long long)need);
+    if (check_every <= 0) check_every = 16;
+    if (max_sweeps <= 0) max_sweeps = 10;
+    char* base = (char*)workspace;
+    Work w;
+    int64_t off = layout(8, cells, n_inst, n_tiles, base, &w);
+    cx<double>* r64 = (cx<double>*)(base + off);
+    off += align256(16 * cells);
+    cx<float>* b32 = (cx<float>*)(base + off);
+    off += align256(8 * cells);
+    cx<float>* x32 = (cx<float>*)(base + off);
+    off += align256(8 * cells);
+    KState* outer = (KState*)(base + off);
+    off += align256(sizeof(KState) * (int64_t)n_inst);
+    unsigned int* ocnt = (unsigned int*)(base + off);
+    off += align256(4 * (int64_t)n_inst);
+    double* opart = (double*)(base + off);
+    hipStream_t s = as_stream(stream);
+    k_zero_u32<<<(n_inst + 255) / 256, 256, 0, s>>>(ocnt, n_inst);
+    std::vector<KState> ho(n_inst), hi;
+    std::vector<int> iters(n_inst, 0), sweeps(n_inst, 0);
+    auto resid = [&](int first) -> int {
+        if (transpose)
+            k_mixed_resid<true><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof64,
+                                                           (const double*)m0_64, (const double*)m1_64,
+                                                           (const cx<double>*)b64, (const cx<double>*)x64, r64,
+                                                           opart, ocnt, outer, tol, first);
+        else
+            k_mixed_resid<false><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof64,
+                                                            (const double*)m0_64, (const double*)m1_64,
+                                                            (const cx<double>*)b64, (const cx<double>*)x64, r64,
+                                                            opart, ocnt, outer, tol, first);
+        CV_LAUNCH_CHECK();
+        CV_HIP(hipMemcpyAsync(ho.data(), outer, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
+        CV_HIP(hipStreamSynchronize(s));
+        return CV_OK;
+    };
+    int rc = resid(1);
+    if (rc != CV_OK) return rc;
+    int inner_status = CV_OK;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        bool any = false;
+        for (int k = 0; k < n_inst; ++k) any |= (ho[k].done == kRunning);
+        if (!any) break;
+        k_mixed_rhs<<<n_tiles, kBlock, 0, s>>>(desc, tiles, r64, b32, x32, outer);
+        CV_LAUNCH_CHECK();
+        // Inner solve over all tiles: instances already converged get a zero rhs
+        // (b32 left from their last sweep is irrelevant: their x64 is not updated).
+        if (transpose)
+            rc = solve_impl<float, true>(desc, desc_host, n_inst, tiles, n_tiles, prof32, m0_32, m1_32, b32, x32,
+                                         inner_tol, max_iter, check_every, w, hi, s);
+        else
+            rc = solve_impl<float, false>(desc, desc_host, n_inst, tiles, n_tiles, prof32, m0_32, m1_32, b32, x32,
+                                          inner_tol, max_iter, check_every, w, hi, s);
+        if (rc != CV_OK) return rc;
+        for (int k = 0; k < n_inst; ++k) {
+            if (ho[k].done == kRunning) {
+                iters[k] += hi[k].iter;
+                sweeps[k] += 1;
+                if (hi[k].done == kNonFinite || hi[k].done == kBreakdown) inner_status = status_of(hi[k]);
+            }
+        }
+        k_mixed_update<<<n_tiles, kBlock, 0, s>>>(desc, tiles, (cx<double>*)x64, x32, outer);
+        CV_LAUNCH_CHECK();
+        rc = resid(0);
+        if (rc != CV_OK) return rc;
+    }
+    int worst = CV_OK;
+    for (int k = 0; k < n_inst; ++k) {
+        info[k].iterations = iters[k];
+        info[k].restarts = sweeps[k];
+        info[k].pad = 0;
+        info[k].rel_residual = ho[k].bb > 0 ? sqrt(ho[k].rr_true / ho[k].bb) : 0.0;
+        info[k].status = (ho[k].done == kConverged) ? CV_OK
+                         : (ho[k].done == kNonFinite) ? CV_ERR_NONFINITE
+                                                      : CV_ERR_NOT_CONVERGED;
+        if (info[k].status != CV_OK && worst == CV_OK) worst = info[k].status;
+    }
+    if (worst == CV_OK && inner_status != CV_OK) worst = CV_OK;  // outer residual is the judge
+    return worst;
+}

@@ -1,0 +1,176 @@
+/*
+ * ceviche_amd C-ABI: MI355X (gfx950) kernels for ceviche's simulation hot path.
+ *
+ * Every entry point takes caller-owned DEVICE pointers (plain void* + sizes),
+ * an explicit hipStream_t (passed as void*), and returns an int status
+ * (CV_OK on success).  The library never allocates device memory: all
+ * workspace is passed in by the caller, sized by the *_workspace_bytes
+ * helpers.  No torch types cross this boundary.
+ *
+ * Layouts
+ *   FDFD fields: complex, C order over (nx, ny) (y contiguous), packed over a
+ *   batch of instances at desc[k].cell_off.  Complex values are interleaved
+ *   (re, im) pairs of float (CV_C64) or double (CV_C128).
+ *   FDTD fields: real float/double, C order over (nx, ny, nz).
+ *
+ * The reference interface each entry point replaces is cited (file:line in
+ * fancompute/ceviche 0.1.3).
+ */
+#ifndef CEVICHE_AMD_H
+#define CEVICHE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define CV_OK                 0
+#define CV_ERR_INVALID        1   /* bad argument / shape                      */
+#define CV_ERR_HIP            2   /* a HIP runtime call failed                 */
+#define CV_ERR_NOT_CONVERGED  3   /* Krylov hit max_iter above tolerance       */
+#define CV_ERR_NONFINITE      4   /* NaN/Inf appeared in a reduction           */
+#define CV_ERR_BREAKDOWN      5   /* Krylov breakdown that restarts could not fix */
+#define CV_ERR_SINGULAR       6   /* zero pivot in the dense LU                */
+
+/* ------------------------------------------------------------------ dtypes */
+#define CV_C64   0   /* complex64 vectors, float32 materials  */
+#define CV_C128  1   /* complex128 vectors, float64 materials */
+#define CV_F32   0   /* FDTD float32 */
+#define CV_F64   1   /* FDTD float64 */
+
+/* ---------------------------------------------------------- FDFD operators */
+#define CV_POL_EZ 0
+#define CV_POL_HZ 1
+#define CV_FLAG_X_IDENTITY 1   /* Nx == 1: Dx* is the identity (derivatives.py:76-77) */
+#define CV_FLAG_Y_IDENTITY 2   /* Ny == 1: Dy* is the identity (derivatives.py:78-79) */
+
+/* One FDFD instance of a batch.  Plain data, 96 bytes, device-resident array.
+ *   pol == CV_POL_EZ:  A = scale*(Dxf Dxb + Dyf Dyb) + diag(dshift*(m0 + i m1))
+ *                      scale = -1/mu0, dshift = -eps0 w^2, m0 = Re eps,
+ *                      m1 = Im eps or NULL for real eps           (fdfd.py:172-185)
+ *   pol == CV_POL_HZ:  A = scale*(Dxb diag(m1) Dxf + Dyb diag(m0) Dyf) + dshift*I
+ *                      scale = 1/eps0, dshift = mu0 w^2,
+ *                      m0 = 1/(eps_xx+1e-5), m1 = 1/(eps_yy+1e-5)      (fdfd.py:211-233)
+ * Profiles at prof_off: kxf[nx], kxb[nx], kyf[ny], kyb[ny] (complex, operator
+ * precision) holding 1/(s*dL), or 1/s on an identity axis (derivatives.py:34-59).
+ * Materials at cell_off in m0 (and m1 for Hz), real, operator precision. */
+typedef struct cv_fdfd_desc {
+    int32_t nx, ny;
+    int32_t pol, flags;
+    int64_t cell_off;
+    int64_t prof_off;
+    int32_t tile_first, tile_count;  /* this instance's range in the tile table   */
+    double  scale, dshift;
+    double  phx_re, phx_im, phy_re, phy_im;   /* Bloch phasors exp(i*theta)       */
+    double  omega;
+} cv_fdfd_desc;
+
+/* Tile table entry: one workgroup processes CV_TILE_CELLS consecutive cells of
+ * one instance starting at cell `first` (instance-local index). */
+typedef struct cv_tile { int32_t inst, first; } cv_tile;
+#define CV_TILE_CELLS 1024
+
+/* y = A x (transpose == 0) or y = A^T x (transpose != 0) for every instance.
+ * Replaces the COO assembly + CSR SpMV of fdfd.py:172-185 / :211-233 and
+ * primitives.py:58-72 on the FDFD operator. */
+int cv_fdfd_apply(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                  const void* prof, const void* m0, const void* m1,
+                  const void* x, void* y, int transpose, void* stream);
+
+/* Hz materials: m0 = 1/((eps + roll(eps,1,axis=1))/2 + 1e-5),
+ *               m1 = 1/((eps + roll(eps,1,axis=0))/2 + 1e-5).   fdfd.py:200-215 */
+int cv_fdfd_hz_materials(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                         const void* eps, void* m0, void* m1, void* stream);
+
+/* One PML-scaled Yee difference operator D in {Dxf, Dxb, Dyf, Dyb}
+ * (which = 0, 1, 2, 3; derivatives.py:34-124), optionally with a real
+ * per-cell material `mat` (may be NULL):
+ *   mode 0:  out = coef * mat .* (D u)
+ *   mode 1:  out = coef * D^T (mat .* u)
+ *   mode 2:  out = conj(coef) * D^H (mat .* u)
+ * accumulate != 0 adds into out.  Field recovery (fdfd.py:135-162) is two
+ * mode-0 calls; its reverse mode uses mode 2.  Replaces the sp_mult of the
+ * D matrices (fdfd.py:96-99 -> primitives.py:58-72). */
+int cv_fdfd_deriv(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                  const void* prof, const void* mat, const void* u, void* out, int which,
+                  int mode, double coef_re, double coef_im, int accumulate, void* stream);
+
+/* Matrix-free gradient of the operator entries that depend on the materials:
+ *   g0[k] += scale * Re( lam^T (dA/dm0[k]) x ),   g1[k] likewise for m1,
+ * i.e. the gather adj[i]*x[j] of primitives.py:121-128 restricted to the
+ * entries that depend on eps (the diagonal for Ez, fdfd.py:179; the
+ * Dxb diag(m1) Dxf / Dyb diag(m0) Dyf products for Hz, fdfd.py:219-223).
+ * For Ez, g1 (the Im eps part) may be NULL.  Real outputs in operator
+ * precision, accumulated.  The caller may scale the whole system (scale and
+ * dshift multiplied by one constant); the kernels use the descriptor as is. */
+int cv_fdfd_material_grad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                          const void* prof, const void* lam, const void* x,
+                          void* g0, void* g1, double scale, void* stream);
+
+/* Reverse mode of cv_fdfd_hz_materials: g_eps += d(m0, m1)/d eps ^T (g_m0, g_m1)
+ * (g_m0 or g_m1 may be NULL).  fdfd.py:200-215. */
+int cv_fdfd_hz_materials_grad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                              const void* m0, const void* m1, const void* g_m0, const void* g_m1,
+                              void* g_eps, void* stream);
+
+/* Reverse mode of cv_fdfd_deriv (mode 0) with respect to its material:
+ * g_mat[k] += Re( conj(coef (D u)[k]) g[k] ).  Used for the eps dependence
+ * of the Hz field recovery (fdfd.py:150-157). */
+int cv_fdfd_deriv_matgrad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                          const void* prof, const void* u, const void* g, void* g_mat, int which,
+                          double coef_re, double coef_im, void* stream);
+
+/* ------------------------------------------------------------- Krylov solve */
+#define CV_KRYLOV_BICGSTAB 0
+
+/* Per-instance solver telemetry written back to a host array. */
+typedef struct cv_solve_info {
+    int32_t iterations;     /* Krylov iterations performed                       */
+    int32_t status;         /* CV_OK / CV_ERR_NOT_CONVERGED / ...                */
+    int32_t restarts;       /* breakdown restarts + refinement sweeps            */
+    int32_t pad;
+    double  rel_residual;   /* final ||b - A x|| / ||b||, recomputed in FP64      */
+} cv_solve_info;
+
+/* Device workspace in bytes for cv_fdfd_solve with `total_cells` packed cells,
+ * `n_inst` instances and `n_tiles` tiles. */
+int64_t cv_fdfd_solve_workspace_bytes(int dtype, int64_t total_cells, int n_inst, int n_tiles);
+
+/* Solve A x = b (or A^T x = b) for every instance of the batch with BiCGSTAB
+ * (fused stencil + axpy + FP64 dot kernels).  x holds the initial guess on
+ * entry.  With dtype CV_C64 and refine != 0, the outer residual and x are kept
+ * in complex128 (x64, b64 given as complex128 buffers) and each sweep solves a
+ * complex64 correction (mixed-precision iterative refinement).
+ * Replaces solvers.py:37-75 (spsolve / scipy Krylov) behind primitives.py:103-119.
+ * info: host array of n_inst entries. */
+int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                  const cv_tile* tiles, int n_tiles,
+                  const void* prof, const void* m0, const void* m1,
+                  const void* b, void* x, int transpose,
+                  double tol, int max_iter, int check_every,
+                  void* workspace, int64_t workspace_bytes,
+                  cv_solve_info* info, void* stream);
+
+/* Mixed precision: complex128 x/b/operator (prof64, m0_64, m1_64) for the
+ * residual, complex64 copies (prof32, m0_32, m1_32) for the inner BiCGSTAB. */
+int64_t cv_fdfd_solve_mixed_workspace_bytes(int64_t total_cells, int n_inst, int n_tiles);
+int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                        const cv_tile* tiles, int n_tiles,
+                        const void* prof64, const void* m0_64, const void* m1_64,
+                        const void* prof32, const void* m0_32, const void* m1_32,
+                        const void* b64, void* x64, int transpose,
+                        double tol, double inner_tol, int max_iter, int max_sweeps, int check_every,
+                        void* workspace, int64_t workspace_bytes,
+                        cv_solve_info* info, void* stream);
+
+/* ------------------------------------------------------------------ misc */
+const char* cv_version(void);
+const char* cv_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CEVICHE_AMD_H */
