@@ -8,3 +8,4 @@ __version__ = '0.1.0'
 
 from .fdfd import fdfd_ez, fdfd_hz  # noqa: E402,F401
 from ._fdfd_op import SolverOptions  # noqa: E402,F401
+from .fdtd import fdtd  # noqa: E402,F401

@@ -1,0 +1,247 @@
+"""Device-side FDTD grid and its differentiable entry points.
+
+``FdtdGrid`` holds the per-axis PML conductivity profiles (the reference's
+sigma tensors vary along one axis each, fdtd.py:224-263) and the time step,
+and wraps the FDTD C-ABI of include/ceviche_amd.h.
+
+Two differentiable paths:
+  * FdtdStep: one Yee step as a torch.autograd.Function (drop-in for the
+    reference's per-step ``forward`` inside an autograd loop); its backward is
+    the hand-written adjoint step kernel.
+  * FdtdRun: many steps with a point source and probes, reverse mode by
+    checkpoint-and-recompute (state saved every ``checkpoint_every`` steps;
+    each segment is re-run forward to rebuild its D history, then swept
+    backward with the adjoint kernels).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_ptr
+from .constants import C_0, EPSILON_0
+
+NSTATE = 18
+COMPONENTS = {'Ex': 0, 'Ey': 1, 'Ez': 2, 'Dx': 3, 'Dy': 4, 'Dz': 5, 'Hx': 6, 'Hy': 7, 'Hz': 8}
+_CODE = {torch.float32: _lib.CV_C64, torch.float64: _lib.CV_C128}  # CV_F32 == 0, CV_F64 == 1
+
+
+def time_step(dL, stability_factor=0.5):
+    """dt = 0.5 dL / (sqrt(3) C0) -- the 3D Courant limit even in 2D; fdtd.py:213-222."""
+    dL_sum = 3 / dL ** 2
+    dL_avg = 1 / np.sqrt(dL_sum)
+    return dL_avg / C_0 * stability_factor
+
+
+def sigma_profiles(shape, npml, dt):
+    """Per-axis (sigH, sigD) 1D profiles sampled from the doubled grid; fdtd.py:224-263.
+
+    On the 2x grid, index 2p - n + 1 (low side) and 2N - 2p + n (high side) get
+    (0.5 eps0/dt) (n / 2p)^3 for n < 2p; H takes odd, D even samples along the
+    component's own axis."""
+    sH, sD = [], []
+    for a in range(3):
+        N, p = int(shape[a]), int(npml[a])
+        s2 = np.zeros(2 * N)
+        for n in range(2 * p):
+            val = (0.5 * EPSILON_0 / dt) * (n / 2 / p) ** 3
+            s2[2 * p - n + 1] = val
+            s2[2 * N - 2 * p + n] = val
+        sH.append(s2[1::2].copy())
+        sD.append(s2[0::2].copy())
+    return sH, sD
+
+
+class FdtdGrid:
+    """Geometry, time step and PML profiles of one FDTD problem on the GPU."""
+
+    def __init__(self, shape, dL, npml, dt, dtype=torch.float64, device=None):
+        self.shape = tuple(int(v) for v in shape)
+        self.nx, self.ny, self.nz = self.shape
+        self.n = self.nx * self.ny * self.nz
+        self.dL = float(dL)
+        self.dt = float(dt)
+        self.npml = list(npml)
+        self.dtype = dtype
+        self.code = _CODE[dtype]
+        self.device = device if device is not None else _lib.device()
+        sH, sD = sigma_profiles(self.shape, self.npml, self.dt)
+        self.sig_host = np.concatenate(sH + sD)
+        self.sig = torch.from_numpy(self.sig_host).to(self.device, dtype)
+        self.desc = _lib.FdtdDesc(self.nx, self.ny, self.nz, 0, self.dt, self.dL, C_0, EPSILON_0)
+        self._tmp = None
+
+    def new_state(self):
+        return torch.zeros((NSTATE,) + self.shape, dtype=self.dtype, device=self.device)
+
+    def _d(self):
+        return ctypes.byref(self.desc)
+
+    def step(self, state, eps, J=(None, None, None), jscale=1.0, n_steps=1, src=None, amp=None,
+             probes=None, rec=None):
+        """Advance ``state`` in place.  src = (comp 0..2, flat cell); amp: host
+        float64 array of n_steps; probes = (comp int32 tensor, cell int64 tensor)."""
+        comp, cell = (-1, 0) if src is None else (int(src[0]), int(src[1]))
+        amp_p = None
+        if comp >= 0:
+            amp = np.ascontiguousarray(amp, dtype=np.float64)
+            assert amp.size >= n_steps
+            amp_p = amp.ctypes.data_as(ctypes.c_void_p)
+        pc, pl = (None, None) if probes is None else probes
+        npb = 0 if probes is None else int(pc.numel())
+        Jx, Jy, Jz = (None if j is None else j.to(self.device, self.dtype).contiguous() for j in J)
+        check(lib().cv_fdtd_step(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(Jx), ptr(Jy),
+                                 ptr(Jz), float(jscale), comp, cell, amp_p, int(n_steps), ptr(pc), ptr(pl), npb,
+                                 ptr(rec), stream_ptr()), 'fdtd step')
+        return state
+
+    def efields(self, state, eps):
+        E = torch.empty((3,) + self.shape, dtype=self.dtype, device=self.device)
+        check(lib().cv_fdtd_efields(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(E),
+                                    stream_ptr()), 'fdtd efields')
+        return E
+
+    def efields_grad(self, state, eps, gE, gD, gmE):
+        check(lib().cv_fdtd_efields_grad(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state),
+                                         ptr(gE.contiguous()), ptr(gD), ptr(gmE), stream_ptr()),
+              'fdtd efields_grad')
+
+    def tmp(self):
+        if self._tmp is None:
+            self._tmp = torch.empty((3,) + self.shape, dtype=self.dtype, device=self.device)
+        return self._tmp
+
+    def adjoint(self, lam, eps, D_prev, gmE, probes=None, g_probe=None, D_t=None):
+        """Reverse one step in place (lam: after step t -> before step t)."""
+        pc, pl = (None, None) if probes is None else probes
+        npb = 0 if probes is None else int(pc.numel())
+        check(lib().cv_fdtd_adjoint_step(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(lam), ptr(self.tmp()),
+                                         ptr(D_prev), ptr(gmE), ptr(pc), ptr(pl), npb, ptr(g_probe), ptr(D_t),
+                                         stream_ptr()), 'fdtd adjoint')
+
+    def eps_grad(self, eps, gmE, geps):
+        check(lib().cv_fdtd_eps_grad(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(gmE), ptr(geps),
+                                     stream_ptr()), 'fdtd eps_grad')
+        return geps
+
+
+class FdtdStep(torch.autograd.Function):
+    """state_t = step(state_{t-1}; eps, J): one reference ``forward`` (fdtd.py:74-144)."""
+
+    @staticmethod
+    def forward(ctx, state, eps, Jx, Jy, Jz, grid):
+        new = state.detach().clone()
+        grid.step(new, eps, J=(Jx, Jy, Jz))
+        ctx.grid = grid
+        ctx.save_for_backward(state, eps)
+        ctx.has_J = (Jx is not None, Jy is not None, Jz is not None)
+        return new
+
+    @staticmethod
+    def backward(ctx, g):
+        state, eps = ctx.saved_tensors
+        grid = ctx.grid
+        lam = g.detach().clone().contiguous()
+        gJ = [lam[3 + a].clone() if (ctx.has_J[a] and ctx.needs_input_grad[2 + a]) else None for a in range(3)]
+        gmE = torch.zeros((3,) + grid.shape, dtype=grid.dtype, device=grid.device)
+        grid.adjoint(lam, eps, state[3:6].contiguous(), gmE)
+        geps = None
+        if ctx.needs_input_grad[1]:
+            geps = grid.eps_grad(eps, gmE, torch.zeros_like(eps))
+        return lam, geps, gJ[0], gJ[1], gJ[2], None
+
+
+class FdtdEFields(torch.autograd.Function):
+    """E = D / eps_aa (fdtd.py:134-137)."""
+
+    @staticmethod
+    def forward(ctx, state, eps, grid):
+        ctx.grid = grid
+        ctx.save_for_backward(state, eps)
+        return grid.efields(state, eps)
+
+    @staticmethod
+    def backward(ctx, gE):
+        state, eps = ctx.saved_tensors
+        grid = ctx.grid
+        gstate = torch.zeros_like(state)
+        gmE = torch.zeros((3,) + grid.shape, dtype=grid.dtype, device=grid.device) if ctx.needs_input_grad[1] else None
+        grid.efields_grad(state, eps, gE, gstate[3:6], gmE)
+        geps = grid.eps_grad(eps, gmE, torch.zeros_like(eps)) if gmE is not None else None
+        return gstate, geps, None
+
+
+def _probe_tensors(grid, probes):
+    comps = []
+    cells = []
+    for name, pos in probes:
+        comps.append(COMPONENTS[name] if isinstance(name, str) else int(name))
+        pos = tuple(pos) + (0,) * (3 - len(pos))
+        i, j, k = (int(v) for v in pos)
+        if not (0 <= i < grid.nx and 0 <= j < grid.ny and 0 <= k < grid.nz):
+            raise ValueError('probe position %s outside the grid %s' % (pos, grid.shape))
+        cells.append((i * grid.ny + j) * grid.nz + k)
+    pc = torch.tensor(comps, dtype=torch.int32, device=grid.device)
+    pl = torch.tensor(cells, dtype=torch.int64, device=grid.device)
+    return pc, pl
+
+
+class FdtdRun(torch.autograd.Function):
+    """rec[t, p] = probe p after step t0+t for a point source amp(t), many steps in
+    one call; reverse mode with checkpoint-and-recompute."""
+
+    @staticmethod
+    def forward(ctx, eps, amp, grid, state0, src, probes, n_steps, checkpoint_every):
+        amp_h = amp.detach().to('cpu', torch.float64).numpy()
+        state = state0.detach().clone()
+        pc, pl = probes
+        rec = torch.empty((n_steps, pc.numel()), dtype=grid.dtype, device=grid.device)
+        ck = max(1, int(checkpoint_every))
+        need_grad = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
+        checkpoints = []
+        t = 0
+        while t < n_steps:
+            m = min(ck, n_steps - t)
+            if need_grad:
+                checkpoints.append(state.clone())
+            grid.step(state, eps, n_steps=m, src=src, amp=amp_h[t:t + m], probes=probes, rec=rec[t:t + m])
+            t += m
+        ctx.grid, ctx.src, ctx.probes, ctx.n_steps, ctx.ck = grid, src, probes, n_steps, ck
+        ctx.amp_h = amp_h
+        ctx.checkpoints = checkpoints
+        ctx.save_for_backward(eps)
+        ctx.final_state = state
+        return rec, state
+
+    @staticmethod
+    def backward(ctx, grec, gstate_final):
+        (eps,) = ctx.saved_tensors
+        grid, src, probes, n_steps, ck = ctx.grid, ctx.src, ctx.probes, ctx.n_steps, ctx.ck
+        lam = grid.new_state()
+        if gstate_final is not None:
+            lam += gstate_final
+        gmE = torch.zeros((3,) + grid.shape, dtype=grid.dtype, device=grid.device)
+        grec = grec.to(grid.dtype).contiguous() if grec is not None else None
+        nseg = len(ctx.checkpoints)
+        # D history of one segment: Dhist[s] = D after step (seg_start + s)
+        Dhist = torch.empty((ck + 1, 3) + grid.shape, dtype=grid.dtype, device=grid.device)
+        for sgi in range(nseg - 1, -1, -1):
+            t0 = sgi * ck
+            m = min(ck, n_steps - t0)
+            st = ctx.checkpoints[sgi].clone()
+            Dhist[0].copy_(st[3:6])
+            for s in range(m):
+                grid.step(st, eps, n_steps=1, src=src, amp=ctx.amp_h[t0 + s:t0 + s + 1])
+                Dhist[s + 1].copy_(st[3:6])
+            for s in range(m - 1, -1, -1):
+                t = t0 + s
+                if grec is not None:
+                    gp = grec[t]
+                    grid.adjoint(lam, eps, Dhist[s], gmE, probes=probes, g_probe=gp, D_t=Dhist[s + 1])
+                else:
+                    grid.adjoint(lam, eps, Dhist[s], gmE)
+            del st
+        geps = grid.eps_grad(eps, gmE, torch.zeros_like(eps)) if ctx.needs_input_grad[0] else None
+        ctx.checkpoints = None
+        return geps, None, None, None, None, None, None, None

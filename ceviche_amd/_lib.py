@@ -59,7 +59,7 @@ class SolveInfo(ctypes.Structure):
 
 class FdtdDesc(ctypes.Structure):
     _fields_ = [('nx', ctypes.c_int32), ('ny', ctypes.c_int32), ('nz', ctypes.c_int32),
-                ('tm2d', ctypes.c_int32), ('dt', ctypes.c_double), ('dL', ctypes.c_double),
+                ('pad', ctypes.c_int32), ('dt', ctypes.c_double), ('dL', ctypes.c_double),
                 ('c0', ctypes.c_double), ('eps0', ctypes.c_double)]
 
 
@@ -95,6 +95,11 @@ _SIGS = {
     'cv_fdfd_solve': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
     'cv_fdfd_solve_mixed_workspace_bytes': (I64, [I64, I, I]),
     'cv_fdfd_solve_mixed': (I, [P, P, I, P, I, P, P, P, P, P, P, P, P, I, D, D, I, I, I, P, I64, P, P]),
+    'cv_fdtd_step': (I, [I, P, P, P, P, P, P, P, D, I, I64, P, I, P, P, I, P, P]),
+    'cv_fdtd_efields': (I, [I, P, P, P, P, P, P]),
+    'cv_fdtd_efields_grad': (I, [I, P, P, P, P, P, P, P, P]),
+    'cv_fdtd_adjoint_step': (I, [I, P, P, P, P, P, P, P, P, P, I, P, P, P]),
+    'cv_fdtd_eps_grad': (I, [I, P, P, P, P, P, P]),
 }
 
 

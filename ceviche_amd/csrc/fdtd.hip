@@ -1,0 +1,625 @@
+// FDTD Yee update with UPML integral terms (ceviche/fdtd.py:74-144) and its
+// exact discrete adjoint, for float32 / float64 state.
+//
+// State (18 arrays of nx*ny*nz, C order, z fastest):
+//   0..2 H   3..5 D   6..8 ICE   9..11 IH   12..14 ICH   15..17 ID
+// E is not stored: E_a = D_a / eps_aa with eps_aa = (eps + roll(eps, 1, a)) / 2
+// (utils.py:153-176), formed on the fly from eps.  The PML integrals are only
+// touched where their coefficient is non-zero (ICE_a where sigmaH_a != 0, IH_a
+// where sigmaH_b sigmaH_c != 0, likewise for D): elsewhere the reference
+// multiplies them by an exact zero.
+//
+// Coefficients (fdtd.py:265-311) are functions of the per-axis sigma profiles
+// sigH_a[n_a], sigD_a[n_a] (fdtd.py:224-263) and are formed in-kernel with the
+// reference's formula; interior cells (all sigma zero) use host-computed
+// constants.
+#include "common.h"
+
+#include <vector>
+
+namespace cv {
+
+template <typename R>
+struct Fdtd {
+    int nx, ny, nz;
+    int64_t n;
+    R inv_dL;
+    double dt, c0, eps0;
+    const R* sig;      // sigHx[nx] sigHy[ny] sigHz[nz] sigDx[nx] sigDy[ny] sigDz[nz]
+    const R* eps;      // relative permittivity, n
+    R* st;             // 18 * n
+    // interior coefficients (sigma = 0): m1 and m2 for H and D
+    R h1, h2, d1, d2;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ const R* sigH(const Fdtd<R>& f, int a) {
+    return f.sig + (a == 0 ? 0 : (a == 1 ? f.nx : f.nx + f.ny));
+}
+template <typename R>
+__host__ __device__ __forceinline__ const R* sigD(const Fdtd<R>& f, int a) {
+    return f.sig + (f.nx + f.ny + f.nz) + (a == 0 ? 0 : (a == 1 ? f.nx : f.nx + f.ny));
+}
+
+struct Coef4 {
+    double m1, m2, m3, m4;
+};
+
+// fdtd.py:272-311 for one component; sa is the component's own-axis sigma,
+// sb, sc the other two.  sign = -1 for H (mH2, mH3 negative), +1 for D.
+template <typename R>
+__host__ __device__ __forceinline__ void coef4(R sa, R sb, R sc, R dt, R c0, R e0, R sign, R& m1, R& m2, R& m3, R& m4) {
+    const R m0 = (R(1) / dt + (sb + sc) / R(2) / e0 + sb * sc * dt / R(4) / (e0 * e0));
+    m1 = (R(1) / m0 * (R(1) / dt - (sb + sc) / R(2) / e0 - sb * sc * dt / R(4) / (e0 * e0)));
+    m2 = sign / m0 * c0;
+    m3 = sign / m0 * c0 * dt * sa / e0;
+    m4 = R(-1) / m0 * dt * sb * sc / (e0 * e0);
+}
+
+__host__ __device__ __forceinline__ int wrapm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__host__ __device__ __forceinline__ int wrapp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
+
+template <typename R>
+__host__ __device__ __forceinline__ int64_t cell(const Fdtd<R>& f, int i, int j, int k) {
+    return ((int64_t)i * f.ny + j) * f.nz + k;
+}
+
+// E_a at (i, j, k) = D_a / ((eps + eps[-e_a]) / 2); D points at the D block (Dx, Dy, Dz)
+template <typename R>
+__host__ __device__ __forceinline__ R efield(const Fdtd<R>& f, const R* __restrict__ D, int a, int i, int j, int k) {
+    const int64_t c = cell(f, i, j, k);
+    D += a * f.n;
+    int64_t cm;
+    if (a == 0) cm = cell(f, wrapm(i, f.nx), j, k);
+    else if (a == 1) cm = cell(f, i, wrapm(j, f.ny), k);
+    else cm = cell(f, i, j, wrapm(k, f.nz));
+    const R eaa = (f.eps[c] + f.eps[cm]) / R(2);
+    return (R(1) / eaa) * D[c];
+}
+
+// ----------------------------------------------------------------- H phase
+template <typename R>
+__host__ __device__ __forceinline__ void h_cell(const Fdtd<R>& f, int64_t c) {
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny);
+    const int i = (int)(ij / f.ny);
+    const int64_t n = f.n;
+    R* H = f.st;
+    const R* D = f.st + 3 * n;
+    R* ICE = f.st + 6 * n;
+    R* IH = f.st + 9 * n;
+    const int ip = wrapp(i, f.nx), jp = wrapp(j, f.ny), kp = wrapp(k, f.nz);
+    const R ex = efield(f, D, 0, i, j, k), ey = efield(f, D, 1, i, j, k), ez = efield(f, D, 2, i, j, k);
+    R ce[3];
+    // curl_E, ceviche/derivatives.py:16-22 (forward differences, periodic)
+    ce[0] = ((efield(f, D, 2, i, jp, k) - ez) - (efield(f, D, 1, i, j, kp) - ey)) * f.inv_dL;
+    ce[1] = ((efield(f, D, 0, i, j, kp) - ex) - (efield(f, D, 2, ip, j, k) - ez)) * f.inv_dL;
+    ce[2] = ((efield(f, D, 1, ip, j, k) - ey) - (efield(f, D, 0, i, jp, k) - ex)) * f.inv_dL;
+    const int pos[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int ax_b = (a == 0) ? 1 : 0;
+        const int ax_c = (a == 2) ? 1 : 2;
+        const R sa = sigH(f, a)[pos[a]], sb = sigH(f, ax_b)[pos[ax_b]], sc = sigH(f, ax_c)[pos[ax_c]];
+        const int64_t ca = a * n + c;
+        const R h = H[ca];
+        R hn;
+        if (sa == R(0) && sb == R(0) && sc == R(0)) {
+            hn = f.h1 * h + f.h2 * ce[a];
+        } else {
+            R m1, m2, m3, m4;
+            coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(-1), m1, m2, m3, m4);
+            hn = m1 * h + m2 * ce[a];
+            if (sa != R(0)) {
+                const R ice = ICE[ca] + ce[a];
+                ICE[ca] = ice;
+                hn += m3 * ice;
+            }
+            if (sb != R(0) && sc != R(0)) {
+                const R ih = IH[ca] + h;
+                IH[ca] = ih;
+                hn += m4 * ih;
+            }
+        }
+        H[ca] = hn;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_h(Fdtd<R> f) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    h_cell(f, c);
+}
+
+// ----------------------------------------------------------------- D phase
+struct PointSrc {
+    int comp;         // 0..2 = Jx, Jy, Jz; -1 none
+    int64_t cell;
+    double amp;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ void d_cell(const Fdtd<R>& f, const R* __restrict__ Jx, const R* __restrict__ Jy,
+                                           const R* __restrict__ Jz, R jscale, PointSrc ps, int64_t c) {
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny);
+    const int i = (int)(ij / f.ny);
+    const int64_t n = f.n;
+    const R* H = f.st;
+    R* D = f.st + 3 * n;
+    R* ICH = f.st + 12 * n;
+    R* ID = f.st + 15 * n;
+    const int64_t cim = cell(f, wrapm(i, f.nx), j, k), cjm = cell(f, i, wrapm(j, f.ny), k),
+                  ckm = cell(f, i, j, wrapm(k, f.nz));
+    const R* Hx = H;
+    const R* Hy = H + n;
+    const R* Hz = H + 2 * n;
+    R ch[3];
+    // curl_H, ceviche/derivatives.py:24-30 (backward differences, periodic)
+    ch[0] = ((Hz[c] - Hz[cjm]) - (Hy[c] - Hy[ckm])) * f.inv_dL;
+    ch[1] = ((Hx[c] - Hx[ckm]) - (Hz[c] - Hz[cim])) * f.inv_dL;
+    ch[2] = ((Hy[c] - Hy[cim]) - (Hx[c] - Hx[cjm])) * f.inv_dL;
+    const int pos[3] = {i, j, k};
+    const R* J[3] = {Jx, Jy, Jz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int ax_b = (a == 0) ? 1 : 0;
+        const int ax_c = (a == 2) ? 1 : 2;
+        const R sa = sigD(f, a)[pos[a]], sb = sigD(f, ax_b)[pos[ax_b]], sc = sigD(f, ax_c)[pos[ax_c]];
+        const int64_t ca = a * n + c;
+        const R d = D[ca];
+        R dn;
+        if (sa == R(0) && sb == R(0) && sc == R(0)) {
+            dn = f.d1 * d + f.d2 * ch[a];
+        } else {
+            R m1, m2, m3, m4;
+            coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(1), m1, m2, m3, m4);
+            dn = m1 * d + m2 * ch[a];
+            if (sa != R(0)) {
+                const R ich = ICH[ca] + ch[a];
+                ICH[ca] = ich;
+                dn += m3 * ich;
+            }
+            if (sb != R(0) && sc != R(0)) {
+                const R idv = ID[ca] + d;
+                ID[ca] = idv;
+                dn += m4 * idv;
+            }
+        }
+        if (J[a]) dn += jscale * J[a][c];
+        if (ps.comp == a && ps.cell == c) dn += R(ps.amp);
+        D[ca] = dn;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_d(Fdtd<R> f, const R* __restrict__ Jx, const R* __restrict__ Jy,
+                                           const R* __restrict__ Jz, R jscale, PointSrc ps) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    d_cell(f, Jx, Jy, Jz, jscale, ps, c);
+}
+
+// ------------------------------------------------------------ probe record
+// comp: 0..2 E, 3..5 D, 6..8 H
+template <typename R>
+__global__ void k_record(Fdtd<R> f, const int32_t* __restrict__ pcomp, const int64_t* __restrict__ pcell,
+                         int np, R* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int comp = pcomp[p];
+    const int64_t c = pcell[p];
+    R v;
+    if (comp < 3) {
+        const int k = (int)(c % f.nz);
+        const int64_t ij = c / f.nz;
+        v = efield(f, f.st + 3 * f.n, comp, (int)(ij / f.ny), (int)(ij % f.ny), k);
+    } else if (comp < 6) {
+        v = f.st[(int64_t)comp * f.n + c];
+    } else {
+        v = f.st[(int64_t)(comp - 6) * f.n + c];
+    }
+    out[p] = v;
+}
+
+// E = D / eps_aa for all cells (the E fields the reference returns)
+template <typename R>
+__global__ void __launch_bounds__(256) k_efields(Fdtd<R> f, R* __restrict__ E) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) E[a * f.n + c] = efield(f, f.st + 3 * f.n, a, i, j, k);
+}
+
+// ============================================================== adjoint
+// Adjoint state lam (18 arrays, same layout) is the cotangent of the state
+// after step t; T holds lam_H_tot between the two phases (3 arrays).
+//   phase A:  muCH = (n2+n3) lamD + lamICH           (cells c, c+e)
+//             T    = lamH + curlE(muCH)
+//             lamICE = m3 T + lamICE;  lamIH = m4 T + lamIH;  lamH = m1 T + lamIH
+//   phase B:  muCE = m2 T + lamICE                    (cells c, c-e)
+//             q    = curlH(muCE)
+//             lamID = n4 lamD + lamID; lamICH = n3 lamD + lamICH
+//             lamD = n1 lamD + lamID + mE q;   g_mE += D_{t-1} q
+// (m*: H coefficients, n*: D coefficients; curl_E^T = curl_H for periodic
+// differences, so the adjoint is a time-reversed Yee step.)
+template <typename R>
+__host__ __device__ __forceinline__ void dcoef(const Fdtd<R>& f, int kind, int a, const int pos[3], R& m1, R& m2, R& m3,
+                                      R& m4, R& sa, R& sb, R& sc) {
+    const int ax_b = (a == 0) ? 1 : 0;
+    const int ax_c = (a == 2) ? 1 : 2;
+    if (kind == 0) {
+        sa = sigH(f, a)[pos[a]];
+        sb = sigH(f, ax_b)[pos[ax_b]];
+        sc = sigH(f, ax_c)[pos[ax_c]];
+    } else {
+        sa = sigD(f, a)[pos[a]];
+        sb = sigD(f, ax_b)[pos[ax_b]];
+        sc = sigD(f, ax_c)[pos[ax_c]];
+    }
+    if (sa == R(0) && sb == R(0) && sc == R(0)) {
+        m1 = kind == 0 ? f.h1 : f.d1;
+        m2 = kind == 0 ? f.h2 : f.d2;
+        m3 = R(0);
+        m4 = R(0);
+    } else {
+        coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), kind == 0 ? R(-1) : R(1), m1, m2, m3, m4);
+    }
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ R mu_ch(const Fdtd<R>& f, const R* __restrict__ lam, int a, int i, int j, int k) {
+    const int pos[3] = {i, j, k};
+    R m1, m2, m3, m4, sa, sb, sc;
+    dcoef<R>(f, 1, a, pos, m1, m2, m3, m4, sa, sb, sc);
+    const int64_t c = cell(f, i, j, k);
+    const R lD = lam[(3 + a) * f.n + c];
+    R v = (m2 + m3) * lD;
+    // lamICH is only ever fed where sigmaD_a != 0 (it is zero elsewhere)
+    if (sa != R(0)) v += lam[(12 + a) * f.n + c];
+    return v;
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void adj_a_cell(const Fdtd<R>& f, R* __restrict__ lam, R* __restrict__ T, int64_t c) {
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    const int ip = wrapp(i, f.nx), jp = wrapp(j, f.ny), kp = wrapp(k, f.nz);
+    const int64_t n = f.n;
+    // curl_E applied to muCH (forward differences)
+    const R mx = mu_ch(f, lam, 0, i, j, k), my = mu_ch(f, lam, 1, i, j, k), mz = mu_ch(f, lam, 2, i, j, k);
+    R ce[3];
+    ce[0] = ((mu_ch(f, lam, 2, i, jp, k) - mz) - (mu_ch(f, lam, 1, i, j, kp) - my)) * f.inv_dL;
+    ce[1] = ((mu_ch(f, lam, 0, i, j, kp) - mx) - (mu_ch(f, lam, 2, ip, j, k) - mz)) * f.inv_dL;
+    ce[2] = ((mu_ch(f, lam, 1, ip, j, k) - my) - (mu_ch(f, lam, 0, i, jp, k) - mx)) * f.inv_dL;
+    const int pos[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        R m1, m2, m3, m4, sa, sb, sc;
+        dcoef<R>(f, 0, a, pos, m1, m2, m3, m4, sa, sb, sc);
+        const int64_t ca = a * n + c;
+        const R t = lam[ca] + ce[a];
+        T[ca] = t;
+        R lih = lam[(9 + a) * n + c];
+        if (sb != R(0) && sc != R(0)) {
+            lih = m4 * t + lih;
+            lam[(9 + a) * n + c] = lih;
+        }
+        if (sa != R(0)) lam[(6 + a) * n + c] = m3 * t + lam[(6 + a) * n + c];
+        lam[ca] = m1 * t + ((sb != R(0) && sc != R(0)) ? lih : R(0));
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_adj_a(Fdtd<R> f, R* __restrict__ lam, R* __restrict__ T) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    adj_a_cell(f, lam, T, c);
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ R mu_ce(const Fdtd<R>& f, const R* __restrict__ lam, const R* __restrict__ T, int a,
+                                   int i, int j, int k) {
+    const int pos[3] = {i, j, k};
+    R m1, m2, m3, m4, sa, sb, sc;
+    dcoef<R>(f, 0, a, pos, m1, m2, m3, m4, sa, sb, sc);
+    const int64_t c = cell(f, i, j, k);
+    R v = m2 * T[a * f.n + c];
+    if (sa != R(0)) v += lam[(6 + a) * f.n + c];  // updated lamICE = m3 T + lamICE'
+    return v;
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void adj_b_cell(const Fdtd<R>& f, R* __restrict__ lam, const R* __restrict__ T,
+                                               const R* __restrict__ Dprev, R* __restrict__ gmE, int64_t c) {
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    const int im = wrapm(i, f.nx), jm = wrapm(j, f.ny), km = wrapm(k, f.nz);
+    const int64_t n = f.n;
+    const R ux = mu_ce(f, lam, T, 0, i, j, k), uy = mu_ce(f, lam, T, 1, i, j, k), uz = mu_ce(f, lam, T, 2, i, j, k);
+    R q[3];
+    // curl_H applied to muCE (backward differences)
+    q[0] = ((uz - mu_ce(f, lam, T, 2, i, jm, k)) - (uy - mu_ce(f, lam, T, 1, i, j, km))) * f.inv_dL;
+    q[1] = ((ux - mu_ce(f, lam, T, 0, i, j, km)) - (uz - mu_ce(f, lam, T, 2, im, j, k))) * f.inv_dL;
+    q[2] = ((uy - mu_ce(f, lam, T, 1, im, j, k)) - (ux - mu_ce(f, lam, T, 0, i, jm, k))) * f.inv_dL;
+    const int pos[3] = {i, j, k};
+    const int64_t cm[3] = {cell(f, im, j, k), cell(f, i, jm, k), cell(f, i, j, km)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        R m1, m2, m3, m4, sa, sb, sc;
+        dcoef<R>(f, 1, a, pos, m1, m2, m3, m4, sa, sb, sc);
+        const int64_t ca = (3 + a) * n + c;
+        const R lD = lam[ca];
+        R lid = R(0);
+        if (sb != R(0) && sc != R(0)) {
+            lid = m4 * lD + lam[(15 + a) * n + c];
+            lam[(15 + a) * n + c] = lid;
+        }
+        if (sa != R(0)) lam[(12 + a) * n + c] = m3 * lD + lam[(12 + a) * n + c];
+        const R eaa = (f.eps[c] + f.eps[cm[a]]) / R(2);
+        const R mE = R(1) / eaa;
+        lam[ca] = m1 * lD + lid + mE * q[a];
+        if (gmE) gmE[a * n + c] += Dprev[a * n + c] * q[a];
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_adj_b(Fdtd<R> f, R* __restrict__ lam, const R* __restrict__ T,
+                                               const R* __restrict__ Dprev, R* __restrict__ gmE) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    adj_b_cell(f, lam, T, Dprev, gmE, c);
+}
+
+// Probe cotangent injection: lam(comp, cell) += g[p]; for E probes also
+// lamD += mE g and g_mE += D_t g.  One thread per probe; atomics make
+// repeated (comp, cell) pairs accumulate.
+template <typename R>
+__global__ void k_inject(Fdtd<R> f, R* __restrict__ lam, const int32_t* __restrict__ pcomp,
+                         const int64_t* __restrict__ pcell, int np, const R* __restrict__ g,
+                         const R* __restrict__ Dt, R* __restrict__ gmE) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int comp = pcomp[p];
+    const int64_t c = pcell[p];
+    const R gv = g[p];
+    if (comp < 3) {
+        const int k = (int)(c % f.nz);
+        const int64_t ij = c / f.nz;
+        const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+        int64_t cm;
+        if (comp == 0) cm = cell(f, wrapm(i, f.nx), j, k);
+        else if (comp == 1) cm = cell(f, i, wrapm(j, f.ny), k);
+        else cm = cell(f, i, j, wrapm(k, f.nz));
+        const R mE = R(1) / ((f.eps[c] + f.eps[cm]) / R(2));
+        atomicAdd(&lam[(3 + comp) * f.n + c], mE * gv);
+        if (gmE) atomicAdd(&gmE[comp * f.n + c], Dt[comp * f.n + c] * gv);
+    } else if (comp < 6) {
+        atomicAdd(&lam[(int64_t)comp * f.n + c], gv);
+    } else {
+        atomicAdd(&lam[(int64_t)(comp - 6) * f.n + c], gv);
+    }
+}
+
+// g_eps from g_mE: mE_a = 1/eps_aa, eps_aa = (eps + roll(eps, 1, a))/2
+//   g_eps[c] += sum_a 1/2 (g_aa[c] + g_aa[c + e_a]),  g_aa = -mE_a^2 g_mE_a
+template <typename R>
+__global__ void __launch_bounds__(256) k_eps_grad(Fdtd<R> f, const R* __restrict__ gmE, R* __restrict__ geps) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    auto gaa = [&](int a, int ii, int jj, int kk) -> R {
+        const int64_t cc = cell(f, ii, jj, kk);
+        int64_t cm;
+        if (a == 0) cm = cell(f, wrapm(ii, f.nx), jj, kk);
+        else if (a == 1) cm = cell(f, ii, wrapm(jj, f.ny), kk);
+        else cm = cell(f, ii, jj, wrapm(kk, f.nz));
+        const R mE = R(1) / ((f.eps[cc] + f.eps[cm]) / R(2));
+        return -mE * mE * gmE[a * f.n + cc];
+    };
+    R g = R(0.5) * (gaa(0, i, j, k) + gaa(0, wrapp(i, f.nx), j, k));
+    g += R(0.5) * (gaa(1, i, j, k) + gaa(1, i, wrapp(j, f.ny), k));
+    g += R(0.5) * (gaa(2, i, j, k) + gaa(2, i, j, wrapp(k, f.nz)));
+    geps[c] += g;
+}
+
+// Reverse of E = D / eps_aa for a dense cotangent gE[3][n]:
+//   gD[a] += mE_a gE[a],  g_mE[a] += D[a] gE[a]
+template <typename R>
+__global__ void __launch_bounds__(256) k_efields_grad(Fdtd<R> f, const R* __restrict__ gE, R* __restrict__ gD,
+                                                      R* __restrict__ gmE) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    const int64_t cm[3] = {cell(f, wrapm(i, f.nx), j, k), cell(f, i, wrapm(j, f.ny), k), cell(f, i, j, wrapm(k, f.nz))};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const R g = gE[a * f.n + c];
+        const R mE = R(1) / ((f.eps[c] + f.eps[cm[a]]) / R(2));
+        gD[a * f.n + c] += mE * g;
+        if (gmE) gmE[a * f.n + c] += f.st[(3 + a) * f.n + c] * g;
+    }
+}
+
+// ------------------------------------------------------------ host glue
+template <typename R>
+static int make_fdtd(const cv_fdtd_desc* d, const void* sig, const void* eps, void* state, Fdtd<R>& f) {
+    CV_CHECK(d && sig && eps && state, "null pointer argument");
+    CV_CHECK(d->nx > 0 && d->ny > 0 && d->nz > 0, "empty grid");
+    f.nx = d->nx;
+    f.ny = d->ny;
+    f.nz = d->nz;
+    f.n = (int64_t)d->nx * d->ny * d->nz;
+    f.inv_dL = R(1.0 / d->dL);
+    f.dt = d->dt;
+    f.c0 = d->c0;
+    f.eps0 = d->eps0;
+    f.sig = (const R*)sig;
+    f.eps = (const R*)eps;
+    f.st = (R*)state;
+    // interior coefficients with the reference formula at sigma = 0, in R
+    const R dt = R(d->dt), c0 = R(d->c0), e0 = R(d->eps0);
+    const R m0 = (R(1) / dt + (R(0) + R(0)) / R(2) / e0 + R(0) * R(0) * dt / R(4) / (e0 * e0));
+    f.h1 = (R(1) / m0 * (R(1) / dt - (R(0) + R(0)) / R(2) / e0 - R(0) * R(0) * dt / R(4) / (e0 * e0)));
+    f.h2 = R(-1) / m0 * c0;
+    f.d1 = f.h1;
+    f.d2 = R(1) / m0 * c0;
+    return CV_OK;
+}
+
+static inline unsigned int nblk(int64_t n) { return (unsigned int)((n + 255) / 256); }
+
+}  // namespace cv
+
+using namespace cv;
+
+extern "C" int cv_fdtd_step(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state,
+                            const void* Jx, const void* Jy, const void* Jz, double jscale, int src_comp,
+                            int64_t src_cell, const double* amp, int n_steps, const int32_t* probe_comp,
+                            const int64_t* probe_cell, int n_probes, void* rec, void* stream) {
+    hipStream_t s = as_stream(stream);
+    CV_CHECK(n_steps >= 0, "n_steps < 0");
+    CV_CHECK(src_comp >= -1 && src_comp < 3, "src_comp must be -1 (none) or 0..2");
+    CV_CHECK(n_probes == 0 || (probe_comp && probe_cell && rec), "probes need comp, cell and rec buffers");
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, state, f);
+        if (rc) return rc;
+        CV_CHECK(src_comp < 0 || (src_cell >= 0 && src_cell < f.n), "source cell out of range");
+        for (int t = 0; t < n_steps; ++t) {
+            PointSrc ps{src_comp, src_cell, (src_comp >= 0 && amp) ? amp[t] : 0.0};
+            k_h<float><<<nblk(f.n), 256, 0, s>>>(f);
+            k_d<float><<<nblk(f.n), 256, 0, s>>>(f, (const float*)Jx, (const float*)Jy, (const float*)Jz,
+                                                 float(jscale), ps);
+            if (n_probes > 0)
+                k_record<float><<<(n_probes + 63) / 64, 64, 0, s>>>(f, probe_comp, probe_cell, n_probes,
+                                                                    (float*)rec + (int64_t)t * n_probes);
+        }
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, state, f);
+        if (rc) return rc;
+        CV_CHECK(src_comp < 0 || (src_cell >= 0 && src_cell < f.n), "source cell out of range");
+        for (int t = 0; t < n_steps; ++t) {
+            PointSrc ps{src_comp, src_cell, (src_comp >= 0 && amp) ? amp[t] : 0.0};
+            k_h<double><<<nblk(f.n), 256, 0, s>>>(f);
+            k_d<double><<<nblk(f.n), 256, 0, s>>>(f, (const double*)Jx, (const double*)Jy, (const double*)Jz,
+                                                  jscale, ps);
+            if (n_probes > 0)
+                k_record<double><<<(n_probes + 63) / 64, 64, 0, s>>>(f, probe_comp, probe_cell, n_probes,
+                                                                     (double*)rec + (int64_t)t * n_probes);
+        }
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdtd_efields(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                               const void* state, void* E, void* stream) {
+    hipStream_t s = as_stream(stream);
+    CV_CHECK(E, "null E");
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, (void*)state, f);
+        if (rc) return rc;
+        k_efields<float><<<nblk(f.n), 256, 0, s>>>(f, (float*)E);
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, (void*)state, f);
+        if (rc) return rc;
+        k_efields<double><<<nblk(f.n), 256, 0, s>>>(f, (double*)E);
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdtd_adjoint_step(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                                    void* lam, void* tmp, const void* D_prev, void* g_mE, const int32_t* probe_comp,
+                                    const int64_t* probe_cell, int n_probes, const void* g_probe,
+                                    const void* D_t, void* stream) {
+    hipStream_t s = as_stream(stream);
+    CV_CHECK(lam && tmp, "null adjoint buffers");
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, lam, f);
+        if (rc) return rc;
+        if (n_probes > 0)
+            k_inject<float><<<(n_probes + 255) / 256, 256, 0, s>>>(f, (float*)lam, probe_comp, probe_cell, n_probes,
+                                             (const float*)g_probe, (const float*)D_t, (float*)g_mE);
+        k_adj_a<float><<<nblk(f.n), 256, 0, s>>>(f, (float*)lam, (float*)tmp);
+        k_adj_b<float><<<nblk(f.n), 256, 0, s>>>(f, (float*)lam, (const float*)tmp, (const float*)D_prev,
+                                                 (float*)g_mE);
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, lam, f);
+        if (rc) return rc;
+        if (n_probes > 0)
+            k_inject<double><<<(n_probes + 255) / 256, 256, 0, s>>>(f, (double*)lam, probe_comp, probe_cell, n_probes,
+                                              (const double*)g_probe, (const double*)D_t, (double*)g_mE);
+        k_adj_a<double><<<nblk(f.n), 256, 0, s>>>(f, (double*)lam, (double*)tmp);
+        k_adj_b<double><<<nblk(f.n), 256, 0, s>>>(f, (double*)lam, (const double*)tmp, (const double*)D_prev,
+                                                  (double*)g_mE);
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdtd_eps_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                                const void* g_mE, void* g_eps, void* stream) {
+    hipStream_t s = as_stream(stream);
+    CV_CHECK(g_mE && g_eps, "null gradient buffers");
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, (void*)g_mE, f);
+        if (rc) return rc;
+        k_eps_grad<float><<<nblk(f.n), 256, 0, s>>>(f, (const float*)g_mE, (float*)g_eps);
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, (void*)g_mE, f);
+        if (rc) return rc;
+        k_eps_grad<double><<<nblk(f.n), 256, 0, s>>>(f, (const double*)g_mE, (double*)g_eps);
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdtd_efields_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                                    const void* state, const void* gE, void* gD, void* g_mE, void* stream) {
+    hipStream_t s = as_stream(stream);
+    CV_CHECK(gE && gD, "null gradient buffers");
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, (void*)state, f);
+        if (rc) return rc;
+        k_efields_grad<float><<<nblk(f.n), 256, 0, s>>>(f, (const float*)gE, (float*)gD, (float*)g_mE);
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, (void*)state, f);
+        if (rc) return rc;
+        k_efields_grad<double><<<nblk(f.n), 256, 0, s>>>(f, (const double*)gE, (double*)gD, (double*)g_mE);
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}

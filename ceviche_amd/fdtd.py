@@ -1,0 +1,166 @@
+"""FDTD simulation class with the reference's API (ceviche/fdtd.py:10-316).
+
+    F = fdtd(eps_r, dL, npml)            # eps_r (Nx, Ny[, Nz]); npml list of 3
+    fields = F.forward(Jx=None, Jy=None, Jz=None)   # one Yee step
+    fields['Ez'], F.t_index, F.dt
+
+Semantics kept from the reference: dt uses the 3D Courant limit even in 2D
+(fdtd.py:213-222); assigning ``eps_r`` recomputes the update coefficients and
+resets the fields and t_index (fdtd.py:63-72); J must have the full grid shape
+(fdtd.py:125-127); arrays of more than 3 dimensions raise (utils.py:206-214).
+
+Values: numpy in / numpy out (float64, like the reference), or torch in /
+torch out on the GPU with autograd through hand-written adjoint kernels.  The
+``precision`` keyword ('float64' default, or 'float32') selects the kernel
+precision.  ``run`` is the fused many-step runner with device-side probes and a
+checkpointed adjoint (no reference counterpart; the reference loops in Python).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from ._fdtd_op import COMPONENTS, FdtdEFields, FdtdGrid, FdtdRun, FdtdStep, _probe_tensors, time_step
+
+_FIELD_ORDER = ('Ex', 'Ey', 'Ez', 'Dx', 'Dy', 'Dz', 'Hx', 'Hy', 'Hz')
+
+
+def reshape_to_ND(arr, N):
+    """ceviche/utils.py:206-214."""
+    ND = len(arr.shape)
+    if ND > N:
+        raise ValueError("array is larger than {} dimensional, given shape {}".format(N, arr.shape))
+    extra_dims = (N - ND) * (1,)
+    return arr.reshape(tuple(arr.shape) + extra_dims)
+
+
+class fdtd:
+
+    def __init__(self, eps_r, dL, npml, precision='float64'):
+        if precision not in ('float64', 'float32'):
+            raise ValueError("precision must be 'float64' or 'float32'")
+        self.precision = precision
+        self._tdtype = torch.float64 if precision == 'float64' else torch.float32
+        eps_r = reshape_to_ND(eps_r, N=3)
+        self.Nx, self.Ny, self.Nz = self.grid_shape = tuple(eps_r.shape)
+        self.dL = dL
+        self.npml = npml
+        self.eps_r = eps_r
+
+    def __repr__(self):
+        return "FDTD(eps_r.shape={}, dL={}, NPML={})".format(self.grid_shape, self.dL, self.npml)
+
+    def __str__(self):
+        return "FDTD object:\n\tdomain size = {}\n\tdL = {}\n\tNPML = {}".format(self.grid_shape, self.dL, self.npml)
+
+    # ------------------------------------------------------------ properties
+    @property
+    def dL(self):
+        return self.__dL
+
+    @dL.setter
+    def dL(self, new_dL):
+        """Resets the time step (fdtd.py:41-45)."""
+        self.__dL = new_dL
+        self.dt = time_step(new_dL)
+
+    @property
+    def npml(self):
+        return self.__npml
+
+    @npml.setter
+    def npml(self, new_npml):
+        """Recomputes the PML profiles with the current dt (fdtd.py:52-56)."""
+        self.__npml = list(new_npml)
+        self._sigma_dt = self.dt
+
+    @property
+    def eps_r(self):
+        return self.__eps_r
+
+    @eps_r.setter
+    def eps_r(self, new_eps):
+        """Recomputes coefficients and resets fields and t_index (fdtd.py:63-72)."""
+        new_eps = reshape_to_ND(new_eps, N=3)
+        self.__eps_r = new_eps
+        self.grid_shape = self.Nx, self.Ny, self.Nz = tuple(new_eps.shape)
+        self.N = self.Nx * self.Ny * self.Nz
+        # The update coefficients are fixed here, with the dt and PML of this moment.
+        self._grid = FdtdGrid(self.grid_shape, self.dL, self.npml, self._sigma_dt, dtype=self._tdtype)
+        self._grid.desc.dt = self.dt
+        self._as_torch = isinstance(new_eps, torch.Tensor)
+        if self._as_torch:
+            self._eps_dev = new_eps.to(self._grid.device, self._tdtype)
+        else:
+            self._eps_dev = torch.from_numpy(np.ascontiguousarray(new_eps, dtype=np.float64)).to(
+                self._grid.device, self._tdtype)
+        self.initialize_fields()
+
+    def initialize_fields(self):
+        """Zero fields and integrals, t_index = 0 (fdtd.py:147-211)."""
+        self.t_index = 0
+        self._state = self._grid.new_state()
+
+    # ---------------------------------------------------------------- steps
+    def _prep_J(self, J):
+        if J is None:
+            return None
+        if isinstance(J, torch.Tensor):
+            self._as_torch = True
+            t = J.to(self._grid.device, self._tdtype)
+        else:
+            t = torch.from_numpy(np.ascontiguousarray(J, dtype=np.float64)).to(self._grid.device, self._tdtype)
+        if tuple(t.shape) != self.grid_shape:
+            raise ValueError('J must have the full grid shape %s (got %s)' % (self.grid_shape, tuple(t.shape)))
+        return t
+
+    def forward(self, Jx=None, Jy=None, Jz=None):
+        """One time step of FDTD (fdtd.py:74-144); returns the 9 fields."""
+        self._grid.desc.dL = self.dL  # curls use the current dL, like the reference
+        Js = [self._prep_J(J) for J in (Jx, Jy, Jz)]
+        self.t_index += 1
+        self._state = FdtdStep.apply(self._state, self._eps_dev, Js[0], Js[1], Js[2], self._grid)
+        return self.fields
+
+    @property
+    def fields(self):
+        E = FdtdEFields.apply(self._state, self._eps_dev, self._grid)
+        out = {}
+        for name in _FIELD_ORDER:
+            c = COMPONENTS[name]
+            out[name] = E[c] if c < 3 else self._state[c if c < 6 else c - 6]
+        if self._as_torch:
+            return out
+        return {k: v.detach().cpu().numpy() for k, v in out.items()}
+
+    def __getattr__(self, name):
+        # Ex, Hy, ... attributes of the reference object
+        if name in COMPONENTS and '_state' in self.__dict__:
+            return self.fields[name]
+        raise AttributeError(name)
+
+    def run(self, steps, source_pos, amplitudes, probes, source_component='Jz', checkpoint_every=50):
+        """Fused runner: ``steps`` Yee steps with a point source
+        J[source_component][source_pos] = amplitudes[t] (t = 0..steps-1, added
+        at every step like forward(J)) recording ``probes`` = [(field name,
+        position), ...] after each step.  Returns a (steps, n_probes) tensor
+        that is differentiable with respect to eps_r (torch) through a
+        checkpointed adjoint; advances t_index and the internal state."""
+        g = self._grid
+        g.desc.dL = self.dL
+        comp = {'Jx': 0, 'Jy': 1, 'Jz': 2}[source_component]
+        pos = tuple(source_pos) + (0,) * (3 - len(source_pos))
+        cell = (int(pos[0]) * g.ny + int(pos[1])) * g.nz + int(pos[2])
+        if not 0 <= cell < g.n:
+            raise ValueError('source position outside the grid')
+        amp = amplitudes if isinstance(amplitudes, torch.Tensor) else torch.as_tensor(np.asarray(amplitudes, float))
+        if amp.numel() < steps:
+            raise ValueError('need one amplitude per step')
+        pt = _probe_tensors(g, probes)
+        rec, state = FdtdRun.apply(self._eps_dev, amp, g, self._state, (comp, cell), pt, int(steps),
+                                   int(checkpoint_every))
+        self._state = state
+        self.t_index += int(steps)
+        return rec
+
+
+__all__ = ['fdtd']

@@ -165,6 +165,56 @@ int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host,
                         void* workspace, int64_t workspace_bytes,
                         cv_solve_info* info, void* stream);
 
+/* -------------------------------------------------------------------- FDTD */
+/* Grid and constants of one FDTD problem (fdtd.py:12-28, 213-222). */
+typedef struct cv_fdtd_desc {
+    int32_t nx, ny, nz;
+    int32_t pad;
+    double  dt, dL, c0, eps0;
+} cv_fdtd_desc;
+
+/* sig: 6 concatenated real profiles sigH_x[nx], sigH_y[ny], sigH_z[nz],
+ *      sigD_x[nx], sigD_y[ny], sigD_z[nz]  (fdtd.py:224-263: the sigma
+ *      tensors vary along their own axis only).
+ * eps: relative permittivity [nx*ny*nz].
+ * state: 18 arrays of nx*ny*nz (C order): H(3) D(3) ICE(3) IH(3) ICH(3) ID(3);
+ *      E = D / eps_aa is formed on demand (cv_fdtd_efields).
+ * All real arrays are float (CV_F32) or double (CV_F64). */
+
+/* n_steps Yee steps (fdtd.py:74-144), each: H update, D update, D += jscale*J
+ * (dense Jx/Jy/Jz, any may be NULL) + amp[t] at (src_comp, src_cell) when
+ * src_comp >= 0 (amp: host array of n_steps), then records n_probes values
+ * into rec[t*n_probes + p] (probe_comp: 0..2 E, 3..5 D, 6..8 H).
+ * Replaces the per-step numpy passes of fdtd.py:77-137. */
+int cv_fdtd_step(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state,
+                 const void* Jx, const void* Jy, const void* Jz, double jscale, int src_comp,
+                 int64_t src_cell, const double* amp, int n_steps, const int32_t* probe_comp,
+                 const int64_t* probe_cell, int n_probes, void* rec, void* stream);
+
+/* E[3][n] = D / eps_aa (fdtd.py:134-137, utils.py:153-176). */
+int cv_fdtd_efields(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                    const void* state, void* E, void* stream);
+
+/* One reverse-time step of the exact discrete adjoint (no reference
+ * counterpart: ceviche differentiates FDTD through autograd's tape).
+ * lam: 18-array adjoint state after step t (updated in place to before step t);
+ * tmp: 3-array scratch; D_prev: D after step t-1; D_t: D after step t (for E
+ * probes); g_mE (3 arrays, may be NULL) accumulates dL/d(1/eps_aa);
+ * g_probe[n_probes]: cotangent of the step-t probe record, injected first. */
+int cv_fdtd_adjoint_step(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                         void* lam, void* tmp, const void* D_prev, void* g_mE,
+                         const int32_t* probe_comp, const int64_t* probe_cell, int n_probes,
+                         const void* g_probe, const void* D_t, void* stream);
+
+/* Reverse of cv_fdtd_efields for a dense cotangent gE[3][n]:
+ * gD[3][n] += gE / eps_aa;  g_mE[3][n] += D * gE (g_mE may be NULL). */
+int cv_fdtd_efields_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                         const void* state, const void* gE, void* gD, void* g_mE, void* stream);
+
+/* g_eps += chain rule of mE_a = 2/(eps + roll(eps, 1, a)) applied to g_mE. */
+int cv_fdtd_eps_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                     const void* g_mE, void* g_eps, void* stream);
+
 /* ------------------------------------------------------------------ misc */
 const char* cv_version(void);
 const char* cv_last_error(void);
