@@ -96,6 +96,15 @@ class FdtdGrid:
                                  ptr(rec), stream_ptr()), 'fdtd step')
         return state
 
+    def run_tm(self, state, eps, src_cell, amp_dev, t0, n_steps, probes, rec, rec_t0, config):
+        """Fused, temporally blocked 2D TM steps (nz == 1, TE fields zero, Jz point source)."""
+        pc, pl = (None, None) if probes is None else probes
+        npb = 0 if probes is None else int(pc.numel())
+        check(lib().cv_fdtd_run_tm(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), int(src_cell),
+                                   ptr(amp_dev), int(t0), int(n_steps), ptr(pc), ptr(pl), npb, ptr(rec),
+                                   int(rec_t0), int(config), stream_ptr()), 'fdtd run_tm')
+        return state
+
     def efields(self, state, eps):
         E = torch.empty((3,) + self.shape, dtype=self.dtype, device=self.device)
         check(lib().cv_fdtd_efields(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(E),
@@ -172,6 +181,41 @@ class FdtdEFields(torch.autograd.Function):
         return gstate, geps, None
 
 
+_TUNE_CACHE = {}
+
+
+def tm_configs():
+    n = lib().cv_fdtd_tm_configs(None, None, 0)
+    rpt = (ctypes.c_int * n)()
+    ts = (ctypes.c_int * n)()
+    lib().cv_fdtd_tm_configs(rpt, ts, n)
+    return [(rpt[i], ts[i]) for i in range(n)]
+
+
+def autotune_tm(grid, eps, steps=64, reps=2):
+    """Times every TM tile configuration on this grid (a scratch copy of the
+    state) and returns the fastest; cached per (shape, dtype)."""
+    key = (grid.shape, str(grid.dtype))
+    if key in _TUNE_CACHE:
+        return _TUNE_CACHE[key]
+    st = grid.new_state()
+    amp = torch.zeros(steps, dtype=torch.float64, device=grid.device)
+    best, best_t = 0, float('inf')
+    for cfg in range(len(tm_configs())):
+        grid.run_tm(st, eps, -1, amp, 0, steps, None, None, 0, cfg)  # warm-up
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            grid.run_tm(st, eps, -1, amp, 0, steps, None, None, 0, cfg)
+        e1.record()
+        e1.synchronize()
+        t = e0.elapsed_time(e1)
+        if t < best_t:
+            best, best_t = cfg, t
+    _TUNE_CACHE[key] = best
+    return best
+
+
 def _probe_tensors(grid, probes):
     comps = []
     cells = []
@@ -192,8 +236,9 @@ class FdtdRun(torch.autograd.Function):
     one call; reverse mode with checkpoint-and-recompute."""
 
     @staticmethod
-    def forward(ctx, eps, amp, grid, state0, src, probes, n_steps, checkpoint_every):
+    def forward(ctx, eps, amp, grid, state0, src, probes, n_steps, checkpoint_every, tm_config=None):
         amp_h = amp.detach().to('cpu', torch.float64).numpy()
+        amp_dev = torch.from_numpy(amp_h).to(grid.device) if tm_config is not None else None
         state = state0.detach().clone()
         pc, pl = probes
         rec = torch.empty((n_steps, pc.numel()), dtype=grid.dtype, device=grid.device)
@@ -202,16 +247,18 @@ class FdtdRun(torch.autograd.Function):
         checkpoints = []
         t = 0
         while t < n_steps:
-            m = min(ck, n_steps - t)
+            m = min(ck, n_steps - t) if need_grad else n_steps - t
             if need_grad:
                 checkpoints.append(state.clone())
-            grid.step(state, eps, n_steps=m, src=src, amp=amp_h[t:t + m], probes=probes, rec=rec[t:t + m])
+            if tm_config is not None:
+                grid.run_tm(state, eps, src[1], amp_dev, t, m, probes, rec, 0, tm_config)
+            else:
+                grid.step(state, eps, n_steps=m, src=src, amp=amp_h[t:t + m], probes=probes, rec=rec[t:t + m])
             t += m
         ctx.grid, ctx.src, ctx.probes, ctx.n_steps, ctx.ck = grid, src, probes, n_steps, ck
-        ctx.amp_h = amp_h
+        ctx.amp_h, ctx.amp_dev, ctx.tm_config = amp_h, amp_dev, tm_config
         ctx.checkpoints = checkpoints
         ctx.save_for_backward(eps)
-        ctx.final_state = state
         return rec, state
 
     @staticmethod
@@ -232,7 +279,10 @@ class FdtdRun(torch.autograd.Function):
             st = ctx.checkpoints[sgi].clone()
             Dhist[0].copy_(st[3:6])
             for s in range(m):
-                grid.step(st, eps, n_steps=1, src=src, amp=ctx.amp_h[t0 + s:t0 + s + 1])
+                if ctx.tm_config is not None:
+                    grid.run_tm(st, eps, src[1], ctx.amp_dev, t0 + s, 1, None, None, 0, ctx.tm_config)
+                else:
+                    grid.step(st, eps, n_steps=1, src=src, amp=ctx.amp_h[t0 + s:t0 + s + 1])
                 Dhist[s + 1].copy_(st[3:6])
             for s in range(m - 1, -1, -1):
                 t = t0 + s
@@ -244,4 +294,4 @@ class FdtdRun(torch.autograd.Function):
             del st
         geps = grid.eps_grad(eps, gmE, torch.zeros_like(eps)) if ctx.needs_input_grad[0] else None
         ctx.checkpoints = None
-        return geps, None, None, None, None, None, None, None
+        return geps, None, None, None, None, None, None, None, None

@@ -100,6 +100,8 @@ _SIGS = {
     'cv_fdtd_efields_grad': (I, [I, P, P, P, P, P, P, P, P]),
     'cv_fdtd_adjoint_step': (I, [I, P, P, P, P, P, P, P, P, P, I, P, P, P]),
     'cv_fdtd_eps_grad': (I, [I, P, P, P, P, P, P]),
+    'cv_fdtd_run_tm': (I, [I, P, P, P, P, I64, P, I, I, P, P, I, P, I, I, P]),
+    'cv_fdtd_tm_configs': (I, [P, P, I]),
 }
 
 

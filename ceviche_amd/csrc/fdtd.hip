@@ -453,6 +453,256 @@ __global__ void __launch_bounds__(256) k_efields_grad(Fdtd<R> f, const R* __rest
     }
 }
 
+// ================================================ 2D TM temporal blocking
+// Fused, temporally blocked Yee update for nz == 1 with only Hx, Hy, Dz live
+// (TM drive: Jz point source, TE fields zero).  One workgroup owns a region of
+// RR = 4*RPT rows (x) by 64 columns (y, contiguous): lane = column, each wave a
+// band of RPT rows.  Cell values (Hx, Hy, Dz, 1/eps and the PML integrals
+// ICEx, ICEy, IDz) live in registers; neighbour values are exchanged through
+// LDS.  TS steps are advanced per launch; after TS steps the cells at least TS
+// away from the region edge are exact (the dependency cone of one step is one
+// cell on each side), so the owned tile is the region minus a TS-wide halo.
+// Region loads use periodic indices, which also makes small domains exact.
+template <typename R>
+struct TmArgs {
+    int nx, ny;
+    int tiles_x, tiles_y;      // tile grid
+    int t0, nsteps;            // first global step of this launch, steps (<= TS)
+    int64_t src;               // flat source cell or -1
+    int np;                    // probes
+    R inv_dL;
+    R h1, h2, d1, d2;          // interior coefficients
+    double dt, c0, eps0;
+    const R* sig;
+    const R* eps;
+    R* st;                     // 18-array state
+    const double* amp;         // amplitude per global step
+    const int32_t* pcomp;      // probe components: 2 Ez, 6 Hx, 7 Hy
+    const int64_t* pcell;      // probe flat cells
+    R* rec;                    // (steps, np), row t - rec_t0
+    int rec_t0;
+};
+
+__host__ __device__ __forceinline__ int pmod(int a, int n) {
+    int r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+constexpr int kTmMaxProbes = 256;
+
+template <typename R, int RPT, int TS>
+__global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
+    constexpr int RR = 4 * RPT;
+    constexpr int RC = 64;
+    constexpr int OX = RR - 2 * TS, OY = RC - 2 * TS;  // owned tile
+    __shared__ R sEz[RR][RC];
+    __shared__ R sHx[RR][RC];
+    __shared__ R sHy[RR][RC];
+    __shared__ int s_np;
+    __shared__ int s_pidx[kTmMaxProbes];
+    __shared__ short s_pr[kTmMaxProbes], s_pc[kTmMaxProbes], s_pk[kTmMaxProbes];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // XCD-aware tile order: consecutive workgroups go to different XCDs, so
+    // give each XCD a contiguous run of tiles (better L2 reuse of halos).
+    const int nb = gridDim.x;
+    int b = blockIdx.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int ti = b / a.tiles_y, tj = b - ti * a.tiles_y;
+    const int i0 = ti * OX, j0 = tj * OY;
+    const int nx = a.nx, ny = a.ny;
+    const int64_t n = (int64_t)nx * ny;
+    const int gj = pmod(j0 - TS + lane, ny);
+    const R* sigHx = a.sig;
+    const R* sigHy = a.sig + nx;
+    const R* sigDx = a.sig + nx + ny + 1;
+    const R* sigDy = a.sig + 2 * nx + ny + 1;
+    const R shy = sigHy[gj], sdy = sigDy[gj];
+    R* Hx = a.st;
+    R* Hy = a.st + n;
+    R* Dz = a.st + 5 * n;
+    R* ICEx = a.st + 6 * n;
+    R* ICEy = a.st + 7 * n;
+    R* IDz = a.st + 17 * n;
+    const R dt = R(a.dt), c0 = R(a.c0), e0 = R(a.eps0);
+
+    if (threadIdx.x == 0) s_np = 0;
+    __syncthreads();
+    for (int p = threadIdx.x; p < a.np; p += 256) {
+        const int64_t c = a.pcell[p];
+        const int pi = (int)(c / ny), pj = (int)(c - (int64_t)pi * ny);
+        const int ri = pi - i0, rj = pj - j0;
+        if (ri >= 0 && ri < OX && rj >= 0 && rj < OY) {
+            const int slot = atomicAdd(&s_np, 1);
+            if (slot < kTmMaxProbes) {
+                s_pidx[slot] = p;
+                s_pr[slot] = (short)(ri + TS);
+                s_pc[slot] = (short)(rj + TS);
+                s_pk[slot] = (short)a.pcomp[p];
+            }
+        }
+    }
+
+    R hx[RPT], hy[RPT], dz[RPT], me[RPT], icx[RPT], icy[RPT], idz[RPT];
+    int gi[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = w * RPT + q;
+        gi[q] = pmod(i0 - TS + r, nx);
+        const int64_t c = (int64_t)gi[q] * ny + gj;
+        hx[q] = Hx[c];
+        hy[q] = Hy[c];
+        dz[q] = Dz[c];
+        const R e = a.eps[c];
+        me[q] = R(1) / ((e + e) / R(2));
+        icx[q] = sigHx[gi[q]] != R(0) ? ICEx[c] : R(0);
+        icy[q] = shy != R(0) ? ICEy[c] : R(0);
+        idz[q] = (sigDx[gi[q]] != R(0) && sdy != R(0)) ? IDz[c] : R(0);
+        sEz[r][lane] = me[q] * dz[q];
+    }
+    const int src_i = a.src >= 0 ? (int)(a.src / ny) : -1;
+    const int src_j = a.src >= 0 ? (int)(a.src - (int64_t)src_i * ny) : -1;
+    __syncthreads();
+    const int np_tile = min(s_np, kTmMaxProbes);
+
+    for (int s = 0; s < a.nsteps; ++s) {
+        const int t = a.t0 + s;
+        // ---- H update: CEx = (Ez[j+1] - Ez)/dL, CEy = -(Ez[i+1] - Ez)/dL
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = w * RPT + q;
+            const R ez = sEz[r][lane];
+            const R ez_ip = sEz[min(r + 1, RR - 1)][lane];
+            const R ez_jp = sEz[r][min(lane + 1, RC - 1)];
+            const R cex = ((ez_jp - ez) - (R(0) - R(0))) * a.inv_dL;
+            const R cey = ((R(0) - R(0)) - (ez_ip - ez)) * a.inv_dL;
+            const R shx = sigHx[gi[q]];
+            if (shx == R(0) && shy == R(0)) {
+                hx[q] = a.h1 * hx[q] + a.h2 * cex;
+                hy[q] = a.h1 * hy[q] + a.h2 * cey;
+            } else {
+                R m1, m2, m3, m4;
+                // Hx: own sigma sHx, others sHy, sHz = 0
+                coef4<R>(shx, shy, R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
+                R v = m1 * hx[q] + m2 * cex;
+                if (shx != R(0)) {
+                    icx[q] += cex;
+                    v += m3 * icx[q];
+                }
+                hx[q] = v;
+                // Hy: own sigma sHy, others sHx, sHz = 0
+                coef4<R>(shy, shx, R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
+                v = m1 * hy[q] + m2 * cey;
+                if (shy != R(0)) {
+                    icy[q] += cey;
+                    v += m3 * icy[q];
+                }
+                hy[q] = v;
+            }
+            sHx[r][lane] = hx[q];
+            sHy[r][lane] = hy[q];
+        }
+        __syncthreads();
+        // ---- D update: CHz = (Hy - Hy[i-1])/dL - (Hx - Hx[j-1])/dL
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = w * RPT + q;
+            const R hy_im = sHy[max(r - 1, 0)][lane];
+            const R hx_jm = sHx[r][max(lane - 1, 0)];
+            const R chz = ((hy[q] - hy_im) - (hx[q] - hx_jm)) * a.inv_dL;
+            const R sdx = sigDx[gi[q]];
+            R v;
+            if (sdx == R(0) && sdy == R(0)) {
+                v = a.d1 * dz[q] + a.d2 * chz;
+            } else {
+                R m1, m2, m3, m4;
+                coef4<R>(R(0), sdx, sdy, dt, c0, e0, R(1), m1, m2, m3, m4);
+                v = m1 * dz[q] + m2 * chz;
+                if (sdx != R(0) && sdy != R(0)) {
+                    idz[q] += dz[q];
+                    v += m4 * idz[q];
+                }
+            }
+            if (gi[q] == src_i && gj == src_j) v += R(a.amp[t]);
+            dz[q] = v;
+            sEz[r][lane] = me[q] * v;
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < np_tile; p += 256) {
+            const int k = s_pk[p];
+            const R v = (k == 2) ? sEz[s_pr[p]][s_pc[p]] : (k == 6 ? sHx[s_pr[p]][s_pc[p]] : sHy[s_pr[p]][s_pc[p]]);
+            a.rec[(int64_t)(t - a.rec_t0) * a.np + s_pidx[p]] = v;
+        }
+    }
+    // ---- store the owned tile
+    const int oj = lane - TS;
+    if (oj >= 0 && oj < OY && j0 + oj < ny) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = w * RPT + q;
+            const int oi = r - TS;
+            if (oi < 0 || oi >= OX || i0 + oi >= nx) continue;
+            const int64_t c = (int64_t)(i0 + oi) * ny + (j0 + oj);
+            Hx[c] = hx[q];
+            Hy[c] = hy[q];
+            Dz[c] = dz[q];
+            if (sigHx[gi[q]] != R(0)) ICEx[c] = icx[q];
+            if (shy != R(0)) ICEy[c] = icy[q];
+            if (sigDx[gi[q]] != R(0) && sdy != R(0)) IDz[c] = idz[q];
+        }
+    }
+}
+
+template <typename R, int RPT, int TS>
+static int launch_tm(const Fdtd<R>& f, const double* amp_dev, int64_t src, int t0, int n_steps,
+                     const int32_t* pcomp, const int64_t* pcell, int np, R* rec, int rec_t0, hipStream_t s) {
+    TmArgs<R> a;
+    a.nx = f.nx;
+    a.ny = f.ny;
+    constexpr int OX = 4 * RPT - 2 * TS, OY = 64 - 2 * TS;
+    a.tiles_x = (f.nx + OX - 1) / OX;
+    a.tiles_y = (f.ny + OY - 1) / OY;
+    a.src = src;
+    a.np = np;
+    a.inv_dL = f.inv_dL;
+    a.h1 = f.h1;
+    a.h2 = f.h2;
+    a.d1 = f.d1;
+    a.d2 = f.d2;
+    a.dt = f.dt;
+    a.c0 = f.c0;
+    a.eps0 = f.eps0;
+    a.sig = f.sig;
+    a.eps = f.eps;
+    a.st = f.st;
+    a.amp = amp_dev;
+    a.pcomp = pcomp;
+    a.pcell = pcell;
+    a.rec = rec;
+    a.rec_t0 = rec_t0;
+    const int nblocks = a.tiles_x * a.tiles_y;
+    for (int t = 0; t < n_steps; t += TS) {
+        a.t0 = t0 + t;
+        a.nsteps = (n_steps - t < TS) ? (n_steps - t) : TS;
+        k_tm_block<R, RPT, TS><<<nblocks, 256, 0, s>>>(a);
+    }
+    return CV_OK;
+}
+
+template <typename R>
+static int dispatch_tm(int cfg, const Fdtd<R>& f, const double* amp_dev, int64_t src, int t0, int n_steps,
+                       const int32_t* pcomp, const int64_t* pcell, int np, R* rec, int rec_t0, hipStream_t s) {
+    switch (cfg) {
+        case 0: return launch_tm<R, 4, 2>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 1: return launch_tm<R, 4, 4>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 2: return launch_tm<R, 8, 4>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 3: return launch_tm<R, 8, 8>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 4: return launch_tm<R, 16, 8>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 5: return launch_tm<R, 16, 4>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        default: break;
+    }
+    CV_CHECK(false, "unknown TM tile configuration %d", cfg);
+}
+
 // ------------------------------------------------------------ host glue
 template <typename R>
 static int make_fdtd(const cv_fdtd_desc* d, const void* sig, const void* eps, void* state, Fdtd<R>& f) {
@@ -617,6 +867,48 @@ extern "C" int cv_fdtd_efields_grad(int dtype, const cv_fdtd_desc* d, const void
         int rc = make_fdtd<double>(d, sig, eps, (void*)state, f);
         if (rc) return rc;
         k_efields_grad<double><<<nblk(f.n), 256, 0, s>>>(f, (const double*)gE, (double*)gD, (double*)g_mE);
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdtd_tm_configs(int* rpt, int* ts, int max_n) {
+    static const int R_[6] = {4, 4, 8, 8, 16, 16}, T_[6] = {2, 4, 4, 8, 8, 4};
+    for (int i = 0; i < 6 && i < max_n; ++i) {
+        if (rpt) rpt[i] = R_[i];
+        if (ts) ts[i] = T_[i];
+    }
+    return 6;
+}
+
+extern "C" int cv_fdtd_run_tm(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state,
+                              int64_t src_cell, const double* amp_dev, int t0, int n_steps,
+                              const int32_t* probe_comp, const int64_t* probe_cell, int n_probes, void* rec,
+                              int rec_t0, int config, void* stream) {
+    hipStream_t s = as_stream(stream);
+    CV_CHECK(d && d->nz == 1, "the TM kernel needs nz == 1");
+    CV_CHECK(n_probes <= kTmMaxProbes, "the TM kernel records at most %d probes", kTmMaxProbes);
+    CV_CHECK(n_steps >= 0 && t0 >= 0, "bad step range");
+    CV_CHECK(n_probes == 0 || (probe_comp && probe_cell && rec), "probes need comp, cell and rec buffers");
+    CV_CHECK(src_cell < 0 || amp_dev, "a source needs its amplitude array");
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, state, f);
+        if (rc) return rc;
+        CV_CHECK(src_cell < f.n, "source cell out of range");
+        rc = dispatch_tm<float>(config, f, amp_dev, src_cell, t0, n_steps, probe_comp, probe_cell, n_probes, (float*)rec,
+                                rec_t0, s);
+        if (rc) return rc;
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, state, f);
+        if (rc) return rc;
+        CV_CHECK(src_cell < f.n, "source cell out of range");
+        rc = dispatch_tm<double>(config, f, amp_dev, src_cell, t0, n_steps, probe_comp, probe_cell, n_probes,
+                                 (double*)rec, rec_t0, s);
+        if (rc) return rc;
     } else {
         CV_CHECK(false, "unknown dtype %d", dtype);
     }

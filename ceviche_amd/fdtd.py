@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._fdtd_op import COMPONENTS, FdtdEFields, FdtdGrid, FdtdRun, FdtdStep, _probe_tensors, time_step
+from ._fdtd_op import COMPONENTS, FdtdEFields, FdtdGrid, FdtdRun, FdtdStep, _probe_tensors, autotune_tm, time_step
 
 _FIELD_ORDER = ('Ex', 'Ey', 'Ez', 'Dx', 'Dy', 'Dz', 'Hx', 'Hy', 'Hz')
 
@@ -99,6 +99,7 @@ class fdtd:
         """Zero fields and integrals, t_index = 0 (fdtd.py:147-211)."""
         self.t_index = 0
         self._state = self._grid.new_state()
+        self._te_live = self.Nz != 1  # TE components can only become non-zero through Jx / Jy
 
     # ---------------------------------------------------------------- steps
     def _prep_J(self, J):
@@ -117,6 +118,8 @@ class fdtd:
         """One time step of FDTD (fdtd.py:74-144); returns the 9 fields."""
         self._grid.desc.dL = self.dL  # curls use the current dL, like the reference
         Js = [self._prep_J(J) for J in (Jx, Jy, Jz)]
+        if Js[0] is not None or Js[1] is not None:
+            self._te_live = True
         self.t_index += 1
         self._state = FdtdStep.apply(self._state, self._eps_dev, Js[0], Js[1], Js[2], self._grid)
         return self.fields
@@ -138,13 +141,19 @@ class fdtd:
             return self.fields[name]
         raise AttributeError(name)
 
-    def run(self, steps, source_pos, amplitudes, probes, source_component='Jz', checkpoint_every=50):
+    def run(self, steps, source_pos, amplitudes, probes, source_component='Jz', checkpoint_every=50,
+            kernel='auto'):
         """Fused runner: ``steps`` Yee steps with a point source
         J[source_component][source_pos] = amplitudes[t] (t = 0..steps-1, added
         at every step like forward(J)) recording ``probes`` = [(field name,
         position), ...] after each step.  Returns a (steps, n_probes) tensor
         that is differentiable with respect to eps_r (torch) through a
-        checkpointed adjoint; advances t_index and the internal state."""
+        checkpointed adjoint; advances t_index and the internal state.
+
+        kernel: 'auto' uses the temporally blocked 2D TM kernel when it applies
+        (nz == 1, Jz source, TE fields zero, Ez/Hx/Hy probes, <= 256 probes;
+        tile shape from the autotuner), 'generic' forces the per-step kernels,
+        or an int selects a TM tile configuration."""
         g = self._grid
         g.desc.dL = self.dL
         comp = {'Jx': 0, 'Jy': 1, 'Jz': 2}[source_component]
@@ -156,8 +165,18 @@ class fdtd:
         if amp.numel() < steps:
             raise ValueError('need one amplitude per step')
         pt = _probe_tensors(g, probes)
+        tm_ok = (g.nz == 1 and comp == 2 and not self._te_live and len(probes) <= 256
+                 and all(COMPONENTS[p[0]] in (2, 6, 7) for p in probes))
+        if kernel == 'generic' or not tm_ok:
+            if isinstance(kernel, int) and not tm_ok:
+                raise ValueError('the TM kernel does not apply to this run')
+            cfg = None
+        elif isinstance(kernel, int):
+            cfg = kernel
+        else:
+            cfg = autotune_tm(g, self._eps_dev.detach())
         rec, state = FdtdRun.apply(self._eps_dev, amp, g, self._state, (comp, cell), pt, int(steps),
-                                   int(checkpoint_every))
+                                   int(checkpoint_every), cfg)
         self._state = state
         self.t_index += int(steps)
         return rec

@@ -1,0 +1,68 @@
+"""Seeded synthetic problem definitions used by the benchmark and the tests.
+
+Each builder returns plain numpy inputs (host setup only) for a named problem:
+
+fdfd_waveguide_box   2D Ez FDFD, 120x120, dL 50 nm, npml 20, lambda 1.55 um; a
+                     straight waveguide along x (eps 12.25, 10 cells) and a
+                     40x40 design box of seeded uniform eps in [1, 12.25];
+                     Jz line source at x-index 30; objective |Ez[90, 60]|^2.
+fdtd_tm_blobs        2D TM FDTD, NxN (400 by default), dL 20 nm, npml 20;
+                     eps = Gaussian-filtered noise (sigma 4 cells) thresholded
+                     to {1, 12.25}; Gaussian-pulse Jz at the centre (centre
+                     step 200, width 50 steps); Ez probes at (N/4, N/2),
+                     (3N/4, N/2), (N/2, N/4), (N/2, 3N/4).
+fdtd_3d_blobs        3D FDTD, N^3 (256 by default), dL 25 nm, npml 10;
+                     background eps 4.41 with a centred (N/4)^3 design region
+                     of seeded binary blobs {1, 12.25}; z-dipole Gaussian
+                     pulse; flux-plane probes.
+"""
+import numpy as np
+
+C_0 = 1 / np.sqrt(8.85418782e-12 * 1.25663706e-6)
+
+
+def fdfd_waveguide_box(N=120):
+    omega = 2 * np.pi * C_0 / 1.55e-6
+    eps = np.ones((N, N))
+    c = N // 2
+    eps[:, c - 5:c + 5] = 12.25
+    q = N // 3
+    eps[q:2 * q, q:2 * q] = np.random.default_rng(0).uniform(1, 12.25, (2 * q - q, 2 * q - q))
+    src = np.zeros((N, N))
+    src[N // 4, :] = 1.0
+    return dict(omega=omega, dL=50e-9, eps=eps, npml=[20, 20], src=src, probe=(3 * N // 4, N // 2),
+                box=(slice(q, 2 * q), slice(q, 2 * q)))
+
+
+def gaussian_pulse(steps, t0, width):
+    t = np.arange(steps)
+    return np.exp(-(t - t0) ** 2 / (2.0 * width ** 2))
+
+
+def fdtd_tm_blobs(N=400, steps=2000, seed=1):
+    from scipy.ndimage import gaussian_filter
+    noise = gaussian_filter(np.random.default_rng(seed).standard_normal((N, N)), 4)
+    eps = np.where(noise > 0, 12.25, 1.0)[:, :, None]
+    c = N // 2
+    probes = [('Ez', (N // 4, c, 0)), ('Ez', (3 * N // 4, c, 0)), ('Ez', (c, N // 4, 0)), ('Ez', (c, 3 * N // 4, 0))]
+    return dict(eps=eps, dL=20e-9, npml=[20, 20, 0], src=(c, c, 0), amp=gaussian_pulse(steps, 200, 50),
+                probes=probes, steps=steps)
+
+
+def fdtd_3d_blobs(N=256, steps=1000, seed=2):
+    from scipy.ndimage import gaussian_filter
+    eps = np.full((N, N, N), 2.1 ** 2)
+    d = N // 4
+    lo = N // 2 - d // 2
+    blob = gaussian_filter(np.random.default_rng(seed).standard_normal((d, d, d)), 3) > 0
+    eps[lo:lo + d, lo:lo + d, lo:lo + d] = np.where(blob, 12.25, 1.0)
+    c = N // 2
+    src = (c, c, max(1, int(round(N * 60 / 256))))
+    zp = int(round(N * 200 / 256))
+    probes = []
+    for comp in ('Ex', 'Ey', 'Hx', 'Hy'):
+        for i in range(lo, lo + d):
+            for j in range(lo, lo + d):
+                probes.append((comp, (i, j, zp)))
+    return dict(eps=eps, dL=25e-9, npml=[10, 10, 10], src=src, amp=gaussian_pulse(steps, 200, 50),
+                probes=probes, steps=steps, plane=(slice(lo, lo + d), slice(lo, lo + d), zp))
