@@ -1,0 +1,301 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X FDTD / FDFD hot path.  Prints ONE JSON line.
+
+Main line (``value``): 2D TM FDTD, 400x400 cells, dL 20 nm, npml 20, binary
+blob permittivity, Gaussian-pulse Jz point source, 2000 time steps, 4 Ez
+probes recorded every step, float32 -- one "step" of this benchmark is one
+full 2000-time-step run from zero fields with inputs resident on the GPU.
+value = cells * time steps / second (Gcell/s).
+
+roofline: the dominant kernel (the temporally blocked TM kernel) timed with
+HIP events on the stream it is launched on; algorithmic bytes per cell-step
+are (2 * 3 fields + eps) * 4 B = 28 B.  Extra measured fields are reported
+under "extras" (float64 error against the oracle's golden trace, adjoint
+gradient time, an HBM-sized 2D run, a 3D run, FDFD solves/s, a copy probe).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras all|none]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FDFD solves/s @1e-6 resid, learned vs plain precond; FDTD Gcell/s + %HBM"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def fdtd_tm_main(steps, warmup):
+    import numpy as np
+    import torch
+    import ceviche_amd as cv
+    from ceviche_amd import workloads
+    from ceviche_amd._fdtd_op import tm_configs
+    c = workloads.fdtd_tm_blobs(400, 2000)
+    eps = torch.tensor(c['eps'])
+    F = cv.fdtd(eps, c['dL'], c['npml'], precision='float32')
+    amp = torch.tensor(c['amp'])
+
+    def one_run():
+        F.initialize_fields()
+        return F.run(c['steps'], c['src'], amp, c['probes'])
+
+    rec = one_run()  # includes autotuning of the tile shape (not timed)
+    for _ in range(warmup):
+        one_run()
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rec = one_run()
+    _sync()
+    dt = time.perf_counter() - t0
+    ms = 1e3 * dt / steps
+    cells = 400 * 400
+    gcells = cells * c['steps'] * steps / dt / 1e9
+    # roofline pass: the dominant kernel alone, HIP events on its stream
+    from ceviche_amd._fdtd_op import autotune_tm
+    cfg = autotune_tm(F._grid, F._eps_dev)
+    rpt, ts = tm_configs()[cfg]
+    n_launch = (c['steps'] + ts - 1) // ts
+    F.initialize_fields()
+    amp_dev = amp.to(F._grid.device)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    from ceviche_amd._fdtd_op import _probe_tensors
+    pc, pl = _probe_tensors(F._grid, c['probes'])
+    si, sj, sk = c['src']
+    src_cell = (si * 400 + sj) * 1 + sk
+    recb = torch.empty((c['steps'], 4), dtype=torch.float32, device=F._grid.device)
+    reps = 5
+    _sync()
+    e0.record()
+    for _ in range(reps):
+        F._grid.run_tm(F._state, F._eps_dev, src_cell, amp_dev, 0, c['steps'], (pc, pl), recb, 0, cfg)
+    e1.record()
+    e1.synchronize()
+    avg_launch_ms = e0.elapsed_time(e1) / (reps * n_launch)
+    bytes_per_launch = 28.0 * cells * ts
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_tm_block<float,%d,%d>" % (rpt, ts), "avg_launch_us": round(avg_launch_ms * 1e3, 2),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "note": "28 B/cell-step x 160000 cells x %d steps per launch; the 400x400 state (2.6 MB) is "
+                    "cache-resident, so the fraction counts compulsory bytes, not HBM traffic" % ts}
+    return dict(value=gcells, ms=ms, rec=rec, roofline=roof, cfg=(rpt, ts))
+
+
+def cpu_baseline_fdtd(seconds=12.0):
+    """The oracle (numpy restatement of fdtd.py, float64, one core) on the same
+    400x400 problem for as many steps as fit in ~``seconds``."""
+    import numpy as np
+    from ceviche_amd import workloads
+    from oracle import fdtd_ref
+    c = workloads.fdtd_tm_blobs(400, 2000)
+    F = fdtd_ref.FDTD(c['eps'], c['dL'], c['npml'])
+    J = np.zeros(F.shape)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds and n < 2000:
+        J[c['src']] = c['amp'][n]
+        F.forward(Jz=J)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(400 * 400 * n / dt / 1e9, 6), "unit": "Gcell/s", "cores": 1, "kind": "port",
+            "sample": "400x400x1 float64 numpy restatement of ceviche fdtd.forward, first %d of 2000 steps "
+                      "(%.1f s)" % (n, dt)}
+
+
+def extras_all(main):
+    import numpy as np
+    import torch
+    import ceviche_amd as cv
+    from ceviche_amd import workloads
+    out = {}
+
+    def guard(name, fn):
+        try:
+            t = time.perf_counter()
+            out[name] = fn()
+            log('extra %s done in %.1f s' % (name, time.perf_counter() - t))
+        except Exception as e:  # a failing extra must not hide the main line
+            out[name] = {"error": repr(e)[:300]}
+            log('extra %s failed: %r' % (name, e))
+
+    gold = os.path.join(ROOT, 'tests', 'golden', 'tm_blobs_400_trace.npy')
+
+    def trace_errors():
+        ref = np.load(gold)
+        c = workloads.fdtd_tm_blobs(400, 2000)
+        F = cv.fdtd(torch.tensor(c['eps']), c['dL'], c['npml'], precision='float64')
+        _sync()
+        t0 = time.perf_counter()
+        r64 = F.run(c['steps'], c['src'], torch.tensor(c['amp']), c['probes']).cpu().numpy()
+        t64 = time.perf_counter() - t0
+        r32 = main['rec'].double().cpu().numpy()
+        rel = lambda a: float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+        return {"fp64_trace_rel_err": rel(r64), "fp32_trace_rel_err": rel(r32),
+                "fp64_gcells": round(400 * 400 * 2000 / t64 / 1e9, 2)}
+
+    def adjoint():
+        c = workloads.fdtd_tm_blobs(400, 2000)
+        eps = torch.tensor(c['eps'], device='cuda', dtype=torch.float32, requires_grad=True)
+        F = cv.fdtd(eps, c['dL'], c['npml'], precision='float32')
+        amp = torch.tensor(c['amp'])
+        F.run(200, c['src'], amp, c['probes'], checkpoint_every=50)  # warm-up / tune
+        F.eps_r = eps
+        _sync()
+        t0 = time.perf_counter()
+        rec = F.run(c['steps'], c['src'], amp, c['probes'], checkpoint_every=50)
+        (rec.double() ** 2).sum().backward()
+        _sync()
+        return {"fwd_plus_adjoint_ms": round(1e3 * (time.perf_counter() - t0), 2),
+                "checkpoint_every": 50, "grad_norm": float(eps.grad.norm())}
+
+    def tm_hbm():
+        N, steps = 8192, 200
+        c = workloads.fdtd_tm_blobs(N, steps)
+        F = cv.fdtd(torch.tensor(c['eps'], dtype=torch.float32), c['dL'], c['npml'], precision='float32')
+        amp = torch.tensor(c['amp'])
+        F.run(steps, c['src'], amp, c['probes'])
+        F.initialize_fields()
+        _sync()
+        t0 = time.perf_counter()
+        F.run(steps, c['src'], amp, c['probes'])
+        _sync()
+        dt = time.perf_counter() - t0
+        g = N * N * steps / dt / 1e9
+        return {"grid": [N, N, 1], "steps": steps, "gcells": round(g, 2),
+                "hbm_frac_compulsory": round(g * 28 / HBM_PEAK_GBS, 4)}
+
+    def fdtd_3d():
+        N, steps = 256, 20
+        c = workloads.fdtd_3d_blobs(N, steps)
+        F = cv.fdtd(torch.tensor(c['eps'], dtype=torch.float32), c['dL'], c['npml'], precision='float32')
+        amp = torch.tensor(c['amp'])
+        F.run(2, c['src'], amp, [('Ez', c['src'])])
+        F.initialize_fields()
+        _sync()
+        t0 = time.perf_counter()
+        F.run(steps, c['src'], amp, [('Ez', c['src'])])
+        _sync()
+        dt = time.perf_counter() - t0
+        g = N ** 3 * steps / dt / 1e9
+        return {"grid": [N, N, N], "steps": steps, "gcells": round(g, 2),
+                "hbm_frac_compulsory": round(g * 52 / HBM_PEAK_GBS, 4), "kernel": "generic two-phase"}
+
+    def fdfd_cfg1():
+        c = workloads.fdfd_waveguide_box(120)
+        gz = np.load(os.path.join(ROOT, 'tests', 'golden', 'waveguide_box_120.npz'))
+        F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'])
+        F.solve(c['src'])
+        _sync()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            _, _, Ez = F.solve(c['src'])
+        dt = (time.perf_counter() - t0) / reps
+        info = F.last_solve_info
+        err = float(np.linalg.norm(Ez - gz['Ez']) / np.linalg.norm(gz['Ez']))
+        return {"grid": [120, 120], "solves_per_s": round(1 / dt, 2), "iterations": info['iterations'],
+                "fp64_rel_residual": info['rel_residual'], "field_rel_err_vs_spsolve": err,
+                "precision": "complex128 BiCGSTAB, FP64 reductions"}
+
+    def fdfd_1024():
+        from ceviche_amd._fdfd_op import SolverOptions
+        c = workloads.fdfd_waveguide_box(1024)
+        c['npml'] = [20, 20]
+        F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'], precision='complex64',
+                       solver_options=SolverOptions(tol=1e-6, max_iter=400000))
+        _sync()
+        t0 = time.perf_counter()
+        F.solve(c['src'])
+        _sync()
+        dt = time.perf_counter() - t0
+        info = F.last_solve_info
+        return {"grid": [1024, 1024], "solves_per_s": round(1 / dt, 4), "iterations": info['iterations'],
+                "fp64_rel_residual": info['rel_residual'],
+                "precision": "complex64 BiCGSTAB in complex128 refinement, FP64 reductions, no preconditioner"}
+
+    def copy_probe():
+        n = 1 << 28  # 1 GiB of float32
+        a = torch.empty(n, dtype=torch.float32, device='cuda')
+        b = torch.empty_like(a)
+        b.copy_(a)
+        _sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        gbs = 2 * 4 * n * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        return {"copy_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
+
+    guard('fdtd_tm_400_trace', trace_errors)
+    guard('fdtd_tm_400_adjoint', adjoint)
+    guard('fdtd_tm_8192', tm_hbm)
+    guard('fdtd_3d_256', fdtd_3d)
+    guard('fdfd_ez_120', fdfd_cfg1)
+    guard('fdfd_ez_1024_plain', fdfd_1024)
+    guard('stream_copy', copy_probe)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--extras', default='all', choices=['all', 'none'])
+    ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU only: every box of this build has one MI355X and the hot path '
+              'runs on one device (see the sharding notes); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py needs a GPU', file=sys.stderr)
+        sys.exit(1)
+    main_res = fdtd_tm_main(args.steps, args.warmup)
+    log('main: %.2f Gcell/s, %.3f ms per 2000-step run' % (main_res['value'], main_res['ms']))
+    extras = extras_all(main_res) if args.extras == 'all' else {}
+    cpu = cpu_baseline_fdtd(args.cpu_seconds)
+    line = {
+        "metric": METRIC,
+        "value": round(main_res['value'], 3),
+        "unit": "Gcell/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_res['ms'], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "2D TM FDTD 400x400x1, dL 20 nm, npml 20, binary-blob eps {1, 12.25}, "
+                               "Gaussian-pulse Jz point source, 2000 time steps, 4 Ez probes per step",
+                   "grid": [400, 400, 1], "time_steps": 2000, "precision": "float32",
+                   "tile_rows_per_thread_steps_per_launch": list(main_res['cfg'])},
+        "roofline": main_res['roofline'],
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()
