@@ -9,3 +9,5 @@ __version__ = '0.1.0'
 from .fdfd import fdfd_ez, fdfd_hz  # noqa: E402,F401
 from ._fdfd_op import SolverOptions  # noqa: E402,F401
 from .fdtd import fdtd  # noqa: E402,F401
+from .jacobians import jacobian  # noqa: E402,F401
+from . import primitives  # noqa: E402,F401

@@ -171,6 +171,23 @@ class FdfdOperator:
               'deriv_matgrad')
         return gmat
 
+    def material_apply(self, dm0, dm1, x, y=None):
+        """(dA/dm . dm) x with the scaled system (c A)."""
+        x = x.to(self.dtype).contiguous()
+        acc = y is not None
+        if y is None:
+            y = torch.empty_like(x)
+        check(lib().cv_fdfd_material_apply(self.code, *self._args(), ptr(self.prof), ptr(dm0), ptr(dm1), ptr(x),
+                                           ptr(y), int(acc), stream_ptr()), 'material_apply')
+        return y
+
+    def materials_jvp(self, m0, m1, d_eps):
+        dm0, dm1 = torch.empty_like(m0), torch.empty_like(m1)
+        check(lib().cv_fdfd_hz_materials_jvp(self.code, *self._args(), ptr(m0), ptr(m1),
+                                             ptr(d_eps.to(self.rdtype).contiguous()), ptr(dm0), ptr(dm1),
+                                             stream_ptr()), 'hz_materials_jvp')
+        return dm0, dm1
+
     def material_grad(self, lam, x, g0, g1, scale):
         check(lib().cv_fdfd_material_grad(self.code, *self._args(), ptr(self.prof), ptr(lam), ptr(x), ptr(g0),
                                           ptr(g1), float(scale), stream_ptr()), 'material_grad')
@@ -247,8 +264,24 @@ class FdfdSolve(torch.autograd.Function):
         x = FdfdSolve._solve(op, op32, opts, m0, m1, b * c, False)
         ctx.op, ctx.op32, ctx.opts = op, op32, opts
         ctx.save_for_backward(m0, m1, x)
+        ctx.save_for_forward(m0, m1, x)
         ctx.has_m1 = m1 is not None
         return x
+
+    @staticmethod
+    def jvp(ctx, dm0, dm1, db, *_):
+        """dx = A^-1 (db - (dA/dm . dm) x)  (primitives.py:137-148)."""
+        m0, m1, x = ctx.saved_tensors
+        op = ctx.op
+        c = op.scale_vector(torch.float64)
+        rhs = torch.zeros_like(x) if db is None else (db.to(x.dtype) * c)
+        if dm0 is not None or dm1 is not None:
+            d0 = torch.zeros_like(m0) if dm0 is None else dm0.to(m0.dtype).contiguous()
+            d1 = None
+            if ctx.has_m1:
+                d1 = torch.zeros_like(m0) if dm1 is None else dm1.to(m0.dtype).contiguous()
+            rhs = rhs - op.material_apply(d0, d1, x)
+        return FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False)
 
     @staticmethod
     def backward(ctx, gx):
@@ -273,8 +306,21 @@ class FdfdDeriv(torch.autograd.Function):
     def forward(ctx, u, mat, op, which, coef):
         ctx.op, ctx.which, ctx.coef = op, which, coef
         ctx.save_for_backward(u, mat)
+        ctx.save_for_forward(u, mat)
         ctx.has_mat = mat is not None
         return op.deriv(u, which, coef, mat=mat, mode=0)
+
+    @staticmethod
+    def jvp(ctx, du, dmat, *_):
+        u, mat = ctx.saved_tensors
+        op = ctx.op
+        out = None
+        if du is not None:
+            out = op.deriv(du, ctx.which, ctx.coef, mat=mat, mode=0)
+        if ctx.has_mat and dmat is not None:
+            t = op.deriv(u, ctx.which, ctx.coef, mat=dmat.to(mat.dtype).contiguous(), mode=0)
+            out = t if out is None else out + t
+        return out if out is not None else torch.zeros_like(op.deriv(u, ctx.which, ctx.coef, mat=mat, mode=0))
 
     @staticmethod
     def backward(ctx, g):
@@ -297,7 +343,13 @@ class HzMaterials(torch.autograd.Function):
         m0, m1 = op.materials(eps)
         ctx.op = op
         ctx.save_for_backward(m0, m1)
+        ctx.save_for_forward(m0, m1)
         return m0, m1
+
+    @staticmethod
+    def jvp(ctx, d_eps, _):
+        m0, m1 = ctx.saved_tensors
+        return ctx.op.materials_jvp(m0, m1, d_eps)
 
     @staticmethod
     def backward(ctx, g0, g1):

@@ -91,6 +91,8 @@ _SIGS = {
     'cv_fdfd_deriv': (I, [I, P, P, I, P, P, P, P, I, I, D, D, I, P]),
     'cv_fdfd_deriv_matgrad': (I, [I, P, P, I, P, P, P, P, I, D, D, P]),
     'cv_fdfd_material_grad': (I, [I, P, P, I, P, P, P, P, P, D, P]),
+    'cv_fdfd_material_apply': (I, [I, P, P, I, P, P, P, P, P, I, P]),
+    'cv_fdfd_hz_materials_jvp': (I, [I, P, P, I, P, P, P, P, P, P]),
     'cv_fdfd_solve_workspace_bytes': (I64, [I, I64, I, I]),
     'cv_fdfd_solve': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
     'cv_fdfd_solve_mixed_workspace_bytes': (I64, [I64, I, I]),
@@ -102,6 +104,9 @@ _SIGS = {
     'cv_fdtd_eps_grad': (I, [I, P, P, P, P, P, P]),
     'cv_fdtd_run_tm': (I, [I, P, P, P, P, I64, P, I, I, P, P, I, P, I, I, P]),
     'cv_fdtd_tm_configs': (I, [P, P, I]),
+    'cv_segsum': (I, [I64, P, P, P, P, P, I, I, D, D, P, I, P, P]),
+    'cv_dense_lu': (I, [I, P, P, P, P]),
+    'cv_dense_lu_solve': (I, [I, P, P, P, I, I, P]),
 }
 
 

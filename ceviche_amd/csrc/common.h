@@ -130,6 +130,12 @@ __device__ __forceinline__ double load_agent(const double* p) {
 // thread 0); it publishes them to partials[tile*NV..] and returns true in
 // the workgroup that arrived last, after that workgroup has summed all the
 // instance's partials (fixed order) into `tot` (valid in all threads).
+//
+// Only the partials and the counter cross workgroups inside the kernel, so
+// they are agent-scope atomics (coherent across the XCDs' L2s without a full
+// L2 write-back / invalidate); s_waitcnt(0) makes the partial stores complete
+// before the counter increment is issued.  The next kernel sees everything
+// else through the usual kernel-boundary cache maintenance.
 template <int NV>
 __device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], double* partials,
                                                   unsigned int* counter, int tile, int tile_first,
@@ -138,13 +144,15 @@ __device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], doub
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) store_agent(&partials[(size_t)tile * NV + c], mine[c]);
-        __threadfence();
-        unsigned int ticket = atomicAdd(counter, 1u);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const unsigned int ticket =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (ticket == (unsigned int)(tile_count - 1));
     }
     __syncthreads();
     if (!s_last) return false;
-    __threadfence();
     double acc[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) acc[c] = 0.0;
@@ -157,7 +165,7 @@ __device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], doub
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) s_tot[c] = acc[c];
-        *counter = 0u;  // ready for the next kernel
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // ready for the next kernel
     }
     __syncthreads();
 #pragma unroll

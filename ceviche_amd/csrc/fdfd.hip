@@ -163,6 +163,84 @@ k_deriv_matgrad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict
     }
 }
 
+
+// ------------------------------------------------- forward-mode pieces
+// y (+)= (dA/dm0 . dm0 + dA/dm1 . dm1) x, the tangent of the operator:
+//   Ez: dshift (dm0 + i dm1) x
+//   Hz: scale (Dxb (dm1 .* Dxf x) + Dyb (dm0 .* Dyf x))
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_material_apply(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+                 const cx<R>* __restrict__ prof, const R* __restrict__ dm0, const R* __restrict__ dm1,
+                 const cx<R>* __restrict__ x, cx<R>* __restrict__ y, int accumulate) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        cx<R> v;
+        if (d.pol == CV_POL_EZ) {
+            const cx<R> dm = mk<R>(dm0[k], dm1 ? dm1[k] : R(0));
+            v = R(d.dshift) * (dm * x[k]);
+        } else {
+            // w_x = dm1 .* Dxf x at (i, j) and (i-1, j); Dxb w_x at (i, j); same along y with dm0
+            const int im = (i == 0) ? d.nx - 1 : i - 1;
+            const int jm = (j == 0) ? d.ny - 1 : j - 1;
+            const int64_t kim = d.cell_off + (int64_t)im * d.ny + j, kjm = d.cell_off + (int64_t)i * d.ny + jm;
+            const cx<R>* kxb = prof + d.prof_off + d.nx;
+            const cx<R>* kyb = prof + d.prof_off + 2 * d.nx + d.ny;
+            const cx<R> wx = dm1[k] * deriv_at<R>(d, prof, x, (const R*)nullptr, 0, 0, i, j);
+            cx<R> vx;
+            if (d.flags & CV_FLAG_X_IDENTITY) {
+                vx = kxb[i] * wx;
+            } else {
+                cx<R> wxm = dm1[kim] * deriv_at<R>(d, prof, x, (const R*)nullptr, 0, 0, im, j);
+                if (i == 0) wxm = wxm * conj(mk<R>(R(d.phx_re), R(d.phx_im)));
+                vx = kxb[i] * (wx - wxm);
+            }
+            const cx<R> wy = dm0[k] * deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, j);
+            cx<R> vy;
+            if (d.flags & CV_FLAG_Y_IDENTITY) {
+                vy = kyb[j] * wy;
+            } else {
+                cx<R> wym = dm0[kjm] * deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, jm);
+                if (j == 0) wym = wym * conj(mk<R>(R(d.phy_re), R(d.phy_im)));
+                vy = kyb[j] * (wy - wym);
+            }
+            v = R(d.scale) * (vx + vy);
+        }
+        y[k] = accumulate ? y[k] + v : v;
+    }
+}
+
+// dm0 = -m0^2 (de + de[j-1]) / 2, dm1 = -m1^2 (de + de[i-1]) / 2  (tangent of the Hz materials)
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_hz_materials_jvp(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+                   const R* __restrict__ m0, const R* __restrict__ m1, const R* __restrict__ de,
+                   R* __restrict__ dm0, R* __restrict__ dm1) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int im = (i == 0) ? d.nx - 1 : i - 1;
+        const int jm = (j == 0) ? d.ny - 1 : j - 1;
+        const int64_t k = d.cell_off + loc;
+        const R dxx = R(0.5) * (de[k] + de[d.cell_off + (int64_t)i * d.ny + jm]);
+        const R dyy = R(0.5) * (de[k] + de[d.cell_off + (int64_t)im * d.ny + j]);
+        dm0[k] = -m0[k] * m0[k] * dxx;
+        dm1[k] = -m1[k] * m1[k] * dyy;
+    }
+}
+
 }  // namespace cv
 
 using namespace cv;
@@ -287,6 +365,44 @@ extern "C" int cv_fdfd_deriv_matgrad(int dtype, const cv_fdfd_desc* desc, const 
         k_deriv_matgrad<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)prof,
                                                           (const cx<float>*)u, (const cx<float>*)g, (float*)gmat,
                                                           which, coef_re, coef_im);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_material_apply(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                                      const void* prof, const void* dm0, const void* dm1, const void* x, void* y,
+                                      int accumulate, void* stream) {
+    CV_CHECK(desc && tiles && prof && dm0 && x && y, "null pointer argument");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_material_apply<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof, (const double*)dm0,
+                                                            (const double*)dm1, (const cx<double>*)x,
+                                                            (cx<double>*)y, accumulate);
+    else if (dtype == CV_C64)
+        k_material_apply<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)prof, (const float*)dm0,
+                                                           (const float*)dm1, (const cx<float>*)x, (cx<float>*)y,
+                                                           accumulate);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_hz_materials_jvp(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                                        const void* m0, const void* m1, const void* d_eps, void* dm0, void* dm1,
+                                        void* stream) {
+    CV_CHECK(desc && tiles && m0 && m1 && d_eps && dm0 && dm1, "null pointer argument");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_hz_materials_jvp<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const double*)m0, (const double*)m1,
+                                                              (const double*)d_eps, (double*)dm0, (double*)dm1);
+    else if (dtype == CV_C64)
+        k_hz_materials_jvp<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const float*)m0, (const float*)m1,
+                                                             (const float*)d_eps, (float*)dm0, (float*)dm1);
     else
         CV_CHECK(false, "unknown dtype %d", dtype);
     CV_LAUNCH_CHECK();

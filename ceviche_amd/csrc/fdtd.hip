@@ -542,7 +542,7 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
         }
     }
 
-    R hx[RPT], hy[RPT], dz[RPT], me[RPT], icx[RPT], icy[RPT], idz[RPT];
+    R hx[RPT], hy[RPT], dz[RPT], me[RPT], icx[RPT], icy[RPT], idz[RPT], shx[RPT], sdx[RPT];
     int gi[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
@@ -554,9 +554,11 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
         dz[q] = Dz[c];
         const R e = a.eps[c];
         me[q] = R(1) / ((e + e) / R(2));
-        icx[q] = sigHx[gi[q]] != R(0) ? ICEx[c] : R(0);
+        shx[q] = sigHx[gi[q]];
+        sdx[q] = sigDx[gi[q]];
+        icx[q] = shx[q] != R(0) ? ICEx[c] : R(0);
         icy[q] = shy != R(0) ? ICEy[c] : R(0);
-        idz[q] = (sigDx[gi[q]] != R(0) && sdy != R(0)) ? IDz[c] : R(0);
+        idz[q] = (sdx[q] != R(0) && sdy != R(0)) ? IDz[c] : R(0);
         sEz[r][lane] = me[q] * dz[q];
     }
     const int src_i = a.src >= 0 ? (int)(a.src / ny) : -1;
@@ -570,27 +572,26 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int r = w * RPT + q;
-            const R ez = sEz[r][lane];
+            const R ez = me[q] * dz[q];
             const R ez_ip = sEz[min(r + 1, RR - 1)][lane];
             const R ez_jp = sEz[r][min(lane + 1, RC - 1)];
             const R cex = ((ez_jp - ez) - (R(0) - R(0))) * a.inv_dL;
             const R cey = ((R(0) - R(0)) - (ez_ip - ez)) * a.inv_dL;
-            const R shx = sigHx[gi[q]];
-            if (shx == R(0) && shy == R(0)) {
+            if (shx[q] == R(0) && shy == R(0)) {
                 hx[q] = a.h1 * hx[q] + a.h2 * cex;
                 hy[q] = a.h1 * hy[q] + a.h2 * cey;
             } else {
                 R m1, m2, m3, m4;
                 // Hx: own sigma sHx, others sHy, sHz = 0
-                coef4<R>(shx, shy, R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
+                coef4<R>(shx[q], shy, R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
                 R v = m1 * hx[q] + m2 * cex;
-                if (shx != R(0)) {
+                if (shx[q] != R(0)) {
                     icx[q] += cex;
                     v += m3 * icx[q];
                 }
                 hx[q] = v;
                 // Hy: own sigma sHy, others sHx, sHz = 0
-                coef4<R>(shy, shx, R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
+                coef4<R>(shy, shx[q], R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
                 v = m1 * hy[q] + m2 * cey;
                 if (shy != R(0)) {
                     icy[q] += cey;
@@ -609,15 +610,14 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
             const R hy_im = sHy[max(r - 1, 0)][lane];
             const R hx_jm = sHx[r][max(lane - 1, 0)];
             const R chz = ((hy[q] - hy_im) - (hx[q] - hx_jm)) * a.inv_dL;
-            const R sdx = sigDx[gi[q]];
             R v;
-            if (sdx == R(0) && sdy == R(0)) {
+            if (sdx[q] == R(0) && sdy == R(0)) {
                 v = a.d1 * dz[q] + a.d2 * chz;
             } else {
                 R m1, m2, m3, m4;
-                coef4<R>(R(0), sdx, sdy, dt, c0, e0, R(1), m1, m2, m3, m4);
+                coef4<R>(R(0), sdx[q], sdy, dt, c0, e0, R(1), m1, m2, m3, m4);
                 v = m1 * dz[q] + m2 * chz;
-                if (sdx != R(0) && sdy != R(0)) {
+                if (sdx[q] != R(0) && sdy != R(0)) {
                     idz[q] += dz[q];
                     v += m4 * idz[q];
                 }
@@ -645,9 +645,9 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
             Hx[c] = hx[q];
             Hy[c] = hy[q];
             Dz[c] = dz[q];
-            if (sigHx[gi[q]] != R(0)) ICEx[c] = icx[q];
+            if (shx[q] != R(0)) ICEx[c] = icx[q];
             if (shy != R(0)) ICEy[c] = icy[q];
-            if (sigDx[gi[q]] != R(0) && sdy != R(0)) IDz[c] = idz[q];
+            if (sdx[q] != R(0) && sdy != R(0)) IDz[c] = idz[q];
         }
     }
 }

@@ -252,10 +252,11 @@ k_bicg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
         st->done = kMaxIter;
     } else {
         const cx<double> om = st->omega;
-        const double rho_mag = sqrt(norm2(rho_new));
         const bool om_zero = (om.re == 0.0 && om.im == 0.0);
-        if (om_zero || rho_mag <= 1e-13 * sqrt(st->rhrh * rr)) {
-            // breakdown / stagnation: restart with rh = r, p = r
+        const bool rho_zero = (rho_new.re == 0.0 && rho_new.im == 0.0);
+        // (rh, r) / (|rh| |r|) decays naturally in BiCGSTAB, so only exact
+        // breakdowns restart (with rh = r, p = r)
+        if (om_zero || rho_zero) {
             st->restart = 1;
             st->rho = mk(rr, 0.0);
             st->rhrh = rr;

@@ -122,6 +122,20 @@ int cv_fdfd_deriv_matgrad(int dtype, const cv_fdfd_desc* desc, const cv_tile* ti
                           const void* prof, const void* u, const void* g, void* g_mat, int which,
                           double coef_re, double coef_im, void* stream);
 
+/* Forward-mode (tangent) pieces of the operator (the defjvp rules of
+ * primitives.py:137-148 specialised to the FDFD operator):
+ *   y (+)= (dA/dm0 . dm0 + dA/dm1 . dm1) x
+ *   Ez: dshift (dm0 + i dm1) .* x;  Hz: scale (Dxb (dm1 .* Dxf x) + Dyb (dm0 .* Dyf x)). */
+int cv_fdfd_material_apply(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                           const void* prof, const void* dm0, const void* dm1, const void* x, void* y,
+                           int accumulate, void* stream);
+
+/* Tangent of cv_fdfd_hz_materials: dm0 = -m0^2 (de + roll(de,1,axis=1))/2,
+ * dm1 = -m1^2 (de + roll(de,1,axis=0))/2. */
+int cv_fdfd_hz_materials_jvp(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                             const void* m0, const void* m1, const void* d_eps, void* dm0, void* dm1,
+                             void* stream);
+
 /* ------------------------------------------------------------- Krylov solve */
 #define CV_KRYLOV_BICGSTAB 0
 
@@ -229,6 +243,32 @@ int cv_fdtd_tm_configs(int* rpt, int* ts, int max_n);
 /* g_eps += chain rule of mE_a = 2/(eps + roll(eps, 1, a)) applied to g_mE. */
 int cv_fdtd_eps_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
                      const void* g_mE, void* g_eps, void* stream);
+
+/* ---------------------------------------------------- generic sparse (COO) */
+/* Segmented product sum over complex128 vectors:
+ *   out[o(s)] (+)= alpha * sum_{q = ptr[s]}^{ptr[s+1]-1} opu(u[iu[q]]) * opv(v[iv[q]])
+ * o(s) = oidx[s] (or s when oidx is NULL), iu / iv NULL mean the identity,
+ * opu / opv conjugate when conj_u / conj_v.  Deterministic (storage order).
+ * With host-built groupings of the COO entries this is the CSR SpMV of
+ * sp_mult (primitives.py:58-72), its A^T / A^H products and the gather
+ * v[i]*x[j] (:74-86), the SpGEMM numerics of spsp_mult (:153-176) and its
+ * VJP/JVP rules (:178-257), and the duplicate summation of make_sparse
+ * (utils.py:13-25). */
+int cv_segsum(int64_t n_seg, const int64_t* ptr, const int64_t* iu, const int64_t* iv,
+              const void* u, const void* v, int conj_u, int conj_v, double alpha_re, double alpha_im,
+              const int64_t* oidx, int accumulate, void* out, void* stream);
+
+/* In-place LU with partial pivoting of a column-major n x n complex128
+ * matrix (P A = L U, unit L), pivot rows in piv[n] (device int32); *info_dev
+ * (device int32) is set to k+1 for the first exactly zero pivot, else 0.
+ * Direct solve for small general systems: spl.spsolve at solvers.py:62. */
+int cv_dense_lu(int n, void* A, int32_t* piv, int32_t* info_dev, void* stream);
+
+/* Solves A X = B (trans 0), A^T X = B (1) or A^H X = B (2) in place with the
+ * cv_dense_lu factors; B is n x nrhs column-major.  The transposed solves
+ * serve the reverse mode of sp_solve (primitives.py:121-135) without a
+ * second factorisation. */
+int cv_dense_lu_solve(int n, const void* LU, const int32_t* piv, void* B, int nrhs, int trans, void* stream);
 
 /* ------------------------------------------------------------------ misc */
 const char* cv_version(void);

@@ -1,5 +1,7 @@
 """GPU parity of the FDFD path (HIP stencils + BiCGSTAB + adjoints) against the
 scipy oracle (ceviche's own algorithm: CSR assembly + SuperLU spsolve)."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -189,3 +191,57 @@ def test_batched_uneven_instances_match_spsolve():
         got = x[op.offsets[k]:op.offsets[k] + e.size]
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-6
         assert op.last_info[k]['rel_residual'] <= 1e-10
+
+
+@pytest.mark.parametrize('pol', ['ez', 'hz'])
+def test_reference_forward_mode_setup(pol):
+    """ceviche tests/test_gradients_fdfd.py:104-132 / :162-190: jacobian(mode='forward')
+    of |E|^2 + |H|^2 (all cells) w.r.t. a scalar c with eps = c eps_r and
+    source = c eps_r src, against central differences of the oracle."""
+    import ceviche_amd as cv
+    rng = np.random.default_rng(31)
+    shape = (20, 18)
+    omega, dL, npml = 2 * np.pi * 200e12, 1e-6, [5, 5]
+    eps_r = rng.random(shape) + 1
+    src = np.zeros(shape)
+    src[10, 9] = 1.0
+    cls = cv.fdfd_ez if pol == 'ez' else cv.fdfd_hz
+    F = cls(omega, dL, eps_r, npml)
+
+    eps_t, src_t = torch.from_numpy(eps_r), torch.from_numpy(src)
+
+    def J(c):
+        c = torch.as_tensor(c, dtype=torch.float64)
+        F.eps_r = c * eps_t
+        a, b, z = F.solve(c * eps_t * src_t)
+        return (z.abs() ** 2 + a.abs() ** 2 + b.abs() ** 2).reshape(-1)
+
+    jf = cv.jacobian(J, mode='forward')(1.0)
+    solver = R.solve_ez if pol == 'ez' else R.solve_hz
+
+    def Jr(c):
+        a, b, z = solver(omega, dL, c * eps_r, npml, c * eps_r * src)
+        return (np.abs(z) ** 2 + np.abs(a) ** 2 + np.abs(b) ** 2).ravel()
+
+    h = 1e-6
+    num = ((Jr(1 + h) - Jr(1 - h)) / (2 * h))[:, None]
+    assert jf.shape == (shape[0] * shape[1], 1)
+    assert np.linalg.norm(jf - num) <= 1e-4 * np.linalg.norm(num)
+    jn = cv.jacobian(J, mode='numerical')(1.0)
+    assert np.linalg.norm(jn - num) <= 1e-3 * np.linalg.norm(num)
+
+
+def test_reverse_jacobian_matches_reference_convention():
+    """jacobian(mode='reverse') of a real objective w.r.t. real eps (one row)."""
+    import ceviche_amd as cv
+    c = cfg1()
+    F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'])
+
+    def J(eps):
+        F.eps_r = eps
+        return F.solve(torch.from_numpy(c['src']))[2][c['probe']].abs() ** 2
+
+    jr = cv.jacobian(J, mode='reverse')(c['eps'])
+    g = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'waveguide_box_120.npz'))['grad_eps']
+    assert jr.shape == (1, 120 * 120)
+    assert np.linalg.norm(jr[0] - g.ravel()) <= 1e-4 * np.linalg.norm(g)
