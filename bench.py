@@ -13,7 +13,7 @@ are (2 * 3 fields + eps) * 4 B = 28 B.  Extra measured fields are reported
 under "extras" (float64 error against the oracle's golden trace, adjoint
 gradient time, an HBM-sized 2D run, a 3D run, FDFD solves/s, a copy probe).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras all|none]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras all|none|name,...]
 """
 import argparse
 import json
@@ -88,12 +88,26 @@ def fdtd_tm_main(steps, warmup):
     bytes_per_launch = 28.0 * cells * ts
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": committed_traffic("k_tm_block<float,%d,%d>" % (rpt, ts)),
             "kernel": "k_tm_block<float,%d,%d>" % (rpt, ts), "avg_launch_us": round(avg_launch_ms * 1e3, 2),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "note": "28 B/cell-step x 160000 cells x %d steps per launch; the 400x400 state (2.6 MB) is "
                     "cache-resident, so the fraction counts compulsory bytes, not HBM traffic" % ts}
     return dict(value=gcells, ms=ms, rec=rec, roofline=roof, cfg=(rpt, ts))
+
+
+def committed_traffic(kernel_tag):
+    """HBM bytes per launch of ``kernel_tag`` from the committed counter
+    summary in profiles/ (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, 'profiles', 'fdtd_tm_400_f32_traffic.md')
+    try:
+        for line in open(path):
+            if line.startswith('traffic_bytes_per_launch') and kernel_tag in line:
+                return float(line.split('=')[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline_fdtd(seconds=12.0):
@@ -117,7 +131,7 @@ def cpu_baseline_fdtd(seconds=12.0):
                       "(%.1f s)" % (n, dt)}
 
 
-def extras_all(main):
+def extras_all(main, which='all'):
     import numpy as np
     import torch
     import ceviche_amd as cv
@@ -196,27 +210,35 @@ def extras_all(main):
                 "hbm_frac_compulsory": round(g * 52 / HBM_PEAK_GBS, 4), "kernel": "generic two-phase"}
 
     def fdfd_cfg1():
+        from oracle import fdfd_ref
         c = workloads.fdfd_waveguide_box(120)
         gz = np.load(os.path.join(ROOT, 'tests', 'golden', 'waveguide_box_120.npz'))
         F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'])
         F.solve(c['src'])
         _sync()
         t0 = time.perf_counter()
-        reps = 3
+        reps = 5
         for _ in range(reps):
             _, _, Ez = F.solve(c['src'])
         dt = (time.perf_counter() - t0) / reps
         info = F.last_solve_info
         err = float(np.linalg.norm(Ez - gz['Ez']) / np.linalg.norm(gz['Ez']))
+        ts = []
+        for _ in range(3):  # the reference's default path: SuperLU spsolve, one core
+            t = time.perf_counter()
+            fdfd_ref.solve_ez(c['omega'], c['dL'], c['eps'], c['npml'], c['src'])
+            ts.append(time.perf_counter() - t)
+        cpu = sorted(ts)[1]
         return {"grid": [120, 120], "solves_per_s": round(1 / dt, 2), "iterations": info['iterations'],
                 "fp64_rel_residual": info['rel_residual'], "field_rel_err_vs_spsolve": err,
-                "precision": "complex128 BiCGSTAB, FP64 reductions"}
+                "precision": "complex128 BiCGSTAB, FP64 reductions",
+                "cpu_spsolve_s": round(cpu, 4), "cpu_solves_per_s": round(1 / cpu, 2), "cpu_cores": 1,
+                "speedup_vs_cpu": round(cpu / dt, 2)}
 
     def fdfd_1024():
         from ceviche_amd._fdfd_op import SolverOptions
         c = workloads.fdfd_waveguide_box(1024)
-        c['npml'] = [20, 20]
-        F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'], precision='complex64',
+        F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'],
                        solver_options=SolverOptions(tol=1e-6, max_iter=400000))
         _sync()
         t0 = time.perf_counter()
@@ -224,9 +246,64 @@ def extras_all(main):
         _sync()
         dt = time.perf_counter() - t0
         info = F.last_solve_info
-        return {"grid": [1024, 1024], "solves_per_s": round(1 / dt, 4), "iterations": info['iterations'],
-                "fp64_rel_residual": info['rel_residual'],
-                "precision": "complex64 BiCGSTAB in complex128 refinement, FP64 reductions, no preconditioner"}
+        gbs = info['iterations'] * 288.0 * 1024 * 1024 / dt / 1e9
+        return {"grid": [1024, 1024], "solves_per_s": round(1 / dt, 4), "seconds": round(dt, 3),
+                "iterations": info['iterations'], "fp64_rel_residual": info['rel_residual'],
+                "us_per_iteration": round(1e6 * dt / max(1, info['iterations']), 1),
+                "krylov_algorithmic_GBs": round(gbs, 1), "krylov_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                "precision": "complex128 BiCGSTAB (288 B/cell/iteration), FP64 reductions, no preconditioner"}
+
+    def fdfd_batch():
+        from ceviche_amd._fdfd_op import FdfdOperator, SolverOptions
+        insts = workloads.fdfd_blob_batch(n_inst=4, n_lambda=4)
+        specs = [dict(omega=d['omega'], dL=d['dL'], shape=d['eps'].shape, npml=d['npml']) for d in insts]
+        op = FdfdOperator('ez', specs)
+        eps = torch.from_numpy(np.concatenate([d['eps'].ravel() for d in insts])).cuda()
+        b = torch.from_numpy(np.concatenate([1j * d['omega'] * d['src'].ravel() for d in insts])).cuda()
+        m0, m1 = op.materials(eps)
+        bs = b * op.scale_vector(torch.float64)
+        opts = SolverOptions(tol=1e-6, max_iter=200000)
+        _sync()
+        t0 = time.perf_counter()
+        op.solve_scaled(bs, m0, m1, opts=opts)
+        _sync()
+        dt = time.perf_counter() - t0
+        its = [i['iterations'] for i in op.last_info]
+        res = max(i['rel_residual'] for i in op.last_info)
+        # the same instances one at a time
+        t0 = time.perf_counter()
+        for d in insts[:4]:
+            F = cv.fdfd_ez(d['omega'], d['dL'], d['eps'], d['npml'], solver_options=opts)
+            F.solve(d['src'])
+        _sync()
+        dt1 = (time.perf_counter() - t0) / 4
+        return {"instances": len(insts), "sizes": sorted(set(int(d['eps'].shape[0]) for d in insts)),
+                "batched_solves_per_s": round(len(insts) / dt, 3), "one_by_one_solves_per_s": round(1 / dt1, 3),
+                "iterations_min_max": [min(its), max(its)], "max_fp64_rel_residual": res,
+                "precision": "complex128 BiCGSTAB, FP64 reductions, all instances in each launch"}
+
+    def sp_mult_1m():
+        import scipy.sparse as sps
+        from ceviche_amd import primitives as cp
+        rng = np.random.default_rng(0)
+        N, M = 1 << 20, 5 << 20
+        ind = np.vstack((rng.integers(0, N, M), rng.integers(0, N, M)))
+        ent = torch.from_numpy(rng.standard_normal(M) + 1j * rng.standard_normal(M)).cuda()
+        x = torch.from_numpy(rng.standard_normal(N) + 1j * rng.standard_normal(N)).cuda()
+        cp.sp_mult(ent, ind, x)
+        _sync()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            y = cp.sp_mult(ent, ind, x)
+        _sync()
+        dt = (time.perf_counter() - t0) / 10
+        A = sps.csr_matrix((ent.cpu().numpy(), (ind[0], ind[1])), shape=(N, N))
+        xh = x.cpu().numpy()
+        t0 = time.perf_counter()
+        A.dot(xh)
+        cpu = time.perf_counter() - t0
+        return {"N": N, "nnz": M, "gpu_ms": round(1e3 * dt, 3), "cpu_scipy_csr_ms": round(1e3 * cpu, 2),
+                "note": "random COO, duplicates summed; host symbolic grouping cached per index set"}
 
     def copy_probe():
         n = 1 << 28  # 1 GiB of float32
@@ -243,13 +320,12 @@ def extras_all(main):
         gbs = 2 * 4 * n * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
         return {"copy_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
 
-    guard('fdtd_tm_400_trace', trace_errors)
-    guard('fdtd_tm_400_adjoint', adjoint)
-    guard('fdtd_tm_8192', tm_hbm)
-    guard('fdtd_3d_256', fdtd_3d)
-    guard('fdfd_ez_120', fdfd_cfg1)
-    guard('fdfd_ez_1024_plain', fdfd_1024)
-    guard('stream_copy', copy_probe)
+    table = [('fdtd_tm_400_trace', trace_errors), ('fdtd_tm_400_adjoint', adjoint), ('fdtd_tm_8192', tm_hbm),
+             ('fdtd_3d_256', fdtd_3d), ('fdfd_ez_120', fdfd_cfg1), ('fdfd_ez_1024_plain', fdfd_1024),
+             ('fdfd_ez_batch', fdfd_batch), ('sp_mult_1M', sp_mult_1m), ('stream_copy', copy_probe)]
+    for name, fn in table:
+        if which == 'all' or name in which:
+            guard(name, fn)
     return out
 
 
@@ -258,7 +334,8 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--extras', default='all', choices=['all', 'none'])
+    ap.add_argument('--extras', default='all',
+                    help="'all', 'none' or a comma-separated list of extra measurements")
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -271,7 +348,8 @@ def main():
         sys.exit(1)
     main_res = fdtd_tm_main(args.steps, args.warmup)
     log('main: %.2f Gcell/s, %.3f ms per 2000-step run' % (main_res['value'], main_res['ms']))
-    extras = extras_all(main_res) if args.extras == 'all' else {}
+    which = 'all' if args.extras == 'all' else set(args.extras.split(','))
+    extras = extras_all(main_res, which) if args.extras != 'none' else {}
     cpu = cpu_baseline_fdtd(args.cpu_seconds)
     line = {
         "metric": METRIC,

@@ -28,7 +28,7 @@ CV_POL_EZ = 0
 CV_POL_HZ = 1
 CV_FLAG_X_IDENTITY = 1
 CV_FLAG_Y_IDENTITY = 2
-CV_TILE_CELLS = 1024
+CV_TILE_CELLS = 256
 
 _STATUS_NAMES = {
     CV_ERR_INVALID: 'invalid argument',

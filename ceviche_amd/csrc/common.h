@@ -156,9 +156,20 @@ __device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], doub
     double acc[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) acc[c] = 0.0;
-    for (int t = threadIdx.x; t < tile_count; t += kBlock) {
+    // 4 independent loads in flight per thread (fixed summation order)
+    for (int t0 = 0; t0 < tile_count; t0 += 4 * kBlock) {
+        double v[4][NV];
 #pragma unroll
-        for (int c = 0; c < NV; ++c) acc[c] += load_agent(&partials[(size_t)(tile_first + t) * NV + c]);
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * kBlock + (int)threadIdx.x;
+#pragma unroll
+            for (int c = 0; c < NV; ++c)
+                v[u][c] = t < tile_count ? load_agent(&partials[(size_t)(tile_first + t) * NV + c]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) acc[c] += v[u][c];
     }
     block_sum<NV>(acc, lds);
     __shared__ double s_tot[NV];

@@ -5,7 +5,7 @@
 
 namespace cv {
 
-constexpr int kCellsPerThread = CV_TILE_CELLS / kBlock;  // 4
+constexpr int kCellsPerThread = CV_TILE_CELLS / kBlock;
 
 // ---------------------------------------------------------------- y = A x
 template <typename R, bool TRANS>

@@ -11,6 +11,11 @@ fdtd_tm_blobs        2D TM FDTD, NxN (400 by default), dL 20 nm, npml 20;
                      to {1, 12.25}; Gaussian-pulse Jz at the centre (centre
                      step 200, width 50 steps); Ez probes at (N/4, N/2),
                      (3N/4, N/2), (N/2, N/4), (N/2, 3N/4).
+fdfd_blob_batch      batch of 2D Ez FDFD instances, sizes drawn from
+                     {200, 300, 400, 600}^2, dL 40 nm, npml 20, wavelengths
+                     evenly spaced over 1.50-1.60 um; eps = waveguide along x
+                     plus a central binary-blob design (sigma 3 cells);
+                     Jz line source across the input waveguide.
 fdtd_3d_blobs        3D FDTD, N^3 (256 by default), dL 25 nm, npml 10;
                      background eps 4.41 with a centred (N/4)^3 design region
                      of seeded binary blobs {1, 12.25}; z-dipole Gaussian
@@ -66,3 +71,24 @@ def fdtd_3d_blobs(N=256, steps=1000, seed=2):
                 probes.append((comp, (i, j, zp)))
     return dict(eps=eps, dL=25e-9, npml=[10, 10, 10], src=src, amp=gaussian_pulse(steps, 200, 50),
                 probes=probes, steps=steps, plane=(slice(lo, lo + d), slice(lo, lo + d), zp))
+
+
+def fdfd_blob_batch(n_inst=64, n_lambda=8, seed=3, sizes=(200, 300, 400, 600)):
+    """List of dicts (omega, dL, eps, npml, src) for a multi-frequency batch."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    lams = np.linspace(1.50e-6, 1.60e-6, n_lambda)
+    out = []
+    for k in range(n_inst):
+        N = int(rng.choice(sizes))
+        eps = np.ones((N, N))
+        c = N // 2
+        eps[:, c - 5:c + 5] = 12.25
+        d = N // 3
+        blob = gaussian_filter(rng.standard_normal((d, d)), 3) > 0
+        eps[c - d // 2:c - d // 2 + d, c - d // 2:c - d // 2 + d] = np.where(blob, 12.25, 1.0)
+        src = np.zeros((N, N))
+        src[25, c - 8:c + 8] = 1.0
+        for lam in lams:
+            out.append(dict(omega=2 * np.pi * C_0 / lam, dL=40e-9, eps=eps, npml=[20, 20], src=src))
+    return out

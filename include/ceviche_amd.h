@@ -70,7 +70,7 @@ typedef struct cv_fdfd_desc {
 /* Tile table entry: one workgroup processes CV_TILE_CELLS consecutive cells of
  * one instance starting at cell `first` (instance-local index). */
 typedef struct cv_tile { int32_t inst, first; } cv_tile;
-#define CV_TILE_CELLS 1024
+#define CV_TILE_CELLS 256
 
 /* y = A x (transpose == 0) or y = A^T x (transpose != 0) for every instance.
  * Replaces the COO assembly + CSR SpMV of fdfd.py:172-185 / :211-233 and
