@@ -105,6 +105,16 @@ class FdtdGrid:
                                    int(rec_t0), int(config), stream_ptr()), 'fdtd run_tm')
         return state
 
+    def run_fused(self, state, other, eps, comp, src_cell, amp_dev, t0, n_steps, probes, rec, rec_t0):
+        """Fused single-pass steps ping-ponging between ``state`` and ``other``;
+        returns (current, spare) after the steps."""
+        pc, pl = (None, None) if probes is None else probes
+        npb = 0 if probes is None else int(pc.numel())
+        check(lib().cv_fdtd_run_fused(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(other),
+                                      int(comp), int(src_cell), ptr(amp_dev), int(t0), int(n_steps), ptr(pc),
+                                      ptr(pl), npb, ptr(rec), int(rec_t0), stream_ptr()), 'fdtd run_fused')
+        return (other, state) if n_steps % 2 else (state, other)
+
     def efields(self, state, eps):
         E = torch.empty((3,) + self.shape, dtype=self.dtype, device=self.device)
         check(lib().cv_fdtd_efields(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(E),
@@ -238,8 +248,9 @@ class FdtdRun(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eps, amp, grid, state0, src, probes, n_steps, checkpoint_every, tm_config=None):
         amp_h = amp.detach().to('cpu', torch.float64).numpy()
-        amp_dev = torch.from_numpy(amp_h).to(grid.device) if tm_config is not None else None
+        amp_dev = torch.from_numpy(amp_h).to(grid.device) if tm_config != 'generic' else None
         state = state0.detach().clone()
+        spare = torch.empty_like(state) if tm_config == 'fused' else None
         pc, pl = probes
         rec = torch.empty((n_steps, pc.numel()), dtype=grid.dtype, device=grid.device)
         ck = max(1, int(checkpoint_every))
@@ -250,10 +261,12 @@ class FdtdRun(torch.autograd.Function):
             m = min(ck, n_steps - t) if need_grad else n_steps - t
             if need_grad:
                 checkpoints.append(state.clone())
-            if tm_config is not None:
-                grid.run_tm(state, eps, src[1], amp_dev, t, m, probes, rec, 0, tm_config)
-            else:
+            if tm_config == 'fused':
+                state, spare = grid.run_fused(state, spare, eps, src[0], src[1], amp_dev, t, m, probes, rec, 0)
+            elif tm_config == 'generic':
                 grid.step(state, eps, n_steps=m, src=src, amp=amp_h[t:t + m], probes=probes, rec=rec[t:t + m])
+            else:
+                grid.run_tm(state, eps, src[1], amp_dev, t, m, probes, rec, 0, tm_config)
             t += m
         ctx.grid, ctx.src, ctx.probes, ctx.n_steps, ctx.ck = grid, src, probes, n_steps, ck
         ctx.amp_h, ctx.amp_dev, ctx.tm_config = amp_h, amp_dev, tm_config
@@ -279,7 +292,7 @@ class FdtdRun(torch.autograd.Function):
             st = ctx.checkpoints[sgi].clone()
             Dhist[0].copy_(st[3:6])
             for s in range(m):
-                if ctx.tm_config is not None:
+                if isinstance(ctx.tm_config, int):
                     grid.run_tm(st, eps, src[1], ctx.amp_dev, t0 + s, 1, None, None, 0, ctx.tm_config)
                 else:
                     grid.step(st, eps, n_steps=1, src=src, amp=ctx.amp_h[t0 + s:t0 + s + 1])

@@ -703,6 +703,122 @@ static int dispatch_tm(int cfg, const Fdtd<R>& f, const double* amp_dev, int64_t
     CV_CHECK(false, "unknown TM tile configuration %d", cfg);
 }
 
+// ================================================ fused single-pass 3D step
+// One kernel per step: the thread of cell c recomputes the new H at c and at
+// its three backward neighbours (c - e_x, c - e_y, c - e_z) from the OLD state
+// A, then forms the new D at c, writing H, D and the PML integrals of c to the
+// NEW state B (ping-pong, so no cell reads a value another thread has already
+// advanced).  DRAM traffic is one read of H, D, eps and one write of H, D per
+// cell-step; the neighbour values come from cache.
+template <typename R>
+__host__ __device__ __forceinline__ R h_comp(const Fdtd<R>& f, const R* __restrict__ A, int a, int i, int j, int k,
+                                             R& ice_out, R& ih_out) {
+    const int64_t n = f.n;
+    const R* D = A + 3 * n;
+    const int ip = wrapp(i, f.nx), jp = wrapp(j, f.ny), kp = wrapp(k, f.nz);
+    R ce;
+    if (a == 0)
+        ce = ((efield(f, D, 2, i, jp, k) - efield(f, D, 2, i, j, k)) -
+              (efield(f, D, 1, i, j, kp) - efield(f, D, 1, i, j, k))) * f.inv_dL;
+    else if (a == 1)
+        ce = ((efield(f, D, 0, i, j, kp) - efield(f, D, 0, i, j, k)) -
+              (efield(f, D, 2, ip, j, k) - efield(f, D, 2, i, j, k))) * f.inv_dL;
+    else
+        ce = ((efield(f, D, 1, ip, j, k) - efield(f, D, 1, i, j, k)) -
+              (efield(f, D, 0, i, jp, k) - efield(f, D, 0, i, j, k))) * f.inv_dL;
+    const int pos[3] = {i, j, k};
+    const int ax_b = (a == 0) ? 1 : 0;
+    const int ax_c = (a == 2) ? 1 : 2;
+    const R sa = sigH(f, a)[pos[a]], sb = sigH(f, ax_b)[pos[ax_b]], sc = sigH(f, ax_c)[pos[ax_c]];
+    const int64_t ca = a * n + cell(f, i, j, k);
+    const R h = A[ca];
+    if (sa == R(0) && sb == R(0) && sc == R(0)) return f.h1 * h + f.h2 * ce;
+    R m1, m2, m3, m4;
+    coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(-1), m1, m2, m3, m4);
+    R hn = m1 * h + m2 * ce;
+    if (sa != R(0)) {
+        ice_out = A[6 * n + ca] + ce;
+        hn += m3 * ice_out;
+    }
+    if (sb != R(0) && sc != R(0)) {
+        ih_out = A[9 * n + ca] + h;
+        hn += m4 * ih_out;
+    }
+    return hn;
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void fused_cell(const Fdtd<R>& f, const R* __restrict__ A, R* __restrict__ B,
+                                                    PointSrc ps, int64_t c) {
+    const int k = (int)(c % f.nz);
+    const int64_t ij = c / f.nz;
+    const int j = (int)(ij % f.ny);
+    const int i = (int)(ij / f.ny);
+    const int64_t n = f.n;
+    const int im = wrapm(i, f.nx), jm = wrapm(j, f.ny), km = wrapm(k, f.nz);
+    R h[3], ice[3], ih[3];
+    for (int a = 0; a < 3; ++a) {
+        ice[a] = R(0);
+        ih[a] = R(0);
+        h[a] = h_comp(f, A, a, i, j, k, ice[a], ih[a]);
+    }
+    R dummy0, dummy1;
+    // backward neighbours' new H components used by curl_H at c
+    const R hz_jm = h_comp(f, A, 2, i, jm, k, dummy0, dummy1);
+    const R hy_km = h_comp(f, A, 1, i, j, km, dummy0, dummy1);
+    const R hx_km = h_comp(f, A, 0, i, j, km, dummy0, dummy1);
+    const R hz_im = h_comp(f, A, 2, im, j, k, dummy0, dummy1);
+    const R hy_im = h_comp(f, A, 1, im, j, k, dummy0, dummy1);
+    const R hx_jm = h_comp(f, A, 0, i, jm, k, dummy0, dummy1);
+    R ch[3];
+    ch[0] = ((h[2] - hz_jm) - (h[1] - hy_km)) * f.inv_dL;
+    ch[1] = ((h[0] - hx_km) - (h[2] - hz_im)) * f.inv_dL;
+    ch[2] = ((h[1] - hy_im) - (h[0] - hx_jm)) * f.inv_dL;
+    const int pos[3] = {i, j, k};
+    for (int a = 0; a < 3; ++a) {
+        const int64_t ca = a * n + c;
+        B[ca] = h[a];
+        const int ax_b = (a == 0) ? 1 : 0;
+        const int ax_c = (a == 2) ? 1 : 2;
+        {
+            const R sa = sigH(f, a)[pos[a]], sb = sigH(f, ax_b)[pos[ax_b]], sc = sigH(f, ax_c)[pos[ax_c]];
+            if (sa != R(0)) B[6 * n + ca] = ice[a];
+            if (sb != R(0) && sc != R(0)) B[9 * n + ca] = ih[a];
+        }
+        const R sa = sigD(f, a)[pos[a]], sb = sigD(f, ax_b)[pos[ax_b]], sc = sigD(f, ax_c)[pos[ax_c]];
+        const R d = A[3 * n + ca];
+        R dn;
+        if (sa == R(0) && sb == R(0) && sc == R(0)) {
+            dn = f.d1 * d + f.d2 * ch[a];
+        } else {
+            R m1, m2, m3, m4;
+            coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(1), m1, m2, m3, m4);
+            dn = m1 * d + m2 * ch[a];
+            if (sa != R(0)) {
+                const R ich = A[12 * n + ca] + ch[a];
+                B[12 * n + ca] = ich;
+                dn += m3 * ich;
+            }
+            if (sb != R(0) && sc != R(0)) {
+                const R idv = A[15 * n + ca] + d;
+                B[15 * n + ca] = idv;
+                dn += m4 * idv;
+            }
+        }
+        if (ps.comp == a && ps.cell == c) dn += R(ps.amp);
+        B[3 * n + ca] = dn;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_fused(Fdtd<R> f, const R* __restrict__ A, R* __restrict__ B, PointSrc ps,
+                                               const double* __restrict__ amp_dev, int t) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    if (ps.comp >= 0) ps.amp = amp_dev[t];
+    fused_cell(f, A, B, ps, c);
+}
+
 // ------------------------------------------------------------ host glue
 template <typename R>
 static int make_fdtd(const cv_fdtd_desc* d, const void* sig, const void* eps, void* state, Fdtd<R>& f) {
@@ -909,6 +1025,62 @@ extern "C" int cv_fdtd_run_tm(int dtype, const cv_fdtd_desc* d, const void* sig,
         rc = dispatch_tm<double>(config, f, amp_dev, src_cell, t0, n_steps, probe_comp, probe_cell, n_probes,
                                  (double*)rec, rec_t0, s);
         if (rc) return rc;
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state_a,
+                                 void* state_b, int comp, int64_t src_cell, const double* amp_dev, int t0,
+                                 int n_steps, const int32_t* probe_comp, const int64_t* probe_cell, int n_probes,
+                                 void* rec, int rec_t0, void* stream) {
+    hipStream_t s = as_stream(stream);
+    CV_CHECK(state_a && state_b && state_a != state_b, "two distinct state buffers are needed");
+    CV_CHECK(comp >= -1 && comp < 3, "comp must be -1 (none) or 0..2");
+    CV_CHECK(comp < 0 || amp_dev, "a source needs its amplitude array");
+    CV_CHECK(n_probes == 0 || (probe_comp && probe_cell && rec), "probes need comp, cell and rec buffers");
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, state_a, f);
+        if (rc) return rc;
+        CV_CHECK(comp < 0 || (src_cell >= 0 && src_cell < f.n), "source cell out of range");
+        float* A = (float*)state_a;
+        float* B = (float*)state_b;
+        for (int t = 0; t < n_steps; ++t) {
+            PointSrc ps{comp, src_cell, 0.0};
+            k_fused<float><<<nblk(f.n), 256, 0, s>>>(f, A, B, ps, amp_dev, t0 + t);
+            if (n_probes > 0) {
+                Fdtd<float> g = f;
+                g.st = B;
+                k_record<float><<<(n_probes + 63) / 64, 64, 0, s>>>(g, probe_comp, probe_cell, n_probes,
+                                                                    (float*)rec + (int64_t)(t0 + t - rec_t0) * n_probes);
+            }
+            float* tmp = A;
+            A = B;
+            B = tmp;
+        }
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, state_a, f);
+        if (rc) return rc;
+        CV_CHECK(comp < 0 || (src_cell >= 0 && src_cell < f.n), "source cell out of range");
+        double* A = (double*)state_a;
+        double* B = (double*)state_b;
+        for (int t = 0; t < n_steps; ++t) {
+            PointSrc ps{comp, src_cell, 0.0};
+            k_fused<double><<<nblk(f.n), 256, 0, s>>>(f, A, B, ps, amp_dev, t0 + t);
+            if (n_probes > 0) {
+                Fdtd<double> g = f;
+                g.st = B;
+                k_record<double><<<(n_probes + 63) / 64, 64, 0, s>>>(g, probe_comp, probe_cell, n_probes,
+                                                                     (double*)rec + (int64_t)(t0 + t - rec_t0) * n_probes);
+            }
+            double* tmp = A;
+            A = B;
+            B = tmp;
+        }
     } else {
         CV_CHECK(false, "unknown dtype %d", dtype);
     }

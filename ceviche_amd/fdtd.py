@@ -152,8 +152,9 @@ class fdtd:
 
         kernel: 'auto' uses the temporally blocked 2D TM kernel when it applies
         (nz == 1, Jz source, TE fields zero, Ez/Hx/Hy probes, <= 256 probes;
-        tile shape from the autotuner), 'generic' forces the per-step kernels,
-        or an int selects a TM tile configuration."""
+        tile shape from the autotuner) and the fused single-pass kernel
+        otherwise; 'generic' forces the two-phase per-step kernels, 'fused'
+        the single-pass kernel, an int a TM tile configuration."""
         g = self._grid
         g.desc.dL = self.dL
         comp = {'Jx': 0, 'Jy': 1, 'Jz': 2}[source_component]
@@ -167,14 +168,16 @@ class fdtd:
         pt = _probe_tensors(g, probes)
         tm_ok = (g.nz == 1 and comp == 2 and not self._te_live and len(probes) <= 256
                  and all(COMPONENTS[p[0]] in (2, 6, 7) for p in probes))
-        if kernel == 'generic' or not tm_ok:
-            if isinstance(kernel, int) and not tm_ok:
+        if isinstance(kernel, int):
+            if not tm_ok:
                 raise ValueError('the TM kernel does not apply to this run')
-            cfg = None
-        elif isinstance(kernel, int):
             cfg = kernel
+        elif kernel in ('generic', 'fused'):
+            cfg = kernel
+        elif kernel == 'auto':
+            cfg = autotune_tm(g, self._eps_dev.detach()) if tm_ok else 'fused'
         else:
-            cfg = autotune_tm(g, self._eps_dev.detach())
+            raise ValueError("kernel must be 'auto', 'generic', 'fused' or a TM configuration index")
         rec, state = FdtdRun.apply(self._eps_dev, amp, g, self._state, (comp, cell), pt, int(steps),
                                    int(checkpoint_every), cfg)
         self._state = state

@@ -236,6 +236,17 @@ int cv_fdtd_run_tm(int dtype, const cv_fdtd_desc* d, const void* sig, const void
                    const int32_t* probe_comp, const int64_t* probe_cell, int n_probes, void* rec,
                    int rec_t0, int config, void* stream);
 
+/* Fused single-pass Yee steps for any grid (3D or 2D, point source J[comp]
+ * at src_cell with amplitude amp_dev[t] (device float64, global step t), or
+ * comp = -1 for none): each step reads state_a and writes state_b (18-array
+ * layout), then the roles swap; after an odd n_steps the result is in state_b.
+ * Probes as cv_fdtd_step, rec row t - rec_t0.  Same arithmetic as
+ * cv_fdtd_step; one kernel per step instead of two. */
+int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state_a,
+                      void* state_b, int comp, int64_t src_cell, const double* amp_dev, int t0,
+                      int n_steps, const int32_t* probe_comp, const int64_t* probe_cell, int n_probes,
+                      void* rec, int rec_t0, void* stream);
+
 /* Tile configurations of cv_fdtd_run_tm: fills rpt[i], ts[i] (rows per
  * thread, steps per launch) for i < max_n and returns their number. */
 int cv_fdtd_tm_configs(int* rpt, int* ts, int max_n);

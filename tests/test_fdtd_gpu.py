@@ -236,3 +236,28 @@ def test_tm_blocked_gradient_matches_generic():
         (rec ** 2).sum().backward()
         grads.append(eps.grad.numpy())
     assert _rel(grads[0], grads[1]) <= 1e-10
+
+
+@pytest.mark.parametrize('shape,npml,comp', [((12, 10, 9), [3, 2, 2], 'Jz'), ((15, 13, 1), [4, 3, 0], 'Jx'),
+                                             ((7, 9, 11), [0, 2, 3], 'Jy')])
+def test_fused_single_pass_kernel_matches_oracle(shape, npml, comp):
+    import ceviche_amd as cv
+    rng = np.random.default_rng(13)
+    eps0 = rng.uniform(1, 5, shape)
+    steps, dL = 37, 30e-9
+    amp = np.exp(-(np.arange(steps) - 12) ** 2 / (2 * 4 ** 2))
+    src = tuple(s // 2 for s in shape)
+    probes = [('Ex', (1, 2, 0)), ('Hy', (shape[0] - 1, 1, shape[2] - 1)), ('Dz', src), ('Hz', (2, 3, shape[2] // 2))]
+    F = cv.fdtd(torch.tensor(eps0), dL, npml)
+    rec = F.run(steps, src, amp, probes, source_component=comp, kernel='fused').cpu().numpy()
+    fields = {k: v.detach().cpu().numpy() for k, v in F.fields.items()}
+    R = T.FDTD(eps0, dL, npml)
+    J = np.zeros(shape)
+    tr = []
+    for t in range(steps):
+        J[src] = amp[t]
+        r = R.forward(**{comp: J})
+        tr.append([r[c][p] for c, p in probes])
+    assert _rel(rec, np.array(tr)) <= 1e-11
+    for k in FIELDS:
+        assert _rel(fields[k], r[k]) <= 1e-11, k
