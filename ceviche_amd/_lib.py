@@ -63,6 +63,17 @@ class FdtdDesc(ctypes.Structure):
                 ('c0', ctypes.c_double), ('eps0', ctypes.c_double)]
 
 
+CV_PRECOND_NONE = 0
+CV_PRECOND_JACOBI = 1
+CV_PRECOND_UNET = 2
+CV_CNN_PACKED_ELEMS = 9216
+
+
+class UNetDesc(ctypes.Structure):
+    _fields_ = [('wpack', ctypes.c_void_p), ('bias', ctypes.c_void_p), ('feat', ctypes.c_void_p),
+                ('nf', ctypes.c_int32), ('levels', ctypes.c_int32)]
+
+
 class CevicheError(RuntimeError):
     """Raised for a non-zero status from the HIP library."""
 
@@ -81,6 +92,7 @@ P = ctypes.c_void_p
 I = ctypes.c_int
 I64 = ctypes.c_int64
 D = ctypes.c_double
+F = ctypes.c_float
 
 _SIGS = {
     'cv_version': (ctypes.c_char_p, []),
@@ -108,6 +120,21 @@ _SIGS = {
     'cv_segsum': (I, [I64, P, P, P, P, P, I, I, D, D, P, I, P, P]),
     'cv_dense_lu': (I, [I, P, P, P, P]),
     'cv_dense_lu_solve': (I, [I, P, P, P, I, I, P]),
+    'cv_cnn_pack_weights': (I, [P, P, I, I, P]),
+    'cv_conv3x3_fwd_bf16': (I, [P, P, P, P, I, I, I, I, P]),
+    'cv_conv3x3_bwd_data_bf16': (I, [P, P, P, P, I, I, I, I, P]),
+    'cv_conv3x3_bwd_weight_workspace_bytes': (I64, [I, I, I]),
+    'cv_conv3x3_bwd_weight_bf16': (I, [P, P, P, P, P, I, I, I, I, P, I64, P]),
+    'cv_cnn_down2': (I, [P, P, I, I, I, F, P]),
+    'cv_cnn_up2': (I, [P, P, P, I, I, I, F, P]),
+    'cv_cnn_pack_input': (I, [I, P, P, P, F, P, P, I, P, I, I, I, P]),
+    'cv_cnn_unpack': (I, [I, P, P, F, P, I, I, I, P]),
+    'cv_cnn_loss': (I, [P, P, P, P, P, P, F, P, F, I, I, I, P]),
+    'cv_fdfd_batch_apply': (I, [P, P, I, P, P, P, P, P, P, P, P, P, D, I, I, P]),
+    'cv_unet_workspace_bytes': (I64, [I, I, I, I]),
+    'cv_unet_apply': (I, [P, I, P, F, F, P, I, I, P, I64, P]),
+    'cv_fdfd_fgmres_workspace_bytes': (I64, [I, I, I, I, I]),
+    'cv_fdfd_fgmres': (I, [P, P, P, P, P, P, P, P, P, P, D, I, I, D, I, P, F, P, I64, P, P]),
 }
 
 

@@ -379,6 +379,33 @@ struct Work {
 
 static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
+// A captured, instantiated loop body.  Capture runs on a private stream of
+// default priority (the caller's stream may be the legacy null stream, which
+// cannot be captured); the executable graph is launched on the caller's stream.
+struct LoopGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+    template <typename Body>
+    hipError_t capture(Body&& body) {
+        hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+        e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        body(cap);
+        const hipError_t le = hipGetLastError();
+        e = hipStreamEndCapture(cap, &graph);
+        if (le != hipSuccess) return le;
+        if (e != hipSuccess) return e;
+        return hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    }
+    ~LoopGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
 static int64_t layout(int64_t cs, int64_t cells, int n_inst, int n_tiles, char* base, Work* w) {
     int64_t off = 0;
     auto take = [&](int64_t bytes) -> void* {
@@ -423,22 +450,31 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
     k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.partials, w.counters,
                                                 w.state, tol, max_iter, 0);
     CV_LAUNCH_CHECK();
-    int cur = 0;
     const int max_checks = 8;
     int checks = 0;
     hs.resize(n_inst);
-    while (true) {
+    // The check_every iterations between host checks (an even count, so the
+    // p / v buffers are back in place at the end) are captured once into a
+    // graph and replayed: every scalar lives in device memory, so the launch
+    // arguments never change, and one graph launch replaces 3 * check_every
+    // kernel launches.
+    auto body = [&](hipStream_t q) {
+        int cur = 0;
         for (int c = 0; c < check_every; ++c) {
             const int nxt = cur ^ 1;
-            k_bicg_p<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, r, rh, p[cur], v[cur], p[nxt],
+            k_bicg_p<R, TRANS><<<n_tiles, kBlock, 0, q>>>(desc, tiles, prof, m0, m1, r, rh, p[cur], v[cur], p[nxt],
                                                          v[nxt], w.partials, w.counters, w.state);
-            k_bicg_s<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, r, v[nxt], tv, w.partials,
+            k_bicg_s<R, TRANS><<<n_tiles, kBlock, 0, q>>>(desc, tiles, prof, m0, m1, r, v[nxt], tv, w.partials,
                                                          w.counters, w.state);
-            k_bicg_x<R><<<n_tiles, kBlock, 0, s>>>(desc, tiles, x, p[nxt], r, v[nxt], tv, rh, w.partials,
+            k_bicg_x<R><<<n_tiles, kBlock, 0, q>>>(desc, tiles, x, p[nxt], r, v[nxt], tv, rh, w.partials,
                                                    w.counters, w.state);
             cur = nxt;
         }
-        CV_LAUNCH_CHECK();
+    };
+    LoopGraph graph;
+    CV_HIP(graph.capture(body));
+    while (true) {
+        CV_HIP(hipGraphLaunch(graph.exec, s));
         CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
         CV_HIP(hipStreamSynchronize(s));
         bool all_done = true;
@@ -508,6 +544,7 @@ extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_
     CV_CHECK(workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
              (long long)need);
     if (check_every <= 0) check_every = 16;
+    check_every += check_every & 1;
     hipStream_t s = as_stream(stream);
     std::vector<KState> hs;
     int rc;
@@ -552,6 +589,7 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
     CV_CHECK(workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
              (long long)need);
     if (check_every <= 0) check_every = 16;
+    check_every += check_every & 1;
     if (max_sweeps <= 0) max_sweeps = 10;
     char* base = (char*)workspace;
     Work w;

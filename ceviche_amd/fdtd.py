@@ -152,9 +152,10 @@ class fdtd:
 
         kernel: 'auto' uses the temporally blocked 2D TM kernel when it applies
         (nz == 1, Jz source, TE fields zero, Ez/Hx/Hy probes, <= 256 probes;
-        tile shape from the autotuner) and the fused single-pass kernel
-        otherwise; 'generic' forces the two-phase per-step kernels, 'fused'
-        the single-pass kernel, an int a TM tile configuration."""
+        tile shape from the autotuner) and the two-phase per-step kernels
+        otherwise (measured faster than the single-pass kernel at 256^3);
+        'generic' forces the two-phase kernels, 'fused' the single-pass
+        kernel, an int a TM tile configuration."""
         g = self._grid
         g.desc.dL = self.dL
         comp = {'Jx': 0, 'Jy': 1, 'Jz': 2}[source_component]
@@ -175,7 +176,7 @@ class fdtd:
         elif kernel in ('generic', 'fused'):
             cfg = kernel
         elif kernel == 'auto':
-            cfg = autotune_tm(g, self._eps_dev.detach()) if tm_ok else 'fused'
+            cfg = autotune_tm(g, self._eps_dev.detach()) if tm_ok else 'generic'
         else:
             raise ValueError("kernel must be 'auto', 'generic', 'fused' or a TM configuration index")
         rec, state = FdtdRun.apply(self._eps_dev, amp, g, self._state, (comp, cell), pt, int(steps),

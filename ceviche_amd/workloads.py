@@ -92,3 +92,26 @@ def fdfd_blob_batch(n_inst=64, n_lambda=8, seed=3, sizes=(200, 300, 400, 600)):
         for lam in lams:
             out.append(dict(omega=2 * np.pi * C_0 / lam, dL=40e-9, eps=eps, npml=[20, 20], src=src))
     return out
+
+
+def fdfd_learned_blobs(N=256, seed=0, lam=None):
+    """2D Ez FDFD instance for the learned preconditioner: NxN, dL 40 nm,
+    npml 20; background eps 1, a waveguide along x (eps 12.25, 10 cells) and
+    a central (N/2)^2 region of seeded binary blobs {1, 12.25} (Gaussian-
+    filtered noise, sigma 3 cells, thresholded at 0); wavelength drawn
+    uniformly from [1.45, 1.65] um unless given; Jz line source across the
+    waveguide at x-index 30.  Seeds 0 .. 9999 are training structures,
+    seeds >= 10000 are held out for evaluation."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    if lam is None:
+        lam = rng.uniform(1.45e-6, 1.65e-6)
+    eps = np.ones((N, N))
+    c = N // 2
+    eps[:, c - 5:c + 5] = 12.25
+    d = N // 2
+    blob = gaussian_filter(rng.standard_normal((d, d)), 3) > 0
+    eps[c - d // 2:c - d // 2 + d, c - d // 2:c - d // 2 + d] = np.where(blob, 12.25, 1.0)
+    src = np.zeros((N, N))
+    src[30, c - 8:c + 8] = 1.0
+    return dict(omega=2 * np.pi * C_0 / lam, dL=40e-9, eps=eps, npml=[20, 20], src=src, lam=lam)

@@ -281,6 +281,80 @@ int cv_dense_lu(int n, void* A, int32_t* piv, int32_t* info_dev, void* stream);
  * second factorisation. */
 int cv_dense_lu_solve(int n, const void* LU, const int32_t* piv, void* B, int nrhs, int trans, void* stream);
 
+/* ------------------------------------------- convolutional preconditioner
+ * No counterpart in the reference: ceviche factorises the sparse system
+ * (solvers.py:37-62); these kernels implement the learned preconditioner used
+ * by cv_fdfd_fgmres.  Activations are NHWC bf16 with 32 channels; weights are
+ * fp32 masters W[layer][tap][ci][co] (tap = 3 dy + dx) packed to bf16 matrix
+ * core operands by cv_cnn_pack_weights (flip = 1 packs the mirrored,
+ * channel-swapped layer whose forward pass is the data gradient). */
+#define CV_UNET_MAX_LEVELS 10
+#define CV_CNN_CHANNELS 32
+#define CV_CNN_PACKED_ELEMS 9216   /* bf16 elements of one packed layer */
+
+typedef struct {
+    const void* wpack;   /* (2 levels + 1) packed layers, forward orientation */
+    const float* bias;   /* (2 levels + 1) x 32 */
+    const float* feat;   /* (H*W) x nf per-cell features (channels 2 .. 2+nf-1) */
+    int nf;
+    int levels;
+} cv_unet;
+
+int cv_cnn_pack_weights(const float* w, void* wpack, int flip, int n_layers, void* stream);
+/* out = act(conv3x3(in) + bias), zero padding; relu != 0 applies max(., 0) */
+int cv_conv3x3_fwd_bf16(const void* in, const void* wpack, const float* bias, void* out, int B, int H, int W,
+                        int relu, void* stream);
+/* gx (+)= conv3x3^T(gy (.) [ymask > 0]) using the flipped packing */
+int cv_conv3x3_bwd_data_bf16(const void* gy, const void* ymask, const void* wpack_flip, void* gx, int B, int H,
+                             int W, int accumulate, void* stream);
+int64_t cv_conv3x3_bwd_weight_workspace_bytes(int B, int H, int W);
+/* dw[tap][ci][co] (+)= sum_p x[p + tap][ci] g[p][co], db (+)= sum_p g[p], g = gy (.) [ymask > 0] */
+int cv_conv3x3_bwd_weight_bf16(const void* x, const void* gy, const void* ymask, float* dw, float* db, int B,
+                               int H, int W, int accumulate, void* workspace, int64_t workspace_bytes,
+                               void* stream);
+/* coarse = scale * sum of each 2x2 block (ceil sizes); fine = scale * up(coarse) + add */
+int cv_cnn_down2(const void* in, void* out, int B, int H, int W, float scale, void* stream);
+int cv_cnn_up2(const void* in, const void* add, void* out, int B, int H, int W, float scale, void* stream);
+int cv_cnn_pack_input(int dtype, const void* r, const int* r_idx, const float* rscale, float rs, const float* feat,
+                      const int* f_idx, int nf, void* out, int B, int H, int W, void* stream);
+int cv_cnn_unpack(int dtype, const void* net, const float* oscale, float os, void* z, int B, int H, int W,
+                  void* stream);
+/* loss += gscale * inv_norm * |y[0:2] - e|^2; g = d loss / d y (bf16, 32
+ * channels), plus extra_scale * (Re, Im) of the complex64 `extra` (may be NULL)
+ * added to channels 0, 1 */
+int cv_cnn_loss(const void* net, const void* e, const int* e_idx, const float* inv_norm, void* g, double* loss,
+                float gscale, const void* extra, float extra_scale, int B, int H, int W, void* stream);
+/* Training-batch operator (complex64): sample b (at x + b n) uses instance
+ * inst[b] of a multi-instance operator whose instances all have n cells.
+ * herm = 0: y_b = A x_b - r[r_idx[b]] (r may be NULL), loss += lscale |y_b|^2;
+ * herm = 1: y_b = A^H x_b.  (loss may be NULL.) */
+int cv_fdfd_batch_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst, const void* prof,
+                        const void* m0, const void* m1, const int* inst, const void* x, const void* r,
+                        const int* r_idx, void* y, double* loss, double lscale, int herm, int B, void* stream);
+int64_t cv_unet_workspace_bytes(int levels, int B, int H, int W);
+/* z = os * net(rs * r) for one H x W complex field (dtype CV_C64 / CV_C128) */
+int cv_unet_apply(const cv_unet* net, int dtype, const void* r, float rs, float os, void* z, int H, int W,
+                  void* workspace, int64_t workspace_bytes, void* stream);
+
+/* ----------------------------------------------------- flexible GMRES
+ * Replaces the sparse solve of sp_solve for one FDFD instance
+ * (ceviche/primitives.py:103-119 -> solvers.py:37-62), with a preconditioner
+ * that may change between applications.  x (complex128) is the initial guess
+ * on entry; the FP64 true residual decides convergence (tol relative to |b|).
+ * `iterations` counts Arnoldi steps, `restarts` restart cycles; with the
+ * network preconditioner, `pad` = 1 reports that three network cycles in a
+ * row reduced the true residual by less than 1 % and the solve finished with
+ * Jacobi. */
+#define CV_PRECOND_NONE 0
+#define CV_PRECOND_JACOBI 1
+#define CV_PRECOND_UNET 2
+int64_t cv_fdfd_fgmres_workspace_bytes(int nx, int ny, int restart, int precond, int levels);
+int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64, const void* m0_64,
+                   const void* m1_64, const void* prof32, const void* m0_32, const void* m1_32, const void* b,
+                   void* x, double tol, int restart, int max_iter, double inner_rtol, int precond,
+                   const cv_unet* net, float net_gain, void* workspace, int64_t workspace_bytes,
+                   cv_solve_info* info, void* stream);
+
 /* ------------------------------------------------------------------ misc */
 const char* cv_version(void);
 const char* cv_last_error(void);

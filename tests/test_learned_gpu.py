@@ -1,0 +1,217 @@
+"""GPU parity of the convolutional preconditioner kernels against the oracle
+(oracle/cnn_ref.py, float64 on the same bf16-rounded operands), of FGMRES
+against scipy's sparse direct solve of the oracle's matrix, and a small
+end-to-end training run on the GPU.
+
+Tolerances: a convolution output stored in bf16 carries one rounding
+(relative 2^-8), so forward / data-gradient outputs are compared to
+|d| <= 8e-3 |ref| + 2e-3 rms(ref); weight gradients are fp32 sums of exact
+bf16 products, compared to 1e-4 of their norm.  The whole network rounds every
+activation to bf16; the oracle does the same (quant=bf16_round), and the
+comparison is 3e-2 of the norm (forward) and 6e-2 (weight gradients), where
+differing roundings of intermediate values accumulate."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cnn_ref as K
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(torch.bfloat16).cuda()
+
+
+def _np(t):
+    return t.float().cpu().numpy().astype(np.float64)
+
+
+def _close(got, ref, rel=8e-3, rms=2e-3):
+    scale = np.sqrt(np.mean(ref ** 2)) + 1e-30
+    err = np.abs(got - ref) - (rel * np.abs(ref) + rms * scale)
+    assert err.max() <= 0, 'max excess %.3e (scale %.3e)' % (err.max(), scale)
+
+
+def _nrel(got, ref):
+    return np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30)
+
+
+def _layer(rng, relu_bias=0.1):
+    w = K.bf16_round(rng.standard_normal((9, 32, 32)) * np.sqrt(2.0 / 288))
+    b = rng.standard_normal(32) * relu_bias
+    return w, b
+
+
+@pytest.mark.parametrize('B,H,W', [(2, 19, 45), (1, 8, 32), (3, 5, 3), (1, 64, 70)])
+def test_conv_forward_matches_oracle(B, H, W):
+    from ceviche_amd import _lib
+    from ceviche_amd.learned import UNet
+    lib = _lib.lib()
+    rng = np.random.default_rng(B * 1000 + H * 10 + W)
+    x = K.bf16_round(rng.standard_normal((B, H, W, 32)))
+    w, b = _layer(rng)
+    net = UNet(levels=1, device=torch.device('cuda'))
+    net.W[0].copy_(torch.from_numpy(w.astype(np.float32)))
+    net.b[0].copy_(torch.from_numpy(b.astype(np.float32)))
+    net.pack()
+    xg = _bf(x)
+    for relu in (0, 1):
+        out = torch.empty_like(xg)
+        _lib.check(lib.cv_conv3x3_fwd_bf16(_lib.ptr(xg), net._wp(0), _lib.ptr(net.b[0]), _lib.ptr(out), B, H, W, relu,
+                                           _lib.stream_ptr()))
+        ref = K.conv3x3(x, w, b.astype(np.float32).astype(np.float64), relu=bool(relu))
+        _close(_np(out), ref)
+
+
+@pytest.mark.parametrize('accumulate', [0, 1])
+def test_conv_backward_data_matches_oracle(accumulate):
+    from ceviche_amd import _lib
+    from ceviche_amd.learned import UNet
+    lib = _lib.lib()
+    rng = np.random.default_rng(11 + accumulate)
+    B, H, W = 2, 21, 37
+    g = K.bf16_round(rng.standard_normal((B, H, W, 32)))
+    y = K.bf16_round(rng.standard_normal((B, H, W, 32)))     # mask source
+    base = K.bf16_round(rng.standard_normal((B, H, W, 32)))
+    w, _ = _layer(rng)
+    net = UNet(levels=1, device=torch.device('cuda'))
+    net.W[0].copy_(torch.from_numpy(w.astype(np.float32)))
+    net.pack()
+    out = _bf(base) if accumulate else torch.empty(B, H, W, 32, dtype=torch.bfloat16, device='cuda')
+    gg, yg = _bf(g), _bf(y)      # keep the device copies alive until the kernel has run
+    _lib.check(lib.cv_conv3x3_bwd_data_bf16(_lib.ptr(gg), _lib.ptr(yg), net._wp(0, True), _lib.ptr(out), B, H, W,
+                                            accumulate, _lib.stream_ptr()))
+    torch.cuda.synchronize()
+    ref = K.conv3x3_bwd_data(g * (y > 0), w) + (base if accumulate else 0.0)
+    _close(_np(out), ref)
+
+
+def test_conv_backward_weight_matches_oracle():
+    from ceviche_amd import _lib
+    lib = _lib.lib()
+    rng = np.random.default_rng(21)
+    for (B, H, W) in [(2, 19, 45), (3, 64, 64), (1, 3, 5)]:
+        x = K.bf16_round(rng.standard_normal((B, H, W, 32)))
+        g = K.bf16_round(rng.standard_normal((B, H, W, 32)))
+        y = K.bf16_round(rng.standard_normal((B, H, W, 32)))
+        need = lib.cv_conv3x3_bwd_weight_workspace_bytes(B, H, W)
+        ws = torch.empty(need, dtype=torch.uint8, device='cuda')
+        dw = torch.zeros(9, 32, 32, device='cuda')
+        db = torch.zeros(32, device='cuda')
+        xg, gg, yg = _bf(x), _bf(g), _bf(y)
+        for mask in (None, y):
+            _lib.check(lib.cv_conv3x3_bwd_weight_bf16(_lib.ptr(xg), _lib.ptr(gg), _lib.ptr(None if mask is None else yg),
+                                                      _lib.ptr(dw), _lib.ptr(db), B, H, W, 0, _lib.ptr(ws), need,
+                                                      _lib.stream_ptr()))
+            gm = g if mask is None else g * (mask > 0)
+            rdw, rdb = K.conv3x3_bwd_weight(x, gm)
+            assert _nrel(dw.cpu().numpy(), rdw) <= 1e-4
+            assert _nrel(db.cpu().numpy(), rdb) <= 1e-4
+
+
+@pytest.mark.parametrize('H,W', [(16, 12), (9, 7), (1, 5)])
+def test_resampling_matches_oracle(H, W):
+    from ceviche_amd.learned import UNet
+    rng = np.random.default_rng(H * W)
+    x = K.bf16_round(rng.standard_normal((2, H, W, 32)))
+    xc = K.bf16_round(rng.standard_normal((2, (H + 1) // 2, (W + 1) // 2, 32)))
+    add = K.bf16_round(rng.standard_normal((2, H, W, 32)))
+    xg, xcg, addg = _bf(x), _bf(xc), _bf(add)
+    _close(_np(UNet._down(xg, 0.25)), K.down2(x, 0.25), rms=1e-6)
+    _close(_np(UNet._up(xcg, H, W, 0.25, add=addg)), K.up2(xc, H, W, 0.25, add), rms=1e-6)
+
+
+def _net_and_oracle(levels, seed=5):
+    from ceviche_amd.learned import UNet
+    net = UNet(levels=levels, seed=seed, device=torch.device('cuda'))
+    rng = np.random.default_rng(seed)
+    net.b.copy_(torch.from_numpy((0.05 * rng.standard_normal(net.b.shape)).astype(np.float32)))
+    Ws = [K.bf16_round(w) for w in net.W.cpu().numpy()]
+    bs = [b.astype(np.float64) for b in net.b.cpu().numpy()]
+    return net, Ws, bs
+
+
+def test_unet_forward_backward_match_oracle():
+    L, B, H, W = 4, 2, 37, 29
+    net, Ws, bs = _net_and_oracle(L)
+    rng = np.random.default_rng(6)
+    x0 = np.zeros((B, H, W, 32))
+    x0[..., :5] = rng.standard_normal((B, H, W, 5))
+    x0 = K.bf16_round(x0)
+    x0g = _bf(x0)
+    y, cache = net.forward(x0g)
+    yr, cr = K.unet_forward(x0, Ws, bs, L, quant=K.bf16_round)
+    assert _nrel(_np(y), yr) <= 3e-2
+    gy = np.zeros((B, H, W, 32))
+    gy[..., :2] = rng.standard_normal((B, H, W, 2))
+    gy = K.bf16_round(gy)
+    gyg = _bf(gy)
+    dW, db = net.backward(cache, gyg)
+    rdW, rdb = K.unet_backward(cr, gy, Ws, L)
+    dW, db = dW.cpu().numpy(), db.cpu().numpy()
+    for layer in range(2 * L + 1):
+        assert _nrel(dW[layer], rdW[layer]) <= 6e-2, layer
+        assert _nrel(db[layer], rdb[layer]) <= 6e-2, layer
+
+
+def test_unet_apply_matches_batched_forward():
+    from ceviche_amd.learned import NF
+    L, H, W = 5, 48, 40
+    net, _, _ = _net_and_oracle(L, seed=9)
+    rng = np.random.default_rng(7)
+    r = torch.from_numpy(rng.standard_normal((H, W)) + 1j * rng.standard_normal((H, W))).to(torch.complex64).cuda()
+    feat = torch.from_numpy(rng.random((H * W, NF)).astype(np.float32)).cuda()
+    z = net.apply(r, feat, 2.0, 0.5)
+    y, _ = net.forward(net.pack_input(r.reshape(1, H, W) * 2.0, feat))
+    zr = 0.5 * (y[0, :, :, 0].float() + 1j * y[0, :, :, 1].float())
+    assert torch.allclose(z, zr.to(z.dtype), rtol=1e-6, atol=1e-7)
+    # and against the oracle's apply at bf16 storage
+    Ws = [K.bf16_round(w) for w in net.W.cpu().numpy()]
+    bs = [b.astype(np.float64) for b in net.b.cpu().numpy()]
+    zo = K.unet_apply(r.cpu().numpy().astype(np.complex128), 2.0, 0.5, feat.cpu().numpy().reshape(H, W, NF), Ws, bs,
+                      L, quant=K.bf16_round)
+    assert _nrel(z.cpu().numpy(), zo) <= 3e-2
+
+
+def _small_problem(N=64, seed=10001):
+    from ceviche_amd import learned, workloads
+    w = workloads.fdfd_learned_blobs(N, seed=seed)
+    pb, src = learned.problem_from(w)
+    return w, pb, pb.rhs(src)
+
+
+@pytest.mark.parametrize('precond', ['none', 'jacobi'])
+def test_fgmres_matches_direct_solve(precond):
+    from scipy.sparse.linalg import spsolve
+    from oracle import fdfd_ref
+    w, pb, bs = _small_problem(48)
+    x, info = pb.fgmres(bs, tol=1e-9, restart=40, max_iter=20000, precond=precond)
+    assert info['status'] == 0 and info['rel_residual'] <= 1e-9
+    A = fdfd_ref.make_A_ez(w['omega'], w['dL'], w['eps'], w['npml'])
+    b = 1j * w['omega'] * w['src'].ravel().astype(np.complex128)
+    ref = spsolve(A.tocsc(), b)
+    got = x.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-7 * np.linalg.norm(ref)
+
+
+def test_training_step_gradients_and_learned_fgmres():
+    """Pairs from plain BiCGSTAB runs, a short training run (loss must fall),
+    then FGMRES with the network: it must reach the FP64 tolerance, and the
+    iteration counts of all three variants are reported."""
+    from ceviche_amd import learned
+    N = 64
+    pairs, stats = learned.generate_pairs(6, N=N, snaps=12, chunk=16, seed0=0)
+    assert pairs.n >= 30 and stats['failed'] == 0
+    net = learned.UNet(levels=4, seed=1, device=torch.device('cuda'))
+    hist = learned.train(net, pairs, steps=120, batch=8, lr=2e-3, log_every=20)
+    assert hist[-1][1] < hist[0][1]
+    w, pb, bs = _small_problem(N)
+    res = {}
+    for kind in ('none', 'jacobi', 'unet'):
+        x, info = pb.fgmres(bs, tol=1e-6, restart=30, max_iter=20000, precond=kind, net=net, gain=pairs.gain)
+        assert info['status'] == 0 and info['rel_residual'] <= 1e-6, (kind, info)
+        r = pb.residual(bs, x)
+        assert torch.linalg.vector_norm(r).item() <= 1.01e-6 * torch.linalg.vector_norm(bs).item()
+        res[kind] = info['iterations']
+    print('FGMRES iterations at %d^2:' % N, res)
