@@ -67,7 +67,8 @@ def fdtd_tm_main(steps, warmup):
     # roofline pass: the dominant kernel alone, HIP events on its stream
     from ceviche_amd._fdtd_op import autotune_tm
     cfg = autotune_tm(F._grid, F._eps_dev)
-    rpt, ts = tm_configs()[cfg]
+    wv, rpt, ts = tm_configs()[cfg]
+    tag = "k_tm_block<float,%d,%d,%d>" % (wv, rpt, ts)
     n_launch = (c['steps'] + ts - 1) // ts
     F.initialize_fields()
     amp_dev = amp.to(F._grid.device)
@@ -89,12 +90,12 @@ def fdtd_tm_main(steps, warmup):
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": committed_traffic("k_tm_block<float,%d,%d>" % (rpt, ts)),
-            "kernel": "k_tm_block<float,%d,%d>" % (rpt, ts), "avg_launch_us": round(avg_launch_ms * 1e3, 2),
+            "traffic": committed_traffic(tag),
+            "kernel": tag, "avg_launch_us": round(avg_launch_ms * 1e3, 2),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "note": "28 B/cell-step x 160000 cells x %d steps per launch; the 400x400 state (2.6 MB) is "
                     "cache-resident, so the fraction counts compulsory bytes, not HBM traffic" % ts}
-    return dict(value=gcells, ms=ms, rec=rec, roofline=roof, cfg=(rpt, ts))
+    return dict(value=gcells, ms=ms, rec=rec, roofline=roof, cfg=(wv, rpt, ts))
 
 
 def committed_traffic(kernel_tag):
@@ -177,6 +178,53 @@ def extras_all(main, which='all'):
         return {"fwd_plus_adjoint_ms": round(1e3 * (time.perf_counter() - t0), 2),
                 "checkpoint_every": 50, "grad_norm": float(eps.grad.norm())}
 
+    def adjoint_3d():
+        """256^3 float32, z-dipole Gaussian pulse, 1000 steps; objective = time-
+        integrated Poynting flux Sz = Ex Hy - Ey Hx through the 64 x 64 monitor
+        plane; reverse-time adjoint with checkpoints every 50 steps; the
+        gradient on the design region is checked against a central difference
+        along a random direction."""
+        N, steps = 256, 1000
+        c = workloads.fdtd_3d_blobs(N, steps)
+        design = tuple(slice(N // 2 - N // 8, N // 2 - N // 8 + N // 4) for _ in range(3))
+        amp = torch.tensor(c['amp'])
+        npl = len(c['probes']) // 4
+
+        def flux(rec):
+            r = rec.double()
+            return (r[:, :npl] * r[:, 3 * npl:] - r[:, npl:2 * npl] * r[:, 2 * npl:3 * npl]).sum()
+
+        eps0 = torch.tensor(c['eps'], device='cuda', dtype=torch.float32)
+        eps = eps0.clone().requires_grad_(True)
+        F = cv.fdtd(eps, c['dL'], c['npml'], precision='float32')
+        _sync()
+        t0 = time.perf_counter()
+        L = flux(F.run(steps, c['src'], amp, c['probes'], checkpoint_every=50))
+        _sync()
+        t_fwd = time.perf_counter() - t0
+        L.backward()
+        _sync()
+        t_all = time.perf_counter() - t0
+        g = eps.grad[design]
+        rng = np.random.default_rng(5)
+        d = torch.from_numpy(rng.choice([-1.0, 1.0], size=g.shape).astype(np.float32)).cuda()
+        h = 1e-2
+        vals = []
+        for sgn in (1.0, -1.0):
+            e2 = eps0.clone()
+            e2[design] += sgn * h * d
+            F2 = cv.fdtd(e2, c['dL'], c['npml'], precision='float32')
+            with torch.no_grad():
+                vals.append(float(flux(F2.run(steps, c['src'], amp, c['probes']))))
+            del F2
+        fd = (vals[0] - vals[1]) / (2 * h)
+        an = float((g.double() * d.double()).sum())
+        return {"grid": [N, N, N], "steps": steps, "checkpoint_every": 50, "monitor_plane": [64, 64],
+                "forward_s": round(t_fwd, 3), "fwd_plus_adjoint_s": round(t_all, 3),
+                "flux": float(L), "directional_derivative_adjoint": an, "directional_derivative_central_fd": fd,
+                "rel_diff": abs(an - fd) / max(abs(fd), 1e-30),
+                "note": "float32; central difference with h = 1e-2 on the 64^3 design region, random +-1 direction"}
+
     def tm_hbm():
         N, steps = 8192, 200
         c = workloads.fdtd_tm_blobs(N, steps)
@@ -257,33 +305,60 @@ def extras_all(main, which='all'):
                 "precision": "complex128 BiCGSTAB (288 B/cell/iteration), FP64 reductions, no preconditioner"}
 
     def fdfd_batch():
-        from ceviche_amd._fdfd_op import FdfdOperator, SolverOptions
+        """Multi-frequency batch (4 blob designs x 4 wavelengths, sizes from
+        {200, 300, 400, 600}^2) solved in one batched launch sequence, in
+        complex128 and in complex64 with FP64 residual refinement, then the
+        per-instance adjoint gradients of a modal-overlap objective
+        |sum_y Ez[N - 25, y] m(y)|^2 (m = the input line-source profile)."""
+        from ceviche_amd._fdfd_op import FdfdOperator, FdfdSolve, SolverOptions
         insts = workloads.fdfd_blob_batch(n_inst=4, n_lambda=4)
         specs = [dict(omega=d['omega'], dL=d['dL'], shape=d['eps'].shape, npml=d['npml']) for d in insts]
         op = FdfdOperator('ez', specs)
+        op32 = FdfdOperator('ez', specs, torch.complex64)
         eps = torch.from_numpy(np.concatenate([d['eps'].ravel() for d in insts])).cuda()
         b = torch.from_numpy(np.concatenate([1j * d['omega'] * d['src'].ravel() for d in insts])).cuda()
-        m0, m1 = op.materials(eps)
-        bs = b * op.scale_vector(torch.float64)
-        opts = SolverOptions(tol=1e-6, max_iter=200000)
-        _sync()
-        t0 = time.perf_counter()
-        op.solve_scaled(bs, m0, m1, opts=opts)
-        _sync()
-        dt = time.perf_counter() - t0
-        its = [i['iterations'] for i in op.last_info]
-        res = max(i['rel_residual'] for i in op.last_info)
-        # the same instances one at a time
+        res = {"instances": len(insts), "sizes": sorted(set(int(d['eps'].shape[0]) for d in insts))}
+        for name, o32 in (('complex128', None), ('complex64_fp64_refined', op32)):
+            # complex64 BiCGSTAB stagnates near its rounding level: short inner solves, more refinement sweeps
+            opts = (SolverOptions(tol=1e-6, max_iter=400000) if o32 is None else
+                    SolverOptions(tol=1e-6, max_iter=20000, inner_tol=1e-3, max_sweeps=40))
+            log('  batch %s: solving' % name)
+            e = eps.clone().requires_grad_(True)
+            _sync()
+            t0 = time.perf_counter()
+            m0, m1 = op.materials(e)
+            x = FdfdSolve.apply(m0, m1, b, op, o32, opts)
+            info = (o32 or op).last_info
+            L = 0.0
+            for k, d in enumerate(insts):
+                N = d['eps'].shape[0]
+                f = x[op.offsets[k]:op.offsets[k] + N * N].reshape(N, N)
+                prof = torch.from_numpy(np.ascontiguousarray(d['src'][25, :])).cuda().to(f.dtype)
+                L = L + torch.abs((f[N - 25, :] * prof).sum()) ** 2
+            _sync()
+            t_fwd = time.perf_counter() - t0
+            log('  batch %s: forward %.2f s, adjoint' % (name, t_fwd))
+            L.backward()
+            _sync()
+            t_all = time.perf_counter() - t0
+            its = [i['iterations'] for i in info]
+            res[name] = {"forward_s": round(t_fwd, 3), "fwd_plus_adjoint_s": round(t_all, 3),
+                         "solves_per_s": round(len(insts) / t_fwd, 3),
+                         "iterations_min_max": [min(its), max(its)],
+                         "max_fp64_rel_residual": max(i['rel_residual'] for i in info),
+                         "grad_norm": float(e.grad.norm())}
+        # the same instances one at a time (complex128, forward only)
+        opts = SolverOptions(tol=1e-6, max_iter=400000)
+        log('  batch: one by one')
         t0 = time.perf_counter()
         for d in insts[:4]:
             F = cv.fdfd_ez(d['omega'], d['dL'], d['eps'], d['npml'], solver_options=opts)
             F.solve(d['src'])
         _sync()
-        dt1 = (time.perf_counter() - t0) / 4
-        return {"instances": len(insts), "sizes": sorted(set(int(d['eps'].shape[0]) for d in insts)),
-                "batched_solves_per_s": round(len(insts) / dt, 3), "one_by_one_solves_per_s": round(1 / dt1, 3),
-                "iterations_min_max": [min(its), max(its)], "max_fp64_rel_residual": res,
-                "precision": "complex128 BiCGSTAB, FP64 reductions, all instances in each launch"}
+        res["one_by_one_complex128_solves_per_s"] = round(4 / (time.perf_counter() - t0), 3)
+        res["note"] = ("BiCGSTAB with FP64 reductions; all instances advance in the same kernels; gradients by "
+                       "one batched adjoint solve (A^T) per precision")
+        return res
 
     def sp_mult_1m():
         import scipy.sparse as sps
@@ -324,6 +399,7 @@ def extras_all(main, which='all'):
         return {"copy_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
 
     table = [('fdtd_tm_400_trace', trace_errors), ('fdtd_tm_400_adjoint', adjoint), ('fdtd_tm_8192', tm_hbm),
+             ('fdtd_3d_256_adjoint', adjoint_3d),
              ('fdtd_3d_256', fdtd_3d), ('fdfd_ez_120', fdfd_cfg1), ('fdfd_ez_1024_plain', fdfd_1024),
              ('fdfd_ez_batch', fdfd_batch), ('sp_mult_1M', sp_mult_1m), ('stream_copy', copy_probe)]
     for name, fn in table:
@@ -370,7 +446,7 @@ def main():
         "config": {"workload": "2D TM FDTD 400x400x1, dL 20 nm, npml 20, binary-blob eps {1, 12.25}, "
                                "Gaussian-pulse Jz point source, 2000 time steps, 4 Ez probes per step",
                    "grid": [400, 400, 1], "time_steps": 2000, "precision": "float32",
-                   "tile_rows_per_thread_steps_per_launch": list(main_res['cfg'])},
+                   "tile_waves_rows_per_thread_steps_per_launch": list(main_res['cfg'])},
         "roofline": main_res['roofline'],
         "cpu_baseline": cpu,
         "extras": extras,

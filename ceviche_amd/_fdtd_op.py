@@ -100,9 +100,12 @@ class FdtdGrid:
         """Fused, temporally blocked 2D TM steps (nz == 1, TE fields zero, Jz point source)."""
         pc, pl = (None, None) if probes is None else probes
         npb = 0 if probes is None else int(pc.numel())
-        check(lib().cv_fdtd_run_tm(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), int(src_cell),
-                                   ptr(amp_dev), int(t0), int(n_steps), ptr(pc), ptr(pl), npb, ptr(rec),
-                                   int(rec_t0), int(config), stream_ptr()), 'fdtd run_tm')
+        alt = getattr(self, '_tm_alt', None)
+        if alt is None or alt.shape != state.shape or alt.dtype != state.dtype:
+            alt = self._tm_alt = torch.empty_like(state)
+        check(lib().cv_fdtd_run_tm(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(alt),
+                                   int(src_cell), ptr(amp_dev), int(t0), int(n_steps), ptr(pc), ptr(pl), npb,
+                                   ptr(rec), int(rec_t0), int(config), stream_ptr()), 'fdtd run_tm')
         return state
 
     def run_fused(self, state, other, eps, comp, src_cell, amp_dev, t0, n_steps, probes, rec, rec_t0):
@@ -195,11 +198,13 @@ _TUNE_CACHE = {}
 
 
 def tm_configs():
-    n = lib().cv_fdtd_tm_configs(None, None, 0)
+    """[(waves per workgroup, rows per thread, steps per launch), ...]"""
+    n = lib().cv_fdtd_tm_configs(None, None, None, 0)
     rpt = (ctypes.c_int * n)()
     ts = (ctypes.c_int * n)()
-    lib().cv_fdtd_tm_configs(rpt, ts, n)
-    return [(rpt[i], ts[i]) for i in range(n)]
+    wv = (ctypes.c_int * n)()
+    lib().cv_fdtd_tm_configs(rpt, ts, wv, n)
+    return [(wv[i], rpt[i], ts[i]) for i in range(n)]
 
 
 def autotune_tm(grid, eps, steps=64, reps=2):

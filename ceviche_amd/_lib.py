@@ -114,8 +114,8 @@ _SIGS = {
     'cv_fdtd_efields_grad': (I, [I, P, P, P, P, P, P, P, P]),
     'cv_fdtd_adjoint_step': (I, [I, P, P, P, P, P, P, P, P, P, I, P, P, P]),
     'cv_fdtd_eps_grad': (I, [I, P, P, P, P, P, P]),
-    'cv_fdtd_run_tm': (I, [I, P, P, P, P, I64, P, I, I, P, P, I, P, I, I, P]),
-    'cv_fdtd_tm_configs': (I, [P, P, I]),
+    'cv_fdtd_run_tm': (I, [I, P, P, P, P, P, I64, P, I, I, P, P, I, P, I, I, P]),
+    'cv_fdtd_tm_configs': (I, [P, P, P, I]),
     'cv_fdtd_run_fused': (I, [I, P, P, P, P, P, I, I64, P, I, I, P, P, I, P, I, P]),
     'cv_segsum': (I, [I64, P, P, P, P, P, I, I, D, D, P, I, P, P]),
     'cv_dense_lu': (I, [I, P, P, P, P]),

@@ -455,9 +455,9 @@ __global__ void __launch_bounds__(256) k_efields_grad(Fdtd<R> f, const R* __rest
 
 // ================================================ 2D TM temporal blocking
 // Fused, temporally blocked Yee update for nz == 1 with only Hx, Hy, Dz live
-// (TM drive: Jz point source, TE fields zero).  One workgroup owns a region of
-// RR = 4*RPT rows (x) by 64 columns (y, contiguous): lane = column, each wave a
-// band of RPT rows.  Cell values (Hx, Hy, Dz, 1/eps and the PML integrals
+// (TM drive: Jz point source, TE fields zero).  One workgroup of WV waves owns
+// a region of RR = WV*RPT rows (x) by 64 columns (y, contiguous): lane =
+// column, each wave a band of RPT rows.  Cell values (Hx, Hy, Dz, 1/eps and the PML integrals
 // ICEx, ICEy, IDz) live in registers; neighbour values are exchanged through
 // LDS.  TS steps are advanced per launch; after TS steps the cells at least TS
 // away from the region edge are exact (the dependency cone of one step is one
@@ -475,7 +475,8 @@ struct TmArgs {
     double dt, c0, eps0;
     const R* sig;
     const R* eps;
-    R* st;                     // 18-array state
+    const R* st_in;            // state at the start of the launch (18 arrays)
+    R* st_out;                 // state after the launch (a different buffer)
     const double* amp;         // amplitude per global step
     const int32_t* pcomp;      // probe components: 2 Ez, 6 Hx, 7 Hy
     const int64_t* pcell;      // probe flat cells
@@ -490,9 +491,10 @@ __host__ __device__ __forceinline__ int pmod(int a, int n) {
 
 constexpr int kTmMaxProbes = 256;
 
-template <typename R, int RPT, int TS>
-__global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
-    constexpr int RR = 4 * RPT;
+template <typename R, int WV, int RPT, int TS>
+__global__ void __launch_bounds__(64 * WV) k_tm_block(TmArgs<R> a) {
+    constexpr int RR = WV * RPT;
+    constexpr int NT = 64 * WV;
     constexpr int RC = 64;
     constexpr int OX = RR - 2 * TS, OY = RC - 2 * TS;  // owned tile
     __shared__ R sEz[RR][RC];
@@ -517,17 +519,26 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
     const R* sigDx = a.sig + nx + ny + 1;
     const R* sigDy = a.sig + 2 * nx + ny + 1;
     const R shy = sigHy[gj], sdy = sigDy[gj];
-    R* Hx = a.st;
-    R* Hy = a.st + n;
-    R* Dz = a.st + 5 * n;
-    R* ICEx = a.st + 6 * n;
-    R* ICEy = a.st + 7 * n;
-    R* IDz = a.st + 17 * n;
+    // Tiles read their halo from st_in and write only their owned cells to
+    // st_out: a tile that finishes early never changes what a later tile of
+    // the same launch reads.
+    const R* Hx = a.st_in;
+    const R* Hy = a.st_in + n;
+    const R* Dz = a.st_in + 5 * n;
+    const R* ICEx = a.st_in + 6 * n;
+    const R* ICEy = a.st_in + 7 * n;
+    const R* IDz = a.st_in + 17 * n;
+    R* oHx = a.st_out;
+    R* oHy = a.st_out + n;
+    R* oDz = a.st_out + 5 * n;
+    R* oICEx = a.st_out + 6 * n;
+    R* oICEy = a.st_out + 7 * n;
+    R* oIDz = a.st_out + 17 * n;
     const R dt = R(a.dt), c0 = R(a.c0), e0 = R(a.eps0);
 
     if (threadIdx.x == 0) s_np = 0;
     __syncthreads();
-    for (int p = threadIdx.x; p < a.np; p += 256) {
+    for (int p = threadIdx.x; p < a.np; p += NT) {
         const int64_t c = a.pcell[p];
         const int pi = (int)(c / ny), pj = (int)(c - (int64_t)pi * ny);
         const int ri = pi - i0, rj = pj - j0;
@@ -543,6 +554,11 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
     }
 
     R hx[RPT], hy[RPT], dz[RPT], me[RPT], icx[RPT], icy[RPT], idz[RPT], shx[RPT], sdx[RPT];
+    // update coefficients per cell, formed once per launch (the PML cells'
+    // divisions stay out of the time loop): Hx = xa hx + xb cex + xc icx,
+    // Hy likewise, Dz = da dz + db chz + dc idz; xc / yc / dc are 0 where the
+    // auxiliary term is absent, so one code path serves every cell.
+    R xa[RPT], xb[RPT], xc[RPT], ya[RPT], yb[RPT], yc[RPT], da[RPT], db[RPT], dc[RPT];
     int gi[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
@@ -560,6 +576,23 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
         icy[q] = shy != R(0) ? ICEy[c] : R(0);
         idz[q] = (sdx[q] != R(0) && sdy != R(0)) ? IDz[c] : R(0);
         sEz[r][lane] = me[q] * dz[q];
+        if (shx[q] == R(0) && shy == R(0)) {
+            xa[q] = a.h1; xb[q] = a.h2; xc[q] = R(0);
+            ya[q] = a.h1; yb[q] = a.h2; yc[q] = R(0);
+        } else {
+            R m1, m2, m3, m4;
+            coef4<R>(shx[q], shy, R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);   // Hx: own sigma sHx
+            xa[q] = m1; xb[q] = m2; xc[q] = shx[q] != R(0) ? m3 : R(0);
+            coef4<R>(shy, shx[q], R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);   // Hy: own sigma sHy
+            ya[q] = m1; yb[q] = m2; yc[q] = shy != R(0) ? m3 : R(0);
+        }
+        if (sdx[q] == R(0) && sdy == R(0)) {
+            da[q] = a.d1; db[q] = a.d2; dc[q] = R(0);
+        } else {
+            R m1, m2, m3, m4;
+            coef4<R>(R(0), sdx[q], sdy, dt, c0, e0, R(1), m1, m2, m3, m4);
+            da[q] = m1; db[q] = m2; dc[q] = (sdx[q] != R(0) && sdy != R(0)) ? m4 : R(0);
+        }
     }
     const int src_i = a.src >= 0 ? (int)(a.src / ny) : -1;
     const int src_j = a.src >= 0 ? (int)(a.src - (int64_t)src_i * ny) : -1;
@@ -577,28 +610,14 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
             const R ez_jp = sEz[r][min(lane + 1, RC - 1)];
             const R cex = ((ez_jp - ez) - (R(0) - R(0))) * a.inv_dL;
             const R cey = ((R(0) - R(0)) - (ez_ip - ez)) * a.inv_dL;
-            if (shx[q] == R(0) && shy == R(0)) {
-                hx[q] = a.h1 * hx[q] + a.h2 * cex;
-                hy[q] = a.h1 * hy[q] + a.h2 * cey;
-            } else {
-                R m1, m2, m3, m4;
-                // Hx: own sigma sHx, others sHy, sHz = 0
-                coef4<R>(shx[q], shy, R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
-                R v = m1 * hx[q] + m2 * cex;
-                if (shx[q] != R(0)) {
-                    icx[q] += cex;
-                    v += m3 * icx[q];
-                }
-                hx[q] = v;
-                // Hy: own sigma sHy, others sHx, sHz = 0
-                coef4<R>(shy, shx[q], R(0), dt, c0, e0, R(-1), m1, m2, m3, m4);
-                v = m1 * hy[q] + m2 * cey;
-                if (shy != R(0)) {
-                    icy[q] += cey;
-                    v += m3 * icy[q];
-                }
-                hy[q] = v;
-            }
+            icx[q] += cex;
+            icy[q] += cey;
+            R v = xa[q] * hx[q] + xb[q] * cex;
+            v += xc[q] * icx[q];
+            hx[q] = v;
+            v = ya[q] * hy[q] + yb[q] * cey;
+            v += yc[q] * icy[q];
+            hy[q] = v;
             sHx[r][lane] = hx[q];
             sHy[r][lane] = hy[q];
         }
@@ -610,27 +629,22 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
             const R hy_im = sHy[max(r - 1, 0)][lane];
             const R hx_jm = sHx[r][max(lane - 1, 0)];
             const R chz = ((hy[q] - hy_im) - (hx[q] - hx_jm)) * a.inv_dL;
-            R v;
-            if (sdx[q] == R(0) && sdy == R(0)) {
-                v = a.d1 * dz[q] + a.d2 * chz;
-            } else {
-                R m1, m2, m3, m4;
-                coef4<R>(R(0), sdx[q], sdy, dt, c0, e0, R(1), m1, m2, m3, m4);
-                v = m1 * dz[q] + m2 * chz;
-                if (sdx[q] != R(0) && sdy != R(0)) {
-                    idz[q] += dz[q];
-                    v += m4 * idz[q];
-                }
-            }
+            idz[q] += dz[q];
+            R v = da[q] * dz[q] + db[q] * chz;
+            v += dc[q] * idz[q];
             if (gi[q] == src_i && gj == src_j) v += R(a.amp[t]);
             dz[q] = v;
             sEz[r][lane] = me[q] * v;
         }
         __syncthreads();
-        for (int p = threadIdx.x; p < np_tile; p += 256) {
-            const int k = s_pk[p];
-            const R v = (k == 2) ? sEz[s_pr[p]][s_pc[p]] : (k == 6 ? sHx[s_pr[p]][s_pc[p]] : sHy[s_pr[p]][s_pc[p]]);
-            a.rec[(int64_t)(t - a.rec_t0) * a.np + s_pidx[p]] = v;
+        if (np_tile > 0) {   // uniform over the workgroup
+            for (int p = threadIdx.x; p < np_tile; p += NT) {
+                const int k = s_pk[p];
+                const R v =
+                    (k == 2) ? sEz[s_pr[p]][s_pc[p]] : (k == 6 ? sHx[s_pr[p]][s_pc[p]] : sHy[s_pr[p]][s_pc[p]]);
+                a.rec[(int64_t)(t - a.rec_t0) * a.np + s_pidx[p]] = v;
+            }
+            __syncthreads();   // the next H phase overwrites sHx / sHy
         }
     }
     // ---- store the owned tile
@@ -642,23 +656,35 @@ __global__ void __launch_bounds__(256) k_tm_block(TmArgs<R> a) {
             const int oi = r - TS;
             if (oi < 0 || oi >= OX || i0 + oi >= nx) continue;
             const int64_t c = (int64_t)(i0 + oi) * ny + (j0 + oj);
-            Hx[c] = hx[q];
-            Hy[c] = hy[q];
-            Dz[c] = dz[q];
-            if (shx[q] != R(0)) ICEx[c] = icx[q];
-            if (shy != R(0)) ICEy[c] = icy[q];
-            if (sdx[q] != R(0) && sdy != R(0)) IDz[c] = idz[q];
+            oHx[c] = hx[q];
+            oHy[c] = hy[q];
+            oDz[c] = dz[q];
+            if (shx[q] != R(0)) oICEx[c] = icx[q];
+            if (shy != R(0)) oICEy[c] = icy[q];
+            if (sdx[q] != R(0) && sdy != R(0)) oIDz[c] = idz[q];
         }
     }
 }
 
-template <typename R, int RPT, int TS>
-static int launch_tm(const Fdtd<R>& f, const double* amp_dev, int64_t src, int t0, int n_steps,
+// The six arrays the TM kernel reads and writes (Hx, Hy, Dz, ICEx, ICEy, IDz).
+static const int kTmArrays[6] = {0, 1, 5, 6, 7, 17};
+
+template <typename R>
+static int copy_tm_arrays(R* dst, const R* src, int64_t n, hipStream_t s) {
+    for (int k = 0; k < 6; ++k)
+        CV_HIP(hipMemcpyAsync(dst + kTmArrays[k] * n, src + kTmArrays[k] * n, sizeof(R) * n,
+                              hipMemcpyDeviceToDevice, s));
+    return CV_OK;
+}
+
+template <typename R, int WV, int RPT, int TS>
+static int launch_tm(const Fdtd<R>& f, R* alt, const double* amp_dev, int64_t src, int t0, int n_steps,
                      const int32_t* pcomp, const int64_t* pcell, int np, R* rec, int rec_t0, hipStream_t s) {
+    static_assert(WV * RPT > 2 * TS && 64 > 2 * TS, "the owned tile must not be empty");
     TmArgs<R> a;
     a.nx = f.nx;
     a.ny = f.ny;
-    constexpr int OX = 4 * RPT - 2 * TS, OY = 64 - 2 * TS;
+    constexpr int OX = WV * RPT - 2 * TS, OY = 64 - 2 * TS;
     a.tiles_x = (f.nx + OX - 1) / OX;
     a.tiles_y = (f.ny + OY - 1) / OY;
     a.src = src;
@@ -673,31 +699,51 @@ static int launch_tm(const Fdtd<R>& f, const double* amp_dev, int64_t src, int t
     a.eps0 = f.eps0;
     a.sig = f.sig;
     a.eps = f.eps;
-    a.st = f.st;
+    a.st_in = f.st;
+    a.st_out = alt;
     a.amp = amp_dev;
     a.pcomp = pcomp;
     a.pcell = pcell;
     a.rec = rec;
     a.rec_t0 = rec_t0;
     const int nblocks = a.tiles_x * a.tiles_y;
+    if (n_steps <= 0) return CV_OK;
+    // alt starts equal to the state, so cells a launch does not write (PML
+    // integrals where sigma is 0) agree in both buffers
+    int rc = copy_tm_arrays<R>(alt, f.st, f.n, s);
+    if (rc != CV_OK) return rc;
+    R* cur = f.st;
+    R* nxt = alt;
     for (int t = 0; t < n_steps; t += TS) {
         a.t0 = t0 + t;
         a.nsteps = (n_steps - t < TS) ? (n_steps - t) : TS;
-        k_tm_block<R, RPT, TS><<<nblocks, 256, 0, s>>>(a);
+        a.st_in = cur;
+        a.st_out = nxt;
+        k_tm_block<R, WV, RPT, TS><<<nblocks, 64 * WV, 0, s>>>(a);
+        R* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
     }
+    if (cur != f.st) return copy_tm_arrays<R>(f.st, cur, f.n, s);
     return CV_OK;
 }
 
 template <typename R>
-static int dispatch_tm(int cfg, const Fdtd<R>& f, const double* amp_dev, int64_t src, int t0, int n_steps,
+static int dispatch_tm(int cfg, const Fdtd<R>& f, R* alt, const double* amp_dev, int64_t src, int t0, int n_steps,
                        const int32_t* pcomp, const int64_t* pcell, int np, R* rec, int rec_t0, hipStream_t s) {
     switch (cfg) {
-        case 0: return launch_tm<R, 4, 2>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
-        case 1: return launch_tm<R, 4, 4>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
-        case 2: return launch_tm<R, 8, 4>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
-        case 3: return launch_tm<R, 8, 8>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
-        case 4: return launch_tm<R, 16, 8>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
-        case 5: return launch_tm<R, 16, 4>(f, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 0: return launch_tm<R, 4, 4, 2>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 1: return launch_tm<R, 4, 4, 4>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 2: return launch_tm<R, 4, 8, 4>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 3: return launch_tm<R, 4, 8, 8>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 4: return launch_tm<R, 4, 16, 8>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 5: return launch_tm<R, 4, 16, 4>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 6: return launch_tm<R, 8, 4, 8>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 7: return launch_tm<R, 16, 4, 16>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 8: return launch_tm<R, 16, 2, 8>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 9: return launch_tm<R, 16, 4, 8>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 10: return launch_tm<R, 8, 4, 4>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
+        case 11: return launch_tm<R, 16, 2, 4>(f, alt, amp_dev, src, t0, n_steps, pcomp, pcell, np, rec, rec_t0, s);
         default: break;
     }
     CV_CHECK(false, "unknown TM tile configuration %d", cfg);
@@ -990,20 +1036,24 @@ extern "C" int cv_fdtd_efields_grad(int dtype, const cv_fdtd_desc* d, const void
     return CV_OK;
 }
 
-extern "C" int cv_fdtd_tm_configs(int* rpt, int* ts, int max_n) {
-    static const int R_[6] = {4, 4, 8, 8, 16, 16}, T_[6] = {2, 4, 4, 8, 8, 4};
-    for (int i = 0; i < 6 && i < max_n; ++i) {
+extern "C" int cv_fdtd_tm_configs(int* rpt, int* ts, int* waves, int max_n) {
+    static const int R_[12] = {4, 4, 8, 8, 16, 16, 4, 4, 2, 4, 4, 2};
+    static const int T_[12] = {2, 4, 4, 8, 8, 4, 8, 16, 8, 8, 4, 4};
+    static const int W_[12] = {4, 4, 4, 4, 4, 4, 8, 16, 16, 16, 8, 16};
+    for (int i = 0; i < 12 && i < max_n; ++i) {
         if (rpt) rpt[i] = R_[i];
         if (ts) ts[i] = T_[i];
+        if (waves) waves[i] = W_[i];
     }
-    return 6;
+    return 12;
 }
 
 extern "C" int cv_fdtd_run_tm(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state,
-                              int64_t src_cell, const double* amp_dev, int t0, int n_steps,
+                              void* state_alt, int64_t src_cell, const double* amp_dev, int t0, int n_steps,
                               const int32_t* probe_comp, const int64_t* probe_cell, int n_probes, void* rec,
                               int rec_t0, int config, void* stream) {
     hipStream_t s = as_stream(stream);
+    CV_CHECK(state && state_alt && state != state_alt, "two distinct state buffers are needed");
     CV_CHECK(d && d->nz == 1, "the TM kernel needs nz == 1");
     CV_CHECK(n_probes <= kTmMaxProbes, "the TM kernel records at most %d probes", kTmMaxProbes);
     CV_CHECK(n_steps >= 0 && t0 >= 0, "bad step range");
@@ -1014,16 +1064,16 @@ extern "C" int cv_fdtd_run_tm(int dtype, const cv_fdtd_desc* d, const void* sig,
         int rc = make_fdtd<float>(d, sig, eps, state, f);
         if (rc) return rc;
         CV_CHECK(src_cell < f.n, "source cell out of range");
-        rc = dispatch_tm<float>(config, f, amp_dev, src_cell, t0, n_steps, probe_comp, probe_cell, n_probes, (float*)rec,
-                                rec_t0, s);
+        rc = dispatch_tm<float>(config, f, (float*)state_alt, amp_dev, src_cell, t0, n_steps, probe_comp, probe_cell,
+                                n_probes, (float*)rec, rec_t0, s);
         if (rc) return rc;
     } else if (dtype == CV_F64) {
         Fdtd<double> f;
         int rc = make_fdtd<double>(d, sig, eps, state, f);
         if (rc) return rc;
         CV_CHECK(src_cell < f.n, "source cell out of range");
-        rc = dispatch_tm<double>(config, f, amp_dev, src_cell, t0, n_steps, probe_comp, probe_cell, n_probes,
-                                 (double*)rec, rec_t0, s);
+        rc = dispatch_tm<double>(config, f, (double*)state_alt, amp_dev, src_cell, t0, n_steps, probe_comp,
+                                 probe_cell, n_probes, (double*)rec, rec_t0, s);
         if (rc) return rc;
     } else {
         CV_CHECK(false, "unknown dtype %d", dtype);

@@ -322,8 +322,8 @@ def generate_pairs(n_struct, N=256, snaps=24, chunk=64, seed0=0, tol=1e-8, max_i
                    device=None, log=None):
     """Runs plain BiCGSTAB (restarted every `chunk` iterations) on n_struct
     seeded structures (workloads.fdfd_learned_blobs, seeds seed0 ..) with
-    random sources, and keeps up to `snaps` iterates x_k per run as pairs
-    (r_k = b - A x_k, e_k = x* - x_k)."""
+    random sources, and keeps up to `snaps` iterates x_k per run, x_0 = 0
+    included, as pairs (r_k = b - A x_k, e_k = x* - x_k)."""
     from . import workloads
     rs, es, structs, feats, ratios = [], [], [], [], []
     specs, epss = [], []
@@ -334,7 +334,7 @@ def generate_pairs(n_struct, N=256, snaps=24, chunk=64, seed0=0, tol=1e-8, max_i
         rng = np.random.default_rng(seed0 + s + 7919)
         bs = pb.rhs(_random_source(rng, N, w['npml'][0]))
         x = torch.zeros_like(bs)
-        traj = []
+        traj = [x.clone()]      # x_0 = 0: the pair (b, x*) is the first one FGMRES meets
         total = 0
         ok = False
         while total < max_iter:

@@ -229,10 +229,13 @@ int cv_fdtd_efields_grad(int dtype, const cv_fdtd_desc* d, const void* sig, cons
  * source): n_steps steps starting at global step t0, D_z[src_cell] += amp_dev[t]
  * (device float64 array indexed by global step), probes (comp 2 Ez, 6 Hx,
  * 7 Hy; at most 256) recorded into rec[(t - rec_t0)*n_probes + p].  config
- * selects the tile (rows per thread, steps per launch) from
- * cv_fdtd_tm_configs.  Same arithmetic as cv_fdtd_step. */
+ * selects the tile (rows per thread, steps per launch, waves per workgroup)
+ * from cv_fdtd_tm_configs.  Launches alternate between `state` and the
+ * caller's second buffer `state_alt` (same layout; its contents are
+ * overwritten), and the result always ends in `state`.  Same arithmetic as
+ * cv_fdtd_step. */
 int cv_fdtd_run_tm(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state,
-                   int64_t src_cell, const double* amp_dev, int t0, int n_steps,
+                   void* state_alt, int64_t src_cell, const double* amp_dev, int t0, int n_steps,
                    const int32_t* probe_comp, const int64_t* probe_cell, int n_probes, void* rec,
                    int rec_t0, int config, void* stream);
 
@@ -247,9 +250,10 @@ int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* sig, const v
                       int n_steps, const int32_t* probe_comp, const int64_t* probe_cell, int n_probes,
                       void* rec, int rec_t0, void* stream);
 
-/* Tile configurations of cv_fdtd_run_tm: fills rpt[i], ts[i] (rows per
- * thread, steps per launch) for i < max_n and returns their number. */
-int cv_fdtd_tm_configs(int* rpt, int* ts, int max_n);
+/* Tile configurations of cv_fdtd_run_tm: fills rpt[i], ts[i], waves[i]
+ * (rows per thread, steps per launch, waves per workgroup; the region is
+ * waves*rpt rows by 64 columns) for i < max_n and returns their number. */
+int cv_fdtd_tm_configs(int* rpt, int* ts, int* waves, int max_n);
 
 /* g_eps += chain rule of mE_a = 2/(eps + roll(eps, 1, a)) applied to g_mE. */
 int cv_fdtd_eps_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,

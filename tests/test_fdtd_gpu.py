@@ -189,7 +189,7 @@ def test_3d_checkpointed_adjoint_with_e_and_h_probes():
     assert np.sum(g * d) == pytest.approx(np.imag(Lc) / h, rel=1e-8)
 
 
-@pytest.mark.parametrize('cfg', range(6))
+@pytest.mark.parametrize('cfg', range(12))
 @pytest.mark.parametrize('shape', [(40, 36, 1), (13, 7, 1), (70, 130, 1)])
 def test_tm_blocked_kernel_matches_oracle(cfg, shape):
     """Every tile configuration of the temporally blocked 2D TM kernel, on
