@@ -360,6 +360,50 @@ def extras_all(main, which='all'):
                        "one batched adjoint solve (A^T) per precision")
         return res
 
+    def fdfd_learned():
+        """Held-out 512^2 blob structure (workloads.fdfd_learned_blobs, seed
+        10000), relative residual 1e-6: plain BiCGSTAB (complex128) against
+        FGMRES(30) with the trained network (weights in
+        ceviche_amd/data/unet_fdfd_ez.npz) and with Jacobi."""
+        from ceviche_amd import learned
+        path = os.path.join(ROOT, 'ceviche_amd', 'data', 'unet_fdfd_ez.npz')
+        if not os.path.exists(path):
+            return {"error": "no trained weights at %s (python -m ceviche_amd.train_precond)" % path}
+        net, meta = learned.UNet.load(path)
+        gain = float(meta['gain'])
+        N = 512
+        w = workloads.fdfd_learned_blobs(N, seed=10000)
+        pb, src = learned.problem_from(w)
+        bs = pb.rhs(src)
+        out = {"grid": [N, N], "wavelength_um": round(w['lam'] * 1e6, 4),
+               "trained_on": "%d pairs at %d^2, %d steps" % (int(meta['train_pairs']), int(meta['train_size']),
+                                                           int(meta['train_steps'])),
+               "train_residual_ratio": float(meta['residual_ratio'])}
+        _sync()
+        t0 = time.perf_counter()
+        x, info = pb.bicgstab(bs, tol=1e-6)
+        _sync()
+        t_plain = time.perf_counter() - t0
+        out["plain_bicgstab"] = {"seconds": round(t_plain, 3), "solves_per_s": round(1 / t_plain, 4),
+                                 "iterations": info['iterations'], "rel_residual": info['rel_residual'],
+                                 "converged": info['status'] == 0}
+        log('  learned: plain BiCGSTAB %.2f s' % t_plain)
+        for pc in ('unet', 'jacobi'):
+            _sync()
+            t0 = time.perf_counter()
+            x, info = pb.fgmres(bs, tol=1e-6, restart=30, max_iter=6000, precond=pc, net=net, gain=gain)
+            _sync()
+            t = time.perf_counter() - t0
+            out["fgmres_" + pc] = {"seconds": round(t, 3), "iterations": info['iterations'],
+                                   "rel_residual": info['rel_residual'], "converged": info['status'] == 0,
+                                   "fell_back_to_jacobi": info['fell_back']}
+            log('  learned: FGMRES %s %.2f s' % (pc, t))
+        u = out["fgmres_unet"]
+        out["gain_vs_plain"] = round(t_plain / u["seconds"], 4) if u["converged"] else None
+        out["note"] = ("FGMRES capped at 6000 Arnoldi steps; gain_vs_plain = plain seconds / learned seconds "
+                       "when the learned solve reaches 1e-6, else null")
+        return out
+
     def sp_mult_1m():
         import scipy.sparse as sps
         from ceviche_amd import primitives as cp
@@ -401,7 +445,8 @@ def extras_all(main, which='all'):
     table = [('fdtd_tm_400_trace', trace_errors), ('fdtd_tm_400_adjoint', adjoint), ('fdtd_tm_8192', tm_hbm),
              ('fdtd_3d_256_adjoint', adjoint_3d),
              ('fdtd_3d_256', fdtd_3d), ('fdfd_ez_120', fdfd_cfg1), ('fdfd_ez_1024_plain', fdfd_1024),
-             ('fdfd_ez_batch', fdfd_batch), ('sp_mult_1M', sp_mult_1m), ('stream_copy', copy_probe)]
+             ('fdfd_ez_batch', fdfd_batch), ('fdfd_learned_512', fdfd_learned), ('sp_mult_1M', sp_mult_1m),
+             ('stream_copy', copy_probe)]
     for name, fn in table:
         if which == 'all' or name in which:
             guard(name, fn)

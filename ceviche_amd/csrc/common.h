@@ -125,22 +125,39 @@ __device__ __forceinline__ double load_agent(const double* p) {
     return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Last-workgroup protocol over the tiles of one instance.  Every workgroup
-// of the instance calls this once with its block-summed values (valid in
-// thread 0); it publishes them to partials[tile*NV..] and returns true in
-// the workgroup that arrived last, after that workgroup has summed all the
-// instance's partials (fixed order) into `tot` (valid in all threads).
+// Last-workgroup protocol over the tiles of one instance, two levels deep.
+// Every workgroup of the instance calls this once with its block-summed
+// values (valid in thread 0).  Level 1: the instance's tiles form groups of
+// kGroup consecutive tiles; each workgroup publishes its values to
+// partials[tile*NV..] and increments its group's counter (gcount at the
+// group's first tile index); the workgroup that arrives last in the group
+// sums the group's partials (fixed order) into gpartials[group first tile*NV..]
+// and increments the instance counter (counter).  Level 2: the workgroup that
+// arrives last there sums the group partials (fixed order) into `tot` (valid
+// in all threads) and returns true; every other workgroup returns false.
 //
-// Only the partials and the counter cross workgroups inside the kernel, so
+// One counter per workgroup of a large instance would serialise thousands of
+// atomics on one address (about 12 ns each on MI355X, 48 µs for 4096
+// workgroups); with groups of 64 the atomics spread over many addresses.
+// Only the partials and the counters cross workgroups inside the kernel, so
 // they are agent-scope atomics (coherent across the XCDs' L2s without a full
 // L2 write-back / invalidate); s_waitcnt(0) makes the partial stores complete
 // before the counter increment is issued.  The next kernel sees everything
-// else through the usual kernel-boundary cache maintenance.
+// else through the usual kernel-boundary cache maintenance.  The last
+// workgroups reset the counters they used, so consecutive kernels can share
+// them; they must be zero before the first kernel.
+constexpr int kGroup = 64;
+
 template <int NV>
-__device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], double* partials,
-                                                  unsigned int* counter, int tile, int tile_first,
-                                                  int tile_count, double* lds, double (&tot)[NV]) {
+__device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], double* partials, double* gpartials,
+                                                  unsigned int* counter, unsigned int* gcount, int tile,
+                                                  int tile_first, int tile_count, double* lds, double (&tot)[NV]) {
     __shared__ int s_last;
+    __shared__ double s_tot[NV];
+    const int g = (tile - tile_first) / kGroup;
+    const int g_first = tile_first + g * kGroup;
+    const int g_size = min(kGroup, tile_count - g * kGroup);
+    const int n_groups = (tile_count + kGroup - 1) / kGroup;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) store_agent(&partials[(size_t)tile * NV + c], mine[c]);
@@ -148,31 +165,35 @@ __device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], doub
         __builtin_amdgcn_s_waitcnt(0);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         const unsigned int ticket =
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (ticket == (unsigned int)(tile_count - 1));
+            __hip_atomic_fetch_add(&gcount[g_first], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (ticket == (unsigned int)(g_size - 1));
     }
     __syncthreads();
     if (!s_last) return false;
     double acc[NV];
 #pragma unroll
+    for (int c = 0; c < NV; ++c)
+        acc[c] = ((int)threadIdx.x < g_size) ? load_agent(&partials[(size_t)(g_first + threadIdx.x) * NV + c]) : 0.0;
+    block_sum<NV>(acc, lds);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&gcount[g_first], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) store_agent(&gpartials[(size_t)g_first * NV + c], acc[c]);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const unsigned int ticket = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (ticket == (unsigned int)(n_groups - 1));
+    }
+    __syncthreads();
+    if (!s_last) return false;
+#pragma unroll
     for (int c = 0; c < NV; ++c) acc[c] = 0.0;
-    // 4 independent loads in flight per thread (fixed summation order)
-    for (int t0 = 0; t0 < tile_count; t0 += 4 * kBlock) {
-        double v[4][NV];
+    for (int gi = threadIdx.x; gi < n_groups; gi += kBlock) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int t = t0 + u * kBlock + (int)threadIdx.x;
-#pragma unroll
-            for (int c = 0; c < NV; ++c)
-                v[u][c] = t < tile_count ? load_agent(&partials[(size_t)(tile_first + t) * NV + c]) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int c = 0; c < NV; ++c) acc[c] += v[u][c];
+        for (int c = 0; c < NV; ++c) acc[c] += load_agent(&gpartials[(size_t)(tile_first + gi * kGroup) * NV + c]);
     }
     block_sum<NV>(acc, lds);
-    __shared__ double s_tot[NV];
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) s_tot[c] = acc[c];

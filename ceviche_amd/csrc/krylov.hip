@@ -31,6 +31,16 @@ struct KState {
     int32_t restarts, pad0, pad1, pad2;
 };
 
+// Scratch of the two-level last-workgroup reductions: per-tile partials,
+// per-group partials and counters (indexed by the group's first tile) and one
+// counter per instance.  Counters must be zero before the first kernel.
+struct RedBuf {
+    double* partials;      // n_tiles * 4
+    double* gpartials;     // n_tiles * 4
+    unsigned int* counters;  // n_inst
+    unsigned int* gcount;    // n_tiles
+};
+
 __global__ void k_zero_u32(unsigned int* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -44,7 +54,7 @@ __global__ void __launch_bounds__(kBlock)
 k_init(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
        const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
        const cx<R>* __restrict__ b, const cx<R>* __restrict__ x, cx<R>* __restrict__ r,
-       cx<R>* __restrict__ rh, double* __restrict__ partials, unsigned int* __restrict__ counters,
+       cx<R>* __restrict__ rh, RedBuf rb,
        KState* __restrict__ state, double tol, int max_iter, int mode) {
     __shared__ double lds[kWaves * 2];
     const cv_tile t = tiles[blockIdx.x];
@@ -73,7 +83,7 @@ k_init(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
     }
     block_sum<2>(acc, lds);
     double tot[2];
-    if (!last_block_reduce<2>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -109,7 +119,7 @@ k_bicg_p(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
          const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
          const cx<R>* __restrict__ r, cx<R>* __restrict__ rh, const cx<R>* __restrict__ p_old,
          const cx<R>* __restrict__ v_old, cx<R>* __restrict__ p_new, cx<R>* __restrict__ v_new,
-         double* __restrict__ partials, unsigned int* __restrict__ counters, KState* __restrict__ state) {
+         RedBuf rb, KState* __restrict__ state) {
     __shared__ double lds[kWaves * 2];
     const cv_tile t = tiles[blockIdx.x];
     KState* st = state + t.inst;
@@ -146,7 +156,7 @@ k_bicg_p(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     }
     block_sum<2>(acc, lds);
     double tot[2];
-    if (!last_block_reduce<2>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -169,7 +179,7 @@ __global__ void __launch_bounds__(kBlock)
 k_bicg_s(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
          const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
          const cx<R>* __restrict__ r, const cx<R>* __restrict__ v, cx<R>* __restrict__ tv,
-         double* __restrict__ partials, unsigned int* __restrict__ counters, KState* __restrict__ state) {
+         RedBuf rb, KState* __restrict__ state) {
     __shared__ double lds[kWaves * 3];
     const cv_tile t = tiles[blockIdx.x];
     KState* st = state + t.inst;
@@ -194,7 +204,7 @@ k_bicg_s(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     }
     block_sum<3>(acc, lds);
     double tot[3];
-    if (!last_block_reduce<3>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<3>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -211,7 +221,7 @@ __global__ void __launch_bounds__(kBlock)
 k_bicg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
          cx<R>* __restrict__ x, const cx<R>* __restrict__ p, cx<R>* __restrict__ r,
          const cx<R>* __restrict__ v, const cx<R>* __restrict__ tv, const cx<R>* __restrict__ rh,
-         double* __restrict__ partials, unsigned int* __restrict__ counters, KState* __restrict__ state) {
+         RedBuf rb, KState* __restrict__ state) {
     __shared__ double lds[kWaves * 3];
     const cv_tile t = tiles[blockIdx.x];
     KState* st = state + t.inst;
@@ -236,7 +246,7 @@ k_bicg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     }
     block_sum<3>(acc, lds);
     double tot[3];
-    if (!last_block_reduce<3>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<3>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -321,8 +331,8 @@ __global__ void __launch_bounds__(kBlock)
 k_mixed_resid(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
               const cx<double>* __restrict__ prof, const double* __restrict__ m0,
               const double* __restrict__ m1, const cx<double>* __restrict__ b,
-              const cx<double>* __restrict__ x, cx<double>* __restrict__ r, double* __restrict__ partials,
-              unsigned int* __restrict__ counters, KState* __restrict__ outer, double tol, int first) {
+              const cx<double>* __restrict__ x, cx<double>* __restrict__ r, RedBuf rb,
+              KState* __restrict__ outer, double tol, int first) {
     __shared__ double lds[kWaves * 2];
     const cv_tile t = tiles[blockIdx.x];
     KState* st = outer + t.inst;
@@ -345,7 +355,7 @@ k_mixed_resid(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__
     }
     block_sum<2>(acc, lds);
     double tot[2];
-    if (!last_block_reduce<2>(acc, partials, counters + t.inst, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -373,8 +383,7 @@ struct Work {
     void* v[2];
     void* t;
     KState* state;
-    unsigned int* counters;
-    double* partials;
+    RedBuf rb;
 };
 
 static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
@@ -423,8 +432,10 @@ static int64_t layout(int64_t cs, int64_t cells, int n_inst, int n_tiles, char* 
     o->v[1] = take(cs * cells);
     o->t = take(cs * cells);
     o->state = (KState*)take(sizeof(KState) * (int64_t)n_inst);
-    o->counters = (unsigned int*)take(sizeof(unsigned int) * (int64_t)n_inst);
-    o->partials = (double*)take(sizeof(double) * 4 * (int64_t)n_tiles);
+    o->rb.counters = (unsigned int*)take(sizeof(unsigned int) * ((int64_t)n_inst + n_tiles));
+    o->rb.gcount = o->rb.counters ? o->rb.counters + n_inst : nullptr;
+    o->rb.partials = (double*)take(sizeof(double) * 4 * (int64_t)n_tiles);
+    o->rb.gpartials = (double*)take(sizeof(double) * 4 * (int64_t)n_tiles);
     return off;
 }
 
@@ -444,10 +455,10 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
     cx<R>* v[2] = {(cx<R>*)w.v[0], (cx<R>*)w.v[1]};
     cx<R>* tv = (cx<R>*)w.t;
     (void)desc_host;
-    k_zero_u32<<<(n_inst + 255) / 256, 256, 0, s>>>(w.counters, n_inst);
+    k_zero_u32<<<(n_inst + n_tiles + 255) / 256, 256, 0, s>>>(w.rb.counters, n_inst + n_tiles);
     // p and v start at zero (they are only read when restart == 0, which never
     // happens before the first K1), so no clear is needed.
-    k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.partials, w.counters,
+    k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.rb,
                                                 w.state, tol, max_iter, 0);
     CV_LAUNCH_CHECK();
     const int max_checks = 8;
@@ -463,11 +474,9 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
         for (int c = 0; c < check_every; ++c) {
             const int nxt = cur ^ 1;
             k_bicg_p<R, TRANS><<<n_tiles, kBlock, 0, q>>>(desc, tiles, prof, m0, m1, r, rh, p[cur], v[cur], p[nxt],
-                                                         v[nxt], w.partials, w.counters, w.state);
-            k_bicg_s<R, TRANS><<<n_tiles, kBlock, 0, q>>>(desc, tiles, prof, m0, m1, r, v[nxt], tv, w.partials,
-                                                         w.counters, w.state);
-            k_bicg_x<R><<<n_tiles, kBlock, 0, q>>>(desc, tiles, x, p[nxt], r, v[nxt], tv, rh, w.partials,
-                                                   w.counters, w.state);
+                                                         v[nxt], w.rb, w.state);
+            k_bicg_s<R, TRANS><<<n_tiles, kBlock, 0, q>>>(desc, tiles, prof, m0, m1, r, v[nxt], tv, w.rb, w.state);
+            k_bicg_x<R><<<n_tiles, kBlock, 0, q>>>(desc, tiles, x, p[nxt], r, v[nxt], tv, rh, w.rb, w.state);
             cur = nxt;
         }
     };
@@ -481,7 +490,7 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
         for (int k = 0; k < n_inst; ++k) all_done &= (hs[k].done != kRunning);
         if (!all_done) continue;
         // verify converged instances with the true residual (restarts them if needed)
-        k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.partials, w.counters,
+        k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.rb,
                                                     w.state, tol, max_iter, 1);
         CV_LAUNCH_CHECK();
         CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
@@ -567,10 +576,10 @@ extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_
 // ------------------------------------------------------------ mixed solve
 extern "C" int64_t cv_fdfd_solve_mixed_workspace_bytes(int64_t total_cells, int n_inst, int n_tiles) {
     int64_t inner = layout(8, total_cells, n_inst, n_tiles, nullptr, nullptr);
-    // r64, b32, x32, outer state, outer counters, outer partials
+    // r64, b32, x32, outer state (the outer residual kernel shares the inner
+    // solver's reduction scratch: the two never run at the same time)
     return inner + align256(16 * total_cells) + 2 * align256(8 * total_cells) +
-           align256(sizeof(KState) * (int64_t)n_inst) + align256(4 * (int64_t)n_inst) +
-           align256(sizeof(double) * 4 * (int64_t)n_tiles);
+           align256(sizeof(KState) * (int64_t)n_inst);
 }
 
 extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
@@ -601,12 +610,8 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
     cx<float>* x32 = (cx<float>*)(base + off);
     off += align256(8 * cells);
     KState* outer = (KState*)(base + off);
-    off += align256(sizeof(KState) * (int64_t)n_inst);
-    unsigned int* ocnt = (unsigned int*)(base + off);
-    off += align256(4 * (int64_t)n_inst);
-    double* opart = (double*)(base + off);
     hipStream_t s = as_stream(stream);
-    k_zero_u32<<<(n_inst + 255) / 256, 256, 0, s>>>(ocnt, n_inst);
+    k_zero_u32<<<(n_inst + n_tiles + 255) / 256, 256, 0, s>>>(w.rb.counters, n_inst + n_tiles);
     std::vector<KState> ho(n_inst), hi;
     std::vector<int> iters(n_inst, 0), sweeps(n_inst, 0);
     auto resid = [&](int first) -> int {
@@ -614,12 +619,12 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
             k_mixed_resid<true><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof64,
                                                            (const double*)m0_64, (const double*)m1_64,
                                                            (const cx<double>*)b64, (const cx<double>*)x64, r64,
-                                                           opart, ocnt, outer, tol, first);
+                                                           w.rb, outer, tol, first);
         else
             k_mixed_resid<false><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof64,
                                                             (const double*)m0_64, (const double*)m1_64,
                                                             (const cx<double>*)b64, (const cx<double>*)x64, r64,
-                                                            opart, ocnt, outer, tol, first);
+                                                            w.rb, outer, tol, first);
         CV_LAUNCH_CHECK();
         CV_HIP(hipMemcpyAsync(ho.data(), outer, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
         CV_HIP(hipStreamSynchronize(s));

@@ -215,3 +215,31 @@ def test_training_step_gradients_and_learned_fgmres():
         assert torch.linalg.vector_norm(r).item() <= 1.01e-6 * torch.linalg.vector_norm(bs).item()
         res[kind] = info['iterations']
     print('FGMRES iterations at %d^2:' % N, res)
+
+
+def test_trained_network_against_untrained_and_jacobi():
+    """The committed network (python -m ceviche_amd.train_precond: 1920 pairs
+    at 256^2, 20 000 steps) on fresh pairs from held-out structures: its
+    correction leaves a smaller residual than the untrained network's, and
+    FGMRES(30) with it converges to the FP64 tolerance on a held-out 128^2
+    instance (the network cycles, then Jacobi if they stall).  The iteration
+    counts of the network, Jacobi and plain BiCGSTAB are reported."""
+    import os
+    from ceviche_amd import learned
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'ceviche_amd', 'data',
+                        'unet_fdfd_ez.npz')
+    net, meta = learned.UNet.load(path)
+    pairs, _ = learned.generate_pairs(2, N=256, snaps=8, chunk=64, seed0=20000, gain=float(meta['gain']))
+    trained = learned.evaluate(net, pairs, count=16)
+    fresh = learned.evaluate(learned.UNet(levels=net.L, seed=3), pairs, count=16)
+    print('residual ratio: trained %.3f, untrained %.3f' % (trained[0], fresh[0]))
+    assert trained[0] < 0.8 < fresh[0]
+    w, pb, bs = _small_problem(128, seed=10003)
+    out = {}
+    for kind in ('unet', 'jacobi'):
+        x, info = pb.fgmres(bs, tol=1e-6, restart=30, max_iter=40000, precond=kind, net=net, gain=float(meta['gain']))
+        assert info['status'] == 0 and info['rel_residual'] <= 1e-6, (kind, info)
+        out[kind] = (info['iterations'], info['fell_back'])
+    x, info = pb.bicgstab(bs, tol=1e-6)
+    out['bicgstab'] = info['iterations']
+    print('held-out 128^2 iterations:', out)
