@@ -175,6 +175,17 @@ __device__ __forceinline__ bool last_block_reduce(const double (&mine)[NV], doub
     for (int c = 0; c < NV; ++c)
         acc[c] = ((int)threadIdx.x < g_size) ? load_agent(&partials[(size_t)(g_first + threadIdx.x) * NV + c]) : 0.0;
     block_sum<NV>(acc, lds);
+    if (n_groups == 1) {   // the group is the whole instance: acc is the total
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) s_tot[c] = acc[c];
+            __hip_atomic_store(&gcount[g_first], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NV; ++c) tot[c] = s_tot[c];
+        return true;
+    }
     if (threadIdx.x == 0) {
         __hip_atomic_store(&gcount[g_first], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll

@@ -16,6 +16,7 @@
 #include "common.h"
 
 #include <vector>
+#include <algorithm>
 
 namespace cv {
 
@@ -64,6 +65,25 @@ __host__ __device__ __forceinline__ int64_t cell(const Fdtd<R>& f, int i, int j,
     return ((int64_t)i * f.ny + j) * f.nz + k;
 }
 
+// (i, j, k) of flat cell c; 32-bit unsigned division when the grid allows it
+// (a 64-bit division by a runtime value is a long instruction sequence)
+template <typename R>
+__host__ __device__ __forceinline__ void decompose(const Fdtd<R>& f, int64_t c, int& i, int& j, int& k) {
+    if (f.n <= (int64_t)0xffffffffLL) {
+        const uint32_t u = (uint32_t)c, nz = (uint32_t)f.nz, ny = (uint32_t)f.ny;
+        const uint32_t ij = u / nz;
+        k = (int)(u - ij * nz);
+        const uint32_t ii = ij / ny;
+        j = (int)(ij - ii * ny);
+        i = (int)ii;
+    } else {
+        const int64_t ij = c / f.nz;
+        k = (int)(c - ij * f.nz);
+        i = (int)(ij / f.ny);
+        j = (int)(ij - (int64_t)i * f.ny);
+    }
+}
+
 // E_a at (i, j, k) = D_a / ((eps + eps[-e_a]) / 2); D points at the D block (Dx, Dy, Dz)
 template <typename R>
 __host__ __device__ __forceinline__ R efield(const Fdtd<R>& f, const R* __restrict__ D, int a, int i, int j, int k) {
@@ -80,10 +100,8 @@ __host__ __device__ __forceinline__ R efield(const Fdtd<R>& f, const R* __restri
 // ----------------------------------------------------------------- H phase
 template <typename R>
 __host__ __device__ __forceinline__ void h_cell(const Fdtd<R>& f, int64_t c) {
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny);
-    const int i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
     const int64_t n = f.n;
     R* H = f.st;
     const R* D = f.st + 3 * n;
@@ -143,10 +161,8 @@ struct PointSrc {
 template <typename R>
 __host__ __device__ __forceinline__ void d_cell(const Fdtd<R>& f, const R* __restrict__ Jx, const R* __restrict__ Jy,
                                            const R* __restrict__ Jz, R jscale, PointSrc ps, int64_t c) {
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny);
-    const int i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
     const int64_t n = f.n;
     const R* H = f.st;
     R* D = f.st + 3 * n;
@@ -214,9 +230,9 @@ __global__ void k_record(Fdtd<R> f, const int32_t* __restrict__ pcomp, const int
     const int64_t c = pcell[p];
     R v;
     if (comp < 3) {
-        const int k = (int)(c % f.nz);
-        const int64_t ij = c / f.nz;
-        v = efield(f, f.st + 3 * f.n, comp, (int)(ij / f.ny), (int)(ij % f.ny), k);
+        int i, j, k;
+        decompose(f, c, i, j, k);
+        v = efield(f, f.st + 3 * f.n, comp, i, j, k);
     } else if (comp < 6) {
         v = f.st[(int64_t)comp * f.n + c];
     } else {
@@ -230,9 +246,8 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_efields(Fdtd<R> f, R* __restrict__ E) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= f.n) return;
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
 #pragma unroll
     for (int a = 0; a < 3; ++a) E[a * f.n + c] = efield(f, f.st + 3 * f.n, a, i, j, k);
 }
@@ -288,9 +303,8 @@ __host__ __device__ __forceinline__ R mu_ch(const Fdtd<R>& f, const R* __restric
 
 template <typename R>
 __host__ __device__ __forceinline__ void adj_a_cell(const Fdtd<R>& f, R* __restrict__ lam, R* __restrict__ T, int64_t c) {
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
     const int ip = wrapp(i, f.nx), jp = wrapp(j, f.ny), kp = wrapp(k, f.nz);
     const int64_t n = f.n;
     // curl_E applied to muCH (forward differences)
@@ -339,9 +353,8 @@ __host__ __device__ __forceinline__ R mu_ce(const Fdtd<R>& f, const R* __restric
 template <typename R>
 __host__ __device__ __forceinline__ void adj_b_cell(const Fdtd<R>& f, R* __restrict__ lam, const R* __restrict__ T,
                                                const R* __restrict__ Dprev, R* __restrict__ gmE, int64_t c) {
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
     const int im = wrapm(i, f.nx), jm = wrapm(j, f.ny), km = wrapm(k, f.nz);
     const int64_t n = f.n;
     const R ux = mu_ce(f, lam, T, 0, i, j, k), uy = mu_ce(f, lam, T, 1, i, j, k), uz = mu_ce(f, lam, T, 2, i, j, k);
@@ -392,9 +405,8 @@ __global__ void k_inject(Fdtd<R> f, R* __restrict__ lam, const int32_t* __restri
     const int64_t c = pcell[p];
     const R gv = g[p];
     if (comp < 3) {
-        const int k = (int)(c % f.nz);
-        const int64_t ij = c / f.nz;
-        const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+        int i, j, k;
+        decompose(f, c, i, j, k);
         int64_t cm;
         if (comp == 0) cm = cell(f, wrapm(i, f.nx), j, k);
         else if (comp == 1) cm = cell(f, i, wrapm(j, f.ny), k);
@@ -415,9 +427,8 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_eps_grad(Fdtd<R> f, const R* __restrict__ gmE, R* __restrict__ geps) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= f.n) return;
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
     auto gaa = [&](int a, int ii, int jj, int kk) -> R {
         const int64_t cc = cell(f, ii, jj, kk);
         int64_t cm;
@@ -440,9 +451,8 @@ __global__ void __launch_bounds__(256) k_efields_grad(Fdtd<R> f, const R* __rest
                                                       R* __restrict__ gmE) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= f.n) return;
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny), i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
     const int64_t cm[3] = {cell(f, wrapm(i, f.nx), j, k), cell(f, i, wrapm(j, f.ny), k), cell(f, i, j, wrapm(k, f.nz))};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -796,10 +806,8 @@ __host__ __device__ __forceinline__ R h_comp(const Fdtd<R>& f, const R* __restri
 template <typename R>
 __host__ __device__ __forceinline__ void fused_cell(const Fdtd<R>& f, const R* __restrict__ A, R* __restrict__ B,
                                                     PointSrc ps, int64_t c) {
-    const int k = (int)(c % f.nz);
-    const int64_t ij = c / f.nz;
-    const int j = (int)(ij % f.ny);
-    const int i = (int)(ij / f.ny);
+    int i, j, k;
+    decompose(f, c, i, j, k);
     const int64_t n = f.n;
     const int im = wrapm(i, f.nx), jm = wrapm(j, f.ny), km = wrapm(k, f.nz);
     R h[3], ice[3], ih[3];
@@ -863,6 +871,254 @@ __global__ void __launch_bounds__(256) k_fused(Fdtd<R> f, const R* __restrict__ 
     if (c >= f.n) return;
     if (ps.comp >= 0) ps.amp = amp_dev[t];
     fused_cell(f, A, B, ps, c);
+}
+
+// =================================== x-marching single-pass 3D step
+// One workgroup (16 / kMRpt waves) owns a region of 16 rows (y) x 64 lanes
+// (z) of (y, z) columns and marches along x over a chunk of slices; each
+// thread keeps kMRpt cells of the region (consecutive rows, its lane) in
+// registers:
+// per slice i it forms E(i+1) from the loads of D(i+1) and eps, the new H(i)
+// from E(i) (own cell, and the y+1 / z+1 neighbours through LDS) and E(i+1),
+// then the new D(i) from H(i) (own, and y-1 / z-1 through LDS) and H(i-1),
+// which it carried from the previous slice.  H is valid on region rows 0..14
+// and lanes 0..62, D on rows 1..14 and lanes 1..62: the owned 14 x 62 cells.
+// Reads come from the old state A only and writes go to B (ping-pong), so
+// workgroup order does not matter; a chunk starts by recomputing H(i0-1).
+// DRAM traffic per cell-step: H, D, eps read and H, D written (52 B in
+// float32) plus the 1 / 14 and 1 / 62 halo fractions.  Same arithmetic as
+// h_cell / d_cell.
+constexpr int kMRows = 16, kMLanes = 64;
+constexpr int kMOwnY = kMRows - 2, kMOwnZ = kMLanes - 2;
+
+template <typename R, int kMRpt>
+__global__ void __launch_bounds__(64 * kMRows / kMRpt) k_march(Fdtd<R> f, const R* __restrict__ A, R* __restrict__ B,
+                                                             PointSrc ps, const double* __restrict__ amp_dev, int t,
+                                                             int tiles_y, int tiles_z, int chunk) {
+    __shared__ R sE[3][kMRows][kMLanes];
+    __shared__ R sH[3][kMRows][kMLanes];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int b = blockIdx.x;
+    const int tz = b % tiles_z;
+    b /= tiles_z;
+    const int ty = b % tiles_y;
+    const int chk = b / tiles_y;
+    const int i0 = chk * chunk;
+    const int i1 = min(f.nx, i0 + chunk);
+    if (i0 >= i1) return;   // uniform over the workgroup
+    if (ps.comp >= 0) ps.amp = amp_dev[t];
+    const int64_t n = f.n;
+    const int ny = f.ny, nz = f.nz;
+    const R* Dx = A + 3 * n;
+    const R* Dy = A + 4 * n;
+    const R* Dz = A + 5 * n;
+    // region coordinates of this thread's cells
+    const int zz = tz * kMOwnZ - 1 + lane;            // unwrapped z
+    const int k = pmod(zz, nz);
+    const int km = wrapm(k, nz);
+    int j[kMRpt], jm[kMRpt];
+    bool own[kMRpt];
+    const bool own_z = lane >= 1 && lane <= kMLanes - 2 && zz < nz;
+#pragma unroll
+    for (int q = 0; q < kMRpt; ++q) {
+        const int r = w * kMRpt + q;
+        const int yy = ty * kMOwnY - 1 + r;
+        j[q] = pmod(yy, ny);
+        jm[q] = wrapm(j[q], ny);
+        own[q] = own_z && r >= 1 && r <= kMRows - 2 && yy < ny;
+    }
+    const R sHy_k = sigH(f, 2)[k], sDz_k = sigD(f, 2)[k];
+    R sHy_j[kMRpt], sDy_j[kMRpt];
+#pragma unroll
+    for (int q = 0; q < kMRpt; ++q) {
+        sHy_j[q] = sigH(f, 1)[j[q]];
+        sDy_j[q] = sigD(f, 1)[j[q]];
+    }
+    auto cidx = [&](int xi, int q) -> int64_t { return ((int64_t)xi * ny + j[q]) * nz + k; };
+
+    // raw loads of slice xi (D and the four eps values E needs), issued
+    // early so that they are in flight during the previous slice's work
+    struct Raw {
+        R d[kMRpt][3];
+        R e0[kMRpt], ex[kMRpt], ey[kMRpt], ez[kMRpt];
+    };
+    auto load_raw = [&](int xi, Raw& rw) {
+        const int xm = wrapm(xi, f.nx);
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q) {
+            const int64_t c = cidx(xi, q);
+            rw.e0[q] = f.eps[c];
+            rw.ex[q] = f.eps[((int64_t)xm * ny + j[q]) * nz + k];
+            rw.ey[q] = f.eps[((int64_t)xi * ny + jm[q]) * nz + k];
+            rw.ez[q] = f.eps[((int64_t)xi * ny + j[q]) * nz + km];
+            rw.d[q][0] = Dx[c];
+            rw.d[q][1] = Dy[c];
+            rw.d[q][2] = Dz[c];
+        }
+    };
+    auto to_E = [&](const Raw& rw, R (&e)[kMRpt][3]) {
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q) {
+            const R ea = (rw.e0[q] + rw.ex[q]) / R(2);
+            const R eb = (rw.e0[q] + rw.ey[q]) / R(2);
+            const R eg = (rw.e0[q] + rw.ez[q]) / R(2);
+            e[q][0] = (R(1) / ea) * rw.d[q][0];
+            e[q][1] = (R(1) / eb) * rw.d[q][1];
+            e[q][2] = (R(1) / eg) * rw.d[q][2];
+        }
+    };
+    auto load_H = [&](int xi, R (&h)[kMRpt][3]) {
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q) {
+            const int64_t c = cidx(xi, q);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) h[q][a] = A[a * n + c];
+        }
+    };
+    // new H of slice xi at this thread's cells; eC = E(xi) (also in sE),
+    // eN = E(xi+1), hO = old H(xi)
+    auto h_new = [&](int xi, const R (&eC)[kMRpt][3], const R (&eN)[kMRpt][3], const R (&hO)[kMRpt][3],
+                     R (&h)[kMRpt][3], bool store) {
+        const R sHx = sigH(f, 0)[xi];
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q) {
+            const int r = w * kMRpt + q;
+            const int rp = min(r + 1, kMRows - 1), lp = min(lane + 1, kMLanes - 1);
+            R ce[3];
+            ce[0] = ((sE[2][rp][lane] - eC[q][2]) - (sE[1][r][lp] - eC[q][1])) * f.inv_dL;
+            ce[1] = ((sE[0][r][lp] - eC[q][0]) - (eN[q][2] - eC[q][2])) * f.inv_dL;
+            ce[2] = ((eN[q][1] - eC[q][1]) - (sE[0][rp][lane] - eC[q][0])) * f.inv_dL;
+            const int64_t c = cidx(xi, q);
+            const R sg[3] = {sHx, sHy_j[q], sHy_k};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ax_b = (a == 0) ? 1 : 0;
+                const int ax_c = (a == 2) ? 1 : 2;
+                const R sa = sg[a], sb = sg[ax_b], sc = sg[ax_c];
+                const int64_t ca = a * n + c;
+                const R hv = hO[q][a];
+                R hn;
+                if (sa == R(0) && sb == R(0) && sc == R(0)) {
+                    hn = f.h1 * hv + f.h2 * ce[a];
+                } else {
+                    R m1, m2, m3, m4;
+                    coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(-1), m1, m2, m3, m4);
+                    hn = m1 * hv + m2 * ce[a];
+                    if (sa != R(0)) {
+                        const R ice = A[6 * n + ca] + ce[a];
+                        if (store && own[q]) B[6 * n + ca] = ice;
+                        hn += m3 * ice;
+                    }
+                    if (sb != R(0) && sc != R(0)) {
+                        const R ih = A[9 * n + ca] + hv;
+                        if (store && own[q]) B[9 * n + ca] = ih;
+                        hn += m4 * ih;
+                    }
+                }
+                h[q][a] = hn;
+                if (store && own[q]) B[ca] = hn;
+            }
+        }
+    };
+
+    Raw rwA, rwB;
+    R eC[kMRpt][3], eN[kMRpt][3], dC[kMRpt][3], hP[kMRpt][3], hC[kMRpt][3], hO[kMRpt][3], hON[kMRpt][3];
+    // H(i0 - 1) from E(i0 - 1), E(i0) and old H(i0 - 1)
+    const int xprev = wrapm(i0, f.nx);
+    load_raw(xprev, rwA);
+    load_raw(i0, rwB);
+    load_H(xprev, hO);
+    to_E(rwA, eC);
+    to_E(rwB, eN);
+#pragma unroll
+    for (int q = 0; q < kMRpt; ++q)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) sE[a][w * kMRpt + q][lane] = eC[q][a];
+    load_raw(wrapp(i0, f.nx), rwA);     // slice i0 + 1 in flight
+    __syncthreads();
+    h_new(xprev, eC, eN, hO, hP, false);
+    load_H(i0, hO);                     // old H(i0) in flight
+#pragma unroll
+    for (int q = 0; q < kMRpt; ++q)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            eC[q][a] = eN[q][a];
+            dC[q][a] = rwB.d[q][a];
+        }
+    for (int xi = i0; xi < i1; ++xi) {
+        __syncthreads();   // every read of sE / sH of the previous slice is done
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sE[a][w * kMRpt + q][lane] = eC[q][a];
+        to_E(rwA, eN);                      // E(xi + 1), loaded one slice ago
+        R dN[kMRpt][3];
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) dN[q][a] = rwA.d[q][a];
+        const int xn2 = wrapp(wrapp(xi, f.nx), f.nx);
+        if (xi + 1 < i1) {
+            load_raw(xn2, rwA);             // slice xi + 2 in flight
+            load_H(wrapp(xi, f.nx), hON);   // old H(xi + 1) in flight
+        }
+        __syncthreads();
+        h_new(xi, eC, eN, hO, hC, true);
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sH[a][w * kMRpt + q][lane] = hC[q][a];
+        __syncthreads();
+        const R sDx = sigD(f, 0)[xi];
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q) {
+            const int r = w * kMRpt + q;
+            const int rm = max(r - 1, 0), lm = max(lane - 1, 0);
+            R ch[3];
+            ch[0] = ((hC[q][2] - sH[2][rm][lane]) - (hC[q][1] - sH[1][r][lm])) * f.inv_dL;
+            ch[1] = ((hC[q][0] - sH[0][r][lm]) - (hC[q][2] - hP[q][2])) * f.inv_dL;
+            ch[2] = ((hC[q][1] - hP[q][1]) - (hC[q][0] - sH[0][rm][lane])) * f.inv_dL;
+            const int64_t c = cidx(xi, q);
+            const R sg[3] = {sDx, sDy_j[q], sDz_k};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ax_b = (a == 0) ? 1 : 0;
+                const int ax_c = (a == 2) ? 1 : 2;
+                const R sa = sg[a], sb = sg[ax_b], sc = sg[ax_c];
+                const int64_t ca = a * n + c;
+                const R dv = dC[q][a];
+                R dn;
+                if (sa == R(0) && sb == R(0) && sc == R(0)) {
+                    dn = f.d1 * dv + f.d2 * ch[a];
+                } else {
+                    R m1, m2, m3, m4;
+                    coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(1), m1, m2, m3, m4);
+                    dn = m1 * dv + m2 * ch[a];
+                    if (sa != R(0)) {
+                        const R ich = A[12 * n + ca] + ch[a];
+                        if (own[q]) B[12 * n + ca] = ich;
+                        dn += m3 * ich;
+                    }
+                    if (sb != R(0) && sc != R(0)) {
+                        const R idv = A[15 * n + ca] + dv;
+                        if (own[q]) B[15 * n + ca] = idv;
+                        dn += m4 * idv;
+                    }
+                }
+                if (ps.comp == a && ps.cell == c) dn += R(ps.amp);
+                if (own[q]) B[3 * n + ca] = dn;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMRpt; ++q)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                hP[q][a] = hC[q][a];
+                eC[q][a] = eN[q][a];
+                dC[q][a] = dN[q][a];
+                hO[q][a] = hON[q][a];
+            }
+    }
 }
 
 // ------------------------------------------------------------ host glue
@@ -1100,7 +1356,15 @@ extern "C" int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* s
         float* B = (float*)state_b;
         for (int t = 0; t < n_steps; ++t) {
             PointSrc ps{comp, src_cell, 0.0};
-            k_fused<float><<<nblk(f.n), 256, 0, s>>>(f, A, B, ps, amp_dev, t0 + t);
+            if (f.nz > 1) {
+                const int ty = (f.ny + kMOwnY - 1) / kMOwnY, tz = (f.nz + kMOwnZ - 1) / kMOwnZ;
+                const int nch = std::max(1, std::min(f.nx / 4 > 0 ? f.nx / 4 : 1, (2048 + ty * tz - 1) / (ty * tz)));
+                const int chunk = (f.nx + nch - 1) / nch;
+                const int nb = ty * tz * ((f.nx + chunk - 1) / chunk);
+                k_march<float, 1><<<nb, 64 * kMRows, 0, s>>>(f, A, B, ps, amp_dev, t0 + t, ty, tz, chunk);
+            } else {
+                k_fused<float><<<nblk(f.n), 256, 0, s>>>(f, A, B, ps, amp_dev, t0 + t);
+            }
             if (n_probes > 0) {
                 Fdtd<float> g = f;
                 g.st = B;
@@ -1120,7 +1384,15 @@ extern "C" int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* s
         double* B = (double*)state_b;
         for (int t = 0; t < n_steps; ++t) {
             PointSrc ps{comp, src_cell, 0.0};
-            k_fused<double><<<nblk(f.n), 256, 0, s>>>(f, A, B, ps, amp_dev, t0 + t);
+            if (f.nz > 1) {
+                const int ty = (f.ny + kMOwnY - 1) / kMOwnY, tz = (f.nz + kMOwnZ - 1) / kMOwnZ;
+                const int nch = std::max(1, std::min(f.nx / 4 > 0 ? f.nx / 4 : 1, (2048 + ty * tz - 1) / (ty * tz)));
+                const int chunk = (f.nx + nch - 1) / nch;
+                const int nb = ty * tz * ((f.nx + chunk - 1) / chunk);
+                k_march<double, 1><<<nb, 64 * kMRows, 0, s>>>(f, A, B, ps, amp_dev, t0 + t, ty, tz, chunk);
+            } else {
+                k_fused<double><<<nblk(f.n), 256, 0, s>>>(f, A, B, ps, amp_dev, t0 + t);
+            }
             if (n_probes > 0) {
                 Fdtd<double> g = f;
                 g.st = B;

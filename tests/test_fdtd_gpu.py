@@ -239,15 +239,21 @@ def test_tm_blocked_gradient_matches_generic():
 
 
 @pytest.mark.parametrize('shape,npml,comp', [((12, 10, 9), [3, 2, 2], 'Jz'), ((15, 13, 1), [4, 3, 0], 'Jx'),
-                                             ((7, 9, 11), [0, 2, 3], 'Jy')])
+                                             ((7, 9, 11), [0, 2, 3], 'Jy'), ((9, 20, 70), [2, 4, 5], 'Jz'),
+                                             ((5, 3, 2), [1, 1, 0], 'Jx'), ((12, 33, 130), [3, 3, 3], 'Jy')])
 def test_fused_single_pass_kernel_matches_oracle(shape, npml, comp):
+    """The single-pass kernels (x-marching tiles in 3D, per-cell otherwise),
+    including (y, z) extents that are not multiples of the 16 x 64 tile and
+    grids smaller than one tile."""
+    print('fused', shape, npml, comp, flush=True)
     import ceviche_amd as cv
     rng = np.random.default_rng(13)
     eps0 = rng.uniform(1, 5, shape)
     steps, dL = 37, 30e-9
     amp = np.exp(-(np.arange(steps) - 12) ** 2 / (2 * 4 ** 2))
     src = tuple(s // 2 for s in shape)
-    probes = [('Ex', (1, 2, 0)), ('Hy', (shape[0] - 1, 1, shape[2] - 1)), ('Dz', src), ('Hz', (2, 3, shape[2] // 2))]
+    probes = [('Ex', (1, min(2, shape[1] - 1), 0)), ('Hy', (shape[0] - 1, 1, shape[2] - 1)), ('Dz', src),
+              ('Hz', (2, min(3, shape[1] - 1), shape[2] // 2))]
     F = cv.fdtd(torch.tensor(eps0), dL, npml)
     rec = F.run(steps, src, amp, probes, source_component=comp, kernel='fused').cpu().numpy()
     fields = {k: v.detach().cpu().numpy() for k, v in F.fields.items()}
