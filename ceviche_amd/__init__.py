@@ -11,3 +11,4 @@ from ._fdfd_op import SolverOptions  # noqa: E402,F401
 from .fdtd import fdtd  # noqa: E402,F401
 from .jacobians import jacobian  # noqa: E402,F401
 from . import primitives  # noqa: E402,F401
+from . import modes  # noqa: E402,F401
