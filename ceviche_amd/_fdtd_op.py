@@ -96,6 +96,35 @@ class FdtdGrid:
                                  ptr(rec), stream_ptr()), 'fdtd step')
         return state
 
+    def step_tangent(self, state, tstate, eps, d_eps, n_steps, src=None, amp=None, d_amp=None, probes=None,
+                     trec=None, rec=None, J=(None, None, None), dJ=(None, None, None), jscale=1.0):
+        """Forward mode: advance ``state`` and its tangent ``tstate`` in place
+        along d_eps (tensor like eps), the dense source tangents dJ and d_amp
+        (host float64 array or None); tangents of the probes go to trec
+        (n_steps, n_probes)."""
+        Jx, Jy, Jz = (None if j is None else j.to(self.device, self.dtype).contiguous() for j in J)
+        dJx, dJy, dJz = (None if j is None else j.to(self.device, self.dtype).contiguous() for j in dJ)
+        comp, cell = (-1, 0) if src is None else (int(src[0]), int(src[1]))
+        amp_p = damp_p = None
+        if comp >= 0:
+            amp = np.ascontiguousarray(amp, dtype=np.float64)
+            amp_p = amp.ctypes.data_as(ctypes.c_void_p)
+            if d_amp is not None:
+                d_amp = np.ascontiguousarray(d_amp, dtype=np.float64)
+                damp_p = d_amp.ctypes.data_as(ctypes.c_void_p)
+        pc, pl = (None, None) if probes is None else probes
+        npb = 0 if probes is None else int(pc.numel())
+        check(lib().cv_fdtd_step_tangent(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(tstate),
+                                         ptr(d_eps), ptr(Jx), ptr(Jy), ptr(Jz), ptr(dJx), ptr(dJy), ptr(dJz),
+                                         float(jscale), comp, cell, amp_p, damp_p, int(n_steps), ptr(pc), ptr(pl),
+                                         npb, ptr(rec), ptr(trec), stream_ptr()), 'fdtd step_tangent')
+
+    def efields_tangent(self, state, tstate, eps, d_eps):
+        dE = torch.empty((3,) + self.shape, dtype=self.dtype, device=self.device)
+        check(lib().cv_fdtd_efields_tangent(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(state), ptr(tstate),
+                                            ptr(d_eps), ptr(dE), stream_ptr()), 'fdtd efields_tangent')
+        return dE
+
     def run_tm(self, state, eps, src_cell, amp_dev, t0, n_steps, probes, rec, rec_t0, config):
         """Fused, temporally blocked 2D TM steps (nz == 1, TE fields zero, Jz point source)."""
         pc, pl = (None, None) if probes is None else probes
@@ -157,8 +186,19 @@ class FdtdStep(torch.autograd.Function):
         grid.step(new, eps, J=(Jx, Jy, Jz))
         ctx.grid = grid
         ctx.save_for_backward(state, eps)
+        ctx.save_for_forward(state, eps, Jx, Jy, Jz)
         ctx.has_J = (Jx is not None, Jy is not None, Jz is not None)
         return new
+
+    @staticmethod
+    def jvp(ctx, d_state, d_eps, dJx, dJy, dJz, _):
+        state, eps, Jx, Jy, Jz = ctx.saved_tensors
+        grid = ctx.grid
+        st = state.detach().clone()
+        tst = grid.new_state() if d_state is None else d_state.to(grid.dtype).contiguous().clone()
+        de = torch.zeros_like(eps) if d_eps is None else d_eps.to(eps.dtype).contiguous()
+        grid.step_tangent(st, tst, eps.detach(), de, 1, J=(Jx, Jy, Jz), dJ=(dJx, dJy, dJz))
+        return tst
 
     @staticmethod
     def backward(ctx, g):
@@ -181,7 +221,16 @@ class FdtdEFields(torch.autograd.Function):
     def forward(ctx, state, eps, grid):
         ctx.grid = grid
         ctx.save_for_backward(state, eps)
+        ctx.save_for_forward(state, eps)
         return grid.efields(state, eps)
+
+    @staticmethod
+    def jvp(ctx, d_state, d_eps, _):
+        state, eps = ctx.saved_tensors
+        grid = ctx.grid
+        tst = grid.new_state() if d_state is None else d_state.to(grid.dtype).contiguous()
+        de = torch.zeros_like(eps) if d_eps is None else d_eps.to(eps.dtype).contiguous()
+        return grid.efields_tangent(state.detach(), tst, eps.detach(), de)
 
     @staticmethod
     def backward(ctx, gE):
@@ -276,8 +325,27 @@ class FdtdRun(torch.autograd.Function):
         ctx.grid, ctx.src, ctx.probes, ctx.n_steps, ctx.ck = grid, src, probes, n_steps, ck
         ctx.amp_h, ctx.amp_dev, ctx.tm_config = amp_h, amp_dev, tm_config
         ctx.checkpoints = checkpoints
+        ctx.state0 = state0.detach().clone()
         ctx.save_for_backward(eps)
+        ctx.save_for_forward(eps)
         return rec, state
+
+    @staticmethod
+    def jvp(ctx, d_eps, d_amp, d_grid, d_state0, *_):
+        """Forward mode: the run is replayed from the initial state with the
+        tangent carried along (cv_fdtd_step_tangent), giving the tangents of
+        the records and of the final state."""
+        (eps,) = ctx.saved_tensors
+        grid, src, probes, n_steps = ctx.grid, ctx.src, ctx.probes, ctx.n_steps
+        pc, _ = probes
+        state = ctx.state0.clone()
+        tstate = grid.new_state() if d_state0 is None else d_state0.to(grid.dtype).contiguous().clone()
+        de = torch.zeros_like(eps) if d_eps is None else d_eps.to(eps.dtype).contiguous()
+        da = None if d_amp is None else d_amp.detach().to('cpu', torch.float64).numpy()
+        trec = torch.empty((n_steps, pc.numel()), dtype=grid.dtype, device=grid.device)
+        grid.step_tangent(state, tstate, eps.detach(), de, n_steps, src=src, amp=ctx.amp_h, d_amp=da, probes=probes,
+                          trec=trec)
+        return trec, tstate
 
     @staticmethod
     def backward(ctx, grec, gstate_final):
@@ -311,5 +379,6 @@ class FdtdRun(torch.autograd.Function):
                     grid.adjoint(lam, eps, Dhist[s], gmE)
             del st
         geps = grid.eps_grad(eps, gmE, torch.zeros_like(eps)) if ctx.needs_input_grad[0] else None
-        ctx.checkpoints = None
+        # the checkpoints stay with the graph: a retained graph (one backward
+        # per output row of a Jacobian) runs this more than once
         return geps, None, None, None, None, None, None, None, None

@@ -111,6 +111,8 @@ _SIGS = {
     'cv_fdfd_solve_mixed': (I, [P, P, I, P, I, P, P, P, P, P, P, P, P, I, D, D, I, I, I, P, I64, P, P]),
     'cv_fdtd_step': (I, [I, P, P, P, P, P, P, P, D, I, I64, P, I, P, P, I, P, P]),
     'cv_fdtd_efields': (I, [I, P, P, P, P, P, P]),
+    'cv_fdtd_step_tangent': (I, [I, P, P, P, P, P, P, P, P, P, P, P, P, D, I, I64, P, P, I, P, P, I, P, P, P]),
+    'cv_fdtd_efields_tangent': (I, [I, P, P, P, P, P, P, P, P]),
     'cv_fdtd_efields_grad': (I, [I, P, P, P, P, P, P, P, P]),
     'cv_fdtd_adjoint_step': (I, [I, P, P, P, P, P, P, P, P, P, I, P, P, P]),
     'cv_fdtd_eps_grad': (I, [I, P, P, P, P, P, P]),

@@ -97,23 +97,40 @@ __host__ __device__ __forceinline__ R efield(const Fdtd<R>& f, const R* __restri
     return (R(1) / eaa) * D[c];
 }
 
-// ----------------------------------------------------------------- H phase
+// Tangent of E_a = D_a / eps_aa along (dD, d_eps): dD_a / eps_aa -
+// D_a d(eps_aa) / eps_aa^2, d(eps_aa) = (d_eps + d_eps[-e_a]) / 2.
 template <typename R>
-__host__ __device__ __forceinline__ void h_cell(const Fdtd<R>& f, int64_t c) {
+__host__ __device__ __forceinline__ R efield_tangent(const Fdtd<R>& f, const R* __restrict__ D,
+                                                     const R* __restrict__ dD, const R* __restrict__ deps, int a,
+                                                     int i, int j, int k) {
+    const int64_t c = cell(f, i, j, k);
+    int64_t cm;
+    if (a == 0) cm = cell(f, wrapm(i, f.nx), j, k);
+    else if (a == 1) cm = cell(f, i, wrapm(j, f.ny), k);
+    else cm = cell(f, i, j, wrapm(k, f.nz));
+    const R inv = R(1) / ((f.eps[c] + f.eps[cm]) / R(2));
+    const R deaa = (deps[c] + deps[cm]) / R(2);
+    return inv * dD[a * f.n + c] - (deaa * inv * inv) * D[a * f.n + c];
+}
+
+// ----------------------------------------------------------------- H phase
+// H update of cell c of the state f.st with E read through `E(a, i, j, k)`
+// (the primal E of f.st, or the tangent E of a forward-mode step).
+template <typename R, typename EGet>
+__host__ __device__ __forceinline__ void h_cell_with(const Fdtd<R>& f, int64_t c, EGet E) {
     int i, j, k;
     decompose(f, c, i, j, k);
     const int64_t n = f.n;
     R* H = f.st;
-    const R* D = f.st + 3 * n;
     R* ICE = f.st + 6 * n;
     R* IH = f.st + 9 * n;
     const int ip = wrapp(i, f.nx), jp = wrapp(j, f.ny), kp = wrapp(k, f.nz);
-    const R ex = efield(f, D, 0, i, j, k), ey = efield(f, D, 1, i, j, k), ez = efield(f, D, 2, i, j, k);
+    const R ex = E(0, i, j, k), ey = E(1, i, j, k), ez = E(2, i, j, k);
     R ce[3];
     // curl_E, ceviche/derivatives.py:16-22 (forward differences, periodic)
-    ce[0] = ((efield(f, D, 2, i, jp, k) - ez) - (efield(f, D, 1, i, j, kp) - ey)) * f.inv_dL;
-    ce[1] = ((efield(f, D, 0, i, j, kp) - ex) - (efield(f, D, 2, ip, j, k) - ez)) * f.inv_dL;
-    ce[2] = ((efield(f, D, 1, ip, j, k) - ey) - (efield(f, D, 0, i, jp, k) - ex)) * f.inv_dL;
+    ce[0] = ((E(2, i, jp, k) - ez) - (E(1, i, j, kp) - ey)) * f.inv_dL;
+    ce[1] = ((E(0, i, j, kp) - ex) - (E(2, ip, j, k) - ez)) * f.inv_dL;
+    ce[2] = ((E(1, ip, j, k) - ey) - (E(0, i, jp, k) - ex)) * f.inv_dL;
     const int pos[3] = {i, j, k};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -145,10 +162,25 @@ __host__ __device__ __forceinline__ void h_cell(const Fdtd<R>& f, int64_t c) {
 }
 
 template <typename R>
+__host__ __device__ __forceinline__ void h_cell(const Fdtd<R>& f, int64_t c) {
+    const R* D = f.st + 3 * f.n;
+    h_cell_with(f, c, [&](int a, int i, int j, int k) { return efield(f, D, a, i, j, k); });
+}
+
+template <typename R>
 __global__ void __launch_bounds__(256) k_h(Fdtd<R> f) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= f.n) return;
     h_cell(f, c);
+}
+
+// tangent H update: ft.st is the tangent state, D the primal D of the same step
+template <typename R>
+__global__ void __launch_bounds__(256) k_h_tangent(Fdtd<R> ft, const R* __restrict__ D, const R* __restrict__ deps) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ft.n) return;
+    const R* dD = ft.st + 3 * ft.n;
+    h_cell_with(ft, c, [&](int a, int i, int j, int k) { return efield_tangent(ft, D, dD, deps, a, i, j, k); });
 }
 
 // ----------------------------------------------------------------- D phase
@@ -237,6 +269,29 @@ __global__ void k_record(Fdtd<R> f, const int32_t* __restrict__ pcomp, const int
         v = f.st[(int64_t)comp * f.n + c];
     } else {
         v = f.st[(int64_t)(comp - 6) * f.n + c];
+    }
+    out[p] = v;
+}
+
+// tangent probes: E comps from efield_tangent (primal D of f.st, tangent of
+// ft.st), D / H comps straight from the tangent state
+template <typename R>
+__global__ void k_record_tangent(Fdtd<R> f, Fdtd<R> ft, const R* __restrict__ deps,
+                                 const int32_t* __restrict__ pcomp, const int64_t* __restrict__ pcell, int np,
+                                 R* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int comp = pcomp[p];
+    const int64_t c = pcell[p];
+    R v;
+    if (comp < 3) {
+        int i, j, k;
+        decompose(f, c, i, j, k);
+        v = efield_tangent(f, f.st + 3 * f.n, ft.st + 3 * f.n, deps, comp, i, j, k);
+    } else if (comp < 6) {
+        v = ft.st[(int64_t)comp * f.n + c];
+    } else {
+        v = ft.st[(int64_t)(comp - 6) * f.n + c];
     }
     out[p] = v;
 }
@@ -1189,6 +1244,95 @@ extern "C" int cv_fdtd_step(int dtype, const cv_fdtd_desc* d, const void* sig, c
                 k_record<double><<<(n_probes + 63) / 64, 64, 0, s>>>(f, probe_comp, probe_cell, n_probes,
                                                                      (double*)rec + (int64_t)t * n_probes);
         }
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+template <typename R>
+static int step_tangent_impl(const cv_fdtd_desc* d, const void* sig, const void* eps, void* state, void* tstate,
+                             const void* deps, const void* Jx, const void* Jy, const void* Jz, const void* dJx,
+                             const void* dJy, const void* dJz, double jscale, int src_comp, int64_t src_cell,
+                             const double* amp, const double* damp, int n_steps, const int32_t* probe_comp,
+                             const int64_t* probe_cell, int n_probes, void* rec, void* trec, hipStream_t s) {
+    Fdtd<R> f, ft;
+    int rc = make_fdtd<R>(d, sig, eps, state, f);
+    if (rc) return rc;
+    rc = make_fdtd<R>(d, sig, eps, tstate, ft);
+    if (rc) return rc;
+    CV_CHECK(src_comp < 0 || (src_cell >= 0 && src_cell < f.n), "source cell out of range");
+    const R* D = f.st + 3 * f.n;
+    for (int t = 0; t < n_steps; ++t) {
+        PointSrc ps{src_comp, src_cell, (src_comp >= 0 && amp) ? amp[t] : 0.0};
+        PointSrc pt{src_comp, src_cell, (src_comp >= 0 && damp) ? damp[t] : 0.0};
+        k_h<R><<<nblk(f.n), 256, 0, s>>>(f);
+        k_h_tangent<R><<<nblk(f.n), 256, 0, s>>>(ft, D, (const R*)deps);   // primal D still at step t
+        k_d<R><<<nblk(f.n), 256, 0, s>>>(f, (const R*)Jx, (const R*)Jy, (const R*)Jz, R(jscale), ps);
+        k_d<R><<<nblk(f.n), 256, 0, s>>>(ft, (const R*)dJx, (const R*)dJy, (const R*)dJz, R(jscale), pt);
+        if (n_probes > 0) {
+            if (rec)
+                k_record<R><<<(n_probes + 63) / 64, 64, 0, s>>>(f, probe_comp, probe_cell, n_probes,
+                                                                (R*)rec + (int64_t)t * n_probes);
+            k_record_tangent<R><<<(n_probes + 63) / 64, 64, 0, s>>>(f, ft, (const R*)deps, probe_comp, probe_cell,
+                                                                    n_probes, (R*)trec + (int64_t)t * n_probes);
+        }
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdtd_step_tangent(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                                    void* state, void* tstate, const void* d_eps, const void* Jx, const void* Jy,
+                                    const void* Jz, const void* dJx, const void* dJy, const void* dJz,
+                                    double jscale, int src_comp, int64_t src_cell, const double* amp,
+                                    const double* d_amp, int n_steps, const int32_t* probe_comp,
+                                    const int64_t* probe_cell, int n_probes, void* rec, void* trec, void* stream) {
+    CV_CHECK(n_steps >= 0, "n_steps < 0");
+    CV_CHECK(state && tstate && d_eps && state != tstate, "null or aliased state / tangent buffers");
+    CV_CHECK(src_comp >= -1 && src_comp < 3, "src_comp must be -1 (none) or 0..2");
+    CV_CHECK(n_probes == 0 || (probe_comp && probe_cell && trec), "probes need comp, cell and trec buffers");
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_F32)
+        return step_tangent_impl<float>(d, sig, eps, state, tstate, d_eps, Jx, Jy, Jz, dJx, dJy, dJz, jscale,
+                                        src_comp, src_cell, amp, d_amp, n_steps, probe_comp, probe_cell, n_probes,
+                                        rec, trec, s);
+    if (dtype == CV_F64)
+        return step_tangent_impl<double>(d, sig, eps, state, tstate, d_eps, Jx, Jy, Jz, dJx, dJy, dJz, jscale,
+                                         src_comp, src_cell, amp, d_amp, n_steps, probe_comp, probe_cell, n_probes,
+                                         rec, trec, s);
+    CV_CHECK(false, "unknown dtype %d", dtype);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_efields_tangent(Fdtd<R> f, const R* __restrict__ dD,
+                                                         const R* __restrict__ deps, R* __restrict__ dE) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    int i, j, k;
+    decompose(f, c, i, j, k);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dE[a * f.n + c] = efield_tangent(f, f.st + 3 * f.n, dD, deps, a, i, j, k);
+}
+
+extern "C" int cv_fdtd_efields_tangent(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
+                                       const void* state, const void* tstate, const void* d_eps, void* dE,
+                                       void* stream) {
+    CV_CHECK(state && tstate && d_eps && dE, "null pointer argument");
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_F32) {
+        Fdtd<float> f;
+        int rc = make_fdtd<float>(d, sig, eps, const_cast<void*>(state), f);
+        if (rc) return rc;
+        k_efields_tangent<float><<<nblk(f.n), 256, 0, s>>>(f, (const float*)tstate + 3 * f.n, (const float*)d_eps,
+                                                           (float*)dE);
+    } else if (dtype == CV_F64) {
+        Fdtd<double> f;
+        int rc = make_fdtd<double>(d, sig, eps, const_cast<void*>(state), f);
+        if (rc) return rc;
+        k_efields_tangent<double><<<nblk(f.n), 256, 0, s>>>(f, (const double*)tstate + 3 * f.n,
+                                                            (const double*)d_eps, (double*)dE);
     } else {
         CV_CHECK(false, "unknown dtype %d", dtype);
     }

@@ -206,6 +206,25 @@ int cv_fdtd_step(int dtype, const cv_fdtd_desc* d, const void* sig, const void* 
                  const int64_t* probe_cell, int n_probes, void* rec, void* stream);
 
 /* E[3][n] = D / eps_aa (fdtd.py:134-137, utils.py:153-176). */
+/* Forward mode (tangent-linear) of cv_fdtd_step: advances the primal `state`
+ * (dense sources Jx/Jy/Jz and point source amp as cv_fdtd_step) and its
+ * tangent `tstate` (same layout) along the permittivity tangent d_eps, the
+ * dense source tangents dJx/dJy/dJz (any may be NULL) and the amplitude
+ * tangent d_amp (host array of n_steps, may be NULL); E_a = D_a / eps_aa gives
+ * dE_a = dD_a / eps_aa - D_a d(eps_aa) / eps_aa^2, the rest of the update is
+ * linear.  Records the primal probes into rec (may be NULL) and their
+ * tangents into trec.  The forward-mode rule the reference gets from
+ * autograd through fdtd.py:74-144 (jacobians.py:38-51). */
+int cv_fdtd_step_tangent(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state,
+                         void* tstate, const void* d_eps, const void* Jx, const void* Jy, const void* Jz,
+                         const void* dJx, const void* dJy, const void* dJz, double jscale, int src_comp,
+                         int64_t src_cell, const double* amp, const double* d_amp, int n_steps,
+                         const int32_t* probe_comp, const int64_t* probe_cell, int n_probes, void* rec, void* trec,
+                         void* stream);
+/* dE = tangent of E = D / eps_aa along (tstate's D, d_eps), all cells (3 x n). */
+int cv_fdtd_efields_tangent(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, const void* state,
+                            const void* tstate, const void* d_eps, void* dE, void* stream);
+
 int cv_fdtd_efields(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
                     const void* state, void* E, void* stream);
 
