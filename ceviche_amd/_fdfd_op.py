@@ -43,7 +43,17 @@ class FdfdOperator:
     """A batch of FDFD instances (``specs``: list of dicts with keys omega, dL,
     shape, npml, bloch) of polarisation 'ez' or 'hz' in precision ``dtype``
     (torch.complex128 or torch.complex64).  Vectors of the batch are packed
-    instance after instance, each in C order over (nx, ny)."""
+    instance after instance, each in C order over (nx, ny).
+
+    Frequency-coupled Ez systems (the multi-frequency operator,
+    ceviche/fdfd.py:284-324): consecutive specs with
+    ``system=(blk, nblk, n_cpl)`` form one linear system of nblk blocks on the
+    same grid; block blk uses ``shift_omega`` (its sideband frequency W_blk)
+    in the eps0 W^2 eps term and ``omega`` for the PML, and couples to blocks
+    blk +- (h+1), h < n_cpl, through a complex table passed to the solve /
+    apply calls (see ``prof_with``): for each h, gp_h then gm_h over the
+    grid's cells, in units of omega^2 * (system scale), multiplied by
+    w2[blk] = (W_blk / omega)^2."""
 
     def __init__(self, pol, specs, dtype=torch.complex128, device=None):
         if pol not in ('ez', 'hz'):
@@ -65,6 +75,8 @@ class FdfdOperator:
         self.sys_scale = np.zeros(n_inst)
         self.shapes = []
         self.offsets = []
+        systems = []       # (first instance, nblk, n_cpl) of every coupled system
+        w2 = []
         for k, sp in enumerate(self.specs):
             nx, ny = (int(v) for v in sp['shape'])
             omega, dL = float(sp['omega']), float(sp['dL'])
@@ -80,9 +92,24 @@ class FdfdOperator:
             # The system is scaled by c = mu0 dL^2 (Ez) or eps0 dL^2 (Hz) so the
             # stencil coefficients are O(1) in either precision; the solution is
             # unchanged and the caller's right-hand side is scaled by c as well.
+            blk, nblk, n_cpl = sp.get('system', (0, 1, 0))
+            w_shift = float(sp.get('shift_omega', omega))
+            if nblk > 1 or n_cpl:
+                if pol != 'ez':
+                    raise ValueError('frequency-coupled systems are Ez only')
+                if not (0 <= blk < nblk and n_cpl >= 0):
+                    raise ValueError('bad system fields %r' % ((blk, nblk, n_cpl),))
+                if blk == 0:
+                    systems.append((k, nblk, n_cpl))
+                else:
+                    r = self.specs[k - blk] if k - blk >= 0 else None
+                    if (r is None or tuple(r.get('system', ())) != (0, nblk, n_cpl)
+                            or tuple(int(v) for v in r['shape']) != (nx, ny)):
+                        raise ValueError('block %d of a coupled system must follow block 0 on the same grid' % k)
+                w2.append((w_shift / omega) ** 2)
             if pol == 'ez':
                 c = MU_0 * dL ** 2
-                scale, dshift = -1 / MU_0 * c, -EPSILON_0 * omega ** 2 * c
+                scale, dshift = -1 / MU_0 * c, -EPSILON_0 * w_shift ** 2 * c
             else:
                 c = EPSILON_0 * dL ** 2
                 scale, dshift = 1 / EPSILON_0 * c, MU_0 * omega ** 2 * c
@@ -90,11 +117,35 @@ class FdfdOperator:
             phx, phy = np.exp(1j * bx), np.exp(1j * by)
             flags = (_lib.CV_FLAG_X_IDENTITY if nx == 1 else 0) | (_lib.CV_FLAG_Y_IDENTITY if ny == 1 else 0)
             desc[k] = (nx, ny, _lib.CV_POL_EZ if pol == 'ez' else _lib.CV_POL_HZ, flags, cell_off, prof_off,
-                       first_tile, ntile, scale, dshift, phx.real, phx.imag, phy.real, phy.imag, omega)
+                       first_tile, ntile, scale, dshift, phx.real, phx.imag, phy.real, phy.imag, omega,
+                       n_cpl, blk, nblk, 0, 0, 0)
             self.shapes.append((nx, ny))
             self.offsets.append(cell_off)
             cell_off += n
             prof_off += 2 * (nx + ny)
+        # coupled systems: shared tile range, table and w2 offsets (tables after
+        # all PML profiles, then every system's w2 entries)
+        self.pml_len = prof_off
+        cpl_cursor = prof_off
+        self.systems = []
+        for k0, nblk, n_cpl in systems:
+            if k0 + nblk > n_inst:
+                raise ValueError('coupled system at instance %d is missing blocks' % k0)
+            n = self.shapes[k0][0] * self.shapes[k0][1]
+            self.systems.append(dict(first=k0, nblk=nblk, n_cpl=n_cpl, cells=n, cpl_off=cpl_cursor,
+                                     cell_off=self.offsets[k0]))
+            cpl_cursor += 2 * n_cpl * n
+        self.cpl_len = cpl_cursor - prof_off
+        w2_cursor = cpl_cursor
+        for sy in self.systems:
+            k0, nblk = sy['first'], sy['nblk']
+            t0 = desc[k0]['tile_first']
+            tn = int(desc[k0 + nblk - 1]['tile_first'] + desc[k0 + nblk - 1]['tile_count'] - t0)
+            for b in range(nblk):
+                d = desc[k0 + b]
+                d['tile_first'], d['tile_count'] = t0, tn
+                d['cpl_off'], d['w2_off'] = sy['cpl_off'], w2_cursor
+            w2_cursor += nblk
         self.total_cells = cell_off
         self.n_inst = n_inst
         self.desc_host = desc
@@ -103,7 +154,7 @@ class FdfdOperator:
         dev = self.device
         self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
         self.tiles = torch.from_numpy(self.tiles_host.view(np.uint8).copy()).to(dev)
-        prof = np.concatenate(profs)
+        prof = np.concatenate(profs + [np.zeros(self.cpl_len), np.asarray(w2, dtype=np.float64)]).astype(np.complex128)
         self.prof = torch.from_numpy(prof).to(dev).to(dtype)
         self.prof64 = self.prof if dtype == torch.complex128 else torch.from_numpy(prof).to(dev)
         self._ws = None
@@ -115,6 +166,37 @@ class FdfdOperator:
     # ------------------------------------------------------------- helpers
     def _args(self):
         return (ptr(self.desc), ptr(self.tiles), self.n_tiles)
+
+    def prof_with(self, cpl, dtype=None):
+        """The coefficient array with the coupling tables ``cpl`` (complex,
+        cpl_len entries: the systems' tables in order) in place."""
+        dtype = dtype or self.dtype
+        base = self.prof if self.prof.dtype == dtype else self.prof64.to(dtype)
+        if self.cpl_len == 0:
+            return base
+        if cpl is None or cpl.numel() != self.cpl_len:
+            raise ValueError('coupling tables must hold %d entries' % self.cpl_len)
+        return torch.cat([base[:self.pml_len], cpl.detach().to(dtype).reshape(-1),
+                          base[self.pml_len + self.cpl_len:]]).contiguous()
+
+    def coupling_apply(self, tab, x, y=None, transpose=False):
+        """y (+)= C x with the inter-block terms only, tables from ``tab``
+        (an array laid out like prof_with's result)."""
+        x = x.to(self.dtype).contiguous()
+        acc = y is not None
+        if y is None:
+            y = torch.empty_like(x)
+        check(lib().cv_fdfd_coupling_apply(self.code, *self._args(), ptr(tab), ptr(x), ptr(y),
+                                           int(bool(transpose)), int(acc), stream_ptr()), 'coupling_apply')
+        return y
+
+    def coupling_grad(self, prof, lam, x, scale):
+        """scale * lam^T (dA/d table) x for every table entry (cpl_len, complex)."""
+        g = torch.zeros_like(prof)
+        check(lib().cv_fdfd_coupling_grad(self.code, *self._args(), ptr(prof), ptr(lam.contiguous()),
+                                          ptr(x.contiguous()), ptr(g), float(scale), stream_ptr()),
+              'coupling_grad')
+        return g[self.pml_len:self.pml_len + self.cpl_len]
 
     def scale_vector(self, dtype=None):
         """Per-cell system scale c (real), for packing right-hand sides."""
@@ -146,11 +228,13 @@ class FdfdOperator:
                                               ptr(out), stream_ptr()), 'hz_materials_grad')
         return out
 
-    def apply(self, x, m0, m1, transpose=False):
-        """c * A x (or c * A^T x) with the per-instance system scale c."""
+    def apply(self, x, m0, m1, transpose=False, prof=None):
+        """c * A x (or c * A^T x) with the per-instance system scale c
+        (``prof``: coefficients with coupling tables, from prof_with)."""
         x = x.to(self.dtype).contiguous()
         y = torch.empty_like(x)
-        check(lib().cv_fdfd_apply(self.code, *self._args(), ptr(self.prof), ptr(m0), ptr(m1), ptr(x), ptr(y),
+        prof = self.prof if prof is None else prof
+        check(lib().cv_fdfd_apply(self.code, *self._args(), ptr(prof), ptr(m0), ptr(m1), ptr(x), ptr(y),
                                   int(bool(transpose)), stream_ptr()), 'apply')
         return y
 
@@ -205,23 +289,27 @@ class FdfdOperator:
             self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def solve_scaled(self, bs, m0, m1, transpose=False, x0=None, opts=None, mixed_ops=None):
+    def solve_scaled(self, bs, m0, m1, transpose=False, x0=None, opts=None, mixed_ops=None, prof=None):
         """Solve (c A) x = bs (bs already multiplied by c).  Returns x.
 
         For complex64 operators, ``mixed_ops`` = (op64, m0_64, m1_64) selects
         mixed-precision iterative refinement: x and the residual in complex128,
-        complex64 BiCGSTAB corrections."""
+        complex64 BiCGSTAB corrections.  ``prof`` (coupled systems) is
+        (prof, prof64) from prof_with for this operator's and op64's
+        precision (prof64 unused without mixed_ops)."""
         opts = opts or SolverOptions()
         info = (_lib.SolveInfo * self.n_inst)()
+        pr, pr64 = (self.prof, None) if prof is None else prof
         if mixed_ops is not None:
             op64, m0_64, m1_64 = mixed_ops
+            pr64 = op64.prof if pr64 is None else pr64
             tol = opts.tol if opts.tol is not None else DEFAULT_TOL[torch.complex64]
             b64 = bs.to(torch.complex128).contiguous()
             x = torch.zeros_like(b64) if x0 is None else x0.to(torch.complex128).clone()
             ws = self._workspace(True)
             rc = lib().cv_fdfd_solve_mixed(
                 ptr(op64.desc), self.desc_host.ctypes.data_as(ctypes.c_void_p), self.n_inst, ptr(self.tiles),
-                self.n_tiles, ptr(op64.prof), ptr(m0_64), ptr(m1_64), ptr(self.prof), ptr(m0), ptr(m1),
+                self.n_tiles, ptr(pr64), ptr(m0_64), ptr(m1_64), ptr(pr), ptr(m0), ptr(m1),
                 ptr(b64), ptr(x), int(bool(transpose)), float(tol), float(opts.inner_tol), int(opts.max_iter),
                 int(opts.max_sweeps), int(opts.check_every), ptr(ws), ws.numel(), info, stream_ptr())
         else:
@@ -231,7 +319,7 @@ class FdfdOperator:
             ws = self._workspace(False)
             rc = lib().cv_fdfd_solve(
                 self.code, ptr(self.desc), self.desc_host.ctypes.data_as(ctypes.c_void_p), self.n_inst,
-                ptr(self.tiles), self.n_tiles, ptr(self.prof), ptr(m0), ptr(m1), ptr(b), ptr(x),
+                ptr(self.tiles), self.n_tiles, ptr(pr), ptr(m0), ptr(m1), ptr(b), ptr(x),
                 int(bool(transpose)), float(tol), int(opts.max_iter), int(opts.check_every), ptr(ws), ws.numel(),
                 info, stream_ptr())
         self.last_info = [dict(iterations=i.iterations, status=i.status, restarts=i.restarts,
@@ -296,6 +384,76 @@ class FdfdSolve(torch.autograd.Function):
             g1 = torch.zeros_like(m0) if ctx.has_m1 else None
             op.material_grad(mu.contiguous(), x.contiguous(), g0, g1, -1.0)
         return g0, g1, gb, None, None, None
+
+
+class FdfdCoupledSolve(torch.autograd.Function):
+    """x = A(m, g)^-1 b for frequency-coupled systems, where g are the
+    coupling tables (complex, op.cpl_len entries, system-scaled units): the
+    sp_solve of the multi-frequency operator (fdfd.py:326-335 ->
+    primitives.py:103-148).  Backward as FdfdSolve, plus
+    grad_g = conj(-mu^T (dA/dg) x) from the coupling-gradient kernel;
+    forward mode subtracts (dA/dm . dm + dA/dg . dg) x."""
+
+    @staticmethod
+    def _profs(op, op32, cpl):
+        p64 = op.prof_with(cpl, torch.complex128)
+        p32 = op32.prof_with(cpl, torch.complex64) if op32 is not None else None
+        return p64, p32
+
+    @staticmethod
+    def _solve(op, op32, opts, m0, m1, bs, transpose, p64, p32):
+        if op32 is None:
+            return op.solve_scaled(bs, m0, m1, transpose=transpose, opts=opts, prof=(p64, None))
+        m0s = m0.to(torch.float32).contiguous()
+        m1s = None if m1 is None else m1.to(torch.float32).contiguous()
+        return op32.solve_scaled(bs, m0s, m1s, transpose=transpose, opts=opts, mixed_ops=(op, m0, m1),
+                                 prof=(p32, p64))
+
+    @staticmethod
+    def forward(ctx, m0, m1, cpl, b, op, op32, opts):
+        c = op.scale_vector(torch.float64)
+        p64, p32 = FdfdCoupledSolve._profs(op, op32, cpl)
+        x = FdfdCoupledSolve._solve(op, op32, opts, m0, m1, b * c, False, p64, p32)
+        ctx.op, ctx.op32, ctx.opts = op, op32, opts
+        ctx.p64, ctx.p32 = p64, p32
+        ctx.save_for_backward(m0, m1, x)
+        ctx.save_for_forward(m0, m1, x)
+        ctx.has_m1 = m1 is not None
+        return x
+
+    @staticmethod
+    def jvp(ctx, dm0, dm1, dcpl, db, *_):
+        m0, m1, x = ctx.saved_tensors
+        op = ctx.op
+        c = op.scale_vector(torch.float64)
+        rhs = torch.zeros_like(x) if db is None else (db.to(x.dtype) * c)
+        if dm0 is not None or dm1 is not None:
+            d0 = torch.zeros_like(m0) if dm0 is None else dm0.to(m0.dtype).contiguous()
+            d1 = None
+            if ctx.has_m1:
+                d1 = torch.zeros_like(m0) if dm1 is None else dm1.to(m0.dtype).contiguous()
+            rhs = rhs - op.material_apply(d0, d1, x)
+        if dcpl is not None:
+            rhs = rhs - op.coupling_apply(op.prof_with(dcpl, torch.complex128), x)
+        return FdfdCoupledSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False, ctx.p64, ctx.p32)
+
+    @staticmethod
+    def backward(ctx, gx):
+        m0, m1, x = ctx.saved_tensors
+        op = ctx.op
+        mu = FdfdCoupledSolve._solve(op, ctx.op32, ctx.opts, m0, m1, torch.conj(gx).resolve_conj().contiguous(),
+                                     True, ctx.p64, ctx.p32)
+        c = op.scale_vector(torch.float64)
+        gb = torch.conj(mu).resolve_conj() * c if ctx.needs_input_grad[3] else None
+        g0 = g1 = None
+        if ctx.needs_input_grad[0] or (ctx.has_m1 and ctx.needs_input_grad[1]):
+            g0 = torch.zeros_like(m0)
+            g1 = torch.zeros_like(m0) if ctx.has_m1 else None
+            op.material_grad(mu.contiguous(), x.contiguous(), g0, g1, -1.0)
+        gc = None
+        if ctx.needs_input_grad[2]:
+            gc = torch.conj(op.coupling_grad(ctx.p64, mu, x, -1.0)).resolve_conj()
+        return g0, g1, gc, gb, None, None, None
 
 
 class FdfdDeriv(torch.autograd.Function):

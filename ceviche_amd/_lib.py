@@ -46,8 +46,10 @@ DESC_DTYPE = np.dtype([
     ('scale', '<f8'), ('dshift', '<f8'),
     ('phx_re', '<f8'), ('phx_im', '<f8'), ('phy_re', '<f8'), ('phy_im', '<f8'),
     ('omega', '<f8'),
+    ('n_cpl', '<i4'), ('blk', '<i4'), ('nblk', '<i4'), ('pad', '<i4'),
+    ('cpl_off', '<i8'), ('w2_off', '<i8'),
 ])
-assert DESC_DTYPE.itemsize == 96
+assert DESC_DTYPE.itemsize == 128
 TILE_DTYPE = np.dtype([('inst', '<i4'), ('first', '<i4')])
 
 
@@ -105,6 +107,8 @@ _SIGS = {
     'cv_fdfd_material_grad': (I, [I, P, P, I, P, P, P, P, P, D, P]),
     'cv_fdfd_material_apply': (I, [I, P, P, I, P, P, P, P, P, I, P]),
     'cv_fdfd_hz_materials_jvp': (I, [I, P, P, I, P, P, P, P, P, P]),
+    'cv_fdfd_coupling_apply': (I, [I, P, P, I, P, P, P, I, I, P]),
+    'cv_fdfd_coupling_grad': (I, [I, P, P, I, P, P, P, P, D, P]),
     'cv_fdfd_solve_workspace_bytes': (I64, [I, I64, I, I]),
     'cv_fdfd_solve': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
     'cv_fdfd_solve_mixed_workspace_bytes': (I64, [I64, I, I]),

@@ -644,6 +644,7 @@ extern "C" int cv_fdfd_batch_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc*
     for (int k = 0; k < n_inst; ++k) {
         CV_CHECK((int64_t)desc_host[k].nx * desc_host[k].ny == n, "instances must have equal sizes");
         CV_CHECK(desc_host[k].pol == CV_POL_EZ || m1, "Hz operator needs m1");
+        CV_CHECK(desc_host[k].n_cpl == 0, "frequency-coupled systems are not supported");
     }
     k_batch_apply<<<grid_of(B * n), kBlock, 0, as_stream(stream)>>>(
         desc, (const cx<float>*)prof, (const float*)m0, (const float*)m1, inst, (const cx<float>*)x,

@@ -241,6 +241,62 @@ k_hz_materials_jvp(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restr
     }
 }
 
+// ------------------------------------------------ frequency coupling
+// y (+)= C x (or C^T x): only the inter-block terms of coupled systems, with
+// the tables read from `tab` (prof's layout).  Instances without coupling
+// write zero (or leave y unchanged when accumulating).
+template <typename R, bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_coupling_apply(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+                 const cx<R>* __restrict__ tab, const cx<R>* __restrict__ x, cx<R>* __restrict__ y, int accumulate) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int n = d.nx * d.ny;
+    auto fx = [&](int64_t k) { return x[k]; };
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> v = (d.n_cpl > 0) ? coupling_row<R, TRANS>(d, tab, i, j, fx) : czero<R>();
+        y[k] = accumulate ? y[k] + v : v;
+    }
+}
+
+// g[table(h, +/-)][c] += scale * sum_m w2[m] lam_m[c] x_{m +/- (h+1)}[c], one
+// thread per cell of block 0 of every coupled system (other tiles return).
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_coupling_grad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+                const cx<R>* __restrict__ prof, const cx<R>* __restrict__ lam, const cx<R>* __restrict__ x,
+                cx<R>* __restrict__ g, R scale) {
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    if (d.blk != 0 || d.n_cpl == 0) return;
+    const int n = d.nx * d.ny;
+    const cx<R>* w2 = prof + d.w2_off;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int64_t k0 = d.cell_off + loc;
+        for (int h = 0; h < d.n_cpl; ++h) {
+            const int dd = h + 1;
+            cx<R> up = czero<R>(), dn = czero<R>();
+            for (int m = 0; m < d.nblk; ++m) {
+                const cx<R> lm = w2[m].re * lam[k0 + (int64_t)m * n];
+                if (m + dd < d.nblk) up = up + lm * x[k0 + (int64_t)(m + dd) * n];
+                if (m - dd >= 0) dn = dn + lm * x[k0 + (int64_t)(m - dd) * n];
+            }
+            cx<R>* gp = g + d.cpl_off + (int64_t)(2 * h) * n + loc;
+            cx<R>* gm = g + d.cpl_off + (int64_t)(2 * h + 1) * n + loc;
+            *gp = *gp + scale * up;
+            *gm = *gm + scale * dn;
+        }
+    }
+}
+
 }  // namespace cv
 
 using namespace cv;
@@ -403,6 +459,55 @@ extern "C" int cv_fdfd_hz_materials_jvp(int dtype, const cv_fdfd_desc* desc, con
     else if (dtype == CV_C64)
         k_hz_materials_jvp<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const float*)m0, (const float*)m1,
                                                              (const float*)d_eps, (float*)dm0, (float*)dm1);
+    else
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_coupling_apply(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                                      const void* tab, const void* x, void* y, int transpose, int accumulate,
+                                      void* stream) {
+    CV_CHECK(desc && tiles && tab && x && y, "null pointer argument");
+    CV_CHECK(n_tiles >= 0, "n_tiles < 0");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128) {
+        if (transpose)
+            k_coupling_apply<double, true><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)tab,
+                                                                     (const cx<double>*)x, (cx<double>*)y, accumulate);
+        else
+            k_coupling_apply<double, false><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)tab,
+                                                                      (const cx<double>*)x, (cx<double>*)y,
+                                                                      accumulate);
+    } else if (dtype == CV_C64) {
+        if (transpose)
+            k_coupling_apply<float, true><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)tab,
+                                                                    (const cx<float>*)x, (cx<float>*)y, accumulate);
+        else
+            k_coupling_apply<float, false><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)tab,
+                                                                     (const cx<float>*)x, (cx<float>*)y, accumulate);
+    } else {
+        CV_CHECK(false, "unknown dtype %d", dtype);
+    }
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_coupling_grad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                                     const void* prof, const void* lam, const void* x, void* g, double scale,
+                                     void* stream) {
+    CV_CHECK(desc && tiles && prof && lam && x && g, "null pointer argument");
+    CV_CHECK(n_tiles >= 0, "n_tiles < 0");
+    if (n_tiles == 0) return CV_OK;
+    hipStream_t s = as_stream(stream);
+    if (dtype == CV_C128)
+        k_coupling_grad<double><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<double>*)prof,
+                                                           (const cx<double>*)lam, (const cx<double>*)x,
+                                                           (cx<double>*)g, scale);
+    else if (dtype == CV_C64)
+        k_coupling_grad<float><<<n_tiles, kBlock, 0, s>>>(desc, tiles, (const cx<float>*)prof, (const cx<float>*)lam,
+                                                          (const cx<float>*)x, (cx<float>*)g, (float)scale);
     else
         CV_CHECK(false, "unknown dtype %d", dtype);
     CV_LAUNCH_CHECK();

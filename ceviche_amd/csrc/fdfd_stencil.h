@@ -130,6 +130,36 @@ __host__ __device__ __forceinline__ Row5<R> row_coefs(const cv_fdfd_desc& d, con
     return r;
 }
 
+// Inter-block terms of a frequency-coupled system at row (i, j) of this
+// block (A, or A^T when TRANS); see cv_fdfd_desc.  Row block m of A couples to
+// block m + (h+1) with w2[m] gp_h and to block m - (h+1) with w2[m] gm_h, so
+// row m of A^T takes w2[m-(h+1)] gp_h from block m - (h+1) and
+// w2[m+(h+1)] gm_h from block m + (h+1).
+template <typename R, bool TRANS, typename Fetch>
+__host__ __device__ __forceinline__ cx<R> coupling_row(const cv_fdfd_desc& d, const cx<R>* __restrict__ tab,
+                                              int i, int j, Fetch f) {
+    const int64_t n = (int64_t)d.nx * d.ny;
+    const int64_t loc = (int64_t)i * d.ny + j;
+    const int64_t k = d.cell_off + loc;
+    const cx<R>* g = tab + d.cpl_off;
+    const cx<R>* w2 = tab + d.w2_off;
+    const R w_self = w2[d.blk].re;
+    cx<R> acc = czero<R>();
+    for (int h = 0; h < d.n_cpl; ++h) {
+        const int dd = h + 1;
+        const int up = d.blk + dd, dn = d.blk - dd;
+        if (up < d.nblk) {
+            const cx<R> c = TRANS ? g[(int64_t)(2 * h + 1) * n + loc] : g[(int64_t)(2 * h) * n + loc];
+            acc = acc + (TRANS ? w2[up].re : w_self) * (c * f(k + dd * n));
+        }
+        if (dn >= 0) {
+            const cx<R> c = TRANS ? g[(int64_t)(2 * h) * n + loc] : g[(int64_t)(2 * h + 1) * n + loc];
+            acc = acc + (TRANS ? w2[dn].re : w_self) * (c * f(k - dd * n));
+        }
+    }
+    return acc;
+}
+
 // (A u)[i,j] with u fetched through `f(k)` (global cell index k).
 template <typename R, typename Fetch>
 __host__ __device__ __forceinline__ cx<R> apply_row(const cv_fdfd_desc& d, const cx<R>* __restrict__ prof,
@@ -151,6 +181,7 @@ __host__ __device__ __forceinline__ cx<R> apply_row(const cv_fdfd_desc& d, const
         acc = acc + r.yp * f(base + (int64_t)i * ny + jp);
         acc = acc + r.ym * f(base + (int64_t)i * ny + jm);
     }
+    if (d.n_cpl > 0) acc = acc + coupling_row<R, false>(d, prof, i, j, f);
     return acc;
 }
 
@@ -180,6 +211,7 @@ __host__ __device__ __forceinline__ cx<R> apply_row_T(const cv_fdfd_desc& d, con
         acc = acc + rm.yp * f(base + (int64_t)i * ny + jm);
         acc = acc + rp.ym * f(base + (int64_t)i * ny + jp);
     }
+    if (d.n_cpl > 0) acc = acc + coupling_row<R, true>(d, prof, i, j, f);
     return acc;
 }
 
@@ -255,5 +287,11 @@ __host__ __device__ __forceinline__ cx<R> deriv_at(const cv_fdfd_desc& d, const 
 __host__ __device__ __forceinline__ cv_fdfd_desc load_desc(const cv_fdfd_desc* __restrict__ desc, int inst) {
     return desc[inst];
 }
+
+// Index of the instance that holds the Krylov state of `inst`'s linear system
+// (block 0 of a frequency-coupled system, the instance itself otherwise).
+__host__ __device__ __forceinline__ int system_of(const cv_fdfd_desc& d, int inst) { return inst - d.blk; }
+
+static_assert(sizeof(cv_fdfd_desc) == 128, "cv_fdfd_desc layout");
 
 }  // namespace cv

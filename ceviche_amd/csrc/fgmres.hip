@@ -237,6 +237,7 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
     const cv_fdfd_desc dh = desc_host[0];
     CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
     CV_CHECK(dh.pol == CV_POL_EZ || (m1_64 && m1_32), "Hz operator needs m1");
+    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported by FGMRES");
     CV_CHECK(restart >= 1 && restart <= kFgMaxM, "restart must be in [1, %d]", kFgMaxM);
     CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
     CV_CHECK(precond == CV_PRECOND_NONE || precond == CV_PRECOND_JACOBI || precond == CV_PRECOND_UNET,

@@ -59,7 +59,8 @@ k_init(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
     __shared__ double lds[kWaves * 2];
     const cv_tile t = tiles[blockIdx.x];
     const cv_fdfd_desc d = load_desc(desc, t.inst);
-    KState* st = state + t.inst;
+    const int sys = system_of(d, t.inst);
+    KState* st = state + sys;
     const int done0 = (mode == 0) ? kRunning : st->done;
     if (mode == 1 && done0 == kRunning) return;
     const bool write_r = (mode == 0) || (done0 == kConverged);
@@ -83,7 +84,7 @@ k_init(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
     }
     block_sum<2>(acc, lds);
     double tot[2];
-    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + sys, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -122,9 +123,10 @@ k_bicg_p(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
          RedBuf rb, KState* __restrict__ state) {
     __shared__ double lds[kWaves * 2];
     const cv_tile t = tiles[blockIdx.x];
-    KState* st = state + t.inst;
-    if (st->done != kRunning) return;
     const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int sys = system_of(d, t.inst);
+    KState* st = state + sys;
+    if (st->done != kRunning) return;
     const bool restart = st->restart != 0;
     const cx<R> beta = ccast<R>(st->beta), omega = ccast<R>(st->omega);
     const int n = d.nx * d.ny;
@@ -156,7 +158,7 @@ k_bicg_p(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     }
     block_sum<2>(acc, lds);
     double tot[2];
-    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + sys, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -182,9 +184,10 @@ k_bicg_s(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
          RedBuf rb, KState* __restrict__ state) {
     __shared__ double lds[kWaves * 3];
     const cv_tile t = tiles[blockIdx.x];
-    KState* st = state + t.inst;
-    if (st->done != kRunning) return;
     const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int sys = system_of(d, t.inst);
+    KState* st = state + sys;
+    if (st->done != kRunning) return;
     const cx<R> alpha = ccast<R>(st->alpha);
     const int n = d.nx * d.ny;
     auto fs = [&](int64_t k) -> cx<R> { return r[k] - alpha * v[k]; };
@@ -204,7 +207,7 @@ k_bicg_s(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     }
     block_sum<3>(acc, lds);
     double tot[3];
-    if (!last_block_reduce<3>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<3>(acc, rb.partials, rb.gpartials, rb.counters + sys, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -224,9 +227,10 @@ k_bicg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
          RedBuf rb, KState* __restrict__ state) {
     __shared__ double lds[kWaves * 3];
     const cv_tile t = tiles[blockIdx.x];
-    KState* st = state + t.inst;
-    if (st->done != kRunning) return;
     const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int sys = system_of(d, t.inst);
+    KState* st = state + sys;
+    if (st->done != kRunning) return;
     const cx<R> alpha = ccast<R>(st->alpha), omega = ccast<R>(st->omega);
     const int n = d.nx * d.ny;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -246,7 +250,7 @@ k_bicg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     }
     block_sum<3>(acc, lds);
     double tot[3];
-    if (!last_block_reduce<3>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<3>(acc, rb.partials, rb.gpartials, rb.counters + sys, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -286,9 +290,9 @@ k_mixed_rhs(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ t
             const cx<double>* __restrict__ r64, cx<float>* __restrict__ b32, cx<float>* __restrict__ x32,
             KState* __restrict__ outer) {
     const cv_tile t = tiles[blockIdx.x];
-    KState* st = outer + t.inst;
-    const bool active = st->done == kRunning;
     const cv_fdfd_desc d = load_desc(desc, t.inst);
+    KState* st = outer + system_of(d, t.inst);
+    const bool active = st->done == kRunning;
     const int n = d.nx * d.ny;
     const double sc = active ? 1.0 / sqrt(st->rr_true) : 0.0;
 #pragma unroll
@@ -308,9 +312,9 @@ k_mixed_update(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict_
                cx<double>* __restrict__ x64, const cx<float>* __restrict__ x32,
                const KState* __restrict__ outer) {
     const cv_tile t = tiles[blockIdx.x];
-    const KState* st = outer + t.inst;
-    if (st->done != kRunning) return;
     const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const KState* st = outer + system_of(d, t.inst);
+    if (st->done != kRunning) return;
     const int n = d.nx * d.ny;
     const double sc = sqrt(st->rr_true);
 #pragma unroll
@@ -335,9 +339,10 @@ k_mixed_resid(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__
               KState* __restrict__ outer, double tol, int first) {
     __shared__ double lds[kWaves * 2];
     const cv_tile t = tiles[blockIdx.x];
-    KState* st = outer + t.inst;
-    if (!first && st->done != kRunning) return;
     const cv_fdfd_desc d = load_desc(desc, t.inst);
+    const int sys = system_of(d, t.inst);
+    KState* st = outer + sys;
+    if (!first && st->done != kRunning) return;
     const int n = d.nx * d.ny;
     auto fx = [&](int64_t k) { return x[k]; };
     double acc[2] = {0.0, 0.0};
@@ -355,7 +360,7 @@ k_mixed_resid(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__
     }
     block_sum<2>(acc, lds);
     double tot[2];
-    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
+    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + sys, rb.gcount, blockIdx.x, d.tile_first, d.tile_count,
                               lds, tot))
         return;
     if (threadIdx.x != 0) return;
@@ -454,7 +459,6 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
     cx<R>* p[2] = {(cx<R>*)w.p[0], (cx<R>*)w.p[1]};
     cx<R>* v[2] = {(cx<R>*)w.v[0], (cx<R>*)w.v[1]};
     cx<R>* tv = (cx<R>*)w.t;
-    (void)desc_host;
     k_zero_u32<<<(n_inst + n_tiles + 255) / 256, 256, 0, s>>>(w.rb.counters, n_inst + n_tiles);
     // p and v start at zero (they are only read when restart == 0, which never
     // happens before the first K1), so no clear is needed.
@@ -487,7 +491,8 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
         CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
         CV_HIP(hipStreamSynchronize(s));
         bool all_done = true;
-        for (int k = 0; k < n_inst; ++k) all_done &= (hs[k].done != kRunning);
+        for (int k = 0; k < n_inst; ++k)
+            if (desc_host[k].blk == 0) all_done &= (hs[k].done != kRunning);
         if (!all_done) continue;
         // verify converged instances with the true residual (restarts them if needed)
         k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.rb,
@@ -496,7 +501,8 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
         CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
         CV_HIP(hipStreamSynchronize(s));
         bool again = false;
-        for (int k = 0; k < n_inst; ++k) again |= (hs[k].done == kRunning);
+        for (int k = 0; k < n_inst; ++k)
+            if (desc_host[k].blk == 0) again |= (hs[k].done == kRunning);
         if (!again || ++checks >= max_checks) break;
     }
     return CV_OK;
@@ -512,14 +518,47 @@ static int status_of(const KState& k) {
     }
 }
 
-static void fill_info(const std::vector<KState>& hs, cv_solve_info* info, int n_inst, int* worst) {
+// Checks the batch layout the kernels rely on; returns the number of packed
+// cells in *cells.  Blocks of a frequency-coupled system must be consecutive
+// instances with equal grids, consecutive cells and the system's tile range,
+// so that every inter-block access stays inside the system's cells.
+static int check_batch(const cv_fdfd_desc* dh, int n_inst, int n_tiles, int64_t* cells) {
+    int64_t total = 0;
+    for (int k = 0; k < n_inst; ++k) {
+        const cv_fdfd_desc& d = dh[k];
+        CV_CHECK(d.nx > 0 && d.ny > 0, "instance %d has an empty grid", k);
+        CV_CHECK(d.tile_first >= 0 && d.tile_count > 0 && d.tile_first + d.tile_count <= n_tiles,
+                 "instance %d: tile range out of the table", k);
+        total = std::max<int64_t>(total, d.cell_off + (int64_t)d.nx * d.ny);
+        CV_CHECK(d.nblk >= 1 && d.blk >= 0 && d.blk < d.nblk && d.n_cpl >= 0, "instance %d: bad block fields", k);
+        if (d.nblk == 1) {
+            CV_CHECK(d.n_cpl == 0, "instance %d: coupling tables on a single-block system", k);
+            continue;
+        }
+        CV_CHECK(d.pol == CV_POL_EZ, "instance %d: coupled blocks must be Ez", k);
+        const int root = k - d.blk;
+        CV_CHECK(root >= 0 && root + d.nblk <= n_inst, "instance %d: coupled blocks out of the batch", k);
+        const cv_fdfd_desc& r = dh[root];
+        const int64_t n = (int64_t)r.nx * r.ny;
+        CV_CHECK(d.nx == r.nx && d.ny == r.ny && d.nblk == r.nblk && d.n_cpl == r.n_cpl &&
+                     d.cell_off == r.cell_off + d.blk * n && d.tile_first == r.tile_first &&
+                     d.tile_count == r.tile_count && d.cpl_off == r.cpl_off && d.w2_off == r.w2_off,
+                 "instance %d: block %d does not match block 0 of its system", k, d.blk);
+    }
+    *cells = total;
+    return CV_OK;
+}
+
+static void fill_info(const cv_fdfd_desc* dh, const std::vector<KState>& hs, cv_solve_info* info, int n_inst,
+                      int* worst) {
     *worst = CV_OK;
     for (int k = 0; k < n_inst; ++k) {
-        info[k].iterations = hs[k].iter;
-        info[k].status = status_of(hs[k]);
-        info[k].restarts = hs[k].restarts;
+        const KState& h = hs[k - dh[k].blk];
+        info[k].iterations = h.iter;
+        info[k].status = status_of(h);
+        info[k].restarts = h.restarts;
         info[k].pad = 0;
-        info[k].rel_residual = hs[k].bb > 0 ? sqrt(hs[k].rr_true / hs[k].bb) : 0.0;
+        info[k].rel_residual = h.bb > 0 ? sqrt(h.rr_true / h.bb) : 0.0;
         if (info[k].status != CV_OK && *worst == CV_OK) *worst = info[k].status;
     }
 }
@@ -542,11 +581,12 @@ extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_
     CV_CHECK(dtype == CV_C64 || dtype == CV_C128, "unknown dtype %d", dtype);
     CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
     int64_t cells = 0;
-    for (int k = 0; k < n_inst; ++k) {
-        CV_CHECK(desc_host[k].nx > 0 && desc_host[k].ny > 0, "instance %d has an empty grid", k);
-        cells = std::max<int64_t>(cells, desc_host[k].cell_off + (int64_t)desc_host[k].nx * desc_host[k].ny);
-        if (desc_host[k].pol == CV_POL_HZ) CV_CHECK(m1 != nullptr, "Hz operator needs m1");
+    {
+        const int rc = check_batch(desc_host, n_inst, n_tiles, &cells);
+        if (rc != CV_OK) return rc;
     }
+    for (int k = 0; k < n_inst; ++k)
+        if (desc_host[k].pol == CV_POL_HZ) CV_CHECK(m1 != nullptr, "Hz operator needs m1");
     const int64_t cs = (dtype == CV_C128) ? 16 : 8;
     Work w;
     const int64_t need = layout(cs, cells, n_inst, n_tiles, (char*)workspace, &w);
@@ -569,7 +609,7 @@ extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_
                                                   max_iter, check_every, w, hs, s);
     if (rc != CV_OK) return rc;
     int worst;
-    fill_info(hs, info, n_inst, &worst);
+    fill_info(desc_host, hs, info, n_inst, &worst);
     return worst;
 }
 
@@ -592,8 +632,10 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
              "null pointer argument");
     CV_CHECK(n_inst > 0 && n_tiles > 0, "empty batch");
     int64_t cells = 0;
-    for (int k = 0; k < n_inst; ++k)
-        cells = std::max<int64_t>(cells, desc_host[k].cell_off + (int64_t)desc_host[k].nx * desc_host[k].ny);
+    {
+        const int rc = check_batch(desc_host, n_inst, n_tiles, &cells);
+        if (rc != CV_OK) return rc;
+    }
     const int64_t need = cv_fdfd_solve_mixed_workspace_bytes(cells, n_inst, n_tiles);
     CV_CHECK(workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
              (long long)need);
@@ -635,7 +677,8 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
     int inner_status = CV_OK;
     for (int sweep = 0; sweep < max_sweeps; ++sweep) {
         bool any = false;
-        for (int k = 0; k < n_inst; ++k) any |= (ho[k].done == kRunning);
+        for (int k = 0; k < n_inst; ++k)
+            if (desc_host[k].blk == 0) any |= (ho[k].done == kRunning);
         if (!any) break;
         k_mixed_rhs<<<n_tiles, kBlock, 0, s>>>(desc, tiles, r64, b32, x32, outer);
         CV_LAUNCH_CHECK();
@@ -649,10 +692,11 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
                                           inner_tol, max_iter, check_every, w, hi, s);
         if (rc != CV_OK) return rc;
         for (int k = 0; k < n_inst; ++k) {
-            if (ho[k].done == kRunning) {
-                iters[k] += hi[k].iter;
+            const int sys = k - desc_host[k].blk;
+            if (ho[sys].done == kRunning) {
+                iters[k] += hi[sys].iter;
                 sweeps[k] += 1;
-                if (hi[k].done == kNonFinite || hi[k].done == kBreakdown) inner_status = status_of(hi[k]);
+                if (hi[sys].done == kNonFinite || hi[sys].done == kBreakdown) inner_status = status_of(hi[sys]);
             }
         }
         k_mixed_update<<<n_tiles, kBlock, 0, s>>>(desc, tiles, (cx<double>*)x64, x32, outer);
@@ -662,13 +706,14 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
     }
     int worst = CV_OK;
     for (int k = 0; k < n_inst; ++k) {
+        const KState& h = ho[k - desc_host[k].blk];
         info[k].iterations = iters[k];
         info[k].restarts = sweeps[k];
         info[k].pad = 0;
-        info[k].rel_residual = ho[k].bb > 0 ? sqrt(ho[k].rr_true / ho[k].bb) : 0.0;
-        info[k].status = (ho[k].done == kConverged) ? CV_OK
-                         : (ho[k].done == kNonFinite) ? CV_ERR_NONFINITE
-                                                      : CV_ERR_NOT_CONVERGED;
+        info[k].rel_residual = h.bb > 0 ? sqrt(h.rr_true / h.bb) : 0.0;
+        info[k].status = (h.done == kConverged) ? CV_OK
+                         : (h.done == kNonFinite) ? CV_ERR_NONFINITE
+                                                  : CV_ERR_NOT_CONVERGED;
         if (info[k].status != CV_OK && worst == CV_OK) worst = info[k].status;
     }
     if (worst == CV_OK && inner_status != CV_OK) worst = CV_OK;  // outer residual is the judge

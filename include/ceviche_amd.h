@@ -46,7 +46,7 @@ extern "C" {
 #define CV_FLAG_X_IDENTITY 1   /* Nx == 1: Dx* is the identity (derivatives.py:76-77) */
 #define CV_FLAG_Y_IDENTITY 2   /* Ny == 1: Dy* is the identity (derivatives.py:78-79) */
 
-/* One FDFD instance of a batch.  Plain data, 96 bytes, device-resident array.
+/* One FDFD instance of a batch.  Plain data, 128 bytes, device-resident array.
  *   pol == CV_POL_EZ:  A = scale*(Dxf Dxb + Dyf Dyb) + diag(dshift*(m0 + i m1))
  *                      scale = -1/mu0, dshift = -eps0 w^2, m0 = Re eps,
  *                      m1 = Im eps or NULL for real eps           (fdfd.py:172-185)
@@ -55,16 +55,29 @@ extern "C" {
  *                      m0 = 1/(eps_xx+1e-5), m1 = 1/(eps_yy+1e-5)      (fdfd.py:211-233)
  * Profiles at prof_off: kxf[nx], kxb[nx], kyf[ny], kyb[ny] (complex, operator
  * precision) holding 1/(s*dL), or 1/s on an identity axis (derivatives.py:34-59).
- * Materials at cell_off in m0 (and m1 for Hz), real, operator precision. */
+ * Materials at cell_off in m0 (and m1 for Hz), real, operator precision.
+ *
+ * Frequency-coupled systems (the multi-frequency Ez operator, fdfd.py:284-324):
+ * the nblk blocks of one linear system are consecutive instances (blk = 0 ..
+ * nblk-1) with equal grids and consecutive cells; each has its own dshift
+ * (-eps0 W_blk^2) and shares the Krylov state and reductions of its block 0
+ * (tile_first / tile_count then span all blocks' tiles).  Row block m adds
+ *   sum_{h < n_cpl} w2[m] * ( gp_h .* x_{m+h+1} + gm_h .* x_{m-h-1} )
+ * (terms whose block falls outside 0 .. nblk-1 dropped), with the complex
+ * tables at prof[cpl_off]: gp_0[n], gm_0[n], gp_1[n], gm_1[n], ... (n = nx*ny)
+ * and w2 at prof[w2_off + m] (real part used).  An ordinary instance has
+ * n_cpl = 0, blk = 0, nblk = 1. */
 typedef struct cv_fdfd_desc {
     int32_t nx, ny;
     int32_t pol, flags;
     int64_t cell_off;
     int64_t prof_off;
-    int32_t tile_first, tile_count;  /* this instance's range in the tile table   */
+    int32_t tile_first, tile_count;  /* tile range of this instance's linear system */
     double  scale, dshift;
     double  phx_re, phx_im, phy_re, phy_im;   /* Bloch phasors exp(i*theta)       */
     double  omega;
+    int32_t n_cpl, blk, nblk, pad;
+    int64_t cpl_off, w2_off;
 } cv_fdfd_desc;
 
 /* Tile table entry: one workgroup processes CV_TILE_CELLS consecutive cells of
@@ -136,6 +149,26 @@ int cv_fdfd_hz_materials_jvp(int dtype, const cv_fdfd_desc* desc, const cv_tile*
                              const void* m0, const void* m1, const void* d_eps, void* dm0, void* dm1,
                              void* stream);
 
+/* Coupling part of frequency-coupled systems alone, with the tables read from
+ * `tab` (same offsets as prof; e.g. a tangent table):
+ *   y (+)= C x  (transpose == 0)  or  C^T x  (transpose != 0)
+ * where C holds only the inter-block terms described at cv_fdfd_desc.  Used for
+ * the forward-mode rule of the multi-frequency solve (the coupling slice of
+ * the defjvp of primitives.py:137-148; entries from fdfd.py:303-324). */
+int cv_fdfd_coupling_apply(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                           const void* tab, const void* x, void* y, int transpose, int accumulate,
+                           void* stream);
+
+/* Gradient of lam^T A x with respect to the coupling tables (complex, no
+ * conjugation): for every system (its block-0 instance) and cell,
+ *   g[cpl_off + 2h n + c]   += scale * sum_m w2[m] lam_m[c] x_{m+h+1}[c]
+ *   g[cpl_off + (2h+1) n + c] += scale * sum_m w2[m] lam_m[c] x_{m-h-1}[c]
+ * g has prof's layout.  The reverse-mode gather adj[i]*x[j] of
+ * primitives.py:121-128 restricted to the coupling entries of fdfd.py:303-324. */
+int cv_fdfd_coupling_grad(int dtype, const cv_fdfd_desc* desc, const cv_tile* tiles, int n_tiles,
+                          const void* prof, const void* lam, const void* x, void* g, double scale,
+                          void* stream);
+
 /* ------------------------------------------------------------- Krylov solve */
 #define CV_KRYLOV_BICGSTAB 0
 
@@ -158,6 +191,8 @@ int64_t cv_fdfd_solve_workspace_bytes(int dtype, int64_t total_cells, int n_inst
  * in complex128 (x64, b64 given as complex128 buffers) and each sweep solves a
  * complex64 correction (mixed-precision iterative refinement).
  * Replaces solvers.py:37-75 (spsolve / scipy Krylov) behind primitives.py:103-119.
+ * A frequency-coupled system (nblk > 1) is one solve over all its blocks; the
+ * info entry of every block reports that solve.
  * info: host array of n_inst entries. */
 int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
                   const cv_tile* tiles, int n_tiles,

@@ -1,4 +1,4 @@
-"""Shared seeded configurations of the parity tests (the BASELINE configs)."""
+"""Shared seeded configurations of the parity tests (the benchmark workloads)."""
 import numpy as np
 
 from oracle.fdfd_ref import C_0

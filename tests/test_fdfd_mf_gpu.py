@@ -1,0 +1,194 @@
+"""GPU parity of the multi-frequency Ez FDFD (fdfd_mf_ez: coupled-block stencil
++ one BiCGSTAB over all sidebands + adjoints) against the scipy oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fdfd_mf_ref as MF
+from oracle import fdfd_ref as R
+
+pytestmark = pytest.mark.gpu
+
+OMEGA = 2 * np.pi * 200e12
+OMEGA_MOD = 2 * np.pi * 2e12
+
+
+def _problem(seed, shape, Nfreq, Nsb):
+    rng = np.random.default_rng(seed)
+    eps = rng.random(shape) + 1
+    delta = rng.random((Nfreq,) + shape)
+    phi = 2 * np.pi * rng.random((Nfreq,) + shape)
+    src = np.zeros((2 * Nsb + 1,) + shape)
+    src[Nsb, shape[0] // 2, shape[1] // 2] = 1.0
+    return eps, delta, phi, src
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize('shape,Nfreq,Nsb,npml,bloch', [
+    ((30, 30), 1, 1, [10, 10], (0.0, 0.0)),
+    ((24, 17), 2, 2, [4, 3], (0.0, 0.0)),
+    ((19, 26), 2, 1, [0, 5], (0.6, 0.0)),
+    ((21, 14), 3, 1, [3, 3], (0.0, -0.4)),     # harmonic 3 couples nothing with 3 sidebands
+    ((16, 16), 1, 0, [4, 4], (0.0, 0.0)),      # one sideband: plain Ez
+])
+def test_fields_match_oracle(shape, Nfreq, Nsb, npml, bloch):
+    import ceviche_amd as cv
+    print('mf fields', shape, Nfreq, Nsb, npml, bloch, flush=True)
+    eps, delta, phi, src = _problem(7, shape, Nfreq, Nsb)
+    # a source on every sideband exercises every block's right-hand side
+    src = src + 0.3 * np.random.default_rng(8).standard_normal(src.shape)
+    F = cv.fdfd_mf_ez(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, Nsb, npml, bloch_phases=list(bloch),
+                      solver_options=cv.SolverOptions(tol=1e-12))
+    hx, hy, ez = F.solve(src)
+    rhx, rhy, rez = MF.solve_mf(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, Nsb, npml, src, bloch)
+    assert ez.shape == (2 * Nsb + 1,) + shape and isinstance(ez, np.ndarray)
+    info = F.last_solve_info
+    assert info['status'] == 0 and info['rel_residual'] <= 1e-12
+    for a, b in ((ez, rez), (hx, rhx), (hy, rhy)):
+        assert _rel(a, b) <= 1e-8
+
+
+def test_one_sideband_equals_single_frequency_solve():
+    import ceviche_amd as cv
+    eps, delta, phi, src = _problem(9, (20, 22), 1, 0)
+    a = cv.fdfd_mf_ez(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, 0, [5, 5]).solve(src)
+    b = cv.fdfd_ez(OMEGA, 1e-6, eps, [5, 5]).solve(src[0])
+    for u, v in zip(a, b):
+        assert _rel(u[0], v) <= 1e-8
+
+
+def test_coupled_operator_matches_assembled_matrix():
+    """The stencil + coupling apply (A and A^T) against the assembled operator."""
+    from ceviche_amd._fdfd_op import FdfdOperator
+    from ceviche_amd.constants import EPSILON_0, MU_0
+    shape, Nsb, Nfreq, npml, dL = (13, 11), 2, 2, [3, 2], 1e-6
+    M, N = 2 * Nsb + 1, 13 * 11
+    eps, delta, phi, _ = _problem(4, shape, Nfreq, Nsb)
+    W = MF.sidebands(OMEGA, OMEGA_MOD, Nsb)
+    specs = [dict(omega=OMEGA, dL=dL, shape=shape, npml=npml, shift_omega=W[m], system=(m, M, Nfreq))
+             for m in range(M)]
+    A = MF.make_A_mf(OMEGA, dL, eps, OMEGA_MOD, delta, phi, Nsb, npml)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(M * N) + 1j * rng.standard_normal(M * N)
+    for dtype, tol in ((torch.complex128, 1e-13), (torch.complex64, 2e-6)):
+        op = FdfdOperator('ez', specs, dtype=dtype)
+        c = MU_0 * dL ** 2
+        kappa = -0.5 * EPSILON_0 * c * OMEGA ** 2
+        d = torch.from_numpy(delta.reshape(Nfreq, N)).to(torch.complex128)
+        p = torch.from_numpy(phi.reshape(Nfreq, N)).to(torch.complex128)
+        cpl = torch.stack([kappa * d * torch.exp(1j * p), kappa * d * torch.exp(-1j * p)], 1).reshape(-1).cuda()
+        prof = op.prof_with(cpl)
+        m0, m1 = op.materials(torch.from_numpy(np.tile(eps.ravel(), M)))
+        for tr in (False, True):
+            y = op.apply(torch.from_numpy(x).cuda(), m0, m1, transpose=tr, prof=prof).cpu().numpy()
+            ref = c * ((A.T if tr else A) @ x)
+            assert np.linalg.norm(y - ref) <= tol * np.linalg.norm(ref), (dtype, tr)
+            # the coupling-only kernel is A minus its diagonal blocks
+            yc = op.coupling_apply(prof, torch.from_numpy(x).cuda().to(dtype), transpose=tr).cpu().numpy()
+            Ab = A.tolil()
+            for m in range(M):
+                Ab[m * N:(m + 1) * N, m * N:(m + 1) * N] = 0
+            refc = c * ((Ab.T if tr else Ab).tocsr() @ x)
+            assert np.linalg.norm(yc - refc) <= tol * np.linalg.norm(refc), (dtype, tr)
+
+
+def test_complex64_refined_solve():
+    import ceviche_amd as cv
+    eps, delta, phi, src = _problem(12, (28, 24), 1, 1)
+    F = cv.fdfd_mf_ez(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, 1, [6, 6], precision='complex64',
+                      solver_options=cv.SolverOptions(tol=1e-9))
+    _, _, ez = F.solve(src)
+    _, _, rez = MF.solve_mf(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, 1, [6, 6], src)
+    assert F.last_solve_info['rel_residual'] <= 1e-9
+    assert _rel(ez, rez) <= 1e-6
+
+
+def _objective_np(params, shape, Nfreq, Nsb, npml, src):
+    N = shape[0] * shape[1]
+    eps = params[:N].reshape(shape)
+    delta = params[N:(Nfreq + 1) * N].reshape((Nfreq,) + shape)
+    phi = params[(Nfreq + 1) * N:].reshape((Nfreq,) + shape)
+    s = (eps + np.sum(delta * np.exp(1j * phi), axis=0)) * src
+    hx, hy, ez = MF.solve_mf(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, Nsb, npml, s)
+    return np.sum(np.abs(ez) ** 2) + np.sum(np.abs(hx) ** 2) + np.sum(np.abs(hy) ** 2)
+
+
+@pytest.mark.parametrize('Nfreq,Nsb', [(1, 1), (2, 2)])
+def test_reverse_gradient_matches_central_differences(Nfreq, Nsb):
+    """ceviche tests/test_gradients_fdfd_mf.py:74-103: the gradient of
+    sum |Ez|^2 + |Hx|^2 + |Hy|^2 with respect to (eps_r, delta, phi), the
+    source scaled by eps_r + sum delta exp(i phi); checked along random
+    directions against central differences of the oracle (ratio <= 1e-4)."""
+    import ceviche_amd as cv
+    shape, npml = (30, 30), [10, 10]
+    N = shape[0] * shape[1]
+    eps, delta, phi, src = _problem(21, shape, Nfreq, Nsb)
+    params = np.concatenate([eps.ravel(), delta.ravel(), phi.ravel()])
+    F = cv.fdfd_mf_ez(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, Nsb, npml,
+                      solver_options=cv.SolverOptions(tol=1e-12))
+    src_t = torch.from_numpy(src).cuda()
+
+    def J(p):
+        e = p[:N].reshape(shape)
+        d = p[N:(Nfreq + 1) * N].reshape((Nfreq,) + shape)
+        f = p[(Nfreq + 1) * N:].reshape((Nfreq,) + shape)
+        F.eps_r, F.delta, F.phi = e, d, f
+        hx, hy, ez = F.solve((e + torch.sum(d * torch.exp(1j * f), 0)) * src_t)
+        return (ez.abs() ** 2).sum() + (hx.abs() ** 2).sum() + (hy.abs() ** 2).sum()
+
+    p = torch.from_numpy(params).cuda().requires_grad_(True)
+    val = J(p)
+    val.backward()
+    g = p.grad.cpu().numpy()
+    assert abs(val.item() - _objective_np(params, shape, Nfreq, Nsb, npml, src)) <= 1e-9 * abs(val.item())
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    for k in range(3):
+        v = rng.standard_normal(params.size)
+        v /= np.linalg.norm(v)
+        num = (_objective_np(params + h * v, shape, Nfreq, Nsb, npml, src)
+               - _objective_np(params - h * v, shape, Nfreq, Nsb, npml, src)) / (2 * h)
+        assert abs(g @ v - num) <= 1e-4 * abs(num), (k, g @ v, num)
+
+
+def test_forward_mode_matches_central_differences():
+    """ceviche tests/test_gradients_fdfd_mf.py:105-130: jacobian(mode='forward')
+    of |Ez|^2 + |Hx|^2 + |Hy|^2 (every cell of every sideband) with respect to
+    c, where eps_r, delta, phi and the source all scale with c."""
+    import ceviche_amd as cv
+    shape, npml, Nfreq, Nsb = (30, 30), [10, 10], 1, 1
+    eps, delta, phi, src = _problem(33, shape, Nfreq, Nsb)
+    F = cv.fdfd_mf_ez(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, Nsb, npml,
+                      solver_options=cv.SolverOptions(tol=1e-12))
+    e_t, d_t, p_t, s_t = (torch.from_numpy(a) for a in (eps, delta, phi, src))
+
+    def J(c):
+        c = torch.as_tensor(c, dtype=torch.float64)
+        F.eps_r, F.delta, F.phi = c * e_t, c * d_t, c * p_t
+        hx, hy, ez = F.solve(c * e_t * s_t)
+        return (ez.abs() ** 2 + hx.abs() ** 2 + hy.abs() ** 2).reshape(-1)
+
+    jf = cv.jacobian(J, mode='forward')(1.0)
+
+    def Jr(c):
+        hx, hy, ez = MF.solve_mf(OMEGA, 1e-6, c * eps, OMEGA_MOD, c * delta, c * phi, Nsb, npml, c * eps * src)
+        return (np.abs(ez) ** 2 + np.abs(hx) ** 2 + np.abs(hy) ** 2).ravel()
+
+    h = 1e-6
+    num = ((Jr(1 + h) - Jr(1 - h)) / (2 * h))[:, None]
+    assert jf.shape == num.shape
+    assert np.linalg.norm(jf - num) <= 1e-4 * np.linalg.norm(num)
+
+
+def test_source_shape_is_checked():
+    import ceviche_amd as cv
+    eps, delta, phi, src = _problem(1, (10, 12), 1, 1)
+    F = cv.fdfd_mf_ez(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, 1, [2, 2])
+    with pytest.raises(ValueError):
+        F.solve(src[0])
+    with pytest.raises(ValueError):
+        F.delta = delta[:, :5]
+        F.solve(src)
