@@ -304,6 +304,48 @@ def extras_all(main, which='all'):
                 "krylov_algorithmic_GBs": round(gbs, 1), "krylov_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                 "precision": "complex128 BiCGSTAB (288 B/cell/iteration), FP64 reductions, no preconditioner"}
 
+    def fdfd_mf():
+        """Multi-frequency Ez (3 coupled sidebands) on the 1100 x 80
+        modulated-waveguide isolator, mode source on the centre sideband:
+        the GPU solve (complex128 BiCGSTAB, FP64 residual 1e-10) against the
+        oracle's spsolve of the assembled system (the reference's default
+        direct path, one core)."""
+        from oracle import fdfd_mf_ref
+        from ceviche_amd.modes import insert_mode
+        c = workloads.fdfd_mf_isolator()
+        nx, ny = c['eps'].shape
+        M = 2 * c['Nsb'] + 1
+        mode = insert_mode(c['omega'], c['dL'], np.array(c['src_x']), c['src_y'], c['eps'], m=1)
+        src = np.zeros((M, nx, ny), dtype=complex)
+        src[c['Nsb']] = mode
+        F = cv.fdfd_mf_ez(c['omega'], c['dL'], c['eps'], c['omega_mod'], c['delta'], c['phi'], c['Nsb'],
+                          c['npml'])
+        F.solve(src)
+        _sync()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hx, hy, ez = F.solve(src)
+        dt = (time.perf_counter() - t0) / reps
+        info = F.last_solve_info
+        log('  mf: GPU %.3f s, %d iterations; spsolve' % (dt, info['iterations']))
+        t = time.perf_counter()
+        _, _, rez = fdfd_mf_ref.solve_mf(c['omega'], c['dL'], c['eps'], c['omega_mod'], c['delta'], c['phi'],
+                                         c['Nsb'], c['npml'], src)
+        cpu = time.perf_counter() - t
+        err = float(np.linalg.norm(ez - rez) / np.linalg.norm(rez))
+        # power in each sideband at the output cross section (the isolator's figure of merit)
+        xo = nx - c['npml'][0] - 1
+        p = [float(np.sum(np.abs(ez[m, xo, :]) ** 2)) for m in range(M)]
+        cells = M * nx * ny
+        return {"grid": [nx, ny], "sidebands": M, "unknowns": cells, "solves_per_s": round(1 / dt, 3),
+                "seconds": round(dt, 4), "iterations": info['iterations'],
+                "us_per_iteration": round(1e6 * dt / max(1, info['iterations']), 1),
+                "fp64_rel_residual": info['rel_residual'], "field_rel_err_vs_spsolve": err,
+                "output_power_by_sideband": p,
+                "cpu_spsolve_s": round(cpu, 3), "cpu_cores": 1, "speedup_vs_cpu": round(cpu / dt, 2),
+                "precision": "complex128 BiCGSTAB over all sidebands, coupling fused into the stencil kernels"}
+
     def fdfd_batch():
         """Multi-frequency batch (4 blob designs x 4 wavelengths, sizes from
         {200, 300, 400, 600}^2) solved in one batched launch sequence, in
@@ -455,7 +497,7 @@ def extras_all(main, which='all'):
     table = [('fdtd_tm_400_trace', trace_errors), ('fdtd_tm_400_adjoint', adjoint), ('fdtd_tm_8192', tm_hbm),
              ('fdtd_3d_256_adjoint', adjoint_3d),
              ('fdtd_3d_256', fdtd_3d), ('fdfd_ez_120', fdfd_cfg1), ('fdfd_ez_1024_plain', fdfd_1024),
-             ('fdfd_ez_batch', fdfd_batch), ('fdfd_learned_512', fdfd_learned), ('sp_mult_1M', sp_mult_1m),
+             ('fdfd_mf_isolator', fdfd_mf), ('fdfd_ez_batch', fdfd_batch), ('fdfd_learned_512', fdfd_learned), ('sp_mult_1M', sp_mult_1m),
              ('stream_copy', copy_probe)]
     for name, fn in table:
         if which == 'all' or name in which:

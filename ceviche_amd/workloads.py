@@ -16,6 +16,13 @@ fdfd_blob_batch      batch of 2D Ez FDFD instances, sizes drawn from
                      evenly spaced over 1.50-1.60 um; eps = waveguide along x
                      plus a central binary-blob design (sigma 3 cells);
                      Jz line source across the input waveguide.
+fdfd_mf_isolator     multi-frequency Ez FDFD, the modulated-waveguide isolator
+                     of ceviche's examples/fdfd_mf_intro.ipynb: 1100x80 cells,
+                     dL 50 nm, npml 10, omega 243e12 rad/s, omega_mod 132e12
+                     rad/s, 3 sidebands (Nsb 1), one modulation harmonic;
+                     eps 12.25 waveguides (22 cells) joined by a 40-cell-wide
+                     centre section; two 380-cell modulated strips on the
+                     upper half of the guide, phases 0 and pi/2, depth 1.
 fdtd_3d_blobs        3D FDTD, N^3 (256 by default), dL 25 nm, npml 10;
                      background eps 4.41 with a centred (N/4)^3 design region
                      of seeded binary blobs {1, 12.25}; z-dipole Gaussian
@@ -115,3 +122,25 @@ def fdfd_learned_blobs(N=256, seed=0, lam=None):
     src = np.zeros((N, N))
     src[30, c - 8:c + 8] = 1.0
     return dict(omega=2 * np.pi * C_0 / lam, dL=40e-9, eps=eps, npml=[20, 20], src=src, lam=lam)
+
+
+def fdfd_mf_isolator():
+    """Geometry of the modulated-waveguide isolator example (sizes in cells
+    from its lengths: L 55 um, W 4 um, waveguide 1.1 um, centre 2 um x
+    4.75 um, modulation 19 um, at dL 50 nm).  The source is the fundamental
+    mode on the input cross section (x = npml + 1, y over 3 waveguide widths),
+    on the centre sideband; build it with modes.insert_mode."""
+    nx, ny, npml = 1100, 80, 10
+    eps = np.ones((nx, ny))
+    l2, r1 = nx // 2 - 47, nx // 2 + 47
+    top, bot = ny // 2 + 11, ny // 2 - 11
+    eps[:l2, bot:top] = 12.25
+    eps[r1:, bot:top] = 12.25
+    eps[l2:r1, ny // 2 - 20:ny // 2 + 20] = 12.25
+    delta = np.zeros((1, nx, ny))
+    phi = np.zeros((1, nx, ny))
+    delta[:, l2 - 380:l2, ny // 2:top] = 1.0
+    delta[:, r1:r1 + 380, ny // 2:top] = 1.0
+    phi[:, r1:r1 + 380, ny // 2:top] = np.pi / 2
+    return dict(omega=243e12, omega_mod=132e12, Nsb=1, dL=50e-9, npml=[npml, npml], eps=eps, delta=delta,
+                phi=phi, src_x=npml + 1, src_y=np.arange(ny // 2 - 33, ny // 2 + 33))
