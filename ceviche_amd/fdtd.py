@@ -20,6 +20,7 @@ import torch
 
 from . import _lib
 from ._fdtd_op import COMPONENTS, FdtdEFields, FdtdGrid, FdtdRun, FdtdStep, _probe_tensors, autotune_tm, time_step
+from ._probes import ProbeSums, ProbeWeights
 
 _FIELD_ORDER = ('Ex', 'Ey', 'Ez', 'Dx', 'Dy', 'Dz', 'Hx', 'Hy', 'Hz')
 
@@ -116,13 +117,24 @@ class fdtd:
 
     def forward(self, Jx=None, Jy=None, Jz=None):
         """One time step of FDTD (fdtd.py:74-144); returns the 9 fields."""
+        self._advance(Jx, Jy, Jz)
+        return self.fields
+
+    def _advance(self, Jx=None, Jy=None, Jz=None):
+        """forward() without building the returned field dictionary."""
         self._grid.desc.dL = self.dL  # curls use the current dL, like the reference
         Js = [self._prep_J(J) for J in (Jx, Jy, Jz)]
         if Js[0] is not None or Js[1] is not None:
             self._te_live = True
         self.t_index += 1
         self._state = FdtdStep.apply(self._state, self._eps_dev, Js[0], Js[1], Js[2], self._grid)
-        return self.fields
+
+    def _device_field(self, name):
+        """One field component as a device tensor of the grid shape."""
+        c = COMPONENTS[name]
+        if c < 3:
+            return FdtdEFields.apply(self._state, self._eps_dev, self._grid)[c]
+        return self._state[c if c < 6 else c - 6]
 
     @property
     def fields(self):
@@ -150,6 +162,13 @@ class fdtd:
         that is differentiable with respect to eps_r (torch) through a
         checkpointed adjoint; advances t_index and the internal state.
 
+        A probe position may also be a weight array of the grid shape (a
+        line, plane or mode profile): the probe then records
+        sum(field * weights), the quantity measure_fields records
+        (ceviche/utils.py:316-332).  The cells of all weighted probes are
+        recorded in-kernel and combined by one weighted-sum kernel after the
+        run.
+
         kernel: 'auto' uses the temporally blocked 2D TM kernel when it applies
         (nz == 1, Jz source, TE fields zero, Ez/Hx/Hy probes, <= 256 probes;
         tile shape from the autotuner) and the two-phase per-step kernels
@@ -166,6 +185,10 @@ class fdtd:
         amp = amplitudes if isinstance(amplitudes, torch.Tensor) else torch.as_tensor(np.asarray(amplitudes, float))
         if amp.numel() < steps:
             raise ValueError('need one amplitude per step')
+        weights = None
+        if any(not isinstance(p[1], (tuple, list)) for p in probes):
+            weights = ProbeWeights(probes, g.shape, g.device)
+            probes = weights.points
         pt = _probe_tensors(g, probes)
         tm_ok = (g.nz == 1 and comp == 2 and not self._te_live and len(probes) <= 256
                  and all(COMPONENTS[p[0]] in (2, 6, 7) for p in probes))
@@ -183,6 +206,8 @@ class fdtd:
                                    int(checkpoint_every), cfg)
         self._state = state
         self.t_index += int(steps)
+        if weights is not None:
+            rec = ProbeSums.apply(rec, weights)
         return rec
 
 

@@ -263,6 +263,17 @@ int cv_fdtd_efields_tangent(int dtype, const cv_fdtd_desc* d, const void* sig, c
 int cv_fdtd_efields(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
                     const void* state, void* E, void* stream);
 
+/* Weighted probe sums over T field snapshots (ceviche/utils.py:316-332: the
+ * sum(fields[component] * probe) of measure_fields, per probe and step):
+ *   out[t*out_stride + p] (+)= sum_{k = rowptr[p]}^{rowptr[p+1]-1} w[k] * field[t*field_stride + col[k]]
+ * for p < n_rows, t < T; the probe weights are a CSR matrix over the field's
+ * cells (col < field_stride is the caller's contract).  field and out are
+ * float (CV_F32) or double (CV_F64); accumulation is FP64.  Used once per step
+ * by measure_fields and once per run for weighted probes recorded in-kernel. */
+int cv_probe_weights(int dtype, const void* field, int64_t field_stride, int T, const int64_t* rowptr,
+                     const int64_t* col, const double* w, int n_rows, void* out, int64_t out_stride,
+                     int accumulate, void* stream);
+
 /* One reverse-time step of the exact discrete adjoint (no reference
  * counterpart: ceviche differentiates FDTD through autograd's tape).
  * lam: 18-array adjoint state after step t (updated in place to before step t);
