@@ -26,14 +26,23 @@ _CODE_OF = {torch.complex128: _lib.CV_C128, torch.complex64: _lib.CV_C64}
 
 
 class SolverOptions:
-    """Krylov settings.  tol is the relative FP64 residual ||b - A x|| / ||b||."""
+    """Solver settings.  tol is the relative FP64 residual ||b - A x|| / ||b||.
 
-    def __init__(self, tol=None, max_iter=200000, check_every=32, inner_tol=1e-4, max_sweeps=12):
+    method: 'auto' (the block-tridiagonal factorisation + FGMRES for a single
+    instance with ny <= 128, BiCGSTAB otherwise), 'direct' (require the
+    former) or 'bicgstab'.  restart: FGMRES cycle length of the direct path."""
+
+    def __init__(self, tol=None, max_iter=200000, check_every=32, inner_tol=1e-4, max_sweeps=12, method='auto',
+                 restart=30):
+        if method not in ('auto', 'direct', 'bicgstab'):
+            raise ValueError("method must be 'auto', 'direct' or 'bicgstab'")
         self.tol = tol
         self.max_iter = max_iter
         self.check_every = check_every
         self.inner_tol = inner_tol
         self.max_sweeps = max_sweeps
+        self.method = method
+        self.restart = restart
 
 
 DEFAULT_TOL = {torch.complex128: 1e-10, torch.complex64: 1e-6}
@@ -328,6 +337,78 @@ class FdfdOperator:
         return x
 
 
+BTD_MAX_NY = 128
+
+
+def direct_applies(op):
+    """True when the block-tridiagonal factorisation handles ``op``: one
+    uncoupled instance whose y extent fits a workgroup."""
+    return op.n_inst == 1 and op.cpl_len == 0 and op.shapes[0][1] <= BTD_MAX_NY
+
+
+class Direct:
+    """Block-tridiagonal factorisation (complex64 Schur-complement inverses
+    along x) of one FDFD instance, and FP64-refined FGMRES solves with it in
+    either orientation (csrc/btd.hip, cv_fdfd_fgmres_btd).  One factorisation
+    serves the forward, adjoint (A^T) and tangent solves of a linear system."""
+
+    def __init__(self, op, m0, m1):
+        if not direct_applies(op):
+            raise ValueError('the block-tridiagonal factorisation needs one uncoupled instance with ny <= %d'
+                             % BTD_MAX_NY)
+        self.op = op
+        nx, ny = op.shapes[0]
+        self.prof32 = op.prof.to(torch.complex64).contiguous()
+        self.m0 = m0.to(torch.float32).contiguous()
+        self.m1 = None if m1 is None else m1.to(torch.float32).contiguous()
+        L = lib()
+        nbytes = L.cv_fdfd_btd_factor_bytes(nx, ny)
+        self.factor = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
+        check(L.cv_fdfd_btd_factor(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(self.prof32),
+                                   ptr(self.m0), ptr(self.m1), ptr(self.factor), nbytes, stream_ptr()),
+              'block-tridiagonal factorisation')
+
+    def apply(self, v, transpose=False):
+        """M^-1 v (complex64) with the factorisation alone."""
+        v = v.to(torch.complex64).contiguous()
+        z = torch.empty_like(v)
+        op = self.op
+        check(lib().cv_fdfd_btd_apply(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(self.prof32),
+                                      ptr(self.m0), ptr(self.m1), ptr(self.factor), ptr(v), ptr(z),
+                                      int(bool(transpose)), stream_ptr()), 'btd_apply')
+        return z
+
+    def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
+        """(c A) x = bs (or (c A)^T x = bs) to the FP64 tolerance."""
+        op = self.op
+        opts = opts or SolverOptions()
+        tol = opts.tol if opts.tol is not None else DEFAULT_TOL[torch.complex128]
+        nx, ny = op.shapes[0]
+        L = lib()
+        need = L.cv_fdfd_fgmres_workspace_bytes(nx, ny, int(opts.restart), _lib.CV_PRECOND_NONE, 0)
+        ws = torch.empty(need, dtype=torch.uint8, device=op.device)
+        b = bs.to(torch.complex128).contiguous()
+        x = torch.zeros_like(b)
+        info = (_lib.SolveInfo * 1)()
+        rc = L.cv_fdfd_fgmres_btd(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(op.prof64),
+                                  ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1),
+                                  ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)), float(tol),
+                                  int(opts.restart), int(opts.max_iter), ptr(ws), need, info, stream_ptr())
+        op.last_info = [dict(iterations=info[0].iterations, status=info[0].status, restarts=info[0].restarts,
+                             rel_residual=info[0].rel_residual, method='direct')]
+        check(rc, 'fdfd direct solve')
+        return x
+
+
+def _use_direct(op, opts):
+    method = (opts or SolverOptions()).method
+    if method == 'bicgstab':
+        return False
+    if method == 'direct' and not direct_applies(op):
+        raise ValueError("method='direct' needs one uncoupled instance with ny <= %d" % BTD_MAX_NY)
+    return direct_applies(op)
+
+
 # ---------------------------------------------------------------- autograd
 class FdfdSolve(torch.autograd.Function):
     """x = A(m)^-1 b for the FDFD operator (sp_solve, primitives.py:103-137).
@@ -339,7 +420,9 @@ class FdfdSolve(torch.autograd.Function):
     of primitives.py:121-135), grad_b = lam, grad_m = -Re(mu^T dA/dm x)."""
 
     @staticmethod
-    def _solve(op, op32, opts, m0, m1, bs, transpose):
+    def _solve(op, op32, opts, m0, m1, bs, transpose, direct=None):
+        if direct is not None:
+            return direct.solve(bs, m0, m1, transpose=transpose, opts=opts)
         if op32 is None:
             return op.solve_scaled(bs, m0, m1, transpose=transpose, opts=opts)
         m0s = m0.to(torch.float32).contiguous()
@@ -349,7 +432,12 @@ class FdfdSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, m0, m1, b, op, op32, opts):
         c = op.scale_vector(torch.float64)
-        x = FdfdSolve._solve(op, op32, opts, m0, m1, b * c, False)
+        # one factorisation serves the forward, adjoint and tangent solves
+        direct = Direct(op, m0, m1) if _use_direct(op, opts) else None
+        x = FdfdSolve._solve(op, op32, opts, m0, m1, b * c, False, direct)
+        if direct is not None and op32 is not None:
+            op32.last_info = op.last_info
+        ctx.direct = direct
         ctx.op, ctx.op32, ctx.opts = op, op32, opts
         ctx.save_for_backward(m0, m1, x)
         ctx.save_for_forward(m0, m1, x)
@@ -369,13 +457,14 @@ class FdfdSolve(torch.autograd.Function):
             if ctx.has_m1:
                 d1 = torch.zeros_like(m0) if dm1 is None else dm1.to(m0.dtype).contiguous()
             rhs = rhs - op.material_apply(d0, d1, x)
-        return FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False)
+        return FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False, ctx.direct)
 
     @staticmethod
     def backward(ctx, gx):
         m0, m1, x = ctx.saved_tensors
         op = ctx.op
-        mu = FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, torch.conj(gx).resolve_conj().contiguous(), True)
+        mu = FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, torch.conj(gx).resolve_conj().contiguous(), True,
+                              ctx.direct)
         c = op.scale_vector(torch.float64)
         gb = torch.conj(mu).resolve_conj() * c if ctx.needs_input_grad[2] else None
         g0 = g1 = None

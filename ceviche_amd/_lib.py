@@ -109,6 +109,10 @@ _SIGS = {
     'cv_fdfd_hz_materials_jvp': (I, [I, P, P, I, P, P, P, P, P, P]),
     'cv_fdfd_coupling_apply': (I, [I, P, P, I, P, P, P, I, I, P]),
     'cv_probe_weights': (I, [I, P, I64, I, P, P, P, I, P, I64, I, P]),
+    'cv_fdfd_btd_factor_bytes': (I64, [I, I]),
+    'cv_fdfd_btd_factor': (I, [P, P, P, P, P, P, I64, P]),
+    'cv_fdfd_btd_apply': (I, [P, P, P, P, P, P, P, P, I, P]),
+    'cv_fdfd_fgmres_btd': (I, [P, P, P, P, P, P, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
     'cv_fdfd_coupling_grad': (I, [I, P, P, I, P, P, P, P, D, P]),
     'cv_fdfd_solve_workspace_bytes': (I64, [I, I64, I, I]),
     'cv_fdfd_solve': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),

@@ -33,7 +33,8 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// r = b - A64 x (row-major single instance), partial[wg] = {|r|^2, |b|^2}
+// r = b - A64 x (or A64^T x; row-major single instance), partial[wg] = {|r|^2, |b|^2}
+template <bool TRANS>
 __global__ void __launch_bounds__(kBlock)
 k_fg_resid(const cv_fdfd_desc* __restrict__ desc, const cx<double>* __restrict__ prof, const double* __restrict__ m0,
            const double* __restrict__ m1, const cx<double>* __restrict__ b, const cx<double>* __restrict__ x,
@@ -49,7 +50,7 @@ k_fg_resid(const cv_fdfd_desc* __restrict__ desc, const cx<double>* __restrict__
         if (k >= n) break;
         const int i = (int)(k / d.ny), j = (int)(k - (int64_t)i * d.ny);
         const cx<double> bk = b[k];
-        const cx<double> rk = bk - apply_any<double, false>(d, prof, m0, m1, i, j, fx, x[k]);
+        const cx<double> rk = bk - apply_any<double, TRANS>(d, prof, m0, m1, i, j, fx, x[k]);
         r[k] = rk;
         acc[0] += norm2(rk);
         acc[1] += norm2(bk);
@@ -83,7 +84,8 @@ k_fg_start(const cx<double>* __restrict__ r, cx<float>* __restrict__ v, double i
     v[k] = mk<float>((float)(a.re * inv), (float)(a.im * inv));
 }
 
-// w = A32 z
+// w = A32 z (or A32^T z)
+template <bool TRANS>
 __global__ void __launch_bounds__(kBlock)
 k_fg_apply(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
            const float* __restrict__ m1, const cx<float>* __restrict__ z, cx<float>* __restrict__ w) {
@@ -93,7 +95,7 @@ k_fg_apply(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ 
     if (k >= n) return;
     const int i = (int)(k / d.ny), j = (int)(k - (int64_t)i * d.ny);
     auto fz = [&](int64_t q) { return z[q]; };
-    w[k] = apply_any<float, false>(d, prof, m0, m1, i, j, fz, z[k]);
+    w[k] = apply_any<float, TRANS>(d, prof, m0, m1, i, j, fz, z[k]);
 }
 
 // z = v / diag(A32)
@@ -227,32 +229,16 @@ extern "C" int64_t cv_fdfd_fgmres_workspace_bytes(int nx, int ny, int restart, i
     return fg_layout((int64_t)nx * ny, restart, net, nullptr, nullptr);
 }
 
-extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
-                              const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
-                              const void* m1_32, const void* b, void* x, double tol, int restart, int max_iter,
-                              double inner_rtol, int precond, const cv_unet* net, float net_gain, void* workspace,
-                              int64_t workspace_bytes, cv_solve_info* info, void* stream) {
-    CV_CHECK(desc && desc_host && prof64 && m0_64 && prof32 && m0_32 && b && x && workspace && info,
-             "null pointer argument");
-    const cv_fdfd_desc dh = desc_host[0];
-    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
-    CV_CHECK(dh.pol == CV_POL_EZ || (m1_64 && m1_32), "Hz operator needs m1");
-    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported by FGMRES");
-    CV_CHECK(restart >= 1 && restart <= kFgMaxM, "restart must be in [1, %d]", kFgMaxM);
-    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
-    CV_CHECK(precond == CV_PRECOND_NONE || precond == CV_PRECOND_JACOBI || precond == CV_PRECOND_UNET,
-             "unknown preconditioner %d", precond);
-    CV_CHECK(precond != CV_PRECOND_UNET || net, "network preconditioner needs a network");
-    const int m = restart;
+namespace cv {
+
+// The restarted flexible GMRES loop shared by every preconditioner.
+// precondition(v, z) writes z ~ M^-1 v; after_cycle(rr, rr_prev) may change it.
+template <bool TRANS, typename Precond, typename AfterCycle>
+static int fgmres_core(const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const void* prof64, const void* m0_64,
+                       const void* m1_64, const void* prof32, const void* m0_32, const void* m1_32, const void* b,
+                       void* x, double tol, int m, int max_iter, double inner_rtol, FgWork& w, Precond&& precondition,
+                       AfterCycle&& after_cycle, cv_solve_info* info, hipStream_t s) {
     const int64_t n = (int64_t)dh.nx * dh.ny;
-    const int levels = (precond == CV_PRECOND_UNET) ? net->levels : 0;
-    const int64_t need = cv_fdfd_fgmres_workspace_bytes(dh.nx, dh.ny, m, precond, levels);
-    CV_CHECK(need > 0 && workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
-             (long long)need);
-    const int64_t net_bytes = (precond == CV_PRECOND_UNET) ? cv_unet_workspace_bytes(levels, 1, dh.nx, dh.ny) : 0;
-    FgWork w;
-    fg_layout(n, m, net_bytes, (char*)workspace, &w);
-    hipStream_t s = as_stream(stream);
     const int n_wg = (int)((n + kFgCells - 1) / kFgCells);
     const int gv = (int)((n + kBlock - 1) / kBlock);
     const cx<double>* b64 = (const cx<double>*)b;
@@ -260,11 +246,9 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
     double* h1 = w.h;                       // first Gram-Schmidt pass, then the sum
     double* h2 = w.h + 2 * (m + 1);         // second pass
     double* nrm = w.h + 4 * (m + 1);        // |w|^2 at [0]; |r|^2, |b|^2 at [2], [3]
-    const float in_scale = (float)sqrt((double)n);     // unit-norm v -> unit rms
-
     auto true_resid = [&](double* rr, double* bb) -> int {
-        k_fg_resid<<<n_wg, kBlock, 0, s>>>(desc, (const cx<double>*)prof64, (const double*)m0_64,
-                                           (const double*)m1_64, b64, x64, w.r, w.partial);
+        k_fg_resid<TRANS><<<n_wg, kBlock, 0, s>>>(desc, (const cx<double>*)prof64, (const double*)m0_64,
+                                                  (const double*)m1_64, b64, x64, w.r, w.partial);
         k_fg_reduce<<<2, kBlock, 0, s>>>(w.partial, n_wg, 2, nrm + 2, 0);
         CV_LAUNCH_CHECK();
         double hv[2];
@@ -274,24 +258,6 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
         *bb = hv[1];
         return CV_OK;
     };
-    // The network is used while its restart cycles keep reducing the true
-    // residual; three cycles in a row that reduce it by less than 1 % switch
-    // the rest of the solve to Jacobi (info.pad = 1 reports the fallback).
-    int use = precond, stalls = 0, fell_back = 0;
-    auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
-        if (use == CV_PRECOND_NONE) {
-            CV_HIP(hipMemcpyAsync(z, v, 8 * n, hipMemcpyDeviceToDevice, s));
-        } else if (use == CV_PRECOND_JACOBI) {
-            k_fg_jacobi<<<gv, kBlock, 0, s>>>(desc, (const cx<float>*)prof32, (const float*)m0_32,
-                                              (const float*)m1_32, v, z);
-            CV_LAUNCH_CHECK();
-        } else {
-            return cv_unet_apply(net, CV_C64, v, in_scale, net_gain / in_scale, z, dh.nx, dh.ny, w.net_ws, net_bytes,
-                                 stream);
-        }
-        return CV_OK;
-    };
-
     double rr = 0, bb = 0;
     int rc = true_resid(&rr, &bb);
     if (rc != CV_OK) return rc;
@@ -316,7 +282,7 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
             cx<float>* zj = w.Z + (int64_t)j * n;
             cx<float>* wv = w.V + (int64_t)(j + 1) * n;
             if ((rc = precondition(vj, zj)) != CV_OK) return rc;
-            k_fg_apply<<<gv, kBlock, 0, s>>>(desc, (const cx<float>*)prof32, (const float*)m0_32,
+            k_fg_apply<TRANS><<<gv, kBlock, 0, s>>>(desc, (const cx<float>*)prof32, (const float*)m0_32,
                                              (const float*)m1_32, zj, wv);
             // two passes of classical Gram-Schmidt against V_0 .. V_j
             k_fg_mdot<<<n_wg, kBlock, 0, s>>>(w.V, j + 1, wv, n, w.partial);
@@ -375,6 +341,65 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
         CV_LAUNCH_CHECK();
         const double rr_prev = rr;
         if ((rc = true_resid(&rr, &bb)) != CV_OK) return rc;
+        after_cycle(rr, rr_prev);
+    }
+    info[0].iterations = iters;
+    info[0].restarts = cycles;
+    info[0].rel_residual = bb > 0 ? sqrt(rr / bb) : 0.0;
+    info[0].status = status;
+    return status;
+}
+
+}  // namespace cv
+
+extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                              const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                              const void* m1_32, const void* b, void* x, double tol, int restart, int max_iter,
+                              double inner_rtol, int precond, const cv_unet* net, float net_gain, void* workspace,
+                              int64_t workspace_bytes, cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && prof64 && m0_64 && prof32 && m0_32 && b && x && workspace && info,
+             "null pointer argument");
+    const cv_fdfd_desc dh = desc_host[0];
+    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
+    CV_CHECK(dh.pol == CV_POL_EZ || (m1_64 && m1_32), "Hz operator needs m1");
+    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported by FGMRES");
+    CV_CHECK(restart >= 1 && restart <= kFgMaxM, "restart must be in [1, %d]", kFgMaxM);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    CV_CHECK(precond == CV_PRECOND_NONE || precond == CV_PRECOND_JACOBI || precond == CV_PRECOND_UNET,
+             "unknown preconditioner %d", precond);
+    CV_CHECK(precond != CV_PRECOND_UNET || net, "network preconditioner needs a network");
+    const int m = restart;
+    const int64_t n = (int64_t)dh.nx * dh.ny;
+    const int levels = (precond == CV_PRECOND_UNET) ? net->levels : 0;
+    const int64_t need = cv_fdfd_fgmres_workspace_bytes(dh.nx, dh.ny, m, precond, levels);
+    CV_CHECK(need > 0 && workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+             (long long)need);
+    const int64_t net_bytes = (precond == CV_PRECOND_UNET) ? cv_unet_workspace_bytes(levels, 1, dh.nx, dh.ny) : 0;
+    FgWork w;
+    fg_layout(n, m, net_bytes, (char*)workspace, &w);
+    hipStream_t s = as_stream(stream);
+    const int gv = (int)((n + kBlock - 1) / kBlock);
+    const float in_scale = (float)sqrt((double)n);     // unit-norm v -> unit rms
+
+    // The network is used while its restart cycles keep reducing the true
+    // residual; three cycles in a row that reduce it by less than 1 % switch
+    // the rest of the solve to Jacobi (info.pad = 1 reports the fallback).
+    int use = precond, stalls = 0, fell_back = 0;
+    auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
+        if (use == CV_PRECOND_NONE) {
+            CV_HIP(hipMemcpyAsync(z, v, 8 * n, hipMemcpyDeviceToDevice, s));
+        } else if (use == CV_PRECOND_JACOBI) {
+            k_fg_jacobi<<<gv, kBlock, 0, s>>>(desc, (const cx<float>*)prof32, (const float*)m0_32,
+                                              (const float*)m1_32, v, z);
+            CV_LAUNCH_CHECK();
+        } else {
+            return cv_unet_apply(net, CV_C64, v, in_scale, net_gain / in_scale, z, dh.nx, dh.ny, w.net_ws, net_bytes,
+                                 stream);
+        }
+        return CV_OK;
+    };
+
+    auto after_cycle = [&](double rr, double rr_prev) {
         if (use == CV_PRECOND_UNET) {
             stalls = (rr > 0.9801 * rr_prev) ? stalls + 1 : 0;
             if (stalls >= 3) {
@@ -382,11 +407,50 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
                 fell_back = 1;
             }
         }
-    }
-    info[0].iterations = iters;
-    info[0].restarts = cycles;
+    };
+    const int status = fgmres_core<false>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter,
+                                          inner_rtol, w, precondition, after_cycle, info, s);
     info[0].pad = fell_back;
-    info[0].rel_residual = bb > 0 ? sqrt(rr / bb) : 0.0;
-    info[0].status = status;
+    return status;
+}
+
+namespace cv {
+int btd_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const void* prof32, const void* m0_32,
+              const void* m1_32, const void* factor, const void* v, void* z, int transpose, hipStream_t s);
+}
+
+extern "C" int cv_fdfd_fgmres_btd(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                                  const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                                  const void* m1_32, const void* factor, const void* b, void* x, int transpose,
+                                  double tol, int restart, int max_iter, void* workspace, int64_t workspace_bytes,
+                                  cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && prof64 && m0_64 && prof32 && m0_32 && factor && b && x && workspace && info,
+             "null pointer argument");
+    const cv_fdfd_desc dh = desc_host[0];
+    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
+    CV_CHECK(dh.ny <= 128, "the block-tridiagonal preconditioner needs ny <= 128");
+    CV_CHECK(dh.pol == CV_POL_EZ || (m1_64 && m1_32), "Hz operator needs m1");
+    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported by FGMRES");
+    CV_CHECK(restart >= 1 && restart <= kFgMaxM, "restart must be in [1, %d]", kFgMaxM);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    const int m = restart;
+    const int64_t need = cv_fdfd_fgmres_workspace_bytes(dh.nx, dh.ny, m, CV_PRECOND_NONE, 0);
+    CV_CHECK(need > 0 && workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+             (long long)need);
+    FgWork w;
+    fg_layout((int64_t)dh.nx * dh.ny, m, 0, (char*)workspace, &w);
+    hipStream_t s = as_stream(stream);
+    auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
+        return btd_apply(desc, dh, prof32, m0_32, m1_32, factor, v, z, transpose, s);
+    };
+    auto after_cycle = [](double, double) {};
+    int status;
+    if (transpose)
+        status = fgmres_core<true>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
+                                   w, precondition, after_cycle, info, s);
+    else
+        status = fgmres_core<false>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
+                                    w, precondition, after_cycle, info, s);
+    info[0].pad = 0;
     return status;
 }

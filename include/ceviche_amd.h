@@ -424,6 +424,33 @@ int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, cons
                    const cv_unet* net, float net_gain, void* workspace, int64_t workspace_bytes,
                    cv_solve_info* info, void* stream);
 
+/* Block-tridiagonal direct factorisation of one 2D FDFD instance with
+ * ny <= 128 (the reference's SuperLU solve, ceviche/solvers.py:50-62, done as
+ * block Gaussian elimination along x): A without its periodic x-wrap entries
+ * is block tridiagonal with diagonal off-diagonal blocks; the dense inverses
+ * of the Schur complements S_i = D_i - diag(lo_i) S_{i-1}^-1 diag(up_{i-1})
+ * are stored in complex64 (nx * ny^2 entries, cv_fdfd_btd_factor_bytes).
+ * Partial pivoting inside each S_i, none across blocks.  Returns
+ * CV_ERR_SINGULAR for a zero pivot. */
+int64_t cv_fdfd_btd_factor_bytes(int nx, int ny);
+int cv_fdfd_btd_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof32,
+                       const void* m0_32, const void* m1_32, void* factor, int64_t factor_bytes, void* stream);
+/* z = M^-1 v (transpose == 0) or M^-T v with the stored factorisation (v, z
+ * complex64, not aliased): one forward and one backward sweep. */
+int cv_fdfd_btd_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof32,
+                      const void* m0_32, const void* m1_32, const void* factor, const void* v, void* z,
+                      int transpose, void* stream);
+/* Solve A x = b (transpose == 0) or A^T x = b to the FP64 relative residual
+ * tol by flexible GMRES preconditioned with the factorisation (complex64
+ * Arnoldi vectors, complex128 x and residual).  The dropped x-wrap and the
+ * complex64 rounding are corrected by the outer iteration.  Workspace:
+ * cv_fdfd_fgmres_workspace_bytes(nx, ny, restart, CV_PRECOND_NONE, 0). */
+int cv_fdfd_fgmres_btd(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                       const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                       const void* m1_32, const void* factor, const void* b, void* x, int transpose, double tol,
+                       int restart, int max_iter, void* workspace, int64_t workspace_bytes, cv_solve_info* info,
+                       void* stream);
+
 /* ------------------------------------------------------------------ misc */
 const char* cv_version(void);
 const char* cv_last_error(void);

@@ -1,0 +1,137 @@
+"""GPU parity of the block-tridiagonal direct path (csrc/btd.hip + FGMRES with
+FP64 true residual) against the scipy oracle: the factorisation blocks
+against a numpy block elimination, the sweep against the block-tridiagonal
+part of the assembled operator, and fields / transposed solves / gradients
+against spsolve."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+import torch
+
+from oracle import fdfd_ref as R
+
+pytestmark = pytest.mark.gpu
+
+OMEGA = 2 * np.pi * R.C_0 / 1.55e-6
+
+
+def _op(pol, shape, npml, bloch=(0.0, 0.0)):
+    from ceviche_amd._fdfd_op import FdfdOperator
+    return FdfdOperator(pol, [dict(omega=OMEGA, dL=50e-9, shape=shape, npml=npml, bloch=bloch)])
+
+
+def _scaled_A(op, pol, eps, npml, bloch):
+    A = (R.make_A_ez if pol == 'ez' else R.make_A_hz)(OMEGA, 50e-9, eps, npml, bloch)
+    return (A * op.sys_scale[0]).tocsr()
+
+
+def _without_x_wrap(A, nx, ny):
+    C = A.tocoo()
+    keep = np.abs(C.row // ny - C.col // ny) <= 1
+    return sp.csr_matrix((C.data[keep], (C.row[keep], C.col[keep])), shape=A.shape)
+
+
+CASES = [('ez', (24, 18), (4, 3), (0.0, 0.0)), ('hz', (19, 23), (3, 4), (0.0, 0.0)),
+         ('ez', (13, 17), (2, 3), (0.5, -0.8)), ('ez', (1, 20), (0, 4), (0.0, 0.0)),
+         ('ez', (20, 1), (4, 0), (0.3, 0.0)), ('hz', (30, 128), (5, 10), (0.0, 0.0)),
+         ('ez', (120, 120), (20, 20), (0.0, 0.0))]
+
+
+@pytest.mark.parametrize('pol,shape,npml,bloch', CASES)
+def test_factor_and_sweep_invert_the_block_tridiagonal_part(pol, shape, npml, bloch):
+    from ceviche_amd._fdfd_op import Direct
+    print('btd', pol, shape, npml, bloch, flush=True)
+    op = _op(pol, shape, npml, bloch)
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(1, 12.25, shape)
+    m0, m1 = op.materials(torch.from_numpy(eps.ravel()))
+    D = Direct(op, m0, m1)
+    nx, ny = shape
+    A = _scaled_A(op, pol, eps, npml, bloch)
+    if nx > 2:
+        Anw = _without_x_wrap(A, nx, ny)
+    else:
+        Anw = A if nx == 1 else None
+    v = rng.standard_normal(nx * ny) + 1j * rng.standard_normal(nx * ny)
+    if Anw is not None:
+        for tr in (False, True):
+            z = D.apply(torch.from_numpy(v).cuda(), transpose=tr).cpu().numpy()
+            ref = spl.spsolve((Anw.T if tr else Anw).tocsc(), v)
+            assert np.linalg.norm(z - ref) <= 1e-3 * np.linalg.norm(ref), tr
+    # the first two stored blocks against a float64 block elimination
+    S = np.frombuffer(D.factor.cpu().numpy()[:nx * ny * ny * 8].tobytes(), dtype=np.complex64).reshape(nx, ny, ny)
+    if Anw is not None:
+        D0 = Anw[:ny, :ny].toarray()
+        np.testing.assert_allclose(S[0], np.linalg.inv(D0), rtol=0, atol=1e-4 * np.abs(np.linalg.inv(D0)).max())
+        if nx > 1:
+            lo = Anw[ny:2 * ny, :ny].diagonal()
+            up = Anw[:ny, ny:2 * ny].diagonal()
+            S1 = np.linalg.inv(Anw[ny:2 * ny, ny:2 * ny].toarray() - (lo[:, None] * np.linalg.inv(D0)) * up[None, :])
+            np.testing.assert_allclose(S[1], S1, rtol=0, atol=1e-4 * np.abs(S1).max())
+
+
+@pytest.mark.parametrize('pol,shape,npml,bloch', CASES)
+def test_direct_solves_match_spsolve(pol, shape, npml, bloch):
+    """A x = b and A^T x = b to the FP64 residual 1e-10 (the x-wrap restored
+    by the outer FGMRES)."""
+    from ceviche_amd._fdfd_op import Direct, SolverOptions
+    op = _op(pol, shape, npml, bloch)
+    rng = np.random.default_rng(4)
+    eps = rng.uniform(1, 12.25, shape)
+    m0, m1 = op.materials(torch.from_numpy(eps.ravel()))
+    D = Direct(op, m0, m1)
+    A = _scaled_A(op, pol, eps, npml, bloch)
+    b = rng.standard_normal(eps.size) + 1j * rng.standard_normal(eps.size)
+    for tr in (False, True):
+        x = D.solve(torch.from_numpy(b).cuda(), m0, m1, transpose=tr, opts=SolverOptions(tol=1e-10)).cpu().numpy()
+        info = op.last_info[0]
+        ref = spl.spsolve((A.T if tr else A).tocsc(), b)
+        assert info['status'] == 0 and info['rel_residual'] <= 1e-10, info
+        assert np.linalg.norm(x - ref) <= 1e-8 * np.linalg.norm(ref), (tr, info)
+        print(pol, shape, 'transpose' if tr else 'forward', info, flush=True)
+
+
+def test_direct_path_fields_gradient_and_forward_mode():
+    """fdfd_ez / fdfd_hz through the direct path: fields vs spsolve, the eps
+    gradient (one factorisation for the forward and adjoint solves) vs the
+    oracle adjoint, forward mode vs central differences."""
+    import ceviche_amd as cv
+    rng = np.random.default_rng(6)
+    shape, npml = (40, 36), [6, 6]
+    eps0 = rng.uniform(1, 6, shape)
+    src = np.zeros(shape)
+    src[10, :] = 1.0
+    F = cv.fdfd_ez(OMEGA, 50e-9, eps0, npml)
+    _, _, ez = F.solve(src)
+    assert F.last_solve_info['method'] == 'direct'
+    _, _, rez = R.solve_ez(OMEGA, 50e-9, eps0, npml, src)
+    assert np.linalg.norm(ez - rez) <= 1e-8 * np.linalg.norm(rez)
+    eps = torch.tensor(eps0, requires_grad=True)
+    F = cv.fdfd_ez(OMEGA, 50e-9, eps, npml)
+    _, _, Ez = F.solve(torch.from_numpy(src))
+    (Ez[30, 18].abs() ** 2).backward()
+    cot = np.zeros(shape, dtype=complex)
+    cot[30, 18] = 2 * Ez.detach().cpu().numpy()[30, 18]
+    gref, _ = R.grad_eps_ez(OMEGA, 50e-9, eps0, npml, src, cot)
+    assert np.linalg.norm(eps.grad.numpy() - gref) <= 1e-6 * np.linalg.norm(gref)
+    G = cv.fdfd_hz(OMEGA, 50e-9, eps0, npml)
+
+    def J(c):
+        c = torch.as_tensor(c, dtype=torch.float64)
+        G.eps_r = c * torch.from_numpy(eps0)
+        a, b, z = G.solve(c * torch.from_numpy(src))
+        return (z.abs() ** 2).reshape(-1)
+
+    jf = cv.jacobian(J, mode='forward')(1.0)
+    h = 1e-6
+    num = (((np.abs(R.solve_hz(OMEGA, 50e-9, (1 + h) * eps0, npml, (1 + h) * src)[2]) ** 2)
+            - (np.abs(R.solve_hz(OMEGA, 50e-9, (1 - h) * eps0, npml, (1 - h) * src)[2]) ** 2)) / (2 * h)).ravel()
+    assert np.linalg.norm(jf[:, 0] - num) <= 1e-5 * np.linalg.norm(num)
+
+
+def test_direct_method_requires_a_narrow_grid():
+    import ceviche_amd as cv
+    F = cv.fdfd_ez(OMEGA, 50e-9, np.ones((10, 130)), [2, 2], solver_options=cv.SolverOptions(method='direct'))
+    with pytest.raises(ValueError):
+        F.solve(np.ones((10, 130)))

@@ -62,13 +62,15 @@ def test_derivative_operators(which, mode):
     assert np.linalg.norm(out - ref) <= 1e-14 * np.linalg.norm(ref)
 
 
-def test_cfg1_ez_fields_match_spsolve():
+@pytest.mark.parametrize('method', ['auto', 'bicgstab'])
+def test_cfg1_ez_fields_match_spsolve(method):
     import ceviche_amd as cv
     c = cfg1()
-    F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'])
+    F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'], solver_options=cv.SolverOptions(method=method))
     Hx, Hy, Ez = F.solve(c['src'])
     info = F.last_solve_info
-    print('cfg1 info', info, flush=True)
+    print('cfg1 info', method, info, flush=True)
+    assert info.get('method', 'bicgstab') == ('direct' if method == 'auto' else 'bicgstab')
     hx, hy, ez = R.solve_ez(c['omega'], c['dL'], c['eps'], c['npml'], c['src'])
     for a, b in ((Ez, ez), (Hx, hx), (Hy, hy)):
         assert isinstance(a, np.ndarray) and a.dtype == np.complex128 and a.shape == (120, 120)
@@ -76,11 +78,12 @@ def test_cfg1_ez_fields_match_spsolve():
     assert info['rel_residual'] <= 1e-10
 
 
-def test_cfg1_ez_gradient_matches_oracle_adjoint():
+@pytest.mark.parametrize('method', ['auto', 'bicgstab'])
+def test_cfg1_ez_gradient_matches_oracle_adjoint(method):
     import ceviche_amd as cv
     c = cfg1()
     eps = torch.tensor(c['eps'], requires_grad=True)
-    F = cv.fdfd_ez(c['omega'], c['dL'], eps, c['npml'])
+    F = cv.fdfd_ez(c['omega'], c['dL'], eps, c['npml'], solver_options=cv.SolverOptions(method=method))
     _, _, Ez = F.solve(torch.from_numpy(c['src']))
     L = Ez[c['probe']].abs() ** 2
     L.backward()
@@ -157,10 +160,12 @@ def test_bloch_and_complex_eps_ez():
     assert np.linalg.norm(Ez - ez) / np.linalg.norm(ez) <= 1e-6
 
 
-def test_complex64_mixed_precision_reaches_fp64_residual():
+@pytest.mark.parametrize('method', ['auto', 'bicgstab'])
+def test_complex64_mixed_precision_reaches_fp64_residual(method):
     import ceviche_amd as cv
     c = cfg1()
-    F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'], precision='complex64')
+    F = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'], precision='complex64',
+                   solver_options=cv.SolverOptions(method=method))
     _, _, Ez = F.solve(c['src'])
     info = F.last_solve_info
     print('mixed info', info, flush=True)
