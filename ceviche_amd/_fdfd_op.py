@@ -347,35 +347,55 @@ def direct_applies(op):
 
 
 class Direct:
-    """Block-tridiagonal factorisation (complex64 Schur-complement inverses
-    along x) of one FDFD instance, and FP64-refined FGMRES solves with it in
-    either orientation (csrc/btd.hip, cv_fdfd_fgmres_btd).  One factorisation
-    serves the forward, adjoint (A^T) and tangent solves of a linear system."""
+    """Block-tridiagonal factorisation of one FDFD instance (A without its
+    x-wrap, complex64) and FP64-refined FGMRES solves with it in either
+    orientation (csrc/btd.hip, cv_fdfd_fgmres_direct).  kind 'bcr' (block
+    cyclic reduction: log2(nx) levels of parallel block inversions, the
+    default for nx >= 8) or 'btd' (the sequential block elimination along x).
+    One factorisation serves the forward, adjoint (A^T) and tangent solves of
+    a linear system."""
 
-    def __init__(self, op, m0, m1):
+    def __init__(self, op, m0, m1, kind=None):
         if not direct_applies(op):
             raise ValueError('the block-tridiagonal factorisation needs one uncoupled instance with ny <= %d'
                              % BTD_MAX_NY)
         self.op = op
         nx, ny = op.shapes[0]
+        if kind is None:
+            kind = 'bcr' if nx >= 8 else 'btd'
+        if kind not in ('bcr', 'btd'):
+            raise ValueError("kind must be 'bcr' or 'btd'")
+        self.kind = kind
+        self.code = _lib.CV_FACTOR_BCR if kind == 'bcr' else _lib.CV_FACTOR_BTD
         self.prof32 = op.prof.to(torch.complex64).contiguous()
         self.m0 = m0.to(torch.float32).contiguous()
         self.m1 = None if m1 is None else m1.to(torch.float32).contiguous()
         L = lib()
-        nbytes = L.cv_fdfd_btd_factor_bytes(nx, ny)
-        self.factor = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
-        check(L.cv_fdfd_btd_factor(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(self.prof32),
-                                   ptr(self.m0), ptr(self.m1), ptr(self.factor), nbytes, stream_ptr()),
-              'block-tridiagonal factorisation')
+        dh = op.desc_host.ctypes.data_as(ctypes.c_void_p)
+        if kind == 'bcr':
+            nbytes = L.cv_fdfd_bcr_factor_bytes(nx, ny)
+            self.factor = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
+            check(L.cv_fdfd_bcr_factor(ptr(op.desc), dh, ptr(self.prof32), ptr(self.m0), ptr(self.m1),
+                                       ptr(self.factor), nbytes, stream_ptr()), 'block cyclic reduction')
+        else:
+            nbytes = L.cv_fdfd_btd_factor_bytes(nx, ny)
+            self.factor = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
+            check(L.cv_fdfd_btd_factor(ptr(op.desc), dh, ptr(self.prof32), ptr(self.m0), ptr(self.m1),
+                                       ptr(self.factor), nbytes, stream_ptr()), 'block-tridiagonal factorisation')
 
     def apply(self, v, transpose=False):
         """M^-1 v (complex64) with the factorisation alone."""
         v = v.to(torch.complex64).contiguous()
         z = torch.empty_like(v)
         op = self.op
-        check(lib().cv_fdfd_btd_apply(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(self.prof32),
-                                      ptr(self.m0), ptr(self.m1), ptr(self.factor), ptr(v), ptr(z),
-                                      int(bool(transpose)), stream_ptr()), 'btd_apply')
+        dh = op.desc_host.ctypes.data_as(ctypes.c_void_p)
+        if self.kind == 'bcr':
+            check(lib().cv_fdfd_bcr_apply(ptr(op.desc), dh, ptr(self.factor), ptr(v), ptr(z), int(bool(transpose)),
+                                          stream_ptr()), 'bcr_apply')
+        else:
+            check(lib().cv_fdfd_btd_apply(ptr(op.desc), dh, ptr(self.prof32), ptr(self.m0), ptr(self.m1),
+                                          ptr(self.factor), ptr(v), ptr(z), int(bool(transpose)), stream_ptr()),
+                  'btd_apply')
         return z
 
     def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
@@ -390,12 +410,12 @@ class Direct:
         b = bs.to(torch.complex128).contiguous()
         x = torch.zeros_like(b)
         info = (_lib.SolveInfo * 1)()
-        rc = L.cv_fdfd_fgmres_btd(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(op.prof64),
-                                  ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1),
-                                  ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)), float(tol),
-                                  int(opts.restart), int(opts.max_iter), ptr(ws), need, info, stream_ptr())
+        rc = L.cv_fdfd_fgmres_direct(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(op.prof64),
+                                     ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1),
+                                     self.code, ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)), float(tol),
+                                     int(opts.restart), int(opts.max_iter), ptr(ws), need, info, stream_ptr())
         op.last_info = [dict(iterations=info[0].iterations, status=info[0].status, restarts=info[0].restarts,
-                             rel_residual=info[0].rel_residual, method='direct')]
+                             rel_residual=info[0].rel_residual, method='direct', factor=self.kind)]
         check(rc, 'fdfd direct solve')
         return x
 

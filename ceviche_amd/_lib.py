@@ -29,6 +29,8 @@ CV_POL_HZ = 1
 CV_FLAG_X_IDENTITY = 1
 CV_FLAG_Y_IDENTITY = 2
 CV_TILE_CELLS = 256
+CV_FACTOR_BTD = 1
+CV_FACTOR_BCR = 2
 
 _STATUS_NAMES = {
     CV_ERR_INVALID: 'invalid argument',
@@ -112,7 +114,10 @@ _SIGS = {
     'cv_fdfd_btd_factor_bytes': (I64, [I, I]),
     'cv_fdfd_btd_factor': (I, [P, P, P, P, P, P, I64, P]),
     'cv_fdfd_btd_apply': (I, [P, P, P, P, P, P, P, P, I, P]),
-    'cv_fdfd_fgmres_btd': (I, [P, P, P, P, P, P, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
+    'cv_fdfd_bcr_factor_bytes': (I64, [I, I]),
+    'cv_fdfd_bcr_factor': (I, [P, P, P, P, P, P, I64, P]),
+    'cv_fdfd_bcr_apply': (I, [P, P, P, P, P, I, P]),
+    'cv_fdfd_fgmres_direct': (I, [P, P, P, P, P, P, P, P, I, P, P, P, I, D, I, I, P, I64, P, P]),
     'cv_fdfd_coupling_grad': (I, [I, P, P, I, P, P, P, P, D, P]),
     'cv_fdfd_solve_workspace_bytes': (I64, [I, I64, I, I]),
     'cv_fdfd_solve': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),

@@ -24,6 +24,8 @@
 // column staged through LDS.
 #include "fdfd_stencil.h"
 
+#include <vector>
+
 namespace cv {
 
 constexpr int kBtdThreads = 1024;
@@ -62,16 +64,163 @@ __device__ __forceinline__ cx<float> cdivf(cx<float> a, cx<float> b) {
 constexpr int kFRows = 8;                     // rows per wave
 constexpr int kFCols = kBtdMax / 64;          // columns per lane (2)
 
+// LDS of one Gauss-Jordan inversion (one per workgroup).
+struct GjLds {
+    cx<float> stage[kBtdMax * kBtdMax];
+    cx<float> colbuf[2][kBtdMax];
+    cx<float> rowbuf[2][kBtdMax];
+    cx<float> swapbuf[kBtdMax];
+    int perm[kBtdMax];
+    int colmap[kBtdMax];
+    int bad;
+};
+
+// Inverts the n x n matrix held in m (wave w: rows 8w .. 8w+7, lane l: columns
+// l and l+64, zero outside n x n) in place: Gauss-Jordan elimination with
+// partial pivoting by physical row swaps, then the column interchanges undone
+// through g.stage.  On return m holds the inverse (zero padding).  A zero pivot
+// sets g.bad = code (if g.bad is still 0).  Called by all threads.
+__device__ void gj_invert(cx<float> (&m)[kFRows][kFCols], int n, int r0, int w, int l, int t, GjLds& g, int code) {
+    cx<float>* const stage = g.stage;
+    cx<float> (&colbuf)[2][kBtdMax] = g.colbuf;
+    cx<float> (&rowbuf)[2][kBtdMax] = g.rowbuf;
+    cx<float>* const swapbuf = g.swapbuf;
+    int* const perm = g.perm;
+    int* const colmap = g.colmap;
+    int& bad = g.bad;
+    const int i = code - 1;
+    // Gauss-Jordan inversion in place, partial pivoting by physical row swaps.
+    // Entries outside the n x n block are zero and stay zero (colbuf / rowbuf
+    // padding is zero), so the update below needs no bounds tests.
+    for (int k = 0; k < n; ++k) {
+        const int b = k & 1;
+        const int qk = k >> 6;
+        if (l == (k & 63)) {
+#pragma unroll
+            for (int p = 0; p < kFRows; ++p) {
+                const cx<float> v0 = m[p][0], v1 = m[p][kFCols - 1];
+                colbuf[b][r0 + p] = (qk == 0) ? v0 : v1;
+            }
+        }
+        __syncthreads();
+        // Pivot: first row of largest |S[r][k]| among k <= r < n, found by
+        // every wave on its own (no barrier): lane l looks at rows l and
+        // l + 64, DPP combines within 16-lane rows, readlane across rows,
+        // ballot picks the smallest such row.
+        int pr;
+        {
+            float best = -1.0f;
+            int arg = k;
+            if (l >= k && l < n) { best = cabs2(colbuf[b][l]); arg = l; }
+            if (l + 64 >= k && l + 64 < n) {
+                const float v2 = cabs2(colbuf[b][l + 64]);
+                if (v2 > best) { best = v2; arg = l + 64; }
+            }
+            float mx = best;
+            mx = fmaxf(mx, dpp_f<0xB1>(mx));    // quad_perm [1,0,3,2]
+            mx = fmaxf(mx, dpp_f<0x4E>(mx));    // quad_perm [2,3,0,1]
+            mx = fmaxf(mx, dpp_f<0x141>(mx));   // row_half_mirror
+            mx = fmaxf(mx, dpp_f<0x140>(mx));   // row_mirror
+            const float gmax = fmaxf(fmaxf(lane_f(mx, 0), lane_f(mx, 16)), fmaxf(lane_f(mx, 32), lane_f(mx, 48)));
+            const unsigned long long low = __ballot(best == gmax && arg < 64);
+            const unsigned long long any = __ballot(best == gmax);
+            if (!(gmax > 0.0f)) pr = k;
+            else if (low) pr = __ffsll(low) - 1;
+            else pr = __ffsll(any) - 1 + 64;
+            pr = __builtin_amdgcn_readfirstlane(pr);
+            if (t == 0) {
+                perm[k] = pr;
+                if (!(gmax > 0.0f) && bad == 0) bad = 1 + i;
+            }
+        }
+        // row extraction by select chains (no indexed access to m)
+        if ((pr >> 3) == w) {
+            const int pp = pr - r0;
+#pragma unroll
+            for (int q = 0; q < kFCols; ++q) {
+                cx<float> v = m[0][q];
+#pragma unroll
+                for (int p = 1; p < kFRows; ++p) v = (p == pp) ? m[p][q] : v;
+                rowbuf[b][l + 64 * q] = v;
+            }
+        }
+        if (pr != k && (k >> 3) == w) {
+            const int pk = k - r0;
+#pragma unroll
+            for (int q = 0; q < kFCols; ++q) {
+                cx<float> v = m[0][q];
+#pragma unroll
+                for (int p = 1; p < kFRows; ++p) v = (p == pk) ? m[p][q] : v;
+                swapbuf[l + 64 * q] = v;
+            }
+        }
+        __syncthreads();
+        // rows k and pr exchange; then row k /= pivot and the other rows
+        // lose their column-k component:
+        //   S[k][c] = rj = (c == k ? 1 : S[k][c]) / piv
+        //   S[r][c] = (c == k ? 0 : S[r][c]) - f_r rj     (f_r = S[r][k] before the step)
+        const cx<float> ck = colbuf[b][pr], cp = colbuf[b][k];
+        const float d2 = ck.re * ck.re + ck.im * ck.im;
+        const float inv_d2 = (d2 > 0.0f) ? 1.0f / d2 : 0.0f;
+        const cx<float> ipiv = mk<float>(ck.re * inv_d2, -ck.im * inv_d2);
+        cx<float> rj[kFCols];
+#pragma unroll
+        for (int q = 0; q < kFCols; ++q) {
+            const int c = l + 64 * q;
+            rj[q] = ((c == k) ? mk<float>(1.0f, 0.0f) : rowbuf[b][c]) * ipiv;
+        }
+        const bool swapped = (pr != k);
+        cx<float> sw[kFCols];
+#pragma unroll
+        for (int q = 0; q < kFCols; ++q) sw[q] = swapbuf[l + 64 * q];
+#pragma unroll
+        for (int p = 0; p < kFRows; ++p) {
+            const int r = r0 + p;           // uniform across the wave
+            const bool is_k = (r == k);
+            const bool is_pr = swapped && (r == pr);
+            const cx<float> f = is_pr ? cp : colbuf[b][r];
+#pragma unroll
+            for (int q = 0; q < kFCols; ++q) {
+                const int c = l + 64 * q;
+                const cx<float> cur = is_pr ? sw[q] : m[p][q];
+                const cx<float> keep = (c == k) ? czero<float>() : cur;
+                const cx<float> upd = keep - f * rj[q];
+                m[p][q] = is_k ? rj[q] : upd;
+            }
+        }
+    }
+    // undo the column interchanges: logical column j is physical column colmap[j]
+    __syncthreads();
+    if (t < n) {
+        int v = t;
+        for (int q = 0; q < n; ++q) {
+            const int pq = perm[q];
+            if (v == q) v = pq;
+            else if (v == pq) v = q;
+        }
+        colmap[t] = v;
+    }
+#pragma unroll
+    for (int p = 0; p < kFRows; ++p)
+#pragma unroll
+        for (int q = 0; q < kFCols; ++q) stage[(r0 + p) * kBtdMax + l + 64 * q] = m[p][q];
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < kFRows; ++p) {
+        const int r = r0 + p;
+#pragma unroll
+        for (int q = 0; q < kFCols; ++q) {
+            const int c = l + 64 * q;
+            m[p][q] = (r < n && c < n) ? stage[r * kBtdMax + colmap[c]] : czero<float>();
+        }
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kBtdThreads) k_btd_factor(BtdFactorArgs a) {
-    __shared__ cx<float> stage[kBtdMax * kBtdMax];
-    __shared__ cx<float> colbuf[2][kBtdMax];
-    __shared__ cx<float> rowbuf[2][kBtdMax];
-    __shared__ cx<float> swapbuf[kBtdMax];
+    __shared__ GjLds g;
     __shared__ cx<float> upprev[kBtdMax];
     __shared__ cx<float> cdg[kBtdMax], clo[kBtdMax], cym[kBtdMax], cyp[kBtdMax];
-    __shared__ int perm[kBtdMax];
-    __shared__ int colmap[kBtdMax];
-    __shared__ int bad;
     const cv_fdfd_desc d = load_desc(a.desc, 0);
     const int n = d.ny, nx = d.nx;
     const int t = threadIdx.x;
@@ -83,11 +232,7 @@ __global__ void __launch_bounds__(kBtdThreads) k_btd_factor(BtdFactorArgs a) {
     for (int p = 0; p < kFRows; ++p)
 #pragma unroll
         for (int q = 0; q < kFCols; ++q) m[p][q] = czero<float>();
-    if (t == 0) bad = 0;
-    if (t < kBtdMax) {
-        colbuf[0][t] = colbuf[1][t] = czero<float>();
-        rowbuf[0][t] = rowbuf[1][t] = swapbuf[t] = czero<float>();
-    }
+    if (t == 0) g.bad = 0;
     const bool xid = d.flags & CV_FLAG_X_IDENTITY;
     for (int i = 0; i < nx; ++i) {
         // S_i = D_i - diag(lo_i) S_{i-1}^{-1} diag(up_{i-1}); m holds S_{i-1}^{-1} (logical layout)
@@ -121,122 +266,7 @@ __global__ void __launch_bounds__(kBtdThreads) k_btd_factor(BtdFactorArgs a) {
                 m[p][q] = v;
             }
         }
-        // Gauss-Jordan inversion in place, partial pivoting by physical row swaps.
-        // Entries outside the n x n block are zero and stay zero (colbuf / rowbuf
-        // padding is zero), so the update below needs no bounds tests.
-        for (int k = 0; k < n; ++k) {
-            const int b = k & 1;
-            const int qk = k >> 6;
-            if (l == (k & 63)) {
-#pragma unroll
-                for (int p = 0; p < kFRows; ++p) {
-                    const cx<float> v0 = m[p][0], v1 = m[p][kFCols - 1];
-                    colbuf[b][r0 + p] = (qk == 0) ? v0 : v1;
-                }
-            }
-            __syncthreads();
-            // Pivot: first row of largest |S[r][k]| among k <= r < n, found by
-            // every wave on its own (no barrier): lane l looks at rows l and
-            // l + 64, DPP combines within 16-lane rows, readlane across rows,
-            // ballot picks the smallest such row.
-            int pr;
-            {
-                float best = -1.0f;
-                int arg = k;
-                if (l >= k && l < n) { best = cabs2(colbuf[b][l]); arg = l; }
-                if (l + 64 >= k && l + 64 < n) {
-                    const float v2 = cabs2(colbuf[b][l + 64]);
-                    if (v2 > best) { best = v2; arg = l + 64; }
-                }
-                float mx = best;
-                mx = fmaxf(mx, dpp_f<0xB1>(mx));    // quad_perm [1,0,3,2]
-                mx = fmaxf(mx, dpp_f<0x4E>(mx));    // quad_perm [2,3,0,1]
-                mx = fmaxf(mx, dpp_f<0x141>(mx));   // row_half_mirror
-                mx = fmaxf(mx, dpp_f<0x140>(mx));   // row_mirror
-                const float g = fmaxf(fmaxf(lane_f(mx, 0), lane_f(mx, 16)), fmaxf(lane_f(mx, 32), lane_f(mx, 48)));
-                const unsigned long long low = __ballot(best == g && arg < 64);
-                const unsigned long long any = __ballot(best == g);
-                if (!(g > 0.0f)) pr = k;
-                else if (low) pr = __ffsll(low) - 1;
-                else pr = __ffsll(any) - 1 + 64;
-                pr = __builtin_amdgcn_readfirstlane(pr);
-                if (t == 0) {
-                    perm[k] = pr;
-                    if (!(g > 0.0f) && bad == 0) bad = 1 + i;
-                }
-            }
-            // row extraction by select chains (no indexed access to m)
-            if ((pr >> 3) == w) {
-                const int pp = pr - r0;
-#pragma unroll
-                for (int q = 0; q < kFCols; ++q) {
-                    cx<float> v = m[0][q];
-#pragma unroll
-                    for (int p = 1; p < kFRows; ++p) v = (p == pp) ? m[p][q] : v;
-                    rowbuf[b][l + 64 * q] = v;
-                }
-            }
-            if (pr != k && (k >> 3) == w) {
-                const int pk = k - r0;
-#pragma unroll
-                for (int q = 0; q < kFCols; ++q) {
-                    cx<float> v = m[0][q];
-#pragma unroll
-                    for (int p = 1; p < kFRows; ++p) v = (p == pk) ? m[p][q] : v;
-                    swapbuf[l + 64 * q] = v;
-                }
-            }
-            __syncthreads();
-            // rows k and pr exchange; then row k /= pivot and the other rows
-            // lose their column-k component:
-            //   S[k][c] = rj = (c == k ? 1 : S[k][c]) / piv
-            //   S[r][c] = (c == k ? 0 : S[r][c]) - f_r rj     (f_r = S[r][k] before the step)
-            const cx<float> ck = colbuf[b][pr], cp = colbuf[b][k];
-            const float d2 = ck.re * ck.re + ck.im * ck.im;
-            const float inv_d2 = (d2 > 0.0f) ? 1.0f / d2 : 0.0f;
-            const cx<float> ipiv = mk<float>(ck.re * inv_d2, -ck.im * inv_d2);
-            cx<float> rj[kFCols];
-#pragma unroll
-            for (int q = 0; q < kFCols; ++q) {
-                const int c = l + 64 * q;
-                rj[q] = ((c == k) ? mk<float>(1.0f, 0.0f) : rowbuf[b][c]) * ipiv;
-            }
-            const bool swapped = (pr != k);
-            cx<float> sw[kFCols];
-#pragma unroll
-            for (int q = 0; q < kFCols; ++q) sw[q] = swapbuf[l + 64 * q];
-#pragma unroll
-            for (int p = 0; p < kFRows; ++p) {
-                const int r = r0 + p;           // uniform across the wave
-                const bool is_k = (r == k);
-                const bool is_pr = swapped && (r == pr);
-                const cx<float> f = is_pr ? cp : colbuf[b][r];
-#pragma unroll
-                for (int q = 0; q < kFCols; ++q) {
-                    const int c = l + 64 * q;
-                    const cx<float> cur = is_pr ? sw[q] : m[p][q];
-                    const cx<float> keep = (c == k) ? czero<float>() : cur;
-                    const cx<float> upd = keep - f * rj[q];
-                    m[p][q] = is_k ? rj[q] : upd;
-                }
-            }
-        }
-        // undo the column interchanges: logical column j is physical column colmap[j]
-        __syncthreads();
-        if (t < n) {
-            int v = t;
-            for (int q = 0; q < n; ++q) {
-                const int pq = perm[q];
-                if (v == q) v = pq;
-                else if (v == pq) v = q;
-            }
-            colmap[t] = v;
-        }
-#pragma unroll
-        for (int p = 0; p < kFRows; ++p)
-#pragma unroll
-            for (int q = 0; q < kFCols; ++q) stage[(r0 + p) * kBtdMax + l + 64 * q] = m[p][q];
-        __syncthreads();
+        gj_invert(m, n, r0, w, l, t, g, 1 + i);
         cx<float>* out = a.sinv + (int64_t)i * n * n;
 #pragma unroll
         for (int p = 0; p < kFRows; ++p) {
@@ -244,16 +274,12 @@ __global__ void __launch_bounds__(kBtdThreads) k_btd_factor(BtdFactorArgs a) {
 #pragma unroll
             for (int q = 0; q < kFCols; ++q) {
                 const int c = l + 64 * q;
-                if (r < n && c < n) {
-                    const cx<float> v = stage[r * kBtdMax + colmap[c]];
-                    m[p][q] = v;
-                    out[(int64_t)r * n + c] = v;
-                }
+                if (r < n && c < n) out[(int64_t)r * n + c] = m[p][q];
             }
         }
-        __syncthreads();
     }
-    if (t == 0) *a.status = bad;
+    __syncthreads();
+    if (t == 0) *a.status = g.bad;
 }
 
 // z = M^-1 v (or M^-T v) with the stored inverses; v and z complex64, z may
@@ -352,6 +378,233 @@ k_btd_apply(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__
     }
 }
 
+
+// --------------------------------------------------------------------------
+// Block cyclic reduction (BCR) of the same block-tridiagonal matrix: level l
+// has N blocks; the odd ones are inverted (all at once, one workgroup each)
+// and eliminated, the even ones form level l+1:
+//   alpha_j = L_j D_{j-1}^-1,  beta_j = U_j D_{j+1}^-1,
+//   gamma_j = D_{j-1}^-1 U_{j-1},  delta_j = D_{j+1}^-1 L_{j+1},
+//   D'_j = D_j - L_j gamma_j - U_j delta_j,  L'_j = -alpha_j L_{j-1},  U'_j = -beta_j U_{j+1}.
+// A solve reduces the right-hand side level by level (b'_j = b_j - alpha_j
+// b_{j-1} - beta_j b_{j+1}; A^T: gamma_j^T, delta_j^T), inverts the last block
+// and substitutes back (x_j = D_j^-1 (b_j - L_j x_{j-1} - U_j x_{j+1}); A^T:
+// D_j^-T (b_j - U_{j-1}^T x_{j-1} - L_{j+1}^T x_{j+1})).  The sequential
+// factorisation's nx dependent inversions become ceil(log2 nx) levels of
+// independent ones; every matrix is an n x n complex64 "slot".
+
+struct BcrGemmOp {     // C = X + s1 A1 B1 + s2 A2 B2 (slot -1: absent)
+    int c, x, a1, b1, a2, b2;
+    float s1, s2;
+};
+
+struct BcrVecOp {      // t = b - M1 u1 - M2 u2 (A^T: M1t^T, M2t^T); out = M0 t (A^T: M0^T t) or t
+    int out, b, u1, u2, m1, m2, m1t, m2t, m0;
+};
+
+// D_i, L_i = diag(lo_i), U_i = diag(up_i) of level 0 (x-wrap dropped);
+// dls[3 i + 0 / 1 / 2] = slots of D_i, L_i, U_i (-1: none)
+__global__ void __launch_bounds__(256)
+k_bcr_build0(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
+             const float* __restrict__ m1, cx<float>* __restrict__ slots, const int* __restrict__ dls) {
+    __shared__ cx<float> cdg[kBtdMax], clo[kBtdMax], cup[kBtdMax], cym[kBtdMax], cyp[kBtdMax];
+    const cv_fdfd_desc d = load_desc(desc, 0);
+    const int n = d.ny, nx = d.nx, i = blockIdx.x;
+    const bool xid = d.flags & CV_FLAG_X_IDENTITY, yid = d.flags & CV_FLAG_Y_IDENTITY;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        const Row5<float> rw = row_coefs<float>(d, prof, m0, m1, i, r);
+        cdg[r] = rw.c;
+        clo[r] = (i > 0 && !xid) ? rw.xm : czero<float>();
+        cup[r] = (i < nx - 1 && !xid) ? rw.xp : czero<float>();
+        cym[r] = yid ? czero<float>() : rw.ym;
+        cyp[r] = yid ? czero<float>() : rw.yp;
+    }
+    __syncthreads();
+    const int64_t nn = (int64_t)n * n;
+    cx<float>* D = slots + (int64_t)dls[3 * i] * nn;
+    cx<float>* L = dls[3 * i + 1] >= 0 ? slots + (int64_t)dls[3 * i + 1] * nn : nullptr;
+    cx<float>* U = dls[3 * i + 2] >= 0 ? slots + (int64_t)dls[3 * i + 2] * nn : nullptr;
+    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) {
+        const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
+        cx<float> v = czero<float>();
+        if (c == r) v = v + cdg[r];
+        if (c == ((r == 0) ? n - 1 : r - 1)) v = v + cym[r];
+        if (c == ((r == n - 1) ? 0 : r + 1)) v = v + cyp[r];
+        D[e] = v;
+        if (L) L[e] = (c == r) ? clo[r] : czero<float>();
+        if (U) U[e] = (c == r) ? cup[r] : czero<float>();
+    }
+}
+
+__global__ void k_zero_u32_btd(unsigned int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// In-place inversion of the listed slots, one workgroup each; a zero pivot
+// records 1 + slot in *status.
+__global__ void __launch_bounds__(kBtdThreads)
+k_bcr_inv(cx<float>* __restrict__ slots, const int* __restrict__ list, int n, int* __restrict__ status) {
+    __shared__ GjLds g;
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
+    const int r0 = w * kFRows;
+    const int slot = list[blockIdx.x];
+    cx<float>* S = slots + (int64_t)slot * n * n;
+    if (t == 0) g.bad = 0;
+    cx<float> m[kFRows][kFCols];
+#pragma unroll
+    for (int p = 0; p < kFRows; ++p)
+#pragma unroll
+        for (int q = 0; q < kFCols; ++q) {
+            const int r = r0 + p, c = l + 64 * q;
+            m[p][q] = (r < n && c < n) ? S[(int64_t)r * n + c] : czero<float>();
+        }
+    __syncthreads();
+    gj_invert(m, n, r0, w, l, t, g, 1 + slot);
+#pragma unroll
+    for (int p = 0; p < kFRows; ++p)
+#pragma unroll
+        for (int q = 0; q < kFCols; ++q) {
+            const int r = r0 + p, c = l + 64 * q;
+            if (r < n && c < n) S[(int64_t)r * n + c] = m[p][q];
+        }
+    if (t == 0 && g.bad != 0) atomicCAS(status, 0, g.bad);
+}
+
+// Batched n x n complex GEMMs (n <= 128): one 256-thread workgroup per op,
+// 8 x 8 outputs per thread, K in chunks of 16 staged through LDS.
+constexpr int kGemmThreads = 256;
+constexpr int kGemmK = 16;
+
+__global__ void __launch_bounds__(kGemmThreads)
+k_bcr_gemm(cx<float>* __restrict__ slots, const BcrGemmOp* __restrict__ ops, int n) {
+    __shared__ cx<float> As[kGemmK][kBtdMax];   // As[k][row]
+    __shared__ cx<float> Bs[kGemmK][kBtdMax];   // Bs[k][col]
+    const BcrGemmOp op = ops[blockIdx.x];
+    const int64_t nn = (int64_t)n * n;
+    const int t = threadIdx.x, ty = t / 16, tx = t % 16;
+    cx<float> acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = czero<float>();
+    for (int term = 0; term < 2; ++term) {
+        const int sa = term ? op.a2 : op.a1, sb = term ? op.b2 : op.b1;
+        if (sa < 0 || sb < 0) continue;
+        const float sg = term ? op.s2 : op.s1;
+        const cx<float>* A = slots + (int64_t)sa * nn;
+        const cx<float>* B = slots + (int64_t)sb * nn;
+        for (int k0 = 0; k0 < n; k0 += kGemmK) {
+            __syncthreads();
+            for (int e = t; e < kGemmK * kBtdMax; e += kGemmThreads) {
+                const int kk = e / kBtdMax, rc = e % kBtdMax, k = k0 + kk;
+                As[kk][rc] = (rc < n && k < n) ? A[(int64_t)rc * n + k] : czero<float>();
+                Bs[kk][rc] = (rc < n && k < n) ? B[(int64_t)k * n + rc] : czero<float>();
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int kk = 0; kk < kGemmK; ++kk) {
+                cx<float> av[8], bv[8];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) av[a] = As[kk][8 * ty + a];
+#pragma unroll
+                for (int b = 0; b < 8; ++b) bv[b] = Bs[kk][8 * tx + b];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const cx<float> sav = mk<float>(sg * av[a].re, sg * av[a].im);
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) acc[a][b] = acc[a][b] + sav * bv[b];
+                }
+            }
+        }
+    }
+    cx<float>* C = slots + (int64_t)op.c * nn;
+    const cx<float>* X = op.x >= 0 ? slots + (int64_t)op.x * nn : nullptr;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int r = 8 * ty + a;
+        if (r >= n) continue;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int c = 8 * tx + b;
+            if (c >= n) continue;
+            cx<float> v = acc[a][b];
+            if (X) v = v + X[(int64_t)r * n + c];
+            C[(int64_t)r * n + c] = v;
+        }
+    }
+}
+
+// part[] += sg M x (or sg M^T x) for one n x n slot, x and part in LDS,
+// 1024 threads (8 per row / column).  Ends with a barrier.
+template <bool T>
+__device__ __forceinline__ void bcr_matvec(const cx<float>* __restrict__ M, const cx<float>* x, cx<float>* part,
+                                           cx<float> (&red)[8][kBtdMax], int n, float sg) {
+    const int t = threadIdx.x;
+    if (!T) {
+        // thread (r = t / 8, cp = t % 8) sums columns cp, cp + 8, ...
+        const int r = t / 8, cp = t % 8;
+        cx<float> acc = czero<float>();
+        if (r < n)
+            for (int c = cp; c < n; c += 8) acc = acc + M[(int64_t)r * n + c] * x[c];
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) {
+            acc.re += __shfl_xor(acc.re, off, 64);
+            acc.im += __shfl_xor(acc.im, off, 64);
+        }
+        if (cp == 0 && r < n) part[r] = part[r] + mk<float>(sg * acc.re, sg * acc.im);
+    } else {
+        // thread (c = t % 128, rp = t / 128) sums rows rp, rp + 8, ...
+        const int c = t % kBtdMax, rp = t / kBtdMax;
+        cx<float> acc = czero<float>();
+        if (c < n)
+            for (int r = rp; r < n; r += 8) acc = acc + M[(int64_t)r * n + c] * x[r];
+        red[rp][c] = acc;
+        __syncthreads();
+        if (t < n) {
+            cx<float> s2 = czero<float>();
+#pragma unroll
+            for (int g = 0; g < 8; ++g) s2 = s2 + red[g][t];
+            part[t] = part[t] + mk<float>(sg * s2.re, sg * s2.im);
+        }
+    }
+    __syncthreads();
+}
+
+// One workgroup per solve op; vectors are n-entry slices of vecs.
+template <bool TRANS>
+__global__ void __launch_bounds__(kBtdThreads)
+k_bcr_vec(const cx<float>* __restrict__ slots, cx<float>* __restrict__ vecs, const BcrVecOp* __restrict__ ops,
+          int n) {
+    __shared__ cx<float> tv[kBtdMax], xs[kBtdMax], res[kBtdMax];
+    __shared__ cx<float> red[8][kBtdMax];
+    const BcrVecOp op = ops[blockIdx.x];
+    const int64_t nn = (int64_t)n * n;
+    const int t = threadIdx.x;
+    if (t < n) tv[t] = vecs[(int64_t)op.b * n + t];
+    __syncthreads();
+    const int m1 = TRANS ? op.m1t : op.m1, m2 = TRANS ? op.m2t : op.m2;
+    if (op.u1 >= 0 && m1 >= 0) {
+        if (t < n) xs[t] = vecs[(int64_t)op.u1 * n + t];
+        __syncthreads();
+        bcr_matvec<TRANS>(slots + (int64_t)m1 * nn, xs, tv, red, n, -1.0f);
+    }
+    if (op.u2 >= 0 && m2 >= 0) {
+        if (t < n) xs[t] = vecs[(int64_t)op.u2 * n + t];
+        __syncthreads();
+        bcr_matvec<TRANS>(slots + (int64_t)m2 * nn, xs, tv, red, n, -1.0f);
+    }
+    if (op.m0 >= 0) {
+        if (t < n) res[t] = czero<float>();
+        __syncthreads();
+        bcr_matvec<TRANS>(slots + (int64_t)op.m0 * nn, tv, res, red, n, 1.0f);
+        if (t < n) vecs[(int64_t)op.out * n + t] = res[t];
+    } else if (t < n) {
+        vecs[(int64_t)op.out * n + t] = tv[t];
+    }
+}
+
 }  // namespace cv
 
 using namespace cv;
@@ -427,4 +680,238 @@ extern "C" int cv_fdfd_btd_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* d
     CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.ny <= kBtdMax && dh.cell_off == 0 && dh.nblk == 1,
              "single instance with ny <= %d required", kBtdMax);
     return btd_apply(desc, dh, prof32, m0_32, m1_32, factor, v, z, transpose, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ BCR host
+namespace cv {
+
+struct BcrPlan {
+    int nx = 0, n = 0;
+    int n_slots = 0, n_vecs = 0;
+    std::vector<int> dls;                         // level-0 D, L, U slots (3 nx)
+    std::vector<std::vector<int>> inv;            // per level: slots to invert
+    std::vector<std::vector<BcrGemmOp>> g1, g2;   // per level: products, then next-level blocks
+    std::vector<std::vector<BcrVecOp>> fwd, bwd;  // per level: RHS reduction, back-substitution
+    int final_slot = -1;
+    BcrVecOp final_op{};
+    // byte offsets inside the factor buffer
+    int64_t off_dls = 0, off_final = 0, off_slots = 0, off_vecs = 0, bytes = 0;
+    std::vector<int64_t> off_inv, off_g1, off_g2, off_fwd, off_bwd;
+};
+
+static int64_t al256b(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+static void bcr_plan(int nx, int n, BcrPlan& P) {
+    P = BcrPlan();
+    P.nx = nx;
+    P.n = n;
+    int ns = 0, nv = 0;
+    // level 0 vectors: b at 0 .. nx-1, x at nx .. 2nx-1 (contiguous for the copies)
+    std::vector<int> b(nx), x(nx), D(nx), L(nx, -1), U(nx, -1);
+    for (int j = 0; j < nx; ++j) b[j] = nv++;
+    for (int j = 0; j < nx; ++j) x[j] = nv++;
+    P.dls.assign(3 * (size_t)nx, -1);
+    for (int j = 0; j < nx; ++j) {
+        D[j] = ns++;
+        if (j >= 1) L[j] = ns++;
+        if (j + 1 < nx) U[j] = ns++;
+        P.dls[3 * j] = D[j];
+        P.dls[3 * j + 1] = L[j];
+        P.dls[3 * j + 2] = U[j];
+    }
+    int N = nx;
+    while (N > 1) {
+        std::vector<int> inv, D2, L2, U2, b2, x2;
+        std::vector<BcrGemmOp> g1, g2;
+        std::vector<BcrVecOp> fw, bw;
+        for (int j = 1; j < N; j += 2) inv.push_back(D[j]);
+        const int N2 = (N + 1) / 2;
+        D2.assign(N2, -1);
+        L2.assign(N2, -1);
+        U2.assign(N2, -1);
+        b2.assign(N2, -1);
+        x2.assign(N2, -1);
+        for (int j = 0; j < N; j += 2) {
+            const int jj = j / 2;
+            int al = -1, be = -1, ga = -1, de = -1;
+            if (j >= 1) {
+                al = ns++;
+                ga = ns++;
+                g1.push_back({al, -1, L[j], D[j - 1], -1, -1, 1.0f, 0.0f});
+                g1.push_back({ga, -1, D[j - 1], U[j - 1], -1, -1, 1.0f, 0.0f});
+            }
+            if (j + 1 < N) {
+                be = ns++;
+                de = ns++;
+                g1.push_back({be, -1, U[j], D[j + 1], -1, -1, 1.0f, 0.0f});
+                g1.push_back({de, -1, D[j + 1], L[j + 1], -1, -1, 1.0f, 0.0f});
+            }
+            D2[jj] = ns++;
+            g2.push_back({D2[jj], D[j], (j >= 1) ? L[j] : -1, ga, (j + 1 < N) ? U[j] : -1, de, -1.0f, -1.0f});
+            if (j >= 2) {
+                L2[jj] = ns++;
+                g2.push_back({L2[jj], -1, al, L[j - 1], -1, -1, -1.0f, 0.0f});
+            }
+            if (j + 2 < N) {
+                U2[jj] = ns++;
+                g2.push_back({U2[jj], -1, be, U[j + 1], -1, -1, -1.0f, 0.0f});
+            }
+            b2[jj] = nv++;
+            x2[jj] = nv++;
+            fw.push_back({b2[jj], b[j], (j >= 1) ? b[j - 1] : -1, (j + 1 < N) ? b[j + 1] : -1, al, be, ga, de, -1});
+            bw.push_back({x[j], x2[jj], -1, -1, -1, -1, -1, -1, -1});     // even: copy from the next level
+        }
+        for (int j = 1; j < N; j += 2) {
+            const bool up = (j + 1 < N);
+            bw.push_back({x[j], b[j], x2[(j - 1) / 2], up ? x2[(j + 1) / 2] : -1, L[j], up ? U[j] : -1, U[j - 1],
+                          up ? L[j + 1] : -1, D[j]});
+        }
+        P.inv.push_back(inv);
+        P.g1.push_back(g1);
+        P.g2.push_back(g2);
+        P.fwd.push_back(fw);
+        P.bwd.push_back(bw);
+        D = D2;
+        L = L2;
+        U = U2;
+        b = b2;
+        x = x2;
+        N = N2;
+    }
+    P.final_slot = D[0];
+    P.final_op = {x[0], b[0], -1, -1, -1, -1, -1, -1, D[0]};
+    P.n_slots = ns;
+    P.n_vecs = nv;
+    // layout: header | dls | per level inv, g1, g2, fwd, bwd | final op | slots | vecs
+    int64_t off = 256;
+    P.off_dls = off;
+    off = al256b(off + 4 * (int64_t)P.dls.size());
+    for (size_t lv = 0; lv < P.inv.size(); ++lv) {
+        P.off_inv.push_back(off);
+        off = al256b(off + 4 * (int64_t)P.inv[lv].size());
+        P.off_g1.push_back(off);
+        off = al256b(off + (int64_t)sizeof(BcrGemmOp) * P.g1[lv].size());
+        P.off_g2.push_back(off);
+        off = al256b(off + (int64_t)sizeof(BcrGemmOp) * P.g2[lv].size());
+        P.off_fwd.push_back(off);
+        off = al256b(off + (int64_t)sizeof(BcrVecOp) * P.fwd[lv].size());
+        P.off_bwd.push_back(off);
+        off = al256b(off + (int64_t)sizeof(BcrVecOp) * P.bwd[lv].size());
+    }
+    P.off_final = off;
+    off = al256b(off + sizeof(BcrVecOp) + 4);
+    P.off_slots = off;
+    off = al256b(off + (int64_t)P.n_slots * n * n * 8);
+    P.off_vecs = off;
+    off = al256b(off + (int64_t)P.n_vecs * n * 8);
+    P.bytes = off;
+}
+
+static int bcr_apply_impl(const cv_fdfd_desc& dh, const void* factor, const void* v, void* z, int transpose,
+                          hipStream_t s) {
+    BcrPlan P;
+    bcr_plan(dh.nx, dh.ny, P);
+    char* base = (char*)factor;
+    const cx<float>* slots = (const cx<float>*)(base + P.off_slots);
+    cx<float>* vecs = (cx<float>*)(base + P.off_vecs);
+    const int n = dh.ny, nx = dh.nx;
+    CV_HIP(hipMemcpyAsync(vecs, v, (size_t)nx * n * 8, hipMemcpyDeviceToDevice, s));
+    auto launch = [&](const BcrVecOp* ops, int count) {
+        if (count <= 0) return;
+        if (transpose)
+            k_bcr_vec<true><<<count, kBtdThreads, 0, s>>>(slots, vecs, ops, n);
+        else
+            k_bcr_vec<false><<<count, kBtdThreads, 0, s>>>(slots, vecs, ops, n);
+    };
+    for (size_t lv = 0; lv < P.fwd.size(); ++lv)
+        launch((const BcrVecOp*)(base + P.off_fwd[lv]), (int)P.fwd[lv].size());
+    launch((const BcrVecOp*)(base + P.off_final), 1);
+    for (size_t lv = P.bwd.size(); lv-- > 0;)
+        launch((const BcrVecOp*)(base + P.off_bwd[lv]), (int)P.bwd[lv].size());
+    CV_LAUNCH_CHECK();
+    CV_HIP(hipMemcpyAsync(z, vecs + (int64_t)nx * n, (size_t)nx * n * 8, hipMemcpyDeviceToDevice, s));
+    return CV_OK;
+}
+
+int direct_apply(int kind, const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const void* prof32, const void* m0_32,
+                 const void* m1_32, const void* factor, const void* v, void* z, int transpose, hipStream_t s) {
+    if (kind == CV_FACTOR_BCR) return bcr_apply_impl(dh, factor, v, z, transpose, s);
+    return btd_apply(desc, dh, prof32, m0_32, m1_32, factor, v, z, transpose, s);
+}
+
+}  // namespace cv
+
+extern "C" int64_t cv_fdfd_bcr_factor_bytes(int nx, int ny) {
+    if (nx <= 0 || ny <= 0 || ny > kBtdMax) return 0;
+    BcrPlan P;
+    bcr_plan(nx, ny, P);
+    return P.bytes;
+}
+
+extern "C" int cv_fdfd_bcr_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof32,
+                                  const void* m0_32, const void* m1_32, void* factor, int64_t factor_bytes,
+                                  void* stream) {
+    CV_CHECK(desc && desc_host && prof32 && m0_32 && factor, "null pointer argument");
+    const cv_fdfd_desc dh = desc_host[0];
+    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
+    CV_CHECK(dh.ny <= kBtdMax, "block cyclic reduction needs ny <= %d (ny = %d)", kBtdMax, dh.ny);
+    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported");
+    CV_CHECK(dh.pol == CV_POL_EZ || m1_32, "Hz operator needs m1");
+    BcrPlan P;
+    bcr_plan(dh.nx, dh.ny, P);
+    CV_CHECK(factor_bytes >= P.bytes, "factor buffer too small: %lld < %lld", (long long)factor_bytes,
+             (long long)P.bytes);
+    hipStream_t s = as_stream(stream);
+    char* base = (char*)factor;
+    // tables: staged in one host buffer, one upload, kept alive until the final synchronisation
+    std::vector<char> host((size_t)P.off_slots, 0);
+    auto put = [&](int64_t off, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + off, src, bytes); };
+    put(P.off_dls, P.dls.data(), 4 * P.dls.size());
+    for (size_t lv = 0; lv < P.inv.size(); ++lv) {
+        put(P.off_inv[lv], P.inv[lv].data(), 4 * P.inv[lv].size());
+        put(P.off_g1[lv], P.g1[lv].data(), sizeof(BcrGemmOp) * P.g1[lv].size());
+        put(P.off_g2[lv], P.g2[lv].data(), sizeof(BcrGemmOp) * P.g2[lv].size());
+        put(P.off_fwd[lv], P.fwd[lv].data(), sizeof(BcrVecOp) * P.fwd[lv].size());
+        put(P.off_bwd[lv], P.bwd[lv].data(), sizeof(BcrVecOp) * P.bwd[lv].size());
+    }
+    put(P.off_final, &P.final_op, sizeof(BcrVecOp));
+    put(P.off_final + sizeof(BcrVecOp), &P.final_slot, 4);
+    CV_HIP(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    int* status = (int*)base;   // header word 0
+    k_zero_u32_btd<<<1, 64, 0, s>>>((unsigned int*)status, 1);
+    cx<float>* slots = (cx<float>*)(base + P.off_slots);
+    const int n = dh.ny;
+    k_bcr_build0<<<dh.nx, 256, 0, s>>>(desc, (const cx<float>*)prof32, (const float*)m0_32, (const float*)m1_32, slots,
+                                      (const int*)(base + P.off_dls));
+    for (size_t lv = 0; lv < P.inv.size(); ++lv) {
+        if (!P.inv[lv].empty())
+            k_bcr_inv<<<(int)P.inv[lv].size(), kBtdThreads, 0, s>>>(slots, (const int*)(base + P.off_inv[lv]), n,
+                                                                    status);
+        if (!P.g1[lv].empty())
+            k_bcr_gemm<<<(int)P.g1[lv].size(), kGemmThreads, 0, s>>>(slots, (const BcrGemmOp*)(base + P.off_g1[lv]),
+                                                                      n);
+        if (!P.g2[lv].empty())
+            k_bcr_gemm<<<(int)P.g2[lv].size(), kGemmThreads, 0, s>>>(slots, (const BcrGemmOp*)(base + P.off_g2[lv]),
+                                                                      n);
+    }
+    k_bcr_inv<<<1, kBtdThreads, 0, s>>>(slots, (const int*)(base + P.off_final + sizeof(BcrVecOp)), n, status);
+    CV_LAUNCH_CHECK();
+    int st = 0;
+    CV_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    CV_HIP(hipStreamSynchronize(s));
+    if (st != 0) {
+        set_error("zero pivot in block slot %d of the block cyclic reduction", st - 1);
+        return CV_ERR_SINGULAR;
+    }
+    return CV_OK;
+}
+
+extern "C" int cv_fdfd_bcr_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* factor,
+                                 const void* v, void* z, int transpose, void* stream) {
+    CV_CHECK(desc && desc_host && factor && v && z, "null pointer argument");
+    CV_CHECK(v != z, "v and z must not alias");
+    const cv_fdfd_desc dh = desc_host[0];
+    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.ny <= kBtdMax && dh.cell_off == 0 && dh.nblk == 1,
+             "single instance with ny <= %d required", kBtdMax);
+    return bcr_apply_impl(dh, factor, v, z, transpose, as_stream(stream));
 }

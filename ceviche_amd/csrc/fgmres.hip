@@ -415,15 +415,16 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
 }
 
 namespace cv {
-int btd_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const void* prof32, const void* m0_32,
-              const void* m1_32, const void* factor, const void* v, void* z, int transpose, hipStream_t s);
+int direct_apply(int kind, const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const void* prof32, const void* m0_32,
+                 const void* m1_32, const void* factor, const void* v, void* z, int transpose, hipStream_t s);
 }
 
-extern "C" int cv_fdfd_fgmres_btd(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
-                                  const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
-                                  const void* m1_32, const void* factor, const void* b, void* x, int transpose,
-                                  double tol, int restart, int max_iter, void* workspace, int64_t workspace_bytes,
-                                  cv_solve_info* info, void* stream) {
+extern "C" int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                                     const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                                     const void* m1_32, int factor_kind, const void* factor, const void* b, void* x,
+                                     int transpose, double tol, int restart, int max_iter, void* workspace,
+                                     int64_t workspace_bytes, cv_solve_info* info, void* stream) {
+    CV_CHECK(factor_kind == CV_FACTOR_BTD || factor_kind == CV_FACTOR_BCR, "unknown factor kind %d", factor_kind);
     CV_CHECK(desc && desc_host && prof64 && m0_64 && prof32 && m0_32 && factor && b && x && workspace && info,
              "null pointer argument");
     const cv_fdfd_desc dh = desc_host[0];
@@ -441,7 +442,7 @@ extern "C" int cv_fdfd_fgmres_btd(const cv_fdfd_desc* desc, const cv_fdfd_desc* 
     fg_layout((int64_t)dh.nx * dh.ny, m, 0, (char*)workspace, &w);
     hipStream_t s = as_stream(stream);
     auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
-        return btd_apply(desc, dh, prof32, m0_32, m1_32, factor, v, z, transpose, s);
+        return direct_apply(factor_kind, desc, dh, prof32, m0_32, m1_32, factor, v, z, transpose, s);
     };
     auto after_cycle = [](double, double) {};
     int status;

@@ -440,16 +440,31 @@ int cv_fdfd_btd_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, 
 int cv_fdfd_btd_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof32,
                       const void* m0_32, const void* m1_32, const void* factor, const void* v, void* z,
                       int transpose, void* stream);
+/* The same block-tridiagonal matrix factorised by block cyclic reduction:
+ * ceil(log2 nx) levels, each inverting its odd blocks in parallel (one
+ * workgroup per block) and forming the next level with batched complex64
+ * products; a solve is one reduction sweep and one back-substitution sweep of
+ * batched matrix-vector products per level (A or A^T).  The factor buffer
+ * holds the plan, the blocks and the solve vectors (one solve at a time). */
+int64_t cv_fdfd_bcr_factor_bytes(int nx, int ny);
+int cv_fdfd_bcr_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof32,
+                       const void* m0_32, const void* m1_32, void* factor, int64_t factor_bytes, void* stream);
+int cv_fdfd_bcr_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* factor, const void* v,
+                      void* z, int transpose, void* stream);
+
 /* Solve A x = b (transpose == 0) or A^T x = b to the FP64 relative residual
- * tol by flexible GMRES preconditioned with the factorisation (complex64
- * Arnoldi vectors, complex128 x and residual).  The dropped x-wrap and the
- * complex64 rounding are corrected by the outer iteration.  Workspace:
- * cv_fdfd_fgmres_workspace_bytes(nx, ny, restart, CV_PRECOND_NONE, 0). */
-int cv_fdfd_fgmres_btd(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
-                       const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
-                       const void* m1_32, const void* factor, const void* b, void* x, int transpose, double tol,
-                       int restart, int max_iter, void* workspace, int64_t workspace_bytes, cv_solve_info* info,
-                       void* stream);
+ * tol by flexible GMRES preconditioned with a factorisation of kind
+ * CV_FACTOR_BTD (cv_fdfd_btd_factor) or CV_FACTOR_BCR (cv_fdfd_bcr_factor)
+ * (complex64 Arnoldi vectors, complex128 x and residual).  The dropped x-wrap
+ * and the complex64 rounding are corrected by the outer iteration.
+ * Workspace: cv_fdfd_fgmres_workspace_bytes(nx, ny, restart, CV_PRECOND_NONE, 0). */
+#define CV_FACTOR_BTD 1
+#define CV_FACTOR_BCR 2
+int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                          const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                          const void* m1_32, int factor_kind, const void* factor, const void* b, void* x,
+                          int transpose, double tol, int restart, int max_iter, void* workspace,
+                          int64_t workspace_bytes, cv_solve_info* info, void* stream);
 
 /* ------------------------------------------------------------------ misc */
 const char* cv_version(void);

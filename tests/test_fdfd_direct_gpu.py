@@ -38,15 +38,20 @@ CASES = [('ez', (24, 18), (4, 3), (0.0, 0.0)), ('hz', (19, 23), (3, 4), (0.0, 0.
          ('ez', (120, 120), (20, 20), (0.0, 0.0))]
 
 
-@pytest.mark.parametrize('pol,shape,npml,bloch', CASES)
-def test_factor_and_sweep_invert_the_block_tridiagonal_part(pol, shape, npml, bloch):
+CASES_BCR = CASES + [('ez', (2, 9), (0, 2), (0.0, 0.0)), ('hz', (3, 14), (1, 3), (0.0, 0.4)),
+                     ('ez', (65, 40), (8, 6), (0.0, 0.0))]
+
+
+@pytest.mark.parametrize('kind', ['btd', 'bcr'])
+@pytest.mark.parametrize('pol,shape,npml,bloch', CASES_BCR)
+def test_factor_and_sweep_invert_the_block_tridiagonal_part(pol, shape, npml, bloch, kind):
     from ceviche_amd._fdfd_op import Direct
-    print('btd', pol, shape, npml, bloch, flush=True)
+    print('direct factor', kind, pol, shape, npml, bloch, flush=True)
     op = _op(pol, shape, npml, bloch)
     rng = np.random.default_rng(3)
     eps = rng.uniform(1, 12.25, shape)
     m0, m1 = op.materials(torch.from_numpy(eps.ravel()))
-    D = Direct(op, m0, m1)
+    D = Direct(op, m0, m1, kind=kind)
     nx, ny = shape
     A = _scaled_A(op, pol, eps, npml, bloch)
     if nx > 2:
@@ -59,6 +64,8 @@ def test_factor_and_sweep_invert_the_block_tridiagonal_part(pol, shape, npml, bl
             z = D.apply(torch.from_numpy(v).cuda(), transpose=tr).cpu().numpy()
             ref = spl.spsolve((Anw.T if tr else Anw).tocsc(), v)
             assert np.linalg.norm(z - ref) <= 1e-3 * np.linalg.norm(ref), tr
+    if kind != 'btd':
+        return
     # the first two stored blocks against a float64 block elimination
     S = np.frombuffer(D.factor.cpu().numpy()[:nx * ny * ny * 8].tobytes(), dtype=np.complex64).reshape(nx, ny, ny)
     if Anw is not None:
@@ -71,8 +78,9 @@ def test_factor_and_sweep_invert_the_block_tridiagonal_part(pol, shape, npml, bl
             np.testing.assert_allclose(S[1], S1, rtol=0, atol=1e-4 * np.abs(S1).max())
 
 
-@pytest.mark.parametrize('pol,shape,npml,bloch', CASES)
-def test_direct_solves_match_spsolve(pol, shape, npml, bloch):
+@pytest.mark.parametrize('kind', ['btd', 'bcr'])
+@pytest.mark.parametrize('pol,shape,npml,bloch', CASES_BCR)
+def test_direct_solves_match_spsolve(pol, shape, npml, bloch, kind):
     """A x = b and A^T x = b to the FP64 residual 1e-10 (the x-wrap restored
     by the outer FGMRES)."""
     from ceviche_amd._fdfd_op import Direct, SolverOptions
@@ -80,7 +88,7 @@ def test_direct_solves_match_spsolve(pol, shape, npml, bloch):
     rng = np.random.default_rng(4)
     eps = rng.uniform(1, 12.25, shape)
     m0, m1 = op.materials(torch.from_numpy(eps.ravel()))
-    D = Direct(op, m0, m1)
+    D = Direct(op, m0, m1, kind=kind)
     A = _scaled_A(op, pol, eps, npml, bloch)
     b = rng.standard_normal(eps.size) + 1j * rng.standard_normal(eps.size)
     for tr in (False, True):
@@ -89,7 +97,7 @@ def test_direct_solves_match_spsolve(pol, shape, npml, bloch):
         ref = spl.spsolve((A.T if tr else A).tocsc(), b)
         assert info['status'] == 0 and info['rel_residual'] <= 1e-10, info
         assert np.linalg.norm(x - ref) <= 1e-8 * np.linalg.norm(ref), (tr, info)
-        print(pol, shape, 'transpose' if tr else 'forward', info, flush=True)
+        print(kind, pol, shape, 'transpose' if tr else 'forward', info, flush=True)
 
 
 def test_direct_path_fields_gradient_and_forward_mode():
