@@ -472,23 +472,27 @@ k_bcr_inv(cx<float>* __restrict__ slots, const int* __restrict__ list, int n, in
     if (t == 0 && g.bad != 0) atomicCAS(status, 0, g.bad);
 }
 
-// Batched n x n complex GEMMs (n <= 128): one 256-thread workgroup per op,
-// 8 x 8 outputs per thread, K in chunks of 16 staged through LDS.
+// Batched n x n complex GEMMs (n <= 128): each op is split into four 64 x 64
+// output tiles, one 256-thread workgroup per tile (4 x 4 outputs per thread),
+// K in chunks of 16 staged through LDS; several workgroups share a CU.
 constexpr int kGemmThreads = 256;
 constexpr int kGemmK = 16;
+constexpr int kGemmTile = 64;
 
 __global__ void __launch_bounds__(kGemmThreads)
 k_bcr_gemm(cx<float>* __restrict__ slots, const BcrGemmOp* __restrict__ ops, int n) {
-    __shared__ cx<float> As[kGemmK][kBtdMax];   // As[k][row]
-    __shared__ cx<float> Bs[kGemmK][kBtdMax];   // Bs[k][col]
-    const BcrGemmOp op = ops[blockIdx.x];
+    __shared__ cx<float> As[kGemmK][kGemmTile];   // As[k][row]
+    __shared__ cx<float> Bs[kGemmK][kGemmTile];   // Bs[k][col]
+    const BcrGemmOp op = ops[blockIdx.x >> 2];
+    const int row0 = ((blockIdx.x >> 1) & 1) * kGemmTile, col0 = (blockIdx.x & 1) * kGemmTile;
+    if (row0 >= n || col0 >= n) return;
     const int64_t nn = (int64_t)n * n;
     const int t = threadIdx.x, ty = t / 16, tx = t % 16;
-    cx<float> acc[8][8];
+    cx<float> acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = czero<float>();
+        for (int b = 0; b < 4; ++b) acc[a][b] = czero<float>();
     for (int term = 0; term < 2; ++term) {
         const int sa = term ? op.a2 : op.a1, sb = term ? op.b2 : op.b1;
         if (sa < 0 || sb < 0) continue;
@@ -497,37 +501,39 @@ k_bcr_gemm(cx<float>* __restrict__ slots, const BcrGemmOp* __restrict__ ops, int
         const cx<float>* B = slots + (int64_t)sb * nn;
         for (int k0 = 0; k0 < n; k0 += kGemmK) {
             __syncthreads();
-            for (int e = t; e < kGemmK * kBtdMax; e += kGemmThreads) {
-                const int kk = e / kBtdMax, rc = e % kBtdMax, k = k0 + kk;
-                As[kk][rc] = (rc < n && k < n) ? A[(int64_t)rc * n + k] : czero<float>();
-                Bs[kk][rc] = (rc < n && k < n) ? B[(int64_t)k * n + rc] : czero<float>();
+            for (int e = t; e < kGemmK * kGemmTile; e += kGemmThreads) {
+                const int kk = e / kGemmTile, rc = e % kGemmTile, k = k0 + kk;
+                const int r = row0 + rc, c = col0 + rc;
+                As[kk][rc] = (r < n && k < n) ? A[(int64_t)r * n + k] : czero<float>();
+                Bs[kk][rc] = (c < n && k < n) ? B[(int64_t)k * n + c] : czero<float>();
             }
             __syncthreads();
-#pragma unroll 4
+#pragma unroll
             for (int kk = 0; kk < kGemmK; ++kk) {
-                cx<float> av[8], bv[8];
+                cx<float> av[4], bv[4];
 #pragma unroll
-                for (int a = 0; a < 8; ++a) av[a] = As[kk][8 * ty + a];
-#pragma unroll
-                for (int b = 0; b < 8; ++b) bv[b] = Bs[kk][8 * tx + b];
-#pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    const cx<float> sav = mk<float>(sg * av[a].re, sg * av[a].im);
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) acc[a][b] = acc[a][b] + sav * bv[b];
+                for (int a = 0; a < 4; ++a) {
+                    const cx<float> v = As[kk][4 * ty + a];
+                    av[a] = mk<float>(sg * v.re, sg * v.im);
                 }
+#pragma unroll
+                for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][4 * tx + b];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] = acc[a][b] + av[a] * bv[b];
             }
         }
     }
     cx<float>* C = slots + (int64_t)op.c * nn;
     const cx<float>* X = op.x >= 0 ? slots + (int64_t)op.x * nn : nullptr;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const int r = 8 * ty + a;
+    for (int a = 0; a < 4; ++a) {
+        const int r = row0 + 4 * ty + a;
         if (r >= n) continue;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const int c = 8 * tx + b;
+        for (int b = 0; b < 4; ++b) {
+            const int c = col0 + 4 * tx + b;
             if (c >= n) continue;
             cx<float> v = acc[a][b];
             if (X) v = v + X[(int64_t)r * n + c];
@@ -888,11 +894,11 @@ extern "C" int cv_fdfd_bcr_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* 
             k_bcr_inv<<<(int)P.inv[lv].size(), kBtdThreads, 0, s>>>(slots, (const int*)(base + P.off_inv[lv]), n,
                                                                     status);
         if (!P.g1[lv].empty())
-            k_bcr_gemm<<<(int)P.g1[lv].size(), kGemmThreads, 0, s>>>(slots, (const BcrGemmOp*)(base + P.off_g1[lv]),
-                                                                      n);
+            k_bcr_gemm<<<4 * (int)P.g1[lv].size(), kGemmThreads, 0, s>>>(
+                slots, (const BcrGemmOp*)(base + P.off_g1[lv]), n);
         if (!P.g2[lv].empty())
-            k_bcr_gemm<<<(int)P.g2[lv].size(), kGemmThreads, 0, s>>>(slots, (const BcrGemmOp*)(base + P.off_g2[lv]),
-                                                                      n);
+            k_bcr_gemm<<<4 * (int)P.g2[lv].size(), kGemmThreads, 0, s>>>(
+                slots, (const BcrGemmOp*)(base + P.off_g2[lv]), n);
     }
     k_bcr_inv<<<1, kBtdThreads, 0, s>>>(slots, (const int*)(base + P.off_final + sizeof(BcrVecOp)), n, status);
     CV_LAUNCH_CHECK();
