@@ -386,9 +386,9 @@ def extras_all(main, which='all'):
     def fdfd_mf():
         """Multi-frequency Ez (3 coupled sidebands) on the 1100 x 80
         modulated-waveguide isolator, mode source on the centre sideband:
-        the GPU solve (complex128 BiCGSTAB, FP64 residual 1e-10) against the
-        oracle's spsolve of the assembled system (the reference's default
-        direct path, one core)."""
+        the GPU solve (FP64 residual 1e-10; default path: FGMRES with the
+        sideband blocks factorised) against the oracle's spsolve of the
+        assembled system (the reference's default direct path, one core)."""
         from oracle import fdfd_mf_ref
         from ceviche_amd.modes import insert_mode
         c = workloads.fdfd_mf_isolator()
@@ -423,7 +423,9 @@ def extras_all(main, which='all'):
                 "fp64_rel_residual": info['rel_residual'], "field_rel_err_vs_spsolve": err,
                 "output_power_by_sideband": p,
                 "cpu_spsolve_s": round(cpu, 3), "cpu_cores": 1, "speedup_vs_cpu": round(cpu / dt, 2),
-                "precision": "complex128 BiCGSTAB over all sidebands, coupling fused into the stencil kernels"}
+                "precision": "FGMRES over all sidebands (complex128 iterate, FP64 residual, the coupling fused "
+                             "into the stencil kernels), preconditioned by a complex64 block-cyclic-reduction "
+                             "factorisation of each sideband block"}
 
     def fdfd_batch():
         """Multi-frequency batch (4 blob designs x 4 wavelengths, sizes from

@@ -420,6 +420,70 @@ class Direct:
         return x
 
 
+def blockdiag_applies(op):
+    """True when ``op`` is one frequency-coupled Ez system whose blocks the
+    block factorisation handles (ny <= 128)."""
+    return (len(op.systems) == 1 and op.n_inst == op.systems[0]['nblk'] and op.systems[0]['nblk'] >= 1
+            and op.shapes[0][1] <= BTD_MAX_NY)
+
+
+class BlockDirect:
+    """Block-diagonal preconditioner of a frequency-coupled system: each
+    sideband block's own factorisation (the block as a single instance at its
+    W_m, block cyclic reduction), inside FP64-refined FGMRES on the whole
+    coupled operator (cv_fdfd_fgmres_blockdiag).  One set of factors serves
+    the forward, adjoint and tangent solves."""
+
+    def __init__(self, op, m0, m1):
+        if not blockdiag_applies(op):
+            raise ValueError('the block-diagonal factorisation needs one coupled system with ny <= %d'
+                             % BTD_MAX_NY)
+        self.op = op
+        nb = op.n_inst
+        n0 = op.shapes[0][0] * op.shapes[0][1]
+        if getattr(op, '_block_ops', None) is None:
+            op._block_ops = [FdfdOperator('ez', [{k: v for k, v in sp.items() if k != 'system'}], dtype=op.dtype,
+                                          device=op.device) for sp in op.specs]
+        self.blocks = [Direct(bop, m0[m * n0:(m + 1) * n0], None if m1 is None else m1[m * n0:(m + 1) * n0])
+                       for m, bop in enumerate(op._block_ops)]
+        self.block_desc = np.concatenate([bop.desc_host for bop in op._block_ops])
+        self.factor_ptrs = (ctypes.c_void_p * nb)(*[d.factor.data_ptr() for d in self.blocks])
+        self.kinds = {d.kind for d in self.blocks}
+        if self.kinds != {'bcr'}:
+            raise ValueError('block-diagonal preconditioning uses block cyclic reduction (nx >= 8)')
+
+    def solve(self, bs, m0_64, m1_64, p64, p32, m0_32, m1_32, transpose=False, opts=None):
+        op = self.op
+        opts = opts or SolverOptions()
+        tol = opts.tol if opts.tol is not None else DEFAULT_TOL[torch.complex128]
+        nx, ny = op.shapes[0]
+        nb = op.n_inst
+        L = lib()
+        need = L.cv_fdfd_fgmres_workspace_bytes(nx * nb, ny, int(opts.restart), _lib.CV_PRECOND_NONE, 0)
+        ws = torch.empty(need, dtype=torch.uint8, device=op.device)
+        b = bs.to(torch.complex128).contiguous()
+        x = torch.zeros_like(b)
+        info = (_lib.SolveInfo * nb)()
+        rc = L.cv_fdfd_fgmres_blockdiag(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(p64),
+                                        ptr(m0_64), ptr(m1_64), ptr(p32), ptr(m0_32), ptr(m1_32),
+                                        self.block_desc.ctypes.data_as(ctypes.c_void_p), self.factor_ptrs, ptr(b),
+                                        ptr(x), int(bool(transpose)), float(tol), int(opts.restart),
+                                        int(opts.max_iter), ptr(ws), need, info, stream_ptr())
+        op.last_info = [dict(iterations=i.iterations, status=i.status, restarts=i.restarts,
+                             rel_residual=i.rel_residual, method='direct', factor='bcr-blockdiag') for i in info]
+        check(rc, 'multi-frequency block-diagonal solve')
+        return x
+
+
+def _use_blockdiag(op, opts):
+    method = (opts or SolverOptions()).method
+    if method == 'bicgstab':
+        return False
+    if method == 'direct' and not (blockdiag_applies(op) and op.shapes[0][0] >= 8):
+        raise ValueError("method='direct' needs one coupled system with ny <= %d and nx >= 8" % BTD_MAX_NY)
+    return blockdiag_applies(op) and op.shapes[0][0] >= 8
+
+
 def _use_direct(op, opts):
     method = (opts or SolverOptions()).method
     if method == 'bicgstab':
@@ -510,7 +574,13 @@ class FdfdCoupledSolve(torch.autograd.Function):
         return p64, p32
 
     @staticmethod
-    def _solve(op, op32, opts, m0, m1, bs, transpose, p64, p32):
+    def _solve(op, op32, opts, m0, m1, bs, transpose, p64, p32, direct=None):
+        if direct is not None:
+            pd32 = p64.to(torch.complex64).contiguous()   # coefficients incl. coupling tables, complex64
+            x = direct.solve(bs, m0, m1, p64, pd32, direct.m0_32, direct.m1_32, transpose=transpose, opts=opts)
+            if op32 is not None:
+                op32.last_info = op.last_info
+            return x
         if op32 is None:
             return op.solve_scaled(bs, m0, m1, transpose=transpose, opts=opts, prof=(p64, None))
         m0s = m0.to(torch.float32).contiguous()
@@ -522,7 +592,13 @@ class FdfdCoupledSolve(torch.autograd.Function):
     def forward(ctx, m0, m1, cpl, b, op, op32, opts):
         c = op.scale_vector(torch.float64)
         p64, p32 = FdfdCoupledSolve._profs(op, op32, cpl)
-        x = FdfdCoupledSolve._solve(op, op32, opts, m0, m1, b * c, False, p64, p32)
+        direct = None
+        if _use_blockdiag(op, opts):
+            direct = BlockDirect(op, m0, m1)
+            direct.m0_32 = m0.to(torch.float32).contiguous()
+            direct.m1_32 = None if m1 is None else m1.to(torch.float32).contiguous()
+        x = FdfdCoupledSolve._solve(op, op32, opts, m0, m1, b * c, False, p64, p32, direct)
+        ctx.direct = direct
         ctx.op, ctx.op32, ctx.opts = op, op32, opts
         ctx.p64, ctx.p32 = p64, p32
         ctx.save_for_backward(m0, m1, x)
@@ -544,14 +620,15 @@ class FdfdCoupledSolve(torch.autograd.Function):
             rhs = rhs - op.material_apply(d0, d1, x)
         if dcpl is not None:
             rhs = rhs - op.coupling_apply(op.prof_with(dcpl, torch.complex128), x)
-        return FdfdCoupledSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False, ctx.p64, ctx.p32)
+        return FdfdCoupledSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False, ctx.p64, ctx.p32,
+                                       ctx.direct)
 
     @staticmethod
     def backward(ctx, gx):
         m0, m1, x = ctx.saved_tensors
         op = ctx.op
         mu = FdfdCoupledSolve._solve(op, ctx.op32, ctx.opts, m0, m1, torch.conj(gx).resolve_conj().contiguous(),
-                                     True, ctx.p64, ctx.p32)
+                                     True, ctx.p64, ctx.p32, ctx.direct)
         c = op.scale_vector(torch.float64)
         gb = torch.conj(mu).resolve_conj() * c if ctx.needs_input_grad[3] else None
         g0 = g1 = None

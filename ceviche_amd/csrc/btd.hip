@@ -813,8 +813,8 @@ static void bcr_plan(int nx, int n, BcrPlan& P) {
     P.bytes = off;
 }
 
-static int bcr_apply_impl(const cv_fdfd_desc& dh, const void* factor, const void* v, void* z, int transpose,
-                          hipStream_t s) {
+int bcr_apply_impl(const cv_fdfd_desc& dh, const void* factor, const void* v, void* z, int transpose,
+                   hipStream_t s) {
     BcrPlan P;
     bcr_plan(dh.nx, dh.ny, P);
     char* base = (char*)factor;

@@ -40,15 +40,20 @@ k_fg_resid(const cv_fdfd_desc* __restrict__ desc, const cx<double>* __restrict__
            const double* __restrict__ m1, const cx<double>* __restrict__ b, const cx<double>* __restrict__ x,
            cx<double>* __restrict__ r, double* __restrict__ partial) {
     __shared__ double lds[kWaves * 2];
-    const cv_fdfd_desc d = load_desc(desc, 0);
-    const int64_t n = (int64_t)d.nx * d.ny;
+    // one instance, or the equal-sized blocks of one frequency-coupled system
+    const cv_fdfd_desc d0 = load_desc(desc, 0);
+    const int64_t n0 = (int64_t)d0.nx * d0.ny;
+    const int64_t n = n0 * d0.nblk;
     auto fx = [&](int64_t k) { return x[k]; };
     double acc[2] = {0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < kFgCPT; ++q) {
         const int64_t k = (int64_t)blockIdx.x * kFgCells + q * kBlock + threadIdx.x;
         if (k >= n) break;
-        const int i = (int)(k / d.ny), j = (int)(k - (int64_t)i * d.ny);
+        const int inst = (int)(k / n0);
+        const cv_fdfd_desc d = (inst == 0) ? d0 : load_desc(desc, inst);
+        const int64_t loc = k - (int64_t)inst * n0;
+        const int i = (int)(loc / d.ny), j = (int)(loc - (int64_t)i * d.ny);
         const cx<double> bk = b[k];
         const cx<double> rk = bk - apply_any<double, TRANS>(d, prof, m0, m1, i, j, fx, x[k]);
         r[k] = rk;
@@ -89,11 +94,14 @@ template <bool TRANS>
 __global__ void __launch_bounds__(kBlock)
 k_fg_apply(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
            const float* __restrict__ m1, const cx<float>* __restrict__ z, cx<float>* __restrict__ w) {
-    const cv_fdfd_desc d = load_desc(desc, 0);
-    const int64_t n = (int64_t)d.nx * d.ny;
+    const cv_fdfd_desc d0 = load_desc(desc, 0);
+    const int64_t n0 = (int64_t)d0.nx * d0.ny;
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    const int i = (int)(k / d.ny), j = (int)(k - (int64_t)i * d.ny);
+    if (k >= n0 * d0.nblk) return;
+    const int inst = (int)(k / n0);
+    const cv_fdfd_desc d = (inst == 0) ? d0 : load_desc(desc, inst);
+    const int64_t loc = k - (int64_t)inst * n0;
+    const int i = (int)(loc / d.ny), j = (int)(loc - (int64_t)i * d.ny);
     auto fz = [&](int64_t q) { return z[q]; };
     w[k] = apply_any<float, TRANS>(d, prof, m0, m1, i, j, fz, z[k]);
 }
@@ -238,7 +246,7 @@ static int fgmres_core(const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const v
                        const void* m1_64, const void* prof32, const void* m0_32, const void* m1_32, const void* b,
                        void* x, double tol, int m, int max_iter, double inner_rtol, FgWork& w, Precond&& precondition,
                        AfterCycle&& after_cycle, cv_solve_info* info, hipStream_t s) {
-    const int64_t n = (int64_t)dh.nx * dh.ny;
+    const int64_t n = (int64_t)dh.nx * dh.ny * dh.nblk;
     const int n_wg = (int)((n + kFgCells - 1) / kFgCells);
     const int gv = (int)((n + kBlock - 1) / kBlock);
     const cx<double>* b64 = (const cx<double>*)b;
@@ -452,6 +460,64 @@ extern "C" int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_des
     else
         status = fgmres_core<false>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
                                     w, precondition, after_cycle, info, s);
+    info[0].pad = 0;
+    return status;
+}
+
+namespace cv {
+int bcr_apply_impl(const cv_fdfd_desc& dh, const void* factor, const void* v, void* z, int transpose, hipStream_t s);
+}
+
+extern "C" int cv_fdfd_fgmres_blockdiag(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                                        const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                                        const void* m1_32, const cv_fdfd_desc* block_desc_host,
+                                        const void* const* factors, const void* b, void* x, int transpose, double tol,
+                                        int restart, int max_iter, void* workspace, int64_t workspace_bytes,
+                                        cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && prof64 && m0_64 && prof32 && m0_32 && block_desc_host && factors && b && x &&
+                 workspace && info,
+             "null pointer argument");
+    const cv_fdfd_desc dh = desc_host[0];
+    const int nb = dh.nblk;
+    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0 && dh.blk == 0 && nb >= 1, "block 0 of a system at cell 0 is required");
+    CV_CHECK(dh.ny <= 128, "the block factorisations need ny <= 128");
+    const int64_t n0 = (int64_t)dh.nx * dh.ny;
+    for (int m = 0; m < nb; ++m) {
+        const cv_fdfd_desc& d = desc_host[m];
+        CV_CHECK(d.nx == dh.nx && d.ny == dh.ny && d.blk == m && d.nblk == nb && d.cell_off == m * n0 &&
+                     d.n_cpl == dh.n_cpl && d.cpl_off == dh.cpl_off && d.w2_off == dh.w2_off && d.pol == CV_POL_EZ,
+                 "block %d does not match block 0 of its system", m);
+        const cv_fdfd_desc& f = block_desc_host[m];
+        CV_CHECK(f.nx == dh.nx && f.ny == dh.ny && f.nblk == 1 && f.cell_off == 0 && factors[m],
+                 "factor %d does not describe one block of the system", m);
+    }
+    CV_CHECK(restart >= 1 && restart <= kFgMaxM, "restart must be in [1, %d]", kFgMaxM);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    const int m = restart;
+    const int64_t need = cv_fdfd_fgmres_workspace_bytes(dh.nx * nb, dh.ny, m, CV_PRECOND_NONE, 0);
+    CV_CHECK(need > 0 && workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+             (long long)need);
+    FgWork w;
+    fg_layout(n0 * nb, m, 0, (char*)workspace, &w);
+    hipStream_t s = as_stream(stream);
+    // block-diagonal preconditioner: each sideband block's own factorisation,
+    // the inter-block coupling left to the outer iteration
+    auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
+        for (int q = 0; q < nb; ++q) {
+            const int rc = bcr_apply_impl(block_desc_host[q], factors[q], v + q * n0, z + q * n0, transpose, s);
+            if (rc != CV_OK) return rc;
+        }
+        return CV_OK;
+    };
+    auto after_cycle = [](double, double) {};
+    int status;
+    if (transpose)
+        status = fgmres_core<true>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
+                                   w, precondition, after_cycle, info, s);
+    else
+        status = fgmres_core<false>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
+                                    w, precondition, after_cycle, info, s);
+    for (int q = 0; q < nb; ++q) info[q] = info[0];
     info[0].pad = 0;
     return status;
 }

@@ -458,6 +458,18 @@ int cv_fdfd_bcr_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, c
  * (complex64 Arnoldi vectors, complex128 x and residual).  The dropped x-wrap
  * and the complex64 rounding are corrected by the outer iteration.
  * Workspace: cv_fdfd_fgmres_workspace_bytes(nx, ny, restart, CV_PRECOND_NONE, 0). */
+/* A frequency-coupled system (desc / desc_host: its nblk consecutive blocks,
+ * prof with the coupling tables) solved by the same FP64-refined FGMRES,
+ * preconditioned block-diagonally: block m with the block-cyclic-reduction
+ * factor factors[m] of that block alone (block_desc_host[m]: the block as a
+ * single instance, ny <= 128); the inter-block coupling of fdfd.py:303-324 is
+ * left to the outer iteration.  info: nblk entries (all the system's). */
+int cv_fdfd_fgmres_blockdiag(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                             const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                             const void* m1_32, const cv_fdfd_desc* block_desc_host, const void* const* factors,
+                             const void* b, void* x, int transpose, double tol, int restart, int max_iter,
+                             void* workspace, int64_t workspace_bytes, cv_solve_info* info, void* stream);
+
 #define CV_FACTOR_BTD 1
 #define CV_FACTOR_BCR 2
 int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,

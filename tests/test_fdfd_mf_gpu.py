@@ -1,5 +1,6 @@
-"""GPU parity of the multi-frequency Ez FDFD (fdfd_mf_ez: coupled-block stencil
-+ one BiCGSTAB over all sidebands + adjoints) against the scipy oracle."""
+"""GPU parity of the multi-frequency Ez FDFD (fdfd_mf_ez: coupled-block stencil,
+FGMRES with block-factorised sidebands or BiCGSTAB over all sidebands,
+adjoints) against the scipy oracle."""
 import numpy as np
 import pytest
 import torch
@@ -34,18 +35,21 @@ def _rel(a, b):
     ((21, 14), 3, 1, [3, 3], (0.0, -0.4)),     # harmonic 3 couples nothing with 3 sidebands
     ((16, 16), 1, 0, [4, 4], (0.0, 0.0)),      # one sideband: plain Ez
 ])
-def test_fields_match_oracle(shape, Nfreq, Nsb, npml, bloch):
+@pytest.mark.parametrize('method', ['auto', 'bicgstab'])
+def test_fields_match_oracle(shape, Nfreq, Nsb, npml, bloch, method):
     import ceviche_amd as cv
-    print('mf fields', shape, Nfreq, Nsb, npml, bloch, flush=True)
+    print('mf fields', method, shape, Nfreq, Nsb, npml, bloch, flush=True)
     eps, delta, phi, src = _problem(7, shape, Nfreq, Nsb)
     # a source on every sideband exercises every block's right-hand side
     src = src + 0.3 * np.random.default_rng(8).standard_normal(src.shape)
     F = cv.fdfd_mf_ez(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, Nsb, npml, bloch_phases=list(bloch),
-                      solver_options=cv.SolverOptions(tol=1e-12))
+                      solver_options=cv.SolverOptions(tol=1e-12, method=method))
     hx, hy, ez = F.solve(src)
     rhx, rhy, rez = MF.solve_mf(OMEGA, 1e-6, eps, OMEGA_MOD, delta, phi, Nsb, npml, src, bloch)
     assert ez.shape == (2 * Nsb + 1,) + shape and isinstance(ez, np.ndarray)
     info = F.last_solve_info
+    if Nsb > 0:
+        assert info.get('factor', 'bicgstab') == ('bcr-blockdiag' if method == 'auto' else 'bicgstab'), info
     assert info['status'] == 0 and info['rel_residual'] <= 1e-12
     for a, b in ((ez, rez), (hx, rhx), (hy, rhy)):
         assert _rel(a, b) <= 1e-8
