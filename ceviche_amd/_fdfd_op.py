@@ -355,7 +355,7 @@ class Direct:
     One factorisation serves the forward, adjoint (A^T) and tangent solves of
     a linear system."""
 
-    def __init__(self, op, m0, m1, kind=None):
+    def __init__(self, op, m0, m1, kind=None, factorise=True):
         if not direct_applies(op):
             raise ValueError('the block-tridiagonal factorisation needs one uncoupled instance with ny <= %d'
                              % BTD_MAX_NY)
@@ -375,8 +375,10 @@ class Direct:
         if kind == 'bcr':
             nbytes = L.cv_fdfd_bcr_factor_bytes(nx, ny)
             self.factor = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
-            check(L.cv_fdfd_bcr_factor(ptr(op.desc), dh, ptr(self.prof32), ptr(self.m0), ptr(self.m1),
-                                       ptr(self.factor), nbytes, stream_ptr()), 'block cyclic reduction')
+            self.factor_bytes = nbytes
+            if factorise:       # otherwise the caller factorises several at once
+                check(L.cv_fdfd_bcr_factor(ptr(op.desc), dh, ptr(self.prof32), ptr(self.m0), ptr(self.m1),
+                                           ptr(self.factor), nbytes, stream_ptr()), 'block cyclic reduction')
         else:
             nbytes = L.cv_fdfd_btd_factor_bytes(nx, ny)
             self.factor = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
@@ -444,13 +446,22 @@ class BlockDirect:
         if getattr(op, '_block_ops', None) is None:
             op._block_ops = [FdfdOperator('ez', [{k: v for k, v in sp.items() if k != 'system'}], dtype=op.dtype,
                                           device=op.device) for sp in op.specs]
-        self.blocks = [Direct(bop, m0[m * n0:(m + 1) * n0], None if m1 is None else m1[m * n0:(m + 1) * n0])
+        self.blocks = [Direct(bop, m0[m * n0:(m + 1) * n0], None if m1 is None else m1[m * n0:(m + 1) * n0],
+                              factorise=False)
                        for m, bop in enumerate(op._block_ops)]
         self.block_desc = np.concatenate([bop.desc_host for bop in op._block_ops])
         self.factor_ptrs = (ctypes.c_void_p * nb)(*[d.factor.data_ptr() for d in self.blocks])
         self.kinds = {d.kind for d in self.blocks}
         if self.kinds != {'bcr'}:
             raise ValueError('block-diagonal preconditioning uses block cyclic reduction (nx >= 8)')
+        # every sideband block factorised in the same launches
+        arr = lambda vals: (ctypes.c_void_p * nb)(*vals)
+        blk = self.blocks
+        check(lib().cv_fdfd_bcr_factor_batch(
+            nb, arr([bop.desc.data_ptr() for bop in op._block_ops]), self.block_desc.ctypes.data_as(ctypes.c_void_p),
+            arr([d.prof32.data_ptr() for d in blk]), arr([d.m0.data_ptr() for d in blk]),
+            arr([0 if d.m1 is None else d.m1.data_ptr() for d in blk]), self.factor_ptrs, blk[0].factor_bytes,
+            stream_ptr()), 'block cyclic reduction of the sideband blocks')
 
     def solve(self, bs, m0_64, m1_64, p64, p32, m0_32, m1_32, transpose=False, opts=None):
         op = self.op

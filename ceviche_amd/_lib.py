@@ -116,6 +116,7 @@ _SIGS = {
     'cv_fdfd_btd_apply': (I, [P, P, P, P, P, P, P, P, I, P]),
     'cv_fdfd_bcr_factor_bytes': (I64, [I, I]),
     'cv_fdfd_bcr_factor': (I, [P, P, P, P, P, P, I64, P]),
+    'cv_fdfd_bcr_factor_batch': (I, [I, P, P, P, P, P, P, I64, P]),
     'cv_fdfd_bcr_apply': (I, [P, P, P, P, P, I, P]),
     'cv_fdfd_fgmres_blockdiag': (I, [P, P, P, P, P, P, P, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
     'cv_fdfd_fgmres_direct': (I, [P, P, P, P, P, P, P, P, I, P, P, P, I, D, I, I, P, I64, P, P]),

@@ -24,6 +24,7 @@
 // column staged through LDS.
 #include "fdfd_stencil.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace cv {
@@ -155,38 +156,56 @@ __device__ void gj_invert(cx<float> (&m)[kFRows][kFCols], int n, int r0, int w, 
             }
         }
         __syncthreads();
-        // rows k and pr exchange; then row k /= pivot and the other rows
-        // lose their column-k component:
-        //   S[k][c] = rj = (c == k ? 1 : S[k][c]) / piv
-        //   S[r][c] = (c == k ? 0 : S[r][c]) - f_r rj     (f_r = S[r][k] before the step)
+        // Row k becomes the pivot row scaled by 1/piv (with 1/piv in column
+        // k); every other row r loses f_r times it, f_r = S[r][k] after the
+        // swap.  Since S[r][k] = f_r, subtracting f_r (rj + e_k) leaves
+        // -f_r / piv in column k, so one update form serves every entry:
+        //   rj[c] = (c == k ? 1 : S[pr][c]) / piv,  rjx = rj + e_k,
+        //   S[r][c] -= f_r rjx[c]  (r != k),  S[k][c] = rj[c].
+        // Row pr first takes the old row k (its owner wave only).
         const cx<float> ck = colbuf[b][pr], cp = colbuf[b][k];
         const float d2 = ck.re * ck.re + ck.im * ck.im;
         const float inv_d2 = (d2 > 0.0f) ? 1.0f / d2 : 0.0f;
         const cx<float> ipiv = mk<float>(ck.re * inv_d2, -ck.im * inv_d2);
-        cx<float> rj[kFCols];
+        cx<float> rj[kFCols], rjx[kFCols];
 #pragma unroll
         for (int q = 0; q < kFCols; ++q) {
             const int c = l + 64 * q;
-            rj[q] = ((c == k) ? mk<float>(1.0f, 0.0f) : rowbuf[b][c]) * ipiv;
+            const bool isk = (c == k);
+            rj[q] = (isk ? mk<float>(1.0f, 0.0f) : rowbuf[b][c]) * ipiv;
+            rjx[q] = mk<float>(rj[q].re + (isk ? 1.0f : 0.0f), rj[q].im);
         }
-        const bool swapped = (pr != k);
-        cx<float> sw[kFCols];
-#pragma unroll
-        for (int q = 0; q < kFCols; ++q) sw[q] = swapbuf[l + 64 * q];
-#pragma unroll
-        for (int p = 0; p < kFRows; ++p) {
-            const int r = r0 + p;           // uniform across the wave
-            const bool is_k = (r == k);
-            const bool is_pr = swapped && (r == pr);
-            const cx<float> f = is_pr ? cp : colbuf[b][r];
+        const bool own_pr = (pr != k) && ((pr >> 3) == w);
+        if (own_pr) {
+            const int pp = pr - r0;
 #pragma unroll
             for (int q = 0; q < kFCols; ++q) {
-                const int c = l + 64 * q;
-                const cx<float> cur = is_pr ? sw[q] : m[p][q];
-                const cx<float> keep = (c == k) ? czero<float>() : cur;
-                const cx<float> upd = keep - f * rj[q];
-                m[p][q] = is_k ? rj[q] : upd;
+                const cx<float> sw = swapbuf[l + 64 * q];
+#pragma unroll
+                for (int p = 0; p < kFRows; ++p) m[p][q] = (p == pp) ? sw : m[p][q];
             }
+        }
+        cx<float> f[kFRows];
+#pragma unroll
+        for (int p = 0; p < kFRows; ++p) f[p] = colbuf[b][r0 + p];
+        if (own_pr) {
+            const int pp = pr - r0;
+#pragma unroll
+            for (int p = 0; p < kFRows; ++p) f[p] = (p == pp) ? cp : f[p];
+        }
+#pragma unroll
+        for (int p = 0; p < kFRows; ++p)
+#pragma unroll
+            for (int q = 0; q < kFCols; ++q) {   // four FMAs per entry
+                m[p][q].re = fmaf(-f[p].re, rjx[q].re, fmaf(f[p].im, rjx[q].im, m[p][q].re));
+                m[p][q].im = fmaf(-f[p].re, rjx[q].im, fmaf(-f[p].im, rjx[q].re, m[p][q].im));
+            }
+        if ((k >> 3) == w) {
+            const int pk = k - r0;
+#pragma unroll
+            for (int q = 0; q < kFCols; ++q)
+#pragma unroll
+                for (int p = 0; p < kFRows; ++p) m[p][q] = (p == pk) ? rj[q] : m[p][q];
         }
     }
     // undo the column interchanges: logical column j is physical column colmap[j]
@@ -402,11 +421,32 @@ struct BcrVecOp {      // t = b - M1 u1 - M2 u2 (A^T: M1t^T, M2t^T); out = M0 t 
     int out, b, u1, u2, m1, m2, m1t, m2t, m0;
 };
 
+// Several factorisations of the same shape run as one (blockIdx.y = which);
+// their op tables are identical, only the slot arrays differ.
+constexpr int kBcrBatch = 8;
+
+struct BcrSlots {
+    cx<float>* slots[kBcrBatch];
+    int* status[kBcrBatch];
+};
+
+struct BcrBuild {
+    const cv_fdfd_desc* desc[kBcrBatch];
+    const cx<float>* prof[kBcrBatch];
+    const float* m0[kBcrBatch];
+    const float* m1[kBcrBatch];
+};
+
 // D_i, L_i = diag(lo_i), U_i = diag(up_i) of level 0 (x-wrap dropped);
 // dls[3 i + 0 / 1 / 2] = slots of D_i, L_i, U_i (-1: none)
 __global__ void __launch_bounds__(256)
-k_bcr_build0(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
-             const float* __restrict__ m1, cx<float>* __restrict__ slots, const int* __restrict__ dls) {
+k_bcr_build0(BcrBuild in, BcrSlots out, const int* __restrict__ dls) {
+    const int fb = blockIdx.y;
+    const cv_fdfd_desc* __restrict__ desc = in.desc[fb];
+    const cx<float>* __restrict__ prof = in.prof[fb];
+    const float* __restrict__ m0 = in.m0[fb];
+    const float* __restrict__ m1 = in.m1[fb];
+    cx<float>* __restrict__ slots = out.slots[fb];
     __shared__ cx<float> cdg[kBtdMax], clo[kBtdMax], cup[kBtdMax], cym[kBtdMax], cyp[kBtdMax];
     const cv_fdfd_desc d = load_desc(desc, 0);
     const int n = d.ny, nx = d.nx, i = blockIdx.x;
@@ -444,8 +484,10 @@ __global__ void k_zero_u32_btd(unsigned int* p, int n) {
 // In-place inversion of the listed slots, one workgroup each; a zero pivot
 // records 1 + slot in *status.
 __global__ void __launch_bounds__(kBtdThreads)
-k_bcr_inv(cx<float>* __restrict__ slots, const int* __restrict__ list, int n, int* __restrict__ status) {
+k_bcr_inv(BcrSlots B, const int* __restrict__ list, int n) {
     __shared__ GjLds g;
+    cx<float>* __restrict__ slots = B.slots[blockIdx.y];
+    int* __restrict__ status = B.status[blockIdx.y];
     const int t = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
     const int r0 = w * kFRows;
@@ -480,7 +522,8 @@ constexpr int kGemmK = 16;
 constexpr int kGemmTile = 64;
 
 __global__ void __launch_bounds__(kGemmThreads)
-k_bcr_gemm(cx<float>* __restrict__ slots, const BcrGemmOp* __restrict__ ops, int n) {
+k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
+    cx<float>* __restrict__ slots = B.slots[blockIdx.y];
     __shared__ cx<float> As[kGemmK][kGemmTile];   // As[k][row]
     __shared__ cx<float> Bs[kGemmK][kGemmTile];   // Bs[k][col]
     const BcrGemmOp op = ops[blockIdx.x >> 2];
@@ -542,73 +585,127 @@ k_bcr_gemm(cx<float>* __restrict__ slots, const BcrGemmOp* __restrict__ ops, int
     }
 }
 
-// part[] += sg M x (or sg M^T x) for one n x n slot, x and part in LDS,
-// 1024 threads (8 per row / column).  Ends with a barrier.
+// One BCR solve op per workgroup of 1024 threads, several factorisations of
+// the same shape at once (blockIdx.y).  Every matrix entry a thread needs is
+// loaded at the start, so each op waits for memory once instead of once per
+// row loop:
+//   A:   thread (row r = t / 8, part = t % 8) holds columns part, part + 8, ...
+//        and 8 lanes combine by shuffles;
+//   A^T: thread (column c = t % 128, part = t / 128) holds rows part, part + 8,
+//        ..., coalesced along c, and the 8 parts combine through LDS.
+// Vector index i < nx is the right-hand side v, nx <= i < 2 nx the solution
+// z (level 0, read from / written to the caller's buffers directly); the
+// others live in the factor's work vectors.
+constexpr int kVecPer = kBtdMax / 8;    // entries per thread and matrix
+
+struct BcrVecBatch {
+    const cx<float>* slots[kBcrBatch];
+    cx<float>* work[kBcrBatch];
+    const cx<float>* v[kBcrBatch];
+    cx<float>* z[kBcrBatch];
+};
+
 template <bool T>
-__device__ __forceinline__ void bcr_matvec(const cx<float>* __restrict__ M, const cx<float>* x, cx<float>* part,
-                                           cx<float> (&red)[8][kBtdMax], int n, float sg) {
+__device__ __forceinline__ void bcr_load(const cx<float>* __restrict__ M, int n, cx<float> (&e)[kVecPer]) {
+    const int t = threadIdx.x;
+    const int a = T ? t % kBtdMax : t / 8, p = T ? t / kBtdMax : t % 8;
+#pragma unroll
+    for (int i = 0; i < kVecPer; ++i) {
+        const int b = p + 8 * i;
+        const int r = T ? b : a, c = T ? a : b;
+        e[i] = (r < n && c < n) ? M[(int64_t)r * n + c] : czero<float>();
+    }
+}
+
+// sum over this thread's held entries of M x (A: along its row; A^T: down its column)
+__device__ __forceinline__ cx<float> bcr_dot(const cx<float> (&e)[kVecPer], const cx<float>* x, int p) {
+    cx<float> acc = czero<float>();
+#pragma unroll
+    for (int i = 0; i < kVecPer; ++i) {
+        const cx<float> xv = x[p + 8 * i];
+        acc.re = fmaf(e[i].re, xv.re, fmaf(-e[i].im, xv.im, acc.re));
+        acc.im = fmaf(e[i].re, xv.im, fmaf(e[i].im, xv.re, acc.im));
+    }
+    return acc;
+}
+
+// out[] = base[] + sg * (sum over the 8 parts), ends with a barrier
+template <bool T>
+__device__ __forceinline__ void bcr_combine(cx<float> acc, const cx<float>* base, cx<float>* out,
+                                            cx<float> (&red)[8][kBtdMax], int n, float sg) {
     const int t = threadIdx.x;
     if (!T) {
-        // thread (r = t / 8, cp = t % 8) sums columns cp, cp + 8, ...
-        const int r = t / 8, cp = t % 8;
-        cx<float> acc = czero<float>();
-        if (r < n)
-            for (int c = cp; c < n; c += 8) acc = acc + M[(int64_t)r * n + c] * x[c];
 #pragma unroll
         for (int off = 1; off < 8; off <<= 1) {
             acc.re += __shfl_xor(acc.re, off, 64);
             acc.im += __shfl_xor(acc.im, off, 64);
         }
-        if (cp == 0 && r < n) part[r] = part[r] + mk<float>(sg * acc.re, sg * acc.im);
+        const int r = t / 8;
+        if ((t % 8) == 0 && r < n) out[r] = base[r] + mk<float>(sg * acc.re, sg * acc.im);
     } else {
-        // thread (c = t % 128, rp = t / 128) sums rows rp, rp + 8, ...
-        const int c = t % kBtdMax, rp = t / kBtdMax;
-        cx<float> acc = czero<float>();
-        if (c < n)
-            for (int r = rp; r < n; r += 8) acc = acc + M[(int64_t)r * n + c] * x[r];
-        red[rp][c] = acc;
+        red[t / kBtdMax][t % kBtdMax] = acc;
         __syncthreads();
         if (t < n) {
             cx<float> s2 = czero<float>();
 #pragma unroll
             for (int g = 0; g < 8; ++g) s2 = s2 + red[g][t];
-            part[t] = part[t] + mk<float>(sg * s2.re, sg * s2.im);
+            out[t] = base[t] + mk<float>(sg * s2.re, sg * s2.im);
         }
     }
     __syncthreads();
 }
 
-// One workgroup per solve op; vectors are n-entry slices of vecs.
 template <bool TRANS>
 __global__ void __launch_bounds__(kBtdThreads)
-k_bcr_vec(const cx<float>* __restrict__ slots, cx<float>* __restrict__ vecs, const BcrVecOp* __restrict__ ops,
-          int n) {
-    __shared__ cx<float> tv[kBtdMax], xs[kBtdMax], res[kBtdMax];
+k_bcr_vec(BcrVecBatch B, const BcrVecOp* __restrict__ ops, int n, int nx) {
+    __shared__ cx<float> bs[kBtdMax], x1[kBtdMax], x2[kBtdMax], tv[kBtdMax], res[kBtdMax];
     __shared__ cx<float> red[8][kBtdMax];
+    const int f = blockIdx.y;
+    const cx<float>* __restrict__ slots = B.slots[f];
+    cx<float>* const work = B.work[f];
     const BcrVecOp op = ops[blockIdx.x];
     const int64_t nn = (int64_t)n * n;
     const int t = threadIdx.x;
-    if (t < n) tv[t] = vecs[(int64_t)op.b * n + t];
-    __syncthreads();
+    const int p = TRANS ? t / kBtdMax : t % 8;
+    auto vec_in = [&](int i) -> const cx<float>* {
+        return i < nx ? B.v[f] + (int64_t)i * n : (i < 2 * nx ? B.z[f] + (int64_t)(i - nx) * n : work + (int64_t)i * n);
+    };
     const int m1 = TRANS ? op.m1t : op.m1, m2 = TRANS ? op.m2t : op.m2;
-    if (op.u1 >= 0 && m1 >= 0) {
-        if (t < n) xs[t] = vecs[(int64_t)op.u1 * n + t];
-        __syncthreads();
-        bcr_matvec<TRANS>(slots + (int64_t)m1 * nn, xs, tv, red, n, -1.0f);
+    const bool h1 = op.u1 >= 0 && m1 >= 0, h2 = op.u2 >= 0 && m2 >= 0, h0 = op.m0 >= 0;
+    cx<float> e1[kVecPer], e2[kVecPer], e0[kVecPer];
+    if (h1) bcr_load<TRANS>(slots + (int64_t)m1 * nn, n, e1);
+    if (h2) bcr_load<TRANS>(slots + (int64_t)m2 * nn, n, e2);
+    if (h0) bcr_load<TRANS>(slots + (int64_t)op.m0 * nn, n, e0);
+    if (t < kBtdMax) {
+        const bool in = t < n;
+        bs[t] = in ? vec_in(op.b)[t] : czero<float>();
+        tv[t] = czero<float>();
+        x1[t] = (in && h1) ? vec_in(op.u1)[t] : czero<float>();
+        x2[t] = (in && h2) ? vec_in(op.u2)[t] : czero<float>();
     }
-    if (op.u2 >= 0 && m2 >= 0) {
-        if (t < n) xs[t] = vecs[(int64_t)op.u2 * n + t];
-        __syncthreads();
-        bcr_matvec<TRANS>(slots + (int64_t)m2 * nn, xs, tv, red, n, -1.0f);
+    __syncthreads();
+    // t = b - M1 u1 - M2 u2
+    cx<float> acc = czero<float>();
+    if (h1) acc = acc + bcr_dot(e1, x1, p);
+    if (h2) acc = acc + bcr_dot(e2, x2, p);
+    // ops never write a right-hand side (index < nx)
+    cx<float>* const outp = (op.out < 2 * nx) ? B.z[f] + (int64_t)(op.out - nx) * n : work + (int64_t)op.out * n;
+    if (!h0) {
+        if (h1 || h2) {
+            bcr_combine<TRANS>(acc, bs, tv, red, n, -1.0f);
+            if (t < n) outp[t] = tv[t];
+        } else if (t < n) {
+            outp[t] = bs[t];
+        }
+        return;
     }
-    if (op.m0 >= 0) {
-        if (t < n) res[t] = czero<float>();
-        __syncthreads();
-        bcr_matvec<TRANS>(slots + (int64_t)op.m0 * nn, tv, res, red, n, 1.0f);
-        if (t < n) vecs[(int64_t)op.out * n + t] = res[t];
-    } else if (t < n) {
-        vecs[(int64_t)op.out * n + t] = tv[t];
-    }
+    if (h1 || h2) bcr_combine<TRANS>(acc, bs, tv, red, n, -1.0f);
+    const cx<float>* const tin = (h1 || h2) ? tv : bs;
+    // out = M0 t
+    if (t < kBtdMax) res[t] = czero<float>();
+    __syncthreads();
+    bcr_combine<TRANS>(bcr_dot(e0, tin, p), res, res, red, n, 1.0f);
+    if (t < n) outp[t] = res[t];
 }
 
 }  // namespace cv
@@ -813,30 +910,49 @@ static void bcr_plan(int nx, int n, BcrPlan& P) {
     P.bytes = off;
 }
 
+// the plan of the last shape applied on this host thread (applies repeat it)
+static const BcrPlan& cached_plan(int nx, int n) {
+    thread_local BcrPlan P;
+    if (P.nx != nx || P.n != n) bcr_plan(nx, n, P);
+    return P;
+}
+
+int bcr_apply_batch(const cv_fdfd_desc& dh, int nb, const void* const* factors, const cx<float>* v, cx<float>* z,
+                    int64_t stride, int transpose, hipStream_t s) {
+    const BcrPlan& P = cached_plan(dh.nx, dh.ny);
+    const int n = dh.ny, nx = dh.nx;
+    const char* base0 = (const char*)factors[0];
+    for (int f0 = 0; f0 < nb; f0 += kBcrBatch) {
+        const int cnt = std::min(kBcrBatch, nb - f0);
+        BcrVecBatch B{};
+        for (int q = 0; q < cnt; ++q) {
+            char* base = (char*)factors[f0 + q];
+            B.slots[q] = (const cx<float>*)(base + P.off_slots);
+            B.work[q] = (cx<float>*)(base + P.off_vecs);
+            B.v[q] = v + (int64_t)(f0 + q) * stride;
+            B.z[q] = z + (int64_t)(f0 + q) * stride;
+        }
+        // every factor of this shape holds the same op tables: use the first one's
+        auto launch = [&](int64_t off, int count) {
+            if (count <= 0) return;
+            const BcrVecOp* ops = (const BcrVecOp*)(base0 + off);
+            const dim3 grid(count, cnt);
+            if (transpose)
+                k_bcr_vec<true><<<grid, kBtdThreads, 0, s>>>(B, ops, n, nx);
+            else
+                k_bcr_vec<false><<<grid, kBtdThreads, 0, s>>>(B, ops, n, nx);
+        };
+        for (size_t lv = 0; lv < P.fwd.size(); ++lv) launch(P.off_fwd[lv], (int)P.fwd[lv].size());
+        launch(P.off_final, 1);
+        for (size_t lv = P.bwd.size(); lv-- > 0;) launch(P.off_bwd[lv], (int)P.bwd[lv].size());
+        CV_LAUNCH_CHECK();
+    }
+    return CV_OK;
+}
+
 int bcr_apply_impl(const cv_fdfd_desc& dh, const void* factor, const void* v, void* z, int transpose,
                    hipStream_t s) {
-    BcrPlan P;
-    bcr_plan(dh.nx, dh.ny, P);
-    char* base = (char*)factor;
-    const cx<float>* slots = (const cx<float>*)(base + P.off_slots);
-    cx<float>* vecs = (cx<float>*)(base + P.off_vecs);
-    const int n = dh.ny, nx = dh.nx;
-    CV_HIP(hipMemcpyAsync(vecs, v, (size_t)nx * n * 8, hipMemcpyDeviceToDevice, s));
-    auto launch = [&](const BcrVecOp* ops, int count) {
-        if (count <= 0) return;
-        if (transpose)
-            k_bcr_vec<true><<<count, kBtdThreads, 0, s>>>(slots, vecs, ops, n);
-        else
-            k_bcr_vec<false><<<count, kBtdThreads, 0, s>>>(slots, vecs, ops, n);
-    };
-    for (size_t lv = 0; lv < P.fwd.size(); ++lv)
-        launch((const BcrVecOp*)(base + P.off_fwd[lv]), (int)P.fwd[lv].size());
-    launch((const BcrVecOp*)(base + P.off_final), 1);
-    for (size_t lv = P.bwd.size(); lv-- > 0;)
-        launch((const BcrVecOp*)(base + P.off_bwd[lv]), (int)P.bwd[lv].size());
-    CV_LAUNCH_CHECK();
-    CV_HIP(hipMemcpyAsync(z, vecs + (int64_t)nx * n, (size_t)nx * n * 8, hipMemcpyDeviceToDevice, s));
-    return CV_OK;
+    return bcr_apply_batch(dh, 1, &factor, (const cx<float>*)v, (cx<float>*)z, 0, transpose, s);
 }
 
 int direct_apply(int kind, const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const void* prof32, const void* m0_32,
@@ -854,22 +970,28 @@ extern "C" int64_t cv_fdfd_bcr_factor_bytes(int nx, int ny) {
     return P.bytes;
 }
 
-extern "C" int cv_fdfd_bcr_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof32,
-                                  const void* m0_32, const void* m1_32, void* factor, int64_t factor_bytes,
-                                  void* stream) {
-    CV_CHECK(desc && desc_host && prof32 && m0_32 && factor, "null pointer argument");
+namespace cv {
+// factorises nb instances of one shape; desc[q] / desc_host[q] describe instance q
+static int bcr_factor_batch(int nb, const cv_fdfd_desc* const* desc, const cv_fdfd_desc* desc_host,
+                            const void* const* prof32, const void* const* m0_32, const void* const* m1_32,
+                            void* const* factor, int64_t factor_bytes, hipStream_t s) {
+    CV_CHECK(nb >= 1 && desc && desc_host && prof32 && m0_32 && m1_32 && factor, "null pointer argument");
     const cv_fdfd_desc dh = desc_host[0];
-    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
-    CV_CHECK(dh.ny <= kBtdMax, "block cyclic reduction needs ny <= %d (ny = %d)", kBtdMax, dh.ny);
-    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported");
-    CV_CHECK(dh.pol == CV_POL_EZ || m1_32, "Hz operator needs m1");
+    for (int q = 0; q < nb; ++q) {
+        const cv_fdfd_desc& d = desc_host[q];
+        CV_CHECK(desc[q] && prof32[q] && m0_32[q] && factor[q], "null pointer argument (instance %d)", q);
+        CV_CHECK(d.nx > 0 && d.ny > 0 && d.cell_off == 0, "one instance starting at cell 0 is required");
+        CV_CHECK(d.nx == dh.nx && d.ny == dh.ny && d.pol == dh.pol, "instance %d differs in shape from instance 0", q);
+        CV_CHECK(d.ny <= kBtdMax, "block cyclic reduction needs ny <= %d (ny = %d)", kBtdMax, d.ny);
+        CV_CHECK(d.nblk == 1 && d.n_cpl == 0, "frequency-coupled systems are not supported");
+        CV_CHECK(d.pol == CV_POL_EZ || m1_32[q], "Hz operator needs m1");
+    }
     BcrPlan P;
     bcr_plan(dh.nx, dh.ny, P);
     CV_CHECK(factor_bytes >= P.bytes, "factor buffer too small: %lld < %lld", (long long)factor_bytes,
              (long long)P.bytes);
-    hipStream_t s = as_stream(stream);
-    char* base = (char*)factor;
-    // tables: staged in one host buffer, one upload, kept alive until the final synchronisation
+    // tables: staged in one host buffer, uploaded into every factor (each
+    // stays self-contained), kept alive until the final synchronisation
     std::vector<char> host((size_t)P.off_slots, 0);
     auto put = [&](int64_t off, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + off, src, bytes); };
     put(P.off_dls, P.dls.data(), 4 * P.dls.size());
@@ -882,34 +1004,65 @@ extern "C" int cv_fdfd_bcr_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* 
     }
     put(P.off_final, &P.final_op, sizeof(BcrVecOp));
     put(P.off_final + sizeof(BcrVecOp), &P.final_slot, 4);
-    CV_HIP(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
-    int* status = (int*)base;   // header word 0
-    k_zero_u32_btd<<<1, 64, 0, s>>>((unsigned int*)status, 1);
-    cx<float>* slots = (cx<float>*)(base + P.off_slots);
+    for (int q = 0; q < nb; ++q) {
+        char* base = (char*)factor[q];
+        CV_HIP(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
+        k_zero_u32_btd<<<1, 64, 0, s>>>((unsigned int*)base, 1);   // status: header word 0
+    }
+    const char* base0 = (const char*)factor[0];
     const int n = dh.ny;
-    k_bcr_build0<<<dh.nx, 256, 0, s>>>(desc, (const cx<float>*)prof32, (const float*)m0_32, (const float*)m1_32, slots,
-                                      (const int*)(base + P.off_dls));
-    for (size_t lv = 0; lv < P.inv.size(); ++lv) {
-        if (!P.inv[lv].empty())
-            k_bcr_inv<<<(int)P.inv[lv].size(), kBtdThreads, 0, s>>>(slots, (const int*)(base + P.off_inv[lv]), n,
-                                                                    status);
-        if (!P.g1[lv].empty())
-            k_bcr_gemm<<<4 * (int)P.g1[lv].size(), kGemmThreads, 0, s>>>(
-                slots, (const BcrGemmOp*)(base + P.off_g1[lv]), n);
-        if (!P.g2[lv].empty())
-            k_bcr_gemm<<<4 * (int)P.g2[lv].size(), kGemmThreads, 0, s>>>(
-                slots, (const BcrGemmOp*)(base + P.off_g2[lv]), n);
+    for (int f0 = 0; f0 < nb; f0 += kBcrBatch) {
+        const int cnt = std::min(kBcrBatch, nb - f0);
+        BcrSlots S{};
+        BcrBuild in{};
+        for (int q = 0; q < cnt; ++q) {
+            char* base = (char*)factor[f0 + q];
+            S.slots[q] = (cx<float>*)(base + P.off_slots);
+            S.status[q] = (int*)base;
+            in.desc[q] = desc[f0 + q];
+            in.prof[q] = (const cx<float>*)prof32[f0 + q];
+            in.m0[q] = (const float*)m0_32[f0 + q];
+            in.m1[q] = (const float*)m1_32[f0 + q];
+        }
+        k_bcr_build0<<<dim3(dh.nx, cnt), 256, 0, s>>>(in, S, (const int*)(base0 + P.off_dls));
+        for (size_t lv = 0; lv < P.inv.size(); ++lv) {
+            if (!P.inv[lv].empty())
+                k_bcr_inv<<<dim3((int)P.inv[lv].size(), cnt), kBtdThreads, 0, s>>>(
+                    S, (const int*)(base0 + P.off_inv[lv]), n);
+            if (!P.g1[lv].empty())
+                k_bcr_gemm<<<dim3(4 * (int)P.g1[lv].size(), cnt), kGemmThreads, 0, s>>>(
+                    S, (const BcrGemmOp*)(base0 + P.off_g1[lv]), n);
+            if (!P.g2[lv].empty())
+                k_bcr_gemm<<<dim3(4 * (int)P.g2[lv].size(), cnt), kGemmThreads, 0, s>>>(
+                    S, (const BcrGemmOp*)(base0 + P.off_g2[lv]), n);
+        }
+        k_bcr_inv<<<dim3(1, cnt), kBtdThreads, 0, s>>>(S, (const int*)(base0 + P.off_final + sizeof(BcrVecOp)), n);
+        CV_LAUNCH_CHECK();
     }
-    k_bcr_inv<<<1, kBtdThreads, 0, s>>>(slots, (const int*)(base + P.off_final + sizeof(BcrVecOp)), n, status);
-    CV_LAUNCH_CHECK();
-    int st = 0;
-    CV_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<int> st(nb, 0);
+    for (int q = 0; q < nb; ++q)
+        CV_HIP(hipMemcpyAsync(&st[q], factor[q], sizeof(int), hipMemcpyDeviceToHost, s));
     CV_HIP(hipStreamSynchronize(s));
-    if (st != 0) {
-        set_error("zero pivot in block slot %d of the block cyclic reduction", st - 1);
-        return CV_ERR_SINGULAR;
-    }
+    for (int q = 0; q < nb; ++q)
+        if (st[q] != 0) {
+            set_error("zero pivot in block slot %d of the block cyclic reduction (instance %d)", st[q] - 1, q);
+            return CV_ERR_SINGULAR;
+        }
     return CV_OK;
+}
+}  // namespace cv
+
+extern "C" int cv_fdfd_bcr_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof32,
+                                  const void* m0_32, const void* m1_32, void* factor, int64_t factor_bytes,
+                                  void* stream) {
+    return bcr_factor_batch(1, &desc, desc_host, &prof32, &m0_32, &m1_32, &factor, factor_bytes, as_stream(stream));
+}
+
+extern "C" int cv_fdfd_bcr_factor_batch(int nb, const cv_fdfd_desc* const* desc, const cv_fdfd_desc* desc_host,
+                                        const void* const* prof32, const void* const* m0_32,
+                                        const void* const* m1_32, void* const* factors, int64_t factor_bytes,
+                                        void* stream) {
+    return bcr_factor_batch(nb, desc, desc_host, prof32, m0_32, m1_32, factors, factor_bytes, as_stream(stream));
 }
 
 extern "C" int cv_fdfd_bcr_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* factor,

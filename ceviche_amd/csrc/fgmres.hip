@@ -465,7 +465,8 @@ extern "C" int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_des
 }
 
 namespace cv {
-int bcr_apply_impl(const cv_fdfd_desc& dh, const void* factor, const void* v, void* z, int transpose, hipStream_t s);
+int bcr_apply_batch(const cv_fdfd_desc& dh, int nb, const void* const* factors, const cx<float>* v, cx<float>* z,
+                    int64_t stride, int transpose, hipStream_t s);
 }
 
 extern "C" int cv_fdfd_fgmres_blockdiag(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
@@ -503,11 +504,7 @@ extern "C" int cv_fdfd_fgmres_blockdiag(const cv_fdfd_desc* desc, const cv_fdfd_
     // block-diagonal preconditioner: each sideband block's own factorisation,
     // the inter-block coupling left to the outer iteration
     auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
-        for (int q = 0; q < nb; ++q) {
-            const int rc = bcr_apply_impl(block_desc_host[q], factors[q], v + q * n0, z + q * n0, transpose, s);
-            if (rc != CV_OK) return rc;
-        }
-        return CV_OK;
+        return bcr_apply_batch(block_desc_host[0], nb, factors, v, z, n0, transpose, s);
     };
     auto after_cycle = [](double, double) {};
     int status;

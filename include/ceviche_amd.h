@@ -451,6 +451,15 @@ int cv_fdfd_bcr_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, 
                        const void* m0_32, const void* m1_32, void* factor, int64_t factor_bytes, void* stream);
 int cv_fdfd_bcr_apply(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* factor, const void* v,
                       void* z, int transpose, void* stream);
+/* nb instances of one shape factorised together (instance q: desc[q],
+ * desc_host[q], prof32[q], m0_32[q], m1_32[q] -> factors[q], each of
+ * factor_bytes): each level's inversions and products of all instances run
+ * in the same launches.  Used for the sideband blocks of a frequency-coupled
+ * system, which the reference factorises as one matrix
+ * (ceviche/fdfd.py:320-329 -> solvers.py:50-62). */
+int cv_fdfd_bcr_factor_batch(int nb, const cv_fdfd_desc* const* desc, const cv_fdfd_desc* desc_host,
+                             const void* const* prof32, const void* const* m0_32, const void* const* m1_32,
+                             void* const* factors, int64_t factor_bytes, void* stream);
 
 /* Solve A x = b (transpose == 0) or A^T x = b to the FP64 relative residual
  * tol by flexible GMRES preconditioned with a factorisation of kind
