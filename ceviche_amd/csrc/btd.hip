@@ -67,22 +67,25 @@ constexpr int kFCols = kBtdMax / 64;          // columns per lane (2)
 
 // LDS of one Gauss-Jordan inversion (one per workgroup).
 struct GjLds {
-    cx<float> stage[kBtdMax * kBtdMax];
     cx<float> colbuf[2][kBtdMax];
     cx<float> rowbuf[2][kBtdMax];
     cx<float> swapbuf[kBtdMax];
     int perm[kBtdMax];
     int colmap[kBtdMax];
+    int invmap[kBtdMax];
     int bad;
 };
 
 // Inverts the n x n matrix held in m (wave w: rows 8w .. 8w+7, lane l: columns
 // l and l+64, zero outside n x n) in place: Gauss-Jordan elimination with
 // partial pivoting by physical row swaps, then the column interchanges undone
-// through g.stage.  On return m holds the inverse (zero padding).  A zero pivot
+// through stage (UNPERMUTE), or left to the caller: physical column c of m is
+// logical column g.invmap[c].  On return m holds the inverse (zero padding),
+// logical or physical column order.  A zero pivot
 // sets g.bad = code (if g.bad is still 0).  Called by all threads.
-__device__ void gj_invert(cx<float> (&m)[kFRows][kFCols], int n, int r0, int w, int l, int t, GjLds& g, int code) {
-    cx<float>* const stage = g.stage;
+template <bool UNPERMUTE>
+__device__ void gj_invert(cx<float> (&m)[kFRows][kFCols], int n, int r0, int w, int l, int t, GjLds& g, int code,
+                          cx<float>* stage) {
     cx<float> (&colbuf)[2][kBtdMax] = g.colbuf;
     cx<float> (&rowbuf)[2][kBtdMax] = g.rowbuf;
     cx<float>* const swapbuf = g.swapbuf;
@@ -219,6 +222,12 @@ __device__ void gj_invert(cx<float> (&m)[kFRows][kFCols], int n, int r0, int w, 
         }
         colmap[t] = v;
     }
+    if (!UNPERMUTE) {
+        __syncthreads();
+        if (t < n) g.invmap[colmap[t]] = t;
+        __syncthreads();
+        return;
+    }
 #pragma unroll
     for (int p = 0; p < kFRows; ++p)
 #pragma unroll
@@ -238,6 +247,7 @@ __device__ void gj_invert(cx<float> (&m)[kFRows][kFCols], int n, int r0, int w, 
 
 __global__ void __launch_bounds__(kBtdThreads) k_btd_factor(BtdFactorArgs a) {
     __shared__ GjLds g;
+    __shared__ cx<float> stage[kBtdMax * kBtdMax];
     __shared__ cx<float> upprev[kBtdMax];
     __shared__ cx<float> cdg[kBtdMax], clo[kBtdMax], cym[kBtdMax], cyp[kBtdMax];
     const cv_fdfd_desc d = load_desc(a.desc, 0);
@@ -285,7 +295,7 @@ __global__ void __launch_bounds__(kBtdThreads) k_btd_factor(BtdFactorArgs a) {
                 m[p][q] = v;
             }
         }
-        gj_invert(m, n, r0, w, l, t, g, 1 + i);
+        gj_invert<true>(m, n, r0, w, l, t, g, 1 + i, stage);
         cx<float>* out = a.sinv + (int64_t)i * n * n;
 #pragma unroll
         for (int p = 0; p < kFRows; ++p) {
@@ -412,13 +422,17 @@ k_btd_apply(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__
 // factorisation's nx dependent inversions become ceil(log2 nx) levels of
 // independent ones; every matrix is an n x n complex64 "slot".
 
+// Level 0's L_i and U_i are diagonal (stored dense); ops mark them so the
+// products and solves touch only their diagonals.
 struct BcrGemmOp {     // C = X + s1 A1 B1 + s2 A2 B2 (slot -1: absent)
     int c, x, a1, b1, a2, b2;
     float s1, s2;
+    int diag;          // diagonal factors: 1 A1, 2 B1, 4 A2, 8 B2
 };
 
 struct BcrVecOp {      // t = b - M1 u1 - M2 u2 (A^T: M1t^T, M2t^T); out = M0 t (A^T: M0^T t) or t
     int out, b, u1, u2, m1, m2, m1t, m2t, m0;
+    int diag;          // diagonal matrices: 1 M1, 2 M2, 4 M1t, 8 M2t
 };
 
 // Several factorisations of the same shape run as one (blockIdx.y = which);
@@ -481,8 +495,8 @@ __global__ void k_zero_u32_btd(unsigned int* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-// In-place inversion of the listed slots, one workgroup each; a zero pivot
-// records 1 + slot in *status.
+// In-place inversion of the listed slots, one workgroup each of 64 ceil(n / 8)
+// threads (the waves that own rows); a zero pivot records 1 + slot in *status.
 __global__ void __launch_bounds__(kBtdThreads)
 k_bcr_inv(BcrSlots B, const int* __restrict__ list, int n) {
     __shared__ GjLds g;
@@ -503,14 +517,18 @@ k_bcr_inv(BcrSlots B, const int* __restrict__ list, int n) {
             m[p][q] = (r < n && c < n) ? S[(int64_t)r * n + c] : czero<float>();
         }
     __syncthreads();
-    gj_invert(m, n, r0, w, l, t, g, 1 + slot);
+    gj_invert<false>(m, n, r0, w, l, t, g, 1 + slot, nullptr);
+    // physical column c is logical column invmap[c]
 #pragma unroll
-    for (int p = 0; p < kFRows; ++p)
+    for (int q = 0; q < kFCols; ++q) {
+        const int c = l + 64 * q;
+        const int cl = (c < n) ? g.invmap[c] : 0;
 #pragma unroll
-        for (int q = 0; q < kFCols; ++q) {
-            const int r = r0 + p, c = l + 64 * q;
-            if (r < n && c < n) S[(int64_t)r * n + c] = m[p][q];
+        for (int p = 0; p < kFRows; ++p) {
+            const int r = r0 + p;
+            if (r < n && c < n) S[(int64_t)r * n + cl] = m[p][q];
         }
+    }
     if (t == 0 && g.bad != 0) atomicCAS(status, 0, g.bad);
 }
 
@@ -531,6 +549,32 @@ k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
     if (row0 >= n || col0 >= n) return;
     const int64_t nn = (int64_t)n * n;
     const int t = threadIdx.x, ty = t / 16, tx = t % 16;
+    const bool has1 = op.a1 >= 0 && op.b1 >= 0, has2 = op.a2 >= 0 && op.b2 >= 0;
+    if ((!has1 || (op.diag & 3)) && (!has2 || (op.diag & 12))) {
+        // every term has a diagonal factor: a row or column scaling, no K loop
+        const int c = col0 + (t % kGemmTile);
+        if (c >= n) return;
+        cx<float>* C = slots + (int64_t)op.c * nn;
+        const cx<float>* X = op.x >= 0 ? slots + (int64_t)op.x * nn : nullptr;
+        for (int i = t / kGemmTile; i < kGemmTile; i += kGemmThreads / kGemmTile) {
+            const int r = row0 + i;
+            if (r >= n) break;
+            const int64_t e = (int64_t)r * n + c;
+            cx<float> v = X ? X[e] : czero<float>();
+#pragma unroll
+            for (int term = 0; term < 2; ++term) {
+                if (!(term ? has2 : has1)) continue;
+                const cx<float>* A = slots + (int64_t)(term ? op.a2 : op.a1) * nn;
+                const cx<float>* Bm = slots + (int64_t)(term ? op.b2 : op.b1) * nn;
+                const bool adiag = op.diag & (term ? 4 : 1);
+                const cx<float> pr = adiag ? A[(int64_t)r * n + r] * Bm[e] : A[e] * Bm[(int64_t)c * n + c];
+                const float sg = term ? op.s2 : op.s1;
+                v = v + mk<float>(sg * pr.re, sg * pr.im);
+            }
+            C[e] = v;
+        }
+        return;
+    }
     cx<float> acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -606,14 +650,14 @@ struct BcrVecBatch {
 };
 
 template <bool T>
-__device__ __forceinline__ void bcr_load(const cx<float>* __restrict__ M, int n, cx<float> (&e)[kVecPer]) {
+__device__ __forceinline__ void bcr_load(const cx<float>* __restrict__ M, int n, bool diag, cx<float> (&e)[kVecPer]) {
     const int t = threadIdx.x;
     const int a = T ? t % kBtdMax : t / 8, p = T ? t / kBtdMax : t % 8;
 #pragma unroll
     for (int i = 0; i < kVecPer; ++i) {
         const int b = p + 8 * i;
         const int r = T ? b : a, c = T ? a : b;
-        e[i] = (r < n && c < n) ? M[(int64_t)r * n + c] : czero<float>();
+        e[i] = (r < n && c < n && (!diag || r == c)) ? M[(int64_t)r * n + c] : czero<float>();
     }
 }
 
@@ -673,9 +717,9 @@ k_bcr_vec(BcrVecBatch B, const BcrVecOp* __restrict__ ops, int n, int nx) {
     const int m1 = TRANS ? op.m1t : op.m1, m2 = TRANS ? op.m2t : op.m2;
     const bool h1 = op.u1 >= 0 && m1 >= 0, h2 = op.u2 >= 0 && m2 >= 0, h0 = op.m0 >= 0;
     cx<float> e1[kVecPer], e2[kVecPer], e0[kVecPer];
-    if (h1) bcr_load<TRANS>(slots + (int64_t)m1 * nn, n, e1);
-    if (h2) bcr_load<TRANS>(slots + (int64_t)m2 * nn, n, e2);
-    if (h0) bcr_load<TRANS>(slots + (int64_t)op.m0 * nn, n, e0);
+    if (h1) bcr_load<TRANS>(slots + (int64_t)m1 * nn, n, op.diag & (TRANS ? 4 : 1), e1);
+    if (h2) bcr_load<TRANS>(slots + (int64_t)m2 * nn, n, op.diag & (TRANS ? 8 : 2), e2);
+    if (h0) bcr_load<TRANS>(slots + (int64_t)op.m0 * nn, n, false, e0);
     if (t < kBtdMax) {
         const bool in = t < n;
         bs[t] = in ? vec_in(op.b)[t] : czero<float>();
@@ -834,40 +878,41 @@ static void bcr_plan(int nx, int n, BcrPlan& P) {
         U2.assign(N2, -1);
         b2.assign(N2, -1);
         x2.assign(N2, -1);
+        const int dg = (N == nx) ? 1 : 0;   // level 0: L and U are diagonal
         for (int j = 0; j < N; j += 2) {
             const int jj = j / 2;
             int al = -1, be = -1, ga = -1, de = -1;
             if (j >= 1) {
                 al = ns++;
                 ga = ns++;
-                g1.push_back({al, -1, L[j], D[j - 1], -1, -1, 1.0f, 0.0f});
-                g1.push_back({ga, -1, D[j - 1], U[j - 1], -1, -1, 1.0f, 0.0f});
+                g1.push_back({al, -1, L[j], D[j - 1], -1, -1, 1.0f, 0.0f, dg * 1});
+                g1.push_back({ga, -1, D[j - 1], U[j - 1], -1, -1, 1.0f, 0.0f, dg * 2});
             }
             if (j + 1 < N) {
                 be = ns++;
                 de = ns++;
-                g1.push_back({be, -1, U[j], D[j + 1], -1, -1, 1.0f, 0.0f});
-                g1.push_back({de, -1, D[j + 1], L[j + 1], -1, -1, 1.0f, 0.0f});
+                g1.push_back({be, -1, U[j], D[j + 1], -1, -1, 1.0f, 0.0f, dg * 1});
+                g1.push_back({de, -1, D[j + 1], L[j + 1], -1, -1, 1.0f, 0.0f, dg * 2});
             }
             D2[jj] = ns++;
-            g2.push_back({D2[jj], D[j], (j >= 1) ? L[j] : -1, ga, (j + 1 < N) ? U[j] : -1, de, -1.0f, -1.0f});
+            g2.push_back({D2[jj], D[j], (j >= 1) ? L[j] : -1, ga, (j + 1 < N) ? U[j] : -1, de, -1.0f, -1.0f, dg * 5});
             if (j >= 2) {
                 L2[jj] = ns++;
-                g2.push_back({L2[jj], -1, al, L[j - 1], -1, -1, -1.0f, 0.0f});
+                g2.push_back({L2[jj], -1, al, L[j - 1], -1, -1, -1.0f, 0.0f, dg * 2});
             }
             if (j + 2 < N) {
                 U2[jj] = ns++;
-                g2.push_back({U2[jj], -1, be, U[j + 1], -1, -1, -1.0f, 0.0f});
+                g2.push_back({U2[jj], -1, be, U[j + 1], -1, -1, -1.0f, 0.0f, dg * 2});
             }
             b2[jj] = nv++;
             x2[jj] = nv++;
-            fw.push_back({b2[jj], b[j], (j >= 1) ? b[j - 1] : -1, (j + 1 < N) ? b[j + 1] : -1, al, be, ga, de, -1});
-            bw.push_back({x[j], x2[jj], -1, -1, -1, -1, -1, -1, -1});     // even: copy from the next level
+            fw.push_back({b2[jj], b[j], (j >= 1) ? b[j - 1] : -1, (j + 1 < N) ? b[j + 1] : -1, al, be, ga, de, -1, 0});
+            bw.push_back({x[j], x2[jj], -1, -1, -1, -1, -1, -1, -1, 0});     // even: copy from the next level
         }
         for (int j = 1; j < N; j += 2) {
             const bool up = (j + 1 < N);
             bw.push_back({x[j], b[j], x2[(j - 1) / 2], up ? x2[(j + 1) / 2] : -1, L[j], up ? U[j] : -1, U[j - 1],
-                          up ? L[j + 1] : -1, D[j]});
+                          up ? L[j + 1] : -1, D[j], dg * 15});
         }
         P.inv.push_back(inv);
         P.g1.push_back(g1);
@@ -882,7 +927,7 @@ static void bcr_plan(int nx, int n, BcrPlan& P) {
         N = N2;
     }
     P.final_slot = D[0];
-    P.final_op = {x[0], b[0], -1, -1, -1, -1, -1, -1, D[0]};
+    P.final_op = {x[0], b[0], -1, -1, -1, -1, -1, -1, D[0], 0};
     P.n_slots = ns;
     P.n_vecs = nv;
     // layout: header | dls | per level inv, g1, g2, fwd, bwd | final op | slots | vecs
@@ -1024,10 +1069,11 @@ static int bcr_factor_batch(int nb, const cv_fdfd_desc* const* desc, const cv_fd
             in.m0[q] = (const float*)m0_32[f0 + q];
             in.m1[q] = (const float*)m1_32[f0 + q];
         }
+        const int inv_threads = 64 * ((n + kFRows - 1) / kFRows);
         k_bcr_build0<<<dim3(dh.nx, cnt), 256, 0, s>>>(in, S, (const int*)(base0 + P.off_dls));
         for (size_t lv = 0; lv < P.inv.size(); ++lv) {
             if (!P.inv[lv].empty())
-                k_bcr_inv<<<dim3((int)P.inv[lv].size(), cnt), kBtdThreads, 0, s>>>(
+                k_bcr_inv<<<dim3((int)P.inv[lv].size(), cnt), inv_threads, 0, s>>>(
                     S, (const int*)(base0 + P.off_inv[lv]), n);
             if (!P.g1[lv].empty())
                 k_bcr_gemm<<<dim3(4 * (int)P.g1[lv].size(), cnt), kGemmThreads, 0, s>>>(
@@ -1036,7 +1082,7 @@ static int bcr_factor_batch(int nb, const cv_fdfd_desc* const* desc, const cv_fd
                 k_bcr_gemm<<<dim3(4 * (int)P.g2[lv].size(), cnt), kGemmThreads, 0, s>>>(
                     S, (const BcrGemmOp*)(base0 + P.off_g2[lv]), n);
         }
-        k_bcr_inv<<<dim3(1, cnt), kBtdThreads, 0, s>>>(S, (const int*)(base0 + P.off_final + sizeof(BcrVecOp)), n);
+        k_bcr_inv<<<dim3(1, cnt), inv_threads, 0, s>>>(S, (const int*)(base0 + P.off_final + sizeof(BcrVecOp)), n);
         CV_LAUNCH_CHECK();
     }
     std::vector<int> st(nb, 0);
