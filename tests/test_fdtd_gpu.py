@@ -20,7 +20,8 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-@pytest.mark.parametrize('shape,npml', [((12, 10, 8), [3, 2, 2]), ((9, 1, 7), [2, 0, 2]), ((14, 11, 1), [3, 3, 0])])
+@pytest.mark.parametrize('shape,npml', [((12, 10, 8), [3, 2, 2]), ((9, 1, 7), [2, 0, 2]), ((14, 11, 1), [3, 3, 0]),
+                                        ((5, 9, 70), [1, 2, 3]), ((18, 4, 128), [2, 1, 4])])
 def test_forward_all_fields_float64(shape, npml):
     import ceviche_amd as cv
     print('fdtd fwd', shape, npml, flush=True)
