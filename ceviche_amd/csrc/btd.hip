@@ -542,7 +542,7 @@ constexpr int kGemmTile = 64;
 __global__ void __launch_bounds__(kGemmThreads)
 k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
     cx<float>* __restrict__ slots = B.slots[blockIdx.y];
-    __shared__ cx<float> As[kGemmK][kGemmTile];   // As[k][row]
+    __shared__ cx<float> As[kGemmK][kGemmTile + 1];   // As[k][row], padded against bank conflicts
     __shared__ cx<float> Bs[kGemmK][kGemmTile];   // Bs[k][col]
     const BcrGemmOp op = ops[blockIdx.x >> 2];
     const int row0 = ((blockIdx.x >> 1) & 1) * kGemmTile, col0 = (blockIdx.x & 1) * kGemmTile;
@@ -580,36 +580,53 @@ k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = czero<float>();
-    for (int term = 0; term < 2; ++term) {
-        const int sa = term ? op.a2 : op.a1, sb = term ? op.b2 : op.b1;
-        if (sa < 0 || sb < 0) continue;
+    // chunks (term, k0) in sequence; the next chunk's entries are loaded into
+    // registers while the current one is multiplied.  A is read along k (16
+    // lanes per row, contiguous), B along the tile's columns.
+    const int nk = (n + kGemmK - 1) / kGemmK;
+    const int first = has1 ? 0 : 1;
+    const int nchunks = ((has1 ? 1 : 0) + (has2 ? 1 : 0)) * nk;
+    cx<float> ra[4], rb[4];
+    auto load = [&](int ch) {
+        const int term = (ch < nk) ? first : 1, k0 = (ch % nk) * kGemmK;
+        const cx<float>* A = slots + (int64_t)(term ? op.a2 : op.a1) * nn;
+        const cx<float>* Bm = slots + (int64_t)(term ? op.b2 : op.b1) * nn;
         const float sg = term ? op.s2 : op.s1;
-        const cx<float>* A = slots + (int64_t)sa * nn;
-        const cx<float>* B = slots + (int64_t)sb * nn;
-        for (int k0 = 0; k0 < n; k0 += kGemmK) {
-            __syncthreads();
-            for (int e = t; e < kGemmK * kGemmTile; e += kGemmThreads) {
-                const int kk = e / kGemmTile, rc = e % kGemmTile, k = k0 + kk;
-                const int r = row0 + rc, c = col0 + rc;
-                As[kk][rc] = (r < n && k < n) ? A[(int64_t)r * n + k] : czero<float>();
-                Bs[kk][rc] = (c < n && k < n) ? B[(int64_t)k * n + c] : czero<float>();
-            }
-            __syncthreads();
 #pragma unroll
-            for (int kk = 0; kk < kGemmK; ++kk) {
-                cx<float> av[4], bv[4];
+        for (int u = 0; u < 4; ++u) {
+            const int e = t + kGemmThreads * u;
+            const int ar = row0 + e / kGemmK, ak = k0 + e % kGemmK;
+            const cx<float> av = (ar < n && ak < n) ? A[(int64_t)ar * n + ak] : czero<float>();
+            ra[u] = mk<float>(sg * av.re, sg * av.im);
+            const int bk = k0 + e / kGemmTile, bc = col0 + e % kGemmTile;
+            rb[u] = (bc < n && bk < n) ? Bm[(int64_t)bk * n + bc] : czero<float>();
+        }
+    };
+    if (nchunks > 0) load(0);
+    for (int ch = 0; ch < nchunks; ++ch) {
+        __syncthreads();
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const cx<float> v = As[kk][4 * ty + a];
-                    av[a] = mk<float>(sg * v.re, sg * v.im);
+        for (int u = 0; u < 4; ++u) {
+            const int e = t + kGemmThreads * u;
+            As[e % kGemmK][e / kGemmK] = ra[u];
+            Bs[e / kGemmTile][e % kGemmTile] = rb[u];
+        }
+        __syncthreads();
+        if (ch + 1 < nchunks) load(ch + 1);
+#pragma unroll
+        for (int kk = 0; kk < kGemmK; ++kk) {
+            cx<float> av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[kk][4 * ty + a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][4 * tx + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    acc[a][b].re = fmaf(av[a].re, bv[b].re, fmaf(-av[a].im, bv[b].im, acc[a][b].re));
+                    acc[a][b].im = fmaf(av[a].re, bv[b].im, fmaf(av[a].im, bv[b].re, acc[a][b].im));
                 }
-#pragma unroll
-                for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][4 * tx + b];
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] = acc[a][b] + av[a] * bv[b];
-            }
         }
     }
     cx<float>* C = slots + (int64_t)op.c * nn;
