@@ -532,34 +532,40 @@ k_bcr_inv(BcrSlots B, const int* __restrict__ list, int n) {
     if (t == 0 && g.bad != 0) atomicCAS(status, 0, g.bad);
 }
 
-// Batched n x n complex GEMMs (n <= 128): each op is split into four 64 x 64
-// output tiles, one 256-thread workgroup per tile (4 x 4 outputs per thread),
-// K in chunks of 16 staged through LDS; several workgroups share a CU.
+// Batched n x n complex GEMMs (n <= 128).  256 threads (16 x 16) own a
+// (16 TM) x (16 TM) output tile, TM x TM outputs per thread: TM = 4 with
+// four 64 x 64 tiles for n > 96, one tile of 80 or 96 (TM = 5, 6) for
+// 64 < n <= 96, so few padded entries are computed.  K in chunks of 16
+// staged through LDS, the next chunk loaded into registers while the
+// current one is multiplied.  Ops whose every term has a diagonal factor
+// (level 0) are row / column scalings over all ops' workgroups instead.
 constexpr int kGemmThreads = 256;
 constexpr int kGemmK = 16;
-constexpr int kGemmTile = 64;
 
+static inline int gemm_tm(int n) { return n > 96 ? 4 : (n > 80 ? 6 : (n > 64 ? 5 : 4)); }
+static inline int gemm_tiles(int n) { const int tile = 16 * gemm_tm(n); return (n + tile - 1) / tile; }
+
+template <int TM>
 __global__ void __launch_bounds__(kGemmThreads)
-k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
+k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n, int tiles, int per_op) {
+    constexpr int kTile = 16 * TM;
     cx<float>* __restrict__ slots = B.slots[blockIdx.y];
-    __shared__ cx<float> As[kGemmK][kGemmTile + 1];   // As[k][row], padded against bank conflicts
-    __shared__ cx<float> Bs[kGemmK][kGemmTile];   // Bs[k][col]
-    const BcrGemmOp op = ops[blockIdx.x >> 2];
-    const int row0 = ((blockIdx.x >> 1) & 1) * kGemmTile, col0 = (blockIdx.x & 1) * kGemmTile;
-    if (row0 >= n || col0 >= n) return;
-    const int64_t nn = (int64_t)n * n;
+    __shared__ cx<float> As[kGemmK][kTile + 1];   // As[k][row], padded against bank conflicts
+    __shared__ cx<float> Bs[kGemmK][kTile];       // Bs[k][col]
+    // per_op workgroups per op: tiles^2 output tiles, or slices of a scaling
+    const int opi = blockIdx.x / per_op, tq = blockIdx.x - opi * per_op;
+    const BcrGemmOp op = ops[opi];
+    const int nn = n * n;
     const int t = threadIdx.x, ty = t / 16, tx = t % 16;
     const bool has1 = op.a1 >= 0 && op.b1 >= 0, has2 = op.a2 >= 0 && op.b2 >= 0;
     if ((!has1 || (op.diag & 3)) && (!has2 || (op.diag & 12))) {
-        // every term has a diagonal factor: a row or column scaling, no K loop
-        const int c = col0 + (t % kGemmTile);
-        if (c >= n) return;
+        // row / column scaling: the op's workgroups take consecutive slices
+        // of the row-major entries
+        const int q0 = (int)(((int64_t)nn * tq) / per_op), q1 = (int)(((int64_t)nn * (tq + 1)) / per_op);
         cx<float>* C = slots + (int64_t)op.c * nn;
         const cx<float>* X = op.x >= 0 ? slots + (int64_t)op.x * nn : nullptr;
-        for (int i = t / kGemmTile; i < kGemmTile; i += kGemmThreads / kGemmTile) {
-            const int r = row0 + i;
-            if (r >= n) break;
-            const int64_t e = (int64_t)r * n + c;
+        for (int e = q0 + t; e < q1; e += kGemmThreads) {
+            const int r = e / n, c = e - r * n;
             cx<float> v = X ? X[e] : czero<float>();
 #pragma unroll
             for (int term = 0; term < 2; ++term) {
@@ -567,7 +573,7 @@ k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
                 const cx<float>* A = slots + (int64_t)(term ? op.a2 : op.a1) * nn;
                 const cx<float>* Bm = slots + (int64_t)(term ? op.b2 : op.b1) * nn;
                 const bool adiag = op.diag & (term ? 4 : 1);
-                const cx<float> pr = adiag ? A[(int64_t)r * n + r] * Bm[e] : A[e] * Bm[(int64_t)c * n + c];
+                const cx<float> pr = adiag ? A[r * n + r] * Bm[e] : A[e] * Bm[c * n + c];
                 const float sg = term ? op.s2 : op.s1;
                 v = v + mk<float>(sg * pr.re, sg * pr.im);
             }
@@ -575,55 +581,56 @@ k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
         }
         return;
     }
-    cx<float> acc[4][4];
+    if (tq >= tiles * tiles) return;   // extra workgroups of a launch sized for scalings
+    const int row0 = (tq / tiles) * kTile, col0 = (tq - (tq / tiles) * tiles) * kTile;
+    cx<float> acc[TM][TM];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = czero<float>();
-    // chunks (term, k0) in sequence; the next chunk's entries are loaded into
-    // registers while the current one is multiplied.  A is read along k (16
-    // lanes per row, contiguous), B along the tile's columns.
+        for (int b = 0; b < TM; ++b) acc[a][b] = czero<float>();
+    // chunks (term, k0) in sequence; A is read along k (16 lanes per row,
+    // contiguous), B along the tile's columns
     const int nk = (n + kGemmK - 1) / kGemmK;
     const int first = has1 ? 0 : 1;
     const int nchunks = ((has1 ? 1 : 0) + (has2 ? 1 : 0)) * nk;
-    cx<float> ra[4], rb[4];
+    cx<float> ra[TM], rb[TM];    // kGemmK * kTile entries of each, over 256 threads
     auto load = [&](int ch) {
         const int term = (ch < nk) ? first : 1, k0 = (ch % nk) * kGemmK;
         const cx<float>* A = slots + (int64_t)(term ? op.a2 : op.a1) * nn;
         const cx<float>* Bm = slots + (int64_t)(term ? op.b2 : op.b1) * nn;
         const float sg = term ? op.s2 : op.s1;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < TM; ++u) {
             const int e = t + kGemmThreads * u;
             const int ar = row0 + e / kGemmK, ak = k0 + e % kGemmK;
-            const cx<float> av = (ar < n && ak < n) ? A[(int64_t)ar * n + ak] : czero<float>();
+            const cx<float> av = (ar < n && ak < n) ? A[ar * n + ak] : czero<float>();
             ra[u] = mk<float>(sg * av.re, sg * av.im);
-            const int bk = k0 + e / kGemmTile, bc = col0 + e % kGemmTile;
-            rb[u] = (bc < n && bk < n) ? Bm[(int64_t)bk * n + bc] : czero<float>();
+            const int bk = k0 + e / kTile, bc = col0 + e % kTile;
+            rb[u] = (bc < n && bk < n) ? Bm[bk * n + bc] : czero<float>();
         }
     };
     if (nchunks > 0) load(0);
     for (int ch = 0; ch < nchunks; ++ch) {
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < TM; ++u) {
             const int e = t + kGemmThreads * u;
             As[e % kGemmK][e / kGemmK] = ra[u];
-            Bs[e / kGemmTile][e % kGemmTile] = rb[u];
+            Bs[e / kTile][e % kTile] = rb[u];
         }
         __syncthreads();
         if (ch + 1 < nchunks) load(ch + 1);
 #pragma unroll
         for (int kk = 0; kk < kGemmK; ++kk) {
-            cx<float> av[4], bv[4];
+            cx<float> av[TM], bv[TM];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = As[kk][4 * ty + a];
+            for (int a = 0; a < TM; ++a) av[a] = As[kk][TM * ty + a];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][4 * tx + b];
+            for (int b = 0; b < TM; ++b) bv[b] = Bs[kk][TM * tx + b];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TM; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
+                for (int b = 0; b < TM; ++b) {
                     acc[a][b].re = fmaf(av[a].re, bv[b].re, fmaf(-av[a].im, bv[b].im, acc[a][b].re));
                     acc[a][b].im = fmaf(av[a].re, bv[b].im, fmaf(av[a].im, bv[b].re, acc[a][b].im));
                 }
@@ -632,17 +639,31 @@ k_bcr_gemm(BcrSlots B, const BcrGemmOp* __restrict__ ops, int n) {
     cx<float>* C = slots + (int64_t)op.c * nn;
     const cx<float>* X = op.x >= 0 ? slots + (int64_t)op.x * nn : nullptr;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int r = row0 + 4 * ty + a;
+    for (int a = 0; a < TM; ++a) {
+        const int r = row0 + TM * ty + a;
         if (r >= n) continue;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int c = col0 + 4 * tx + b;
+        for (int b = 0; b < TM; ++b) {
+            const int c = col0 + TM * tx + b;
             if (c >= n) continue;
             cx<float> v = acc[a][b];
-            if (X) v = v + X[(int64_t)r * n + c];
-            C[(int64_t)r * n + c] = v;
+            if (X) v = v + X[r * n + c];
+            C[r * n + c] = v;
         }
+    }
+}
+
+// scaling: every op of the launch is a level-0 row / column scaling (eight
+// workgroups per op); otherwise every op is a dense product (one per tile)
+static void launch_gemm(const BcrSlots& S, const BcrGemmOp* ops, int count, int cnt, int n, bool scaling,
+                        hipStream_t s) {
+    const int tiles = gemm_tiles(n);
+    const int per_op = scaling ? 8 : tiles * tiles;
+    const dim3 grid(per_op * count, cnt);
+    switch (gemm_tm(n)) {
+        case 5: k_bcr_gemm<5><<<grid, kGemmThreads, 0, s>>>(S, ops, n, tiles, per_op); break;
+        case 6: k_bcr_gemm<6><<<grid, kGemmThreads, 0, s>>>(S, ops, n, tiles, per_op); break;
+        default: k_bcr_gemm<4><<<grid, kGemmThreads, 0, s>>>(S, ops, n, tiles, per_op); break;
     }
 }
 
@@ -1093,11 +1114,9 @@ static int bcr_factor_batch(int nb, const cv_fdfd_desc* const* desc, const cv_fd
                 k_bcr_inv<<<dim3((int)P.inv[lv].size(), cnt), inv_threads, 0, s>>>(
                     S, (const int*)(base0 + P.off_inv[lv]), n);
             if (!P.g1[lv].empty())
-                k_bcr_gemm<<<dim3(4 * (int)P.g1[lv].size(), cnt), kGemmThreads, 0, s>>>(
-                    S, (const BcrGemmOp*)(base0 + P.off_g1[lv]), n);
+                launch_gemm(S, (const BcrGemmOp*)(base0 + P.off_g1[lv]), (int)P.g1[lv].size(), cnt, n, lv == 0, s);
             if (!P.g2[lv].empty())
-                k_bcr_gemm<<<dim3(4 * (int)P.g2[lv].size(), cnt), kGemmThreads, 0, s>>>(
-                    S, (const BcrGemmOp*)(base0 + P.off_g2[lv]), n);
+                launch_gemm(S, (const BcrGemmOp*)(base0 + P.off_g2[lv]), (int)P.g2[lv].size(), cnt, n, lv == 0, s);
         }
         k_bcr_inv<<<dim3(1, cnt), inv_threads, 0, s>>>(S, (const int*)(base0 + P.off_final + sizeof(BcrVecOp)), n);
         CV_LAUNCH_CHECK();

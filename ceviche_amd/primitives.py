@@ -33,6 +33,35 @@ def _i64(a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.int64)).to(_dev())
 
 
+try:
+    import xxhash
+
+    def _digest(buf):
+        return xxhash.xxh3_128_intdigest(buf)
+except ImportError:      # pragma: no cover - xxhash ships with this environment
+    import hashlib
+
+    def _digest(buf):
+        return hashlib.blake2b(buf, digest_size=16).digest()
+
+
+def _index_np(indices):
+    """(2, M) int64 numpy view of an index set (copied only when needed)."""
+    a = indices.detach().cpu().numpy() if isinstance(indices, torch.Tensor) else np.asarray(indices)
+    return np.ascontiguousarray(a, dtype=np.int64).reshape(2, -1)
+
+
+def _index_key(indices):
+    """Cache key of an index set.  A torch tensor is identified by its memory,
+    layout and in-place version counter (no copy to the host); any other array
+    by a 128-bit digest of its int64 contents."""
+    if isinstance(indices, torch.Tensor):
+        return ('t', indices.data_ptr(), tuple(indices.shape), tuple(indices.stride()), str(indices.dtype),
+                str(indices.device), indices._version)
+    a = _index_np(indices)
+    return ('n', a.shape, _digest(memoryview(a).cast('B')))
+
+
 class _Groups:
     """Entries grouped by a key: ptr (n_groups + 1), order (entry indices)."""
 
@@ -70,12 +99,13 @@ class Pattern:
 
     @classmethod
     def get(cls, indices, N):
-        idx = np.asarray(indices.detach().cpu().numpy() if isinstance(indices, torch.Tensor) else indices)
-        idx = idx.astype(np.int64).reshape(2, -1)
-        k = (int(N), idx.shape[1], hash(idx.tobytes()))
+        k = (int(N),) + _index_key(indices)
         p = cls._cache.get(k)
         if p is None:
+            idx = _index_np(indices)
             p = cls(idx[0].copy(), idx[1].copy(), N)
+            # a tensor key names its memory: keep the tensor alive with the entry
+            p.keep = indices if isinstance(indices, torch.Tensor) else None
             if len(cls._cache) > 64:
                 cls._cache.clear()
             cls._cache[k] = p
@@ -235,16 +265,16 @@ class ProductPattern:
         self.Ma, self.Mx = int(ra.size), int(rx.size)
 
     @classmethod
-    def get(cls, ia, ix, N):
-        ia = np.asarray(ia).astype(np.int64).reshape(2, -1)
-        ix = np.asarray(ix).astype(np.int64).reshape(2, -1)
-        k = (int(N), hash(ia.tobytes()), hash(ix.tobytes()), ia.shape[1], ix.shape[1])
+    def get(cls, ia_in, ix_in, N):
+        k = (int(N),) + _index_key(ia_in) + _index_key(ix_in)
         pp = cls._cache.get(k)
         if pp is None:
+            ia, ix = _index_np(ia_in), _index_np(ix_in)
             for a, name in ((ia, 'A'), (ix, 'X')):
                 if a.size and (a.min() < 0 or a.max() >= N):
                     raise ValueError('indices of %s out of range for an %d x %d matrix' % (name, N, N))
             pp = cls(ia, ix, N)
+            pp.keep = tuple(a for a in (ia_in, ix_in) if isinstance(a, torch.Tensor))
             if len(cls._cache) > 64:
                 cls._cache.clear()
             cls._cache[k] = pp
@@ -327,9 +357,7 @@ def spsp_mult(entries_a, indices_a, entries_x, indices_x, N):
     """(entries_b, indices_b) of B = A X (N x N), entries in row-major order of
     the unique output positions; primitives.py:153-176."""
     as_t = _is_t(entries_a, entries_x)
-    ia = indices_a.detach().cpu().numpy() if isinstance(indices_a, torch.Tensor) else indices_a
-    ix = indices_x.detach().cpu().numpy() if isinstance(indices_x, torch.Tensor) else indices_x
-    PP = ProductPattern.get(ia, ix, N)
+    PP = ProductPattern.get(indices_a, indices_x, N)
     eb = SpspMult.apply(_to_t(entries_a), _to_t(entries_x), PP)
     ib = PP.indices_b
     if as_t:

@@ -51,6 +51,29 @@ def test_values_match_oracle():
     assert np.allclose(P.make_sparse(eb, ib, (d['N'], d['N'])).toarray(), rb.toarray(), rtol=1e-14, atol=1e-14)
 
 
+def test_index_sets_modified_in_place_are_reanalysed():
+    """The symbolic grouping is cached per index set: an index array (numpy
+    or torch) changed in place must give the new matrix's product."""
+    from ceviche_amd import primitives as cp
+    d = _setup()
+    ind = d['ind'].copy()
+    for as_torch in (False, True):
+        idx = torch.from_numpy(ind.copy()).cuda() if as_torch else ind.copy()
+        ent = d['ent']
+        x = d['x']
+        y0 = cp.sp_mult(ent, idx, x)
+        idx[1, :5] = (idx[1, :5] + 3) % d['N']          # in place
+        y1 = cp.sp_mult(ent, idx, x)
+        host = idx.cpu().numpy() if as_torch else idx
+        assert np.allclose(y0, P.sp_mult(ent, ind, x), rtol=1e-14, atol=1e-14)
+        assert np.allclose(y1, P.sp_mult(ent, host, x), rtol=1e-14, atol=1e-14)
+        eb, ib = cp.spsp_mult(ent, idx, ent, idx, d['N'])
+        ib = ib.cpu().numpy() if isinstance(ib, torch.Tensor) else ib
+        rb = P.make_sparse(*P.spsp_mult(ent, host, ent, host, d['N']), (d['N'], d['N']))
+        eb = eb.cpu().numpy() if isinstance(eb, torch.Tensor) else eb
+        assert np.allclose(P.make_sparse(eb, ib, (d['N'], d['N'])).toarray(), rb.toarray(), rtol=1e-14, atol=1e-14)
+
+
 def test_mult_entries():
     from ceviche_amd.primitives import sp_mult
     d = _setup()
