@@ -496,37 +496,141 @@ __global__ void k_zero_u32_btd(unsigned int* p, int n) {
 }
 
 // In-place inversion of the listed slots, one workgroup each of 64 ceil(n / 8)
-// threads (the waves that own rows); a zero pivot records 1 + slot in *status.
+// threads; a zero pivot records 1 + slot in *status.
+//
+// Gauss-Jordan with partial pivoting and neither row swaps nor row scaling
+// during the elimination: wave w owns columns 8w .. 8w+7, lane l rows l and
+// l + 64.  Step k (column k, held by one wave in one register slot):
+//   the owner wave picks the largest |S[r][k]| among rows not yet used as
+//   pivots (pr, 1/piv = ipiv), publishes f_r = S[r][k] ipiv (0 for r = pr)
+//   and writes column k itself: -f_r, and 1 in row pr (whose scale ipiv is
+//   recorded instead of applied);
+//   after one barrier every wave reads the stored pivot row u_c from lane
+//   pr & 63 (readlane, scalar) and updates S[r][c] -= f_r u_c (u_k = 0).
+// Row pr, scaled by ipiv, is the pivot row of textbook Gauss-Jordan; since
+// every later update of a row is linear in it, the scale can wait: the
+// inverse is A^-1[i][j] = scale[piv[i]] S[piv[i]][inv[j]] with piv[k] = pr of
+// step k and inv its inverse permutation (logical row i lives in physical
+// row piv[i], logical column j in physical column inv[j]).
+struct GjcLds {
+    cx<float> fbuf[2][kBtdMax];   // f_r of the step, double-buffered
+    int prbuf[2];
+    cx<float> scale[kBtdMax];     // per physical row: 1 / its pivot
+    int piv[kBtdMax];             // step -> pivot row
+    int inv[kBtdMax];             // pivot row -> step
+    int bad;
+};
+
 __global__ void __launch_bounds__(kBtdThreads)
 k_bcr_inv(BcrSlots B, const int* __restrict__ list, int n) {
-    __shared__ GjLds g;
+    __shared__ GjcLds g;
     cx<float>* __restrict__ slots = B.slots[blockIdx.y];
     int* __restrict__ status = B.status[blockIdx.y];
     const int t = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
-    const int r0 = w * kFRows;
+    const int nw = (n + 7) / 8;
     const int slot = list[blockIdx.x];
+    const int code = 1 + slot;
     cx<float>* S = slots + (int64_t)slot * n * n;
     if (t == 0) g.bad = 0;
-    cx<float> m[kFRows][kFCols];
+    cx<float> m[2][8];           // rows l, l + 64; columns 8w .. 8w+7
 #pragma unroll
-    for (int p = 0; p < kFRows; ++p)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int q = 0; q < kFCols; ++q) {
-            const int r = r0 + p, c = l + 64 * q;
-            m[p][q] = (r < n && c < n) ? S[(int64_t)r * n + c] : czero<float>();
+        for (int c = 0; c < 8; ++c) {
+            const int r = l + 64 * q, cc = 8 * w + c;
+            m[q][c] = (r < n && cc < n) ? S[(int64_t)r * n + cc] : czero<float>();
         }
+    bool used0 = false, used1 = false;
     __syncthreads();
-    gj_invert<false>(m, n, r0, w, l, t, g, 1 + slot, nullptr);
-    // physical column c is logical column invmap[c]
+    for (int kb = 0; kb < nw; ++kb) {
 #pragma unroll
-    for (int q = 0; q < kFCols; ++q) {
-        const int c = l + 64 * q;
-        const int cl = (c < n) ? g.invmap[c] : 0;
+        for (int cc = 0; cc < 8; ++cc) {
+            const int k = 8 * kb + cc;
+            if (k >= n) break;     // uniform
+            const int b = k & 1;
+            if (w == kb) {
+                const cx<float> c0 = m[0][cc], c1 = m[1][cc];
+                const int ra = l, rb = l + 64;
+                const float mag0 = (ra < n && !used0) ? cabs2(c0) : -1.0f;
+                const float mag1 = (rb < n && !used1) ? cabs2(c1) : -1.0f;
+                const float best = fmaxf(mag0, mag1);
+                const int arg = (mag1 > mag0) ? rb : ra;
+                float mx = best;
+                mx = fmaxf(mx, dpp_f<0xB1>(mx));    // quad_perm [1,0,3,2]
+                mx = fmaxf(mx, dpp_f<0x4E>(mx));    // quad_perm [2,3,0,1]
+                mx = fmaxf(mx, dpp_f<0x141>(mx));   // row_half_mirror
+                mx = fmaxf(mx, dpp_f<0x140>(mx));   // row_mirror
+                const float gmax = fmaxf(fmaxf(lane_f(mx, 0), lane_f(mx, 16)), fmaxf(lane_f(mx, 32), lane_f(mx, 48)));
+                // lowest row attaining the maximum: rows < 64 first
+                const unsigned long long low = __ballot(best == gmax && arg < 64);
+                const unsigned long long any = __ballot(best == gmax);
+                int pr = low ? (__ffsll(low) - 1) : (__ffsll(any) - 1 + 64);
+                pr = __builtin_amdgcn_readfirstlane(pr);
+                const int pl = pr & 63;
+                const bool hi = pr >= 64;
+                const cx<float> pv = hi ? c1 : c0;
+                const cx<float> piv = mk<float>(lane_f(pv.re, pl), lane_f(pv.im, pl));
+                const float d2 = piv.re * piv.re + piv.im * piv.im;
+                const float inv_d2 = (d2 > 0.0f) ? 1.0f / d2 : 0.0f;
+                const cx<float> ipiv = mk<float>(piv.re * inv_d2, -piv.im * inv_d2);
+                const cx<float> f0 = (ra == pr) ? czero<float>() : c0 * ipiv;
+                const cx<float> f1 = (rb == pr) ? czero<float>() : c1 * ipiv;
+                g.fbuf[b][ra] = f0;
+                g.fbuf[b][rb] = f1;
+                m[0][cc] = (ra == pr) ? mk<float>(1.0f, 0.0f) : mk<float>(-f0.re, -f0.im);
+                m[1][cc] = (rb == pr) ? mk<float>(1.0f, 0.0f) : mk<float>(-f1.re, -f1.im);
+                if (l == 0) {
+                    g.prbuf[b] = pr;
+                    g.scale[pr] = ipiv;
+                    g.piv[k] = pr;
+                    if (!(d2 > 0.0f) && g.bad == 0) g.bad = code;
+                }
+            }
+            __syncthreads();
+            const int pr = __builtin_amdgcn_readfirstlane(g.prbuf[b]);
+            const int pl = pr & 63;
+            const bool hi = pr >= 64;
+            if (l == pl) {
+                if (hi) used1 = true;
+                else used0 = true;
+            }
+            const cx<float> f0 = g.fbuf[b][l], f1 = g.fbuf[b][l + 64];
+            // the stored pivot row in this wave's columns (column k itself excluded)
+            float ur[8], ui[8];
 #pragma unroll
-        for (int p = 0; p < kFRows; ++p) {
-            const int r = r0 + p;
-            if (r < n && c < n) S[(int64_t)r * n + cl] = m[p][q];
+            for (int c = 0; c < 8; ++c) {
+                const cx<float> v = hi ? m[1][c] : m[0][c];
+                ur[c] = lane_f(v.re, pl);
+                ui[c] = lane_f(v.im, pl);
+            }
+            if (w == kb) {
+                ur[cc] = 0.0f;
+                ui[cc] = 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                m[0][c].re = fmaf(-f0.re, ur[c], fmaf(f0.im, ui[c], m[0][c].re));
+                m[0][c].im = fmaf(-f0.re, ui[c], fmaf(-f0.im, ur[c], m[0][c].im));
+                m[1][c].re = fmaf(-f1.re, ur[c], fmaf(f1.im, ui[c], m[1][c].re));
+                m[1][c].im = fmaf(-f1.re, ui[c], fmaf(-f1.im, ur[c], m[1][c].im));
+            }
+        }
+    }
+    __syncthreads();
+    if (t < n) g.inv[g.piv[t]] = t;
+    __syncthreads();
+    // physical (r, c) -> logical (inv[r], piv[c]), times the row's scale
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = l + 64 * q;
+        if (r >= n) continue;
+        const int li = g.inv[r];
+        const cx<float> sc = g.scale[r];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cc = 8 * w + c;
+            if (cc < n) S[(int64_t)li * n + g.piv[cc]] = sc * m[q][c];
         }
     }
     if (t == 0 && g.bad != 0) atomicCAS(status, 0, g.bad);
