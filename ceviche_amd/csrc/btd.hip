@@ -791,15 +791,28 @@ struct BcrVecBatch {
     cx<float>* z[kBcrBatch];
 };
 
+// Thread roles, 64 ceil(8 n / 64) threads: A: (row a = t / 8, part p = t % 8);
+// A^T: (column a = t % n, part p = t / n), threads with p >= 8 idle.
 template <bool T>
-__device__ __forceinline__ void bcr_load(const cx<float>* __restrict__ M, int n, bool diag, cx<float> (&e)[kVecPer]) {
-    const int t = threadIdx.x;
-    const int a = T ? t % kBtdMax : t / 8, p = T ? t / kBtdMax : t % 8;
+struct VecRole {
+    int a, p;
+    bool on;
+    __device__ __forceinline__ VecRole(int n) {
+        const int t = threadIdx.x;
+        a = T ? t % n : t / 8;
+        p = T ? t / n : t % 8;
+        on = (p < 8) && (a < n);
+    }
+};
+
+template <bool T>
+__device__ __forceinline__ void bcr_load(const cx<float>* __restrict__ M, int n, bool diag, const VecRole<T>& v,
+                                         cx<float> (&e)[kVecPer]) {
 #pragma unroll
     for (int i = 0; i < kVecPer; ++i) {
-        const int b = p + 8 * i;
-        const int r = T ? b : a, c = T ? a : b;
-        e[i] = (r < n && c < n && (!diag || r == c)) ? M[(int64_t)r * n + c] : czero<float>();
+        const int b = v.p + 8 * i;
+        const int r = T ? b : v.a, c = T ? v.a : b;
+        e[i] = (v.on && b < n && (!diag || r == c)) ? M[(int64_t)r * n + c] : czero<float>();
     }
 }
 
@@ -817,8 +830,8 @@ __device__ __forceinline__ cx<float> bcr_dot(const cx<float> (&e)[kVecPer], cons
 
 // out[] = base[] + sg * (sum over the 8 parts), ends with a barrier
 template <bool T>
-__device__ __forceinline__ void bcr_combine(cx<float> acc, const cx<float>* base, cx<float>* out,
-                                            cx<float> (&red)[8][kBtdMax], int n, float sg) {
+__device__ __forceinline__ void bcr_combine(cx<float> acc, const VecRole<T>& v, const cx<float>* base,
+                                            cx<float>* out, cx<float> (&red)[8][kBtdMax], int n, float sg) {
     const int t = threadIdx.x;
     if (!T) {
 #pragma unroll
@@ -826,10 +839,9 @@ __device__ __forceinline__ void bcr_combine(cx<float> acc, const cx<float>* base
             acc.re += __shfl_xor(acc.re, off, 64);
             acc.im += __shfl_xor(acc.im, off, 64);
         }
-        const int r = t / 8;
-        if ((t % 8) == 0 && r < n) out[r] = base[r] + mk<float>(sg * acc.re, sg * acc.im);
+        if (v.p == 0 && v.a < n) out[v.a] = base[v.a] + mk<float>(sg * acc.re, sg * acc.im);
     } else {
-        red[t / kBtdMax][t % kBtdMax] = acc;
+        if (v.on) red[v.p][v.a] = acc;
         __syncthreads();
         if (t < n) {
             cx<float> s2 = czero<float>();
@@ -841,7 +853,13 @@ __device__ __forceinline__ void bcr_combine(cx<float> acc, const cx<float>* base
     __syncthreads();
 }
 
-template <bool TRANS>
+// Solve-op kinds of a launch: every op's matrices are in registers before
+// the first barrier; the kinds bound the registers a launch needs.
+constexpr int kVecGeneral = 0;   // M1, M2 and M0 (any present)
+constexpr int kVecReduce = 1;    // M1, M2 only: the reduction sweep
+constexpr int kVecDiag = 2;      // diagonal M1, M2 and a dense M0: level 0 back-substitution
+
+template <bool TRANS, int KIND>
 __global__ void __launch_bounds__(kBtdThreads)
 k_bcr_vec(BcrVecBatch B, const BcrVecOp* __restrict__ ops, int n, int nx) {
     __shared__ cx<float> bs[kBtdMax], x1[kBtdMax], x2[kBtdMax], tv[kBtdMax], res[kBtdMax];
@@ -852,45 +870,61 @@ k_bcr_vec(BcrVecBatch B, const BcrVecOp* __restrict__ ops, int n, int nx) {
     const BcrVecOp op = ops[blockIdx.x];
     const int64_t nn = (int64_t)n * n;
     const int t = threadIdx.x;
-    const int p = TRANS ? t / kBtdMax : t % 8;
+    const VecRole<TRANS> role(n);
     auto vec_in = [&](int i) -> const cx<float>* {
         return i < nx ? B.v[f] + (int64_t)i * n : (i < 2 * nx ? B.z[f] + (int64_t)(i - nx) * n : work + (int64_t)i * n);
     };
     const int m1 = TRANS ? op.m1t : op.m1, m2 = TRANS ? op.m2t : op.m2;
-    const bool h1 = op.u1 >= 0 && m1 >= 0, h2 = op.u2 >= 0 && m2 >= 0, h0 = op.m0 >= 0;
+    const bool h1 = op.u1 >= 0 && m1 >= 0, h2 = op.u2 >= 0 && m2 >= 0;
+    const bool h0 = (KIND != kVecReduce) && op.m0 >= 0;
     cx<float> e1[kVecPer], e2[kVecPer], e0[kVecPer];
-    if (h1) bcr_load<TRANS>(slots + (int64_t)m1 * nn, n, op.diag & (TRANS ? 4 : 1), e1);
-    if (h2) bcr_load<TRANS>(slots + (int64_t)m2 * nn, n, op.diag & (TRANS ? 8 : 2), e2);
-    if (h0) bcr_load<TRANS>(slots + (int64_t)op.m0 * nn, n, false, e0);
-    if (t < kBtdMax) {
-        const bool in = t < n;
-        bs[t] = in ? vec_in(op.b)[t] : czero<float>();
-        tv[t] = czero<float>();
-        x1[t] = (in && h1) ? vec_in(op.u1)[t] : czero<float>();
-        x2[t] = (in && h2) ? vec_in(op.u2)[t] : czero<float>();
+    cx<float> d1 = czero<float>(), d2 = czero<float>();    // kVecDiag: this thread's diagonal entries
+    // the thread holding the diagonal entry of its row (A) / column (A^T)
+    const bool holder = role.on && (role.p == (role.a & 7));
+    if (KIND == kVecDiag) {
+        const int64_t dd = (int64_t)role.a * n + role.a;
+        if (h1 && holder) d1 = slots[(int64_t)m1 * nn + dd];
+        if (h2 && holder) d2 = slots[(int64_t)m2 * nn + dd];
+    } else {
+        if (h1) bcr_load<TRANS>(slots + (int64_t)m1 * nn, n, op.diag & (TRANS ? 4 : 1), role, e1);
+        if (h2) bcr_load<TRANS>(slots + (int64_t)m2 * nn, n, op.diag & (TRANS ? 8 : 2), role, e2);
+    }
+    if (h0) bcr_load<TRANS>(slots + (int64_t)op.m0 * nn, n, false, role, e0);
+    for (int i = t; i < kBtdMax; i += blockDim.x) {
+        const bool in = i < n;
+        bs[i] = in ? vec_in(op.b)[i] : czero<float>();
+        tv[i] = czero<float>();
+        res[i] = czero<float>();
+        x1[i] = (in && h1) ? vec_in(op.u1)[i] : czero<float>();
+        x2[i] = (in && h2) ? vec_in(op.u2)[i] : czero<float>();
     }
     __syncthreads();
     // t = b - M1 u1 - M2 u2
     cx<float> acc = czero<float>();
-    if (h1) acc = acc + bcr_dot(e1, x1, p);
-    if (h2) acc = acc + bcr_dot(e2, x2, p);
+    if (role.on) {
+        if (KIND == kVecDiag) {
+            acc = acc + d1 * x1[role.a] + d2 * x2[role.a];
+        } else {
+            if (h1) acc = acc + bcr_dot(e1, x1, role.p);
+            if (h2) acc = acc + bcr_dot(e2, x2, role.p);
+        }
+    }
     // ops never write a right-hand side (index < nx)
     cx<float>* const outp = (op.out < 2 * nx) ? B.z[f] + (int64_t)(op.out - nx) * n : work + (int64_t)op.out * n;
     if (!h0) {
         if (h1 || h2) {
-            bcr_combine<TRANS>(acc, bs, tv, red, n, -1.0f);
+            bcr_combine<TRANS>(acc, role, bs, tv, red, n, -1.0f);
             if (t < n) outp[t] = tv[t];
         } else if (t < n) {
             outp[t] = bs[t];
         }
         return;
     }
-    if (h1 || h2) bcr_combine<TRANS>(acc, bs, tv, red, n, -1.0f);
+    if (h1 || h2) bcr_combine<TRANS>(acc, role, bs, tv, red, n, -1.0f);
     const cx<float>* const tin = (h1 || h2) ? tv : bs;
     // out = M0 t
-    if (t < kBtdMax) res[t] = czero<float>();
-    __syncthreads();
-    bcr_combine<TRANS>(bcr_dot(e0, tin, p), res, res, red, n, 1.0f);
+    const cx<float> acc0 = role.on ? bcr_dot(e0, tin, role.p) : czero<float>();
+    bcr_combine<TRANS>(acc0, role, res, res, red, n, 1.0f);
     if (t < n) outp[t] = res[t];
 }
 
@@ -1120,18 +1154,25 @@ int bcr_apply_batch(const cv_fdfd_desc& dh, int nb, const void* const* factors, 
             B.z[q] = z + (int64_t)(f0 + q) * stride;
         }
         // every factor of this shape holds the same op tables: use the first one's
-        auto launch = [&](int64_t off, int count) {
+        const int threads = 64 * ((8 * n + 63) / 64);
+        auto launch = [&](int64_t off, int count, int kind) {
             if (count <= 0) return;
             const BcrVecOp* ops = (const BcrVecOp*)(base0 + off);
             const dim3 grid(count, cnt);
-            if (transpose)
-                k_bcr_vec<true><<<grid, kBtdThreads, 0, s>>>(B, ops, n, nx);
-            else
-                k_bcr_vec<false><<<grid, kBtdThreads, 0, s>>>(B, ops, n, nx);
+            if (transpose) {
+                if (kind == kVecReduce) k_bcr_vec<true, kVecReduce><<<grid, threads, 0, s>>>(B, ops, n, nx);
+                else if (kind == kVecDiag) k_bcr_vec<true, kVecDiag><<<grid, threads, 0, s>>>(B, ops, n, nx);
+                else k_bcr_vec<true, kVecGeneral><<<grid, threads, 0, s>>>(B, ops, n, nx);
+            } else {
+                if (kind == kVecReduce) k_bcr_vec<false, kVecReduce><<<grid, threads, 0, s>>>(B, ops, n, nx);
+                else if (kind == kVecDiag) k_bcr_vec<false, kVecDiag><<<grid, threads, 0, s>>>(B, ops, n, nx);
+                else k_bcr_vec<false, kVecGeneral><<<grid, threads, 0, s>>>(B, ops, n, nx);
+            }
         };
-        for (size_t lv = 0; lv < P.fwd.size(); ++lv) launch(P.off_fwd[lv], (int)P.fwd[lv].size());
-        launch(P.off_final, 1);
-        for (size_t lv = P.bwd.size(); lv-- > 0;) launch(P.off_bwd[lv], (int)P.bwd[lv].size());
+        for (size_t lv = 0; lv < P.fwd.size(); ++lv) launch(P.off_fwd[lv], (int)P.fwd[lv].size(), kVecReduce);
+        launch(P.off_final, 1, kVecGeneral);
+        for (size_t lv = P.bwd.size(); lv-- > 0;)
+            launch(P.off_bwd[lv], (int)P.bwd[lv].size(), lv == 0 ? kVecDiag : kVecGeneral);
         CV_LAUNCH_CHECK();
     }
     return CV_OK;
