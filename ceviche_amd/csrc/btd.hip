@@ -771,13 +771,13 @@ static void launch_gemm(const BcrSlots& S, const BcrGemmOp* ops, int count, int 
     }
 }
 
-// One BCR solve op per workgroup of 1024 threads, several factorisations of
-// the same shape at once (blockIdx.y).  Every matrix entry a thread needs is
-// loaded at the start, so each op waits for memory once instead of once per
-// row loop:
+// One BCR solve op per workgroup of 8 threads per block row (rounded up to
+// whole waves), several factorisations of the same shape at once
+// (blockIdx.y).  Every matrix entry a thread needs is loaded at the start,
+// so each op waits for memory once instead of once per row loop:
 //   A:   thread (row r = t / 8, part = t % 8) holds columns part, part + 8, ...
 //        and 8 lanes combine by shuffles;
-//   A^T: thread (column c = t % 128, part = t / 128) holds rows part, part + 8,
+//   A^T: thread (column c = t % n, part = t / n) holds rows part, part + 8,
 //        ..., coalesced along c, and the 8 parts combine through LDS.
 // Vector index i < nx is the right-hand side v, nx <= i < 2 nx the solution
 // z (level 0, read from / written to the caller's buffers directly); the
