@@ -342,8 +342,28 @@ BTD_MAX_NY = 128
 
 def direct_applies(op):
     """True when the block-tridiagonal factorisation handles ``op``: one
-    uncoupled instance whose y extent fits a workgroup."""
-    return op.n_inst == 1 and op.cpl_len == 0 and op.shapes[0][1] <= BTD_MAX_NY
+    uncoupled instance whose y extent fits a workgroup, or an Ez instance
+    whose x extent does (factorised on the transposed grid)."""
+    if op.n_inst != 1 or op.cpl_len != 0:
+        return False
+    nx, ny = op.shapes[0]
+    return ny <= BTD_MAX_NY or (op.pol == 'ez' and nx <= BTD_MAX_NY)
+
+
+def transposed_operator(op):
+    """The same single-instance Ez operator on the transposed grid (x and y
+    swapped with their PML widths and Bloch phases): A_T = P A P^T for the
+    cell permutation P (i, j) -> (j, i), since the Ez operator
+    -(1/mu0)(Dxf Dxb + Dyf Dyb) - eps0 w^2 eps treats both axes alike
+    (ceviche/fdfd.py:172-185, derivatives.py:94-124)."""
+    if getattr(op, '_transposed', None) is None:
+        sp = dict(op.specs[0])
+        nx, ny = (int(v) for v in sp['shape'])
+        px, py = sp['npml']
+        bx, by = sp.get('bloch', (0.0, 0.0))
+        sp.update(shape=(ny, nx), npml=(py, px), bloch=(by, bx))
+        op._transposed = FdfdOperator(op.pol, [sp], dtype=op.dtype, device=op.device)
+    return op._transposed
 
 
 class Direct:
@@ -357,8 +377,16 @@ class Direct:
 
     def __init__(self, op, m0, m1, kind=None, factorise=True):
         if not direct_applies(op):
-            raise ValueError('the block-tridiagonal factorisation needs one uncoupled instance with ny <= %d'
-                             % BTD_MAX_NY)
+            raise ValueError('the block-tridiagonal factorisation needs one uncoupled instance with ny <= %d '
+                             '(or an Ez instance with nx <= %d)' % (BTD_MAX_NY, BTD_MAX_NY))
+        self.user_op = op
+        self.shape = tuple(op.shapes[0])
+        # a tall Ez grid (ny > 128 >= nx) is factorised and solved on its
+        # transpose; vectors are transposed on the way in and out
+        self.tr = self.shape[1] > BTD_MAX_NY
+        if self.tr:
+            op = transposed_operator(op)
+            m0, m1 = self._t(m0), self._t(m1)
         self.op = op
         nx, ny = op.shapes[0]
         if kind is None:
@@ -385,8 +413,25 @@ class Direct:
             check(L.cv_fdfd_btd_factor(ptr(op.desc), dh, ptr(self.prof32), ptr(self.m0), ptr(self.m1),
                                        ptr(self.factor), nbytes, stream_ptr()), 'block-tridiagonal factorisation')
 
+    def _t(self, v):
+        """A vector of the user's (nx, ny) grid in the factorised grid's order."""
+        if v is None or not self.tr:
+            return v
+        return v.reshape(self.shape).transpose(0, 1).contiguous().reshape(-1)
+
+    def _tb(self, v):
+        """The inverse of _t."""
+        if not self.tr:
+            return v
+        return v.reshape(self.shape[1], self.shape[0]).transpose(0, 1).contiguous().reshape(-1)
+
     def apply(self, v, transpose=False):
         """M^-1 v (complex64) with the factorisation alone."""
+        if self.tr:
+            return self._tb(self._apply(self._t(v), transpose))
+        return self._apply(v, transpose)
+
+    def _apply(self, v, transpose):
         v = v.to(torch.complex64).contiguous()
         z = torch.empty_like(v)
         op = self.op
@@ -402,6 +447,12 @@ class Direct:
 
     def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
         """(c A) x = bs (or (c A)^T x = bs) to the FP64 tolerance."""
+        if self.tr:
+            x = self._solve(self._t(bs), self._t(m0_64), self._t(m1_64), transpose, opts)
+            return self._tb(x)
+        return self._solve(bs, m0_64, m1_64, transpose, opts)
+
+    def _solve(self, bs, m0_64, m1_64, transpose, opts):
         op = self.op
         opts = opts or SolverOptions()
         tol = opts.tol if opts.tol is not None else DEFAULT_TOL[torch.complex128]
@@ -416,8 +467,9 @@ class Direct:
                                      ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1),
                                      self.code, ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)), float(tol),
                                      int(opts.restart), int(opts.max_iter), ptr(ws), need, info, stream_ptr())
-        op.last_info = [dict(iterations=info[0].iterations, status=info[0].status, restarts=info[0].restarts,
-                             rel_residual=info[0].rel_residual, method='direct', factor=self.kind)]
+        self.user_op.last_info = [dict(iterations=info[0].iterations, status=info[0].status,
+                                       restarts=info[0].restarts, rel_residual=info[0].rel_residual,
+                                       method='direct', factor=self.kind + ('-transposed' if self.tr else ''))]
         check(rc, 'fdfd direct solve')
         return x
 
@@ -500,7 +552,8 @@ def _use_direct(op, opts):
     if method == 'bicgstab':
         return False
     if method == 'direct' and not direct_applies(op):
-        raise ValueError("method='direct' needs one uncoupled instance with ny <= %d" % BTD_MAX_NY)
+        raise ValueError("method='direct' needs one uncoupled instance with ny <= %d (Ez: nx or ny <= %d)"
+                         % (BTD_MAX_NY, BTD_MAX_NY))
     return direct_applies(op)
 
 

@@ -102,6 +102,59 @@ def test_direct_solves_match_spsolve(pol, shape, npml, bloch, kind):
         print(kind, pol, shape, 'transpose' if tr else 'forward', info, flush=True)
 
 
+# tall Ez grids (ny > 128 >= nx): factorised and solved on the transposed grid
+CASES_TALL = [('ez', (40, 200), (5, 12), (0.3, -0.5)), ('ez', (9, 150), (2, 10), (0.0, 0.0)),
+              ('ez', (128, 129), (10, 10), (0.0, 0.0))]
+
+
+@pytest.mark.parametrize('kind', ['btd', 'bcr'])
+@pytest.mark.parametrize('pol,shape,npml,bloch', CASES_TALL)
+def test_tall_grids_solve_on_the_transposed_grid(pol, shape, npml, bloch, kind):
+    """A x = b and A^T x = b for tall grids (the factor and the FGMRES on the
+    transposed operator, vectors transposed in and out) vs spsolve of the
+    user's operator."""
+    from ceviche_amd._fdfd_op import Direct, SolverOptions, direct_applies
+    op = _op(pol, shape, npml, bloch)
+    assert direct_applies(op)
+    rng = np.random.default_rng(5)
+    eps = rng.uniform(1, 12.25, shape)
+    m0, m1 = op.materials(torch.from_numpy(eps.ravel()))
+    D = Direct(op, m0, m1, kind=kind)
+    assert D.tr and D.op.shapes[0] == (shape[1], shape[0])
+    A = _scaled_A(op, pol, eps, npml, bloch)
+    b = rng.standard_normal(eps.size) + 1j * rng.standard_normal(eps.size)
+    for tr in (False, True):
+        x = D.solve(torch.from_numpy(b).cuda(), m0, m1, transpose=tr, opts=SolverOptions(tol=1e-10)).cpu().numpy()
+        info = op.last_info[0]
+        ref = spl.spsolve((A.T if tr else A).tocsc(), b)
+        assert info['status'] == 0 and info['rel_residual'] <= 1e-10, info
+        assert info['factor'] == kind + '-transposed'
+        assert np.linalg.norm(x - ref) <= 1e-8 * np.linalg.norm(ref), (tr, info)
+
+
+def test_tall_grid_fields_and_gradient():
+    """fdfd_ez on a 48 x 180 grid takes the transposed direct path by default:
+    fields vs spsolve, the eps gradient vs the oracle adjoint."""
+    import ceviche_amd as cv
+    rng = np.random.default_rng(8)
+    shape, npml = (48, 180), [6, 10]
+    eps0 = rng.uniform(1, 6, shape)
+    src = np.zeros(shape)
+    src[:, 30] = 1.0
+    eps = torch.tensor(eps0, requires_grad=True)
+    F = cv.fdfd_ez(OMEGA, 50e-9, eps, npml)
+    _, _, Ez = F.solve(torch.from_numpy(src))
+    assert F.last_solve_info['method'] == 'direct' and F.last_solve_info['factor'] == 'bcr-transposed'
+    _, _, rez = R.solve_ez(OMEGA, 50e-9, eps0, npml, src)
+    ez = Ez.detach().cpu().numpy()
+    assert np.linalg.norm(ez - rez) <= 1e-8 * np.linalg.norm(rez)
+    (Ez[30, 150].abs() ** 2).backward()
+    cot = np.zeros(shape, dtype=complex)
+    cot[30, 150] = 2 * ez[30, 150]
+    gref, _ = R.grad_eps_ez(OMEGA, 50e-9, eps0, npml, src, cot)
+    assert np.linalg.norm(eps.grad.numpy() - gref) <= 1e-6 * np.linalg.norm(gref)
+
+
 def test_direct_path_fields_gradient_and_forward_mode():
     """fdfd_ez / fdfd_hz through the direct path: fields vs spsolve, the eps
     gradient (one factorisation for the forward and adjoint solves) vs the
@@ -141,7 +194,10 @@ def test_direct_path_fields_gradient_and_forward_mode():
 
 
 def test_direct_method_requires_a_narrow_grid():
+    """method='direct' needs ny <= 128, or nx <= 128 for Ez (the transposed
+    grid); otherwise it is an error, not a silent fallback."""
     import ceviche_amd as cv
-    F = cv.fdfd_ez(OMEGA, 50e-9, np.ones((10, 130)), [2, 2], solver_options=cv.SolverOptions(method='direct'))
-    with pytest.raises(ValueError):
-        F.solve(np.ones((10, 130)))
+    for cls, shape in ((cv.fdfd_ez, (130, 131)), (cv.fdfd_hz, (10, 130))):
+        F = cls(OMEGA, 50e-9, np.ones(shape), [2, 2], solver_options=cv.SolverOptions(method='direct'))
+        with pytest.raises(ValueError):
+            F.solve(np.ones(shape))
