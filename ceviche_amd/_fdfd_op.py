@@ -342,20 +342,22 @@ BTD_MAX_NY = 128
 
 def direct_applies(op):
     """True when the block-tridiagonal factorisation handles ``op``: one
-    uncoupled instance whose y extent fits a workgroup, or an Ez instance
-    whose x extent does (factorised on the transposed grid)."""
+    uncoupled instance whose y extent fits a workgroup, or whose x extent
+    does (factorised on the transposed grid)."""
     if op.n_inst != 1 or op.cpl_len != 0:
         return False
     nx, ny = op.shapes[0]
-    return ny <= BTD_MAX_NY or (op.pol == 'ez' and nx <= BTD_MAX_NY)
+    return ny <= BTD_MAX_NY or nx <= BTD_MAX_NY
 
 
 def transposed_operator(op):
-    """The same single-instance Ez operator on the transposed grid (x and y
+    """The same single-instance operator on the transposed grid (x and y
     swapped with their PML widths and Bloch phases): A_T = P A P^T for the
-    cell permutation P (i, j) -> (j, i), since the Ez operator
-    -(1/mu0)(Dxf Dxb + Dyf Dyb) - eps0 w^2 eps treats both axes alike
-    (ceviche/fdfd.py:172-185, derivatives.py:94-124)."""
+    cell permutation P (i, j) -> (j, i).  Ez: -(1/mu0)(Dxf Dxb + Dyf Dyb) -
+    eps0 w^2 eps treats both axes alike (ceviche/fdfd.py:172-185,
+    derivatives.py:94-124); Hz: the same with the x- and y-derivative
+    materials exchanged, as eps_xx / eps_yy average along y / x
+    (fdfd.py:200-244)."""
     if getattr(op, '_transposed', None) is None:
         sp = dict(op.specs[0])
         nx, ny = (int(v) for v in sp['shape'])
@@ -377,16 +379,17 @@ class Direct:
 
     def __init__(self, op, m0, m1, kind=None, factorise=True):
         if not direct_applies(op):
-            raise ValueError('the block-tridiagonal factorisation needs one uncoupled instance with ny <= %d '
-                             '(or an Ez instance with nx <= %d)' % (BTD_MAX_NY, BTD_MAX_NY))
+            raise ValueError('the block-tridiagonal factorisation needs one uncoupled instance with nx or ny '
+                             '<= %d' % BTD_MAX_NY)
         self.user_op = op
         self.shape = tuple(op.shapes[0])
-        # a tall Ez grid (ny > 128 >= nx) is factorised and solved on its
-        # transpose; vectors are transposed on the way in and out
+        # a tall grid (ny > 128 >= nx) is factorised and solved on its
+        # transpose; vectors are transposed on the way in and out (Hz: the
+        # two materials exchanged as well)
         self.tr = self.shape[1] > BTD_MAX_NY
         if self.tr:
             op = transposed_operator(op)
-            m0, m1 = self._t(m0), self._t(m1)
+            m0, m1 = self._tm(m0, m1)
         self.op = op
         nx, ny = op.shapes[0]
         if kind is None:
@@ -419,6 +422,14 @@ class Direct:
             return v
         return v.reshape(self.shape).transpose(0, 1).contiguous().reshape(-1)
 
+    def _tm(self, m0, m1):
+        """The materials (m0, m1) of the user's grid for the factorised grid."""
+        if not self.tr:
+            return m0, m1
+        if self.user_op.pol == 'hz':     # m1 goes with x differences, m0 with y
+            return self._t(m1), self._t(m0)
+        return self._t(m0), self._t(m1)
+
     def _tb(self, v):
         """The inverse of _t."""
         if not self.tr:
@@ -448,7 +459,7 @@ class Direct:
     def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
         """(c A) x = bs (or (c A)^T x = bs) to the FP64 tolerance."""
         if self.tr:
-            x = self._solve(self._t(bs), self._t(m0_64), self._t(m1_64), transpose, opts)
+            x = self._solve(self._t(bs), *self._tm(m0_64, m1_64), transpose, opts)
             return self._tb(x)
         return self._solve(bs, m0_64, m1_64, transpose, opts)
 
@@ -552,8 +563,7 @@ def _use_direct(op, opts):
     if method == 'bicgstab':
         return False
     if method == 'direct' and not direct_applies(op):
-        raise ValueError("method='direct' needs one uncoupled instance with ny <= %d (Ez: nx or ny <= %d)"
-                         % (BTD_MAX_NY, BTD_MAX_NY))
+        raise ValueError("method='direct' needs one uncoupled instance with nx or ny <= %d" % BTD_MAX_NY)
     return direct_applies(op)
 
 

@@ -102,9 +102,10 @@ def test_direct_solves_match_spsolve(pol, shape, npml, bloch, kind):
         print(kind, pol, shape, 'transpose' if tr else 'forward', info, flush=True)
 
 
-# tall Ez grids (ny > 128 >= nx): factorised and solved on the transposed grid
+# tall grids (ny > 128 >= nx): factorised and solved on the transposed grid
 CASES_TALL = [('ez', (40, 200), (5, 12), (0.3, -0.5)), ('ez', (9, 150), (2, 10), (0.0, 0.0)),
-              ('ez', (128, 129), (10, 10), (0.0, 0.0))]
+              ('ez', (128, 129), (10, 10), (0.0, 0.0)), ('hz', (33, 170), (4, 9), (0.0, 0.7)),
+              ('hz', (12, 140), (0, 6), (0.2, 0.0))]
 
 
 @pytest.mark.parametrize('kind', ['btd', 'bcr'])
@@ -194,10 +195,10 @@ def test_direct_path_fields_gradient_and_forward_mode():
 
 
 def test_direct_method_requires_a_narrow_grid():
-    """method='direct' needs ny <= 128, or nx <= 128 for Ez (the transposed
-    grid); otherwise it is an error, not a silent fallback."""
+    """method='direct' needs ny <= 128, or nx <= 128 (the transposed grid);
+    otherwise it is an error, not a silent fallback."""
     import ceviche_amd as cv
-    for cls, shape in ((cv.fdfd_ez, (130, 131)), (cv.fdfd_hz, (10, 130))):
+    for cls, shape in ((cv.fdfd_ez, (130, 131)), (cv.fdfd_hz, (129, 140))):
         F = cls(OMEGA, 50e-9, np.ones(shape), [2, 2], solver_options=cv.SolverOptions(method='direct'))
         with pytest.raises(ValueError):
             F.solve(np.ones(shape))
