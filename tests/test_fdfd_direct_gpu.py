@@ -41,7 +41,10 @@ CASES = [('ez', (24, 18), (4, 3), (0.0, 0.0)), ('hz', (19, 23), (3, 4), (0.0, 0.
 CASES_BCR = CASES + [('ez', (2, 9), (0, 2), (0.0, 0.0)), ('hz', (3, 14), (1, 3), (0.0, 0.4)),
                      ('ez', (65, 40), (8, 6), (0.0, 0.0)),
                      # y extents whose products use the 80- and 96-wide tiles
-                     ('ez', (40, 80), (6, 8), (0.0, 0.0)), ('hz', (33, 90), (5, 8), (0.2, 0.0))]
+                     ('ez', (40, 80), (6, 8), (0.0, 0.0)), ('hz', (33, 90), (5, 8), (0.2, 0.0)),
+                     # the smallest x extent that defaults to reduction; a long one with odd
+                     # block counts on most levels
+                     ('ez', (8, 30), (2, 4), (0.0, 0.0)), ('hz', (1025, 12), (20, 2), (0.1, 0.0))]
 
 
 @pytest.mark.parametrize('kind', ['btd', 'bcr'])
