@@ -29,8 +29,9 @@ class SolverOptions:
     """Solver settings.  tol is the relative FP64 residual ||b - A x|| / ||b||.
 
     method: 'auto' (the block-tridiagonal factorisation + FGMRES for a single
-    instance with ny <= 128, BiCGSTAB otherwise), 'direct' (require the
-    former) or 'bicgstab'.  restart: FGMRES cycle length of the direct path."""
+    instance with ny <= 128, or nx <= 128 on the transposed grid, BiCGSTAB
+    otherwise), 'direct' (require the former) or 'bicgstab'.  restart: FGMRES
+    cycle length of the direct path."""
 
     def __init__(self, tol=None, max_iter=200000, check_every=32, inner_tol=1e-4, max_sweeps=12, method='auto',
                  restart=30):
