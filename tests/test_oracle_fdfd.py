@@ -150,3 +150,97 @@ def test_ez_adjoint_gradient_matches_central_differences():
     num = np.array([der_num(lambda e: obj(e), eps.copy(), i, 1e-4) for i in idx])
     ana = np.array([grad[i] for i in idx])
     assert np.linalg.norm(num - ana) <= 1e-4 * np.linalg.norm(num)
+
+
+def test_hz_grid_average_hand_values():
+    # ceviche/fdfd.py:200-209: eps_xx averages each cell with its neighbour at
+    # y - 1 (axis 1), eps_yy with its neighbour at x - 1 (axis 0), periodically.
+    eps = np.array([[1.0, 2.0, 3.0],
+                    [4.0, 5.0, 6.0]])
+    exx, eyy = R.grid_average_hz(eps)
+    # by hand: exx[0, 0] = (1 + 3) / 2 (wraps to y = 2), exx[0, 1] = (2 + 1) / 2, ...
+    assert np.array_equal(exx, np.array([[2.0, 1.5, 2.5],
+                                         [5.0, 4.5, 5.5]]))
+    # eyy[0, j] = (eps[0, j] + eps[1, j]) / 2 (wraps to x = 1), eyy[1, j] the same pair
+    assert np.array_equal(eyy, np.array([[2.5, 3.5, 4.5],
+                                         [2.5, 3.5, 4.5]]))
+
+
+def _hz_layered_1d(omega, dL, e):
+    """Hz operator of a medium layered along one axis, for fields constant
+    along the other, written out entry by entry: the forward difference
+    h[i+1] - h[i] is divided by the average of cells i and i-1 (ceviche's
+    index convention, fdfd.py:200-215: the average stored at index i is
+    formed with the neighbour at i-1), then differenced backward; periodic,
+    no PML."""
+    n = e.size
+    A = np.zeros((n, n), dtype=complex)
+    avg = np.array([(e[i] + e[i - 1]) / 2 + 1e-5 for i in range(n)])
+    c = 1 / R.EPSILON_0 / dL ** 2
+    for i in range(n):
+        ip, im = (i + 1) % n, (i - 1) % n
+        # forward difference at i, scaled by 1/avg[i]; backward difference of that flux
+        A[i, ip] += c / avg[i]
+        A[i, i] -= c / avg[i]
+        A[i, i] -= c / avg[im]
+        A[i, im] += c / avg[im]
+        A[i, i] += R.MU_0 * omega ** 2
+    return A
+
+
+@pytest.mark.parametrize('axis', [0, 1])
+def test_hz_operator_pairs_each_average_with_the_right_derivative(axis):
+    # eps layered along x (axis 0) only acts through eps_yy and Dx; layered
+    # along y only through eps_xx and Dy.  A y-uniform (x-uniform) field sees
+    # exactly the 1D operator; a swapped pairing or averaging axis fails.
+    omega, dL = 2 * np.pi * R.C_0 / 1.55e-6, 50e-9
+    rng = np.random.default_rng(3)
+    n, m = 11, 7
+    prof = rng.uniform(1, 12.25, n)
+    eps = np.repeat(prof[:, None], m, axis=1) if axis == 0 else np.repeat(prof[None, :], m, axis=0)
+    A = R.make_A_hz(omega, dL, eps, [0, 0])
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    field = np.repeat(u[:, None], m, axis=1) if axis == 0 else np.repeat(u[None, :], m, axis=0)
+    got = (A @ field.ravel()).reshape(eps.shape)
+    want = _hz_layered_1d(omega, dL, prof) @ u
+    want = np.repeat(want[:, None], m, axis=1) if axis == 0 else np.repeat(want[None, :], m, axis=0)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_hz_regulariser_closed_form():
+    # eps = 0 everywhere: 1 / (eps + 1e-5) = 1e5 exactly, so a DFT mode of the
+    # periodic grid (no PML) has the symbol
+    #   -(1e5 / eps0) (4 / dL^2) (sin^2(kx dL / 2) + sin^2(ky dL / 2)) + mu0 w^2.
+    omega, dL = 2 * np.pi * R.C_0 / 1.55e-6, 50e-9
+    nx, ny = 8, 6
+    A = R.make_A_hz(omega, dL, np.zeros((nx, ny)), [0, 0])
+    for p, q in ((1, 2), (3, 0), (0, 5)):
+        kx, ky = 2 * np.pi * p / (nx * dL), 2 * np.pi * q / (ny * dL)
+        X, Y = np.meshgrid(np.arange(nx) * dL, np.arange(ny) * dL, indexing='ij')
+        v = np.exp(1j * (kx * X + ky * Y)).ravel()
+        lam = -(1e5 / R.EPSILON_0) * (4 / dL ** 2) * (np.sin(kx * dL / 2) ** 2 + np.sin(ky * dL / 2) ** 2) \
+            + R.MU_0 * omega ** 2
+        assert np.linalg.norm(A @ v - lam * v) <= 1e-10 * abs(lam) * np.linalg.norm(v)
+
+
+def test_hz_field_recovery_is_faraday_law():
+    # Ex, Ey from Hz (fdfd.py:150-162) with A Hz = i w Mz satisfy Faraday's law
+    # Dxb Ey - Dyb Ex = -i w mu0 Hz - Mz, and Ampere's law per component:
+    # i w eps0 (eps_xx + 1e-5) Ex = Dyf Hz, i w eps0 (eps_yy + 1e-5) Ey = -Dxf Hz.
+    omega = 2 * np.pi * R.C_0 / 1.55e-6
+    rng = np.random.default_rng(4)
+    shape = (28, 23)
+    eps = rng.uniform(1, 6, shape)
+    src = np.zeros(shape)
+    src[14, 11] = 1
+    ex, ey, hz = R.solve_hz(omega, 50e-9, eps, [6, 6], src)
+    Dxf, Dxb, Dyf, Dyb = R.derivative_matrices(omega, shape, [6, 6], 50e-9)
+    lhs = Dxb @ ey.ravel() - Dyb @ ex.ravel()
+    rhs = -1j * omega * R.MU_0 * hz.ravel() - src.ravel()
+    assert np.linalg.norm(lhs - rhs) <= 1e-9 * np.linalg.norm(rhs)
+    exx = 0.5 * (eps + np.roll(eps, 1, axis=1))
+    eyy = 0.5 * (eps + np.roll(eps, 1, axis=0))
+    assert np.allclose(1j * omega * R.EPSILON_0 * (exx.ravel() + 1e-5) * ex.ravel(), Dyf @ hz.ravel(),
+                       rtol=1e-12, atol=1e-12 * np.abs(Dyf @ hz.ravel()).max())
+    assert np.allclose(1j * omega * R.EPSILON_0 * (eyy.ravel() + 1e-5) * ey.ravel(), -(Dxf @ hz.ravel()),
+                       rtol=1e-12, atol=1e-12 * np.abs(Dxf @ hz.ravel()).max())
