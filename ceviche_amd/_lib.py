@@ -29,6 +29,8 @@ CV_POL_HZ = 1
 CV_FLAG_X_IDENTITY = 1
 CV_FLAG_Y_IDENTITY = 2
 CV_TILE_CELLS = 256
+CV_NOALLOC_SCRATCH_BYTES = 32 << 20
+CV_NOALLOC_MAX_INST = 64
 CV_FACTOR_BTD = 1
 CV_FACTOR_BCR = 2
 
@@ -123,6 +125,8 @@ _SIGS = {
     'cv_fdfd_coupling_grad': (I, [I, P, P, I, P, P, P, P, D, P]),
     'cv_fdfd_solve_workspace_bytes': (I64, [I, I64, I, I]),
     'cv_fdfd_solve': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
+    'cv_noalloc_init': (I, []),
+    'cv_fdfd_solve_noalloc': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, P]),
     'cv_fdfd_solve_mixed_workspace_bytes': (I64, [I64, I, I]),
     'cv_fdfd_solve_mixed': (I, [P, P, I, P, I, P, P, P, P, P, P, P, P, I, D, D, I, I, I, P, I64, P, P]),
     'cv_fdtd_step': (I, [I, P, P, P, P, P, P, P, D, I, I64, P, I, P, P, I, P, P]),

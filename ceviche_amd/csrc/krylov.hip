@@ -16,6 +16,7 @@
 
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 namespace cv {
 
@@ -447,8 +448,8 @@ static int64_t layout(int64_t cs, int64_t cells, int n_inst, int n_tiles, char* 
 template <typename R, bool TRANS>
 static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst, const cv_tile* tiles,
                       int n_tiles, const void* prof_, const void* m0_, const void* m1_, const void* b_, void* x_,
-                      double tol, int max_iter, int check_every, Work& w, std::vector<KState>& hs,
-                      hipStream_t s) {
+                      double tol, int max_iter, int check_every, Work& w, KState* hs,
+                      hipStream_t s, bool use_graph = true) {
     const cx<R>* prof = (const cx<R>*)prof_;
     const R* m0 = (const R*)m0_;
     const R* m1 = (const R*)m1_;
@@ -467,7 +468,6 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
     CV_LAUNCH_CHECK();
     const int max_checks = 8;
     int checks = 0;
-    hs.resize(n_inst);
     // The check_every iterations between host checks (an even count, so the
     // p / v buffers are back in place at the end) are captured once into a
     // graph and replayed: every scalar lives in device memory, so the launch
@@ -484,11 +484,19 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
             cur = nxt;
         }
     };
+    // Without the graph (use_graph false: the allocation-free entry point,
+    // since instantiating a graph allocates) the same launches are issued
+    // directly on the caller's stream.
     LoopGraph graph;
-    CV_HIP(graph.capture(body));
+    if (use_graph) CV_HIP(graph.capture(body));
     while (true) {
-        CV_HIP(hipGraphLaunch(graph.exec, s));
-        CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
+        if (use_graph) {
+            CV_HIP(hipGraphLaunch(graph.exec, s));
+        } else {
+            body(s);
+            CV_LAUNCH_CHECK();
+        }
+        CV_HIP(hipMemcpyAsync(hs, w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
         CV_HIP(hipStreamSynchronize(s));
         bool all_done = true;
         for (int k = 0; k < n_inst; ++k)
@@ -498,7 +506,7 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
         k_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, r, rh, w.rb,
                                                     w.state, tol, max_iter, 1);
         CV_LAUNCH_CHECK();
-        CV_HIP(hipMemcpyAsync(hs.data(), w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
+        CV_HIP(hipMemcpyAsync(hs, w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
         CV_HIP(hipStreamSynchronize(s));
         bool again = false;
         for (int k = 0; k < n_inst; ++k)
@@ -549,7 +557,7 @@ static int check_batch(const cv_fdfd_desc* dh, int n_inst, int n_tiles, int64_t*
     return CV_OK;
 }
 
-static void fill_info(const cv_fdfd_desc* dh, const std::vector<KState>& hs, cv_solve_info* info, int n_inst,
+static void fill_info(const cv_fdfd_desc* dh, const KState* hs, cv_solve_info* info, int n_inst,
                       int* worst) {
     *worst = CV_OK;
     for (int k = 0; k < n_inst; ++k) {
@@ -572,14 +580,14 @@ extern "C" int64_t cv_fdfd_solve_workspace_bytes(int dtype, int64_t total_cells,
     return layout(cs, total_cells, n_inst, n_tiles, nullptr, nullptr);
 }
 
-extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
-                             const cv_tile* tiles, int n_tiles, const void* prof, const void* m0, const void* m1,
-                             const void* b, void* x, int transpose, double tol, int max_iter, int check_every,
-                             void* workspace, int64_t workspace_bytes, cv_solve_info* info, void* stream) {
-    CV_CHECK(desc && desc_host && tiles && prof && m0 && b && x && workspace && info, "null pointer argument");
-    CV_CHECK(n_inst > 0 && n_tiles > 0, "empty batch");
-    CV_CHECK(dtype == CV_C64 || dtype == CV_C128, "unknown dtype %d", dtype);
-    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+// Shared by cv_fdfd_solve and cv_fdfd_solve_noalloc: checks the arguments,
+// lays the workspace out at `workspace`, runs the batched BiCGSTAB and fills
+// `info` from the host copy `hs` of the n_inst solver states.
+static int solve_entry(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                       const cv_tile* tiles, int n_tiles, const void* prof, const void* m0, const void* m1,
+                       const void* b, void* x, int transpose, double tol, int max_iter, int check_every,
+                       void* workspace, int64_t workspace_bytes, cv_solve_info* info, KState* hs, bool use_graph,
+                       hipStream_t s) {
     int64_t cells = 0;
     {
         const int rc = check_batch(desc_host, n_inst, n_tiles, &cells);
@@ -594,23 +602,101 @@ extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_
              (long long)need);
     if (check_every <= 0) check_every = 16;
     check_every += check_every & 1;
-    hipStream_t s = as_stream(stream);
-    std::vector<KState> hs;
     int rc;
     if (dtype == CV_C128)
         rc = transpose ? solve_impl<double, true>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
-                                                  max_iter, check_every, w, hs, s)
+                                                  max_iter, check_every, w, hs, s, use_graph)
                        : solve_impl<double, false>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x,
-                                                   tol, max_iter, check_every, w, hs, s);
+                                                   tol, max_iter, check_every, w, hs, s, use_graph);
     else
         rc = transpose ? solve_impl<float, true>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
-                                                 max_iter, check_every, w, hs, s)
+                                                 max_iter, check_every, w, hs, s, use_graph)
                        : solve_impl<float, false>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
-                                                  max_iter, check_every, w, hs, s);
+                                                  max_iter, check_every, w, hs, s, use_graph);
     if (rc != CV_OK) return rc;
     int worst;
     fill_info(desc_host, hs, info, n_inst, &worst);
     return worst;
+}
+
+extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                             const cv_tile* tiles, int n_tiles, const void* prof, const void* m0, const void* m1,
+                             const void* b, void* x, int transpose, double tol, int max_iter, int check_every,
+                             void* workspace, int64_t workspace_bytes, cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && tiles && prof && m0 && b && x && workspace && info, "null pointer argument");
+    CV_CHECK(n_inst > 0 && n_tiles > 0, "empty batch");
+    CV_CHECK(dtype == CV_C64 || dtype == CV_C128, "unknown dtype %d", dtype);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    std::vector<KState> hs(n_inst);
+    return solve_entry(dtype, desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, transpose, tol, max_iter,
+                       check_every, workspace, workspace_bytes, info, hs.data(), true, as_stream(stream));
+}
+
+// ------------------------------------------------ allocation-free solve
+// The workspace of cv_fdfd_solve_noalloc lives in this file-scope device
+// array (one copy per GPU, held until the process ends).  Host code never
+// passes `scratch` itself to a kernel or a copy: on the host the name stands
+// for a host-side shadow, not the device copy.  The device address of each
+// GPU's copy is looked up once per device with hipGetSymbolAddress and kept
+// in g_scratch_ptr.
+static __device__ __align__(256) char scratch[CV_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of the current device's scratch array (looked up on the
+// first call on that device); CV_OK or CV_ERR_HIP / CV_ERR_INVALID.
+static int scratch_ptr(void** out) {
+    int dev = -1;
+    CV_HIP(hipGetDevice(&dev));
+    CV_CHECK(dev >= 0 && dev < kMaxDevices, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        CV_CHECK(p != nullptr, "no device address for the scratch array on device %d", dev);
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CV_OK;
+}
+
+extern "C" int cv_noalloc_init(void) {
+    void* p = nullptr;
+    return scratch_ptr(&p);
+}
+
+extern "C" int cv_fdfd_solve_noalloc(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                                     const cv_tile* tiles, int n_tiles, const void* prof, const void* m0,
+                                     const void* m1, const void* b, void* x, int transpose, double tol, int max_iter,
+                                     int check_every, cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && tiles && prof && m0 && b && x && info, "null pointer argument");
+    CV_CHECK(n_inst > 0 && n_tiles > 0, "empty batch");
+    CV_CHECK(n_inst <= CV_NOALLOC_MAX_INST, "%d instances exceed the allocation-free limit of %d", n_inst,
+             CV_NOALLOC_MAX_INST);
+    CV_CHECK(dtype == CV_C64 || dtype == CV_C128, "unknown dtype %d", dtype);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    int64_t cells = 0;
+    {
+        const int rc = check_batch(desc_host, n_inst, n_tiles, &cells);
+        if (rc != CV_OK) return rc;
+    }
+    const int64_t need = layout(dtype == CV_C128 ? 16 : 8, cells, n_inst, n_tiles, nullptr, nullptr);
+    CV_CHECK(need <= (int64_t)CV_NOALLOC_SCRATCH_BYTES,
+             "problem needs %lld bytes of solver scratch, above the allocation-free limit of %lld bytes "
+             "(%lld packed cells, %d tiles); use cv_fdfd_solve with a workspace",
+             (long long)need, (long long)CV_NOALLOC_SCRATCH_BYTES, (long long)cells, n_tiles);
+    void* ws = nullptr;
+    {
+        const int rc = scratch_ptr(&ws);
+        if (rc != CV_OK) return rc;
+    }
+    KState hs[CV_NOALLOC_MAX_INST];
+    return solve_entry(dtype, desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, transpose, tol, max_iter,
+                       check_every, ws, need, info, hs, false, as_stream(stream));
 }
 
 // ------------------------------------------------------------ mixed solve
@@ -654,7 +740,7 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
     KState* outer = (KState*)(base + off);
     hipStream_t s = as_stream(stream);
     k_zero_u32<<<(n_inst + n_tiles + 255) / 256, 256, 0, s>>>(w.rb.counters, n_inst + n_tiles);
-    std::vector<KState> ho(n_inst), hi;
+    std::vector<KState> ho(n_inst), hi(n_inst);
     std::vector<int> iters(n_inst, 0), sweeps(n_inst, 0);
     auto resid = [&](int first) -> int {
         if (transpose)
@@ -686,10 +772,10 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
         // (b32 left from their last sweep is irrelevant: their x64 is not updated).
         if (transpose)
             rc = solve_impl<float, true>(desc, desc_host, n_inst, tiles, n_tiles, prof32, m0_32, m1_32, b32, x32,
-                                         inner_tol, max_iter, check_every, w, hi, s);
+                                         inner_tol, max_iter, check_every, w, hi.data(), s);
         else
             rc = solve_impl<float, false>(desc, desc_host, n_inst, tiles, n_tiles, prof32, m0_32, m1_32, b32, x32,
-                                          inner_tol, max_iter, check_every, w, hi, s);
+                                          inner_tol, max_iter, check_every, w, hi.data(), s);
         if (rc != CV_OK) return rc;
         for (int k = 0; k < n_inst; ++k) {
             const int sys = k - desc_host[k].blk;

@@ -3,9 +3,13 @@
  *
  * Every entry point takes caller-owned DEVICE pointers (plain void* + sizes),
  * an explicit hipStream_t (passed as void*), and returns an int status
- * (CV_OK on success).  The library never allocates device memory: all
+ * (CV_OK on success).  Data buffers are never allocated by the library: all
  * workspace is passed in by the caller, sized by the *_workspace_bytes
- * helpers.  No torch types cross this boundary.
+ * helpers.  The iterative solvers do ask the HIP runtime for resources of
+ * their own on each call: cv_fdfd_solve captures its loop body on a private
+ * stream and instantiates a HIP graph (both released before it returns), and
+ * the FGMRES entry points keep host-side vectors.  cv_fdfd_solve_noalloc is
+ * the variant that allocates nothing.  No torch types cross this boundary.
  *
  * Layouts
  *   FDFD fields: complex, C order over (nx, ny) (y contiguous), packed over a
@@ -201,6 +205,32 @@ int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_
                   double tol, int max_iter, int check_every,
                   void* workspace, int64_t workspace_bytes,
                   cv_solve_info* info, void* stream);
+
+/* Allocation-free variant of cv_fdfd_solve.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  It calls no hipMalloc /
+ * hipFree, creates no stream and instantiates no graph: the BiCGSTAB vectors,
+ * solver states and reduction partials live in one file-scope device array of
+ * CV_NOALLOC_SCRATCH_BYTES (one copy per GPU, held until the process ends),
+ * and the loop kernels are launched directly on `stream`.  A batch whose
+ * solver scratch (cv_fdfd_solve_workspace_bytes) exceeds that size, or with
+ * more than CV_NOALLOC_MAX_INST instances, is rejected with CV_ERR_INVALID
+ * (complex128: up to about 290 000 packed cells, e.g. one 512 x 512 grid).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (two streams, two host threads) race and give wrong results; serialise
+ * them.  The counters the kernels need at zero are cleared at the start of
+ * every call; nothing else carries over between calls.
+ * cv_noalloc_init looks the array's device address up on the current device
+ * (done by the first call on each device otherwise). */
+#define CV_NOALLOC_SCRATCH_BYTES (32ll << 20)
+#define CV_NOALLOC_MAX_INST 64
+int cv_noalloc_init(void);
+int cv_fdfd_solve_noalloc(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                          const cv_tile* tiles, int n_tiles,
+                          const void* prof, const void* m0, const void* m1,
+                          const void* b, void* x, int transpose,
+                          double tol, int max_iter, int check_every,
+                          cv_solve_info* info, void* stream);
 
 /* Mixed precision: complex128 x/b/operator (prof64, m0_64, m1_64) for the
  * residual, complex64 copies (prof32, m0_32, m1_32) for the inner BiCGSTAB. */
