@@ -12,12 +12,13 @@ the cotangent of a complex z is dL/dRe z + i dL/dIm z, so reverse mode uses
 A^H = conj(A)^T where the reference (autograd convention) uses A^T.
 """
 import ctypes
+import warnings
 
 import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib, ptr, stream_ptr
+from ._lib import CevicheError, check, lib, ptr, stream_ptr
 from .constants import EPSILON_0, MU_0
 from .derivatives import axis_profiles
 
@@ -31,14 +32,18 @@ class SolverOptions:
     method: 'auto' (the block-tridiagonal factorisation + FGMRES for a single
     instance with ny <= 128, or nx <= 128 on the transposed grid, BiCGSTAB
     otherwise), 'direct' (require the former) or 'bicgstab'.  restart: FGMRES
-    cycle length of the direct path."""
+    cycle length of the direct path; direct_max_iter: its FGMRES step limit
+    (max_iter when None).  Under 'auto' a direct solve that fails (singular
+    factor, FGMRES not converged, out of memory) falls back to BiCGSTAB with a
+    warning; under 'direct' it raises."""
 
     def __init__(self, tol=None, max_iter=200000, check_every=32, inner_tol=1e-4, max_sweeps=12, method='auto',
-                 restart=30):
+                 restart=30, direct_max_iter=None):
         if method not in ('auto', 'direct', 'bicgstab'):
             raise ValueError("method must be 'auto', 'direct' or 'bicgstab'")
         self.tol = tol
         self.max_iter = max_iter
+        self.direct_max_iter = direct_max_iter
         self.check_every = check_every
         self.inner_tol = inner_tol
         self.max_sweeps = max_sweeps
@@ -65,9 +70,12 @@ class FdfdOperator:
     grid's cells, in units of omega^2 * (system scale), multiplied by
     w2[blk] = (W_blk / omega)^2."""
 
-    def __init__(self, pol, specs, dtype=torch.complex128, device=None):
+    def __init__(self, pol, specs, dtype=torch.complex128, device=None, complex_materials=False):
         if pol not in ('ez', 'hz'):
             raise ValueError("pol must be 'ez' or 'hz'")
+        # Hz with complex eps: m0 = 1/(eps_xx + 1e-5) and m1 = 1/(eps_yy + 1e-5)
+        # are complex arrays (the instances carry CV_FLAG_CPLX_MAT)
+        self.cmat = bool(complex_materials) and pol == 'hz'
         if dtype not in _CODE_OF:
             raise ValueError('dtype must be torch.complex128 or torch.complex64')
         self.pol = pol
@@ -126,6 +134,8 @@ class FdfdOperator:
             self.sys_scale[k] = c
             phx, phy = np.exp(1j * bx), np.exp(1j * by)
             flags = (_lib.CV_FLAG_X_IDENTITY if nx == 1 else 0) | (_lib.CV_FLAG_Y_IDENTITY if ny == 1 else 0)
+            if self.cmat:
+                flags |= _lib.CV_FLAG_CPLX_MAT
             desc[k] = (nx, ny, _lib.CV_POL_EZ if pol == 'ez' else _lib.CV_POL_HZ, flags, cell_off, prof_off,
                        first_tile, ntile, scale, dshift, phx.real, phx.imag, phy.real, phy.imag, omega,
                        n_cpl, blk, nblk, 0, 0, 0)
@@ -222,9 +232,10 @@ class FdfdOperator:
             if torch.is_complex(eps):
                 return eps.real.to(self.rdtype).contiguous(), eps.imag.to(self.rdtype).contiguous()
             return eps.to(self.rdtype).contiguous(), None
-        if torch.is_complex(eps):
-            raise NotImplementedError('fdfd_hz with complex eps_r is not supported')
-        e = eps.to(self.rdtype).contiguous()
+        if torch.is_complex(eps) != self.cmat:
+            raise ValueError('this Hz operator was built for %s eps; use complex_materials=%s'
+                             % ('complex' if self.cmat else 'real', torch.is_complex(eps)))
+        e = eps.to(self.dtype if self.cmat else self.rdtype).contiguous()
         m0 = torch.empty_like(e)
         m1 = torch.empty_like(e)
         check(lib().cv_fdfd_hz_materials(self.code, *self._args(), ptr(e), ptr(m0), ptr(m1), stream_ptr()),
@@ -278,7 +289,7 @@ class FdfdOperator:
     def materials_jvp(self, m0, m1, d_eps):
         dm0, dm1 = torch.empty_like(m0), torch.empty_like(m1)
         check(lib().cv_fdfd_hz_materials_jvp(self.code, *self._args(), ptr(m0), ptr(m1),
-                                             ptr(d_eps.to(self.rdtype).contiguous()), ptr(dm0), ptr(dm1),
+                                             ptr(d_eps.to(m0.dtype).contiguous()), ptr(dm0), ptr(dm1),
                                              stream_ptr()), 'hz_materials_jvp')
         return dm0, dm1
 
@@ -345,10 +356,22 @@ def direct_applies(op):
     """True when the block-tridiagonal factorisation handles ``op``: one
     uncoupled instance whose y extent fits a workgroup, or whose x extent
     does (factorised on the transposed grid)."""
-    if op.n_inst != 1 or op.cpl_len != 0:
+    if op.n_inst != 1 or op.cpl_len != 0 or op.cmat:
         return False
     nx, ny = op.shapes[0]
     return ny <= BTD_MAX_NY or nx <= BTD_MAX_NY
+
+
+def direct_factor_bytes(op):
+    """Device bytes of the block-cyclic-reduction factor of ``op`` (in the
+    orientation the direct path factorises)."""
+    nx, ny = op.shapes[0]
+    if ny > BTD_MAX_NY:
+        nx, ny = ny, nx
+    return int(lib().cv_fdfd_bcr_factor_bytes(nx, ny))
+
+
+DIRECT_MEMORY_FRACTION = 0.5
 
 
 def transposed_operator(op):
@@ -475,10 +498,11 @@ class Direct:
         b = bs.to(torch.complex128).contiguous()
         x = torch.zeros_like(b)
         info = (_lib.SolveInfo * 1)()
+        max_iter = opts.max_iter if opts.direct_max_iter is None else opts.direct_max_iter
         rc = L.cv_fdfd_fgmres_direct(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(op.prof64),
                                      ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1),
                                      self.code, ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)), float(tol),
-                                     int(opts.restart), int(opts.max_iter), ptr(ws), need, info, stream_ptr())
+                                     int(opts.restart), int(max_iter), ptr(ws), need, info, stream_ptr())
         self.user_op.last_info = [dict(iterations=info[0].iterations, status=info[0].status,
                                        restarts=info[0].restarts, rel_residual=info[0].rel_residual,
                                        method='direct', factor=self.kind + ('-transposed' if self.tr else ''))]
@@ -560,12 +584,38 @@ def _use_blockdiag(op, opts):
 
 
 def _use_direct(op, opts):
+    """method 'direct' requires the direct path; 'auto' takes it when it
+    applies and its factor fits in DIRECT_MEMORY_FRACTION of the free device
+    memory."""
     method = (opts or SolverOptions()).method
     if method == 'bicgstab':
         return False
-    if method == 'direct' and not direct_applies(op):
-        raise ValueError("method='direct' needs one uncoupled instance with nx or ny <= %d" % BTD_MAX_NY)
-    return direct_applies(op)
+    if method == 'direct':
+        if not direct_applies(op):
+            raise ValueError("method='direct' needs one uncoupled instance with nx or ny <= %d and real Hz "
+                             "materials" % BTD_MAX_NY)
+        return True
+    if not direct_applies(op):
+        return False
+    free, _ = torch.cuda.mem_get_info(op.device)
+    return direct_factor_bytes(op) <= DIRECT_MEMORY_FRACTION * free
+
+
+# status codes after which the direct path gives way to BiCGSTAB under 'auto'
+_DIRECT_FALLBACK = (_lib.CV_ERR_SINGULAR, _lib.CV_ERR_NOT_CONVERGED, _lib.CV_ERR_BREAKDOWN, _lib.CV_ERR_NONFINITE)
+
+
+def _direct_failed(err, opts):
+    """True when a direct-path error should fall back to BiCGSTAB (method
+    'auto'); warns.  Other errors, and any error under 'direct', propagate."""
+    if (opts or SolverOptions()).method == 'direct':
+        return False
+    if isinstance(err, torch.cuda.OutOfMemoryError) or (isinstance(err, CevicheError)
+                                                         and err.status in _DIRECT_FALLBACK):
+        warnings.warn('direct FDFD solve failed (%s); falling back to Jacobi-preconditioned BiCGSTAB' % err,
+                      RuntimeWarning)
+        return True
+    return False
 
 
 # ---------------------------------------------------------------- autograd
@@ -579,24 +629,41 @@ class FdfdSolve(torch.autograd.Function):
     of primitives.py:121-135), grad_b = lam, grad_m = -Re(mu^T dA/dm x)."""
 
     @staticmethod
-    def _solve(op, op32, opts, m0, m1, bs, transpose, direct=None):
-        if direct is not None:
-            return direct.solve(bs, m0, m1, transpose=transpose, opts=opts)
+    def _krylov(op, op32, opts, m0, m1, bs, transpose):
         if op32 is None:
             return op.solve_scaled(bs, m0, m1, transpose=transpose, opts=opts)
-        m0s = m0.to(torch.float32).contiguous()
-        m1s = None if m1 is None else m1.to(torch.float32).contiguous()
-        return op32.solve_scaled(bs, m0s, m1s, transpose=transpose, opts=opts, mixed_ops=(op, m0, m1))
+        lo = lambda m: m.to(torch.complex64 if m.is_complex() else torch.float32).contiguous()
+        m1s = None if m1 is None else lo(m1)
+        return op32.solve_scaled(bs, lo(m0), m1s, transpose=transpose, opts=opts, mixed_ops=(op, m0, m1))
+
+    @staticmethod
+    def _solve(op, op32, opts, m0, m1, bs, transpose, direct=None, ctx=None):
+        if direct is not None:
+            try:
+                return direct.solve(bs, m0, m1, transpose=transpose, opts=opts)
+            except (CevicheError, torch.cuda.OutOfMemoryError) as err:
+                if not _direct_failed(err, opts):
+                    raise
+                if ctx is not None:
+                    ctx.direct = None
+        return FdfdSolve._krylov(op, op32, opts, m0, m1, bs, transpose)
 
     @staticmethod
     def forward(ctx, m0, m1, b, op, op32, opts):
         c = op.scale_vector(torch.float64)
         # one factorisation serves the forward, adjoint and tangent solves
-        direct = Direct(op, m0, m1) if _use_direct(op, opts) else None
-        x = FdfdSolve._solve(op, op32, opts, m0, m1, b * c, False, direct)
+        direct = None
+        if _use_direct(op, opts):
+            try:
+                direct = Direct(op, m0, m1)
+            except (CevicheError, torch.cuda.OutOfMemoryError) as err:
+                if not _direct_failed(err, opts):
+                    raise
+        ctx.direct = direct
+        x = FdfdSolve._solve(op, op32, opts, m0, m1, b * c, False, direct, ctx)
+        direct = ctx.direct
         if direct is not None and op32 is not None:
             op32.last_info = op.last_info
-        ctx.direct = direct
         ctx.op, ctx.op32, ctx.opts = op, op32, opts
         ctx.save_for_backward(m0, m1, x)
         ctx.save_for_forward(m0, m1, x)
@@ -616,14 +683,14 @@ class FdfdSolve(torch.autograd.Function):
             if ctx.has_m1:
                 d1 = torch.zeros_like(m0) if dm1 is None else dm1.to(m0.dtype).contiguous()
             rhs = rhs - op.material_apply(d0, d1, x)
-        return FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False, ctx.direct)
+        return FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, rhs.contiguous(), False, ctx.direct, ctx)
 
     @staticmethod
     def backward(ctx, gx):
         m0, m1, x = ctx.saved_tensors
         op = ctx.op
         mu = FdfdSolve._solve(op, ctx.op32, ctx.opts, m0, m1, torch.conj(gx).resolve_conj().contiguous(), True,
-                              ctx.direct)
+                              ctx.direct, ctx)
         c = op.scale_vector(torch.float64)
         gb = torch.conj(mu).resolve_conj() * c if ctx.needs_input_grad[2] else None
         g0 = g1 = None

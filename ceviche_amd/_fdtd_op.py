@@ -163,13 +163,17 @@ class FdtdGrid:
             self._tmp = torch.empty((3,) + self.shape, dtype=self.dtype, device=self.device)
         return self._tmp
 
-    def adjoint(self, lam, eps, D_prev, gmE, probes=None, g_probe=None, D_t=None):
-        """Reverse one step in place (lam: after step t -> before step t)."""
+    def adjoint(self, lam, eps, D_prev, gmE, probes=None, g_probe=None, D_t=None, src=None, g_src=None):
+        """Reverse one step in place (lam: after step t -> before step t).
+        With src = (comp, cell) and g_src (a one-element device tensor of the
+        grid dtype), g_src receives the cotangent of the point-source
+        amplitude added at this step."""
         pc, pl = (None, None) if probes is None else probes
         npb = 0 if probes is None else int(pc.numel())
+        comp, cell = (-1, 0) if src is None else (int(src[0]), int(src[1]))
         check(lib().cv_fdtd_adjoint_step(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(lam), ptr(self.tmp()),
                                          ptr(D_prev), ptr(gmE), ptr(pc), ptr(pl), npb, ptr(g_probe), ptr(D_t),
-                                         stream_ptr()), 'fdtd adjoint')
+                                         comp, cell, ptr(g_src), stream_ptr()), 'fdtd adjoint')
 
     def eps_grad(self, eps, gmE, geps):
         check(lib().cv_fdtd_eps_grad(self.code, self._d(), ptr(self.sig), ptr(eps), ptr(gmE), ptr(geps),
@@ -301,8 +305,16 @@ class FdtdRun(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, eps, amp, grid, state0, src, probes, n_steps, checkpoint_every, tm_config=None):
-        amp_h = amp.detach().to('cpu', torch.float64).numpy()
-        amp_dev = torch.from_numpy(amp_h).to(grid.device) if tm_config != 'generic' else None
+        # The amplitude series is read by the temporally blocked and fused
+        # kernels from device memory (a float64 device tensor is used as it
+        # is, anything else is uploaded once) and by the per-step kernels from
+        # host memory (copied once, only for those).
+        a = amp.detach()
+        amp_dev = amp_h = None
+        if tm_config != 'generic':
+            amp_dev = a.to(grid.device, torch.float64).contiguous()
+        if tm_config in ('generic', 'fused'):
+            amp_h = a.to('cpu', torch.float64).numpy()
         state = state0.detach().clone()
         spare = torch.empty_like(state) if tm_config == 'fused' else None
         pc, pl = probes
@@ -324,6 +336,7 @@ class FdtdRun(torch.autograd.Function):
             t += m
         ctx.grid, ctx.src, ctx.probes, ctx.n_steps, ctx.ck = grid, src, probes, n_steps, ck
         ctx.amp_h, ctx.amp_dev, ctx.tm_config = amp_h, amp_dev, tm_config
+        ctx.amp_meta = (amp.shape, amp.dtype, amp.device)
         ctx.checkpoints = checkpoints
         ctx.state0 = state0.detach().clone()
         ctx.save_for_backward(eps)
@@ -342,8 +355,9 @@ class FdtdRun(torch.autograd.Function):
         tstate = grid.new_state() if d_state0 is None else d_state0.to(grid.dtype).contiguous().clone()
         de = torch.zeros_like(eps) if d_eps is None else d_eps.to(eps.dtype).contiguous()
         da = None if d_amp is None else d_amp.detach().to('cpu', torch.float64).numpy()
+        amp_h = ctx.amp_h if ctx.amp_h is not None else ctx.amp_dev.cpu().numpy()
         trec = torch.empty((n_steps, pc.numel()), dtype=grid.dtype, device=grid.device)
-        grid.step_tangent(state, tstate, eps.detach(), de, n_steps, src=src, amp=ctx.amp_h, d_amp=da, probes=probes,
+        grid.step_tangent(state, tstate, eps.detach(), de, n_steps, src=src, amp=amp_h, d_amp=da, probes=probes,
                           trec=trec)
         return trec, tstate
 
@@ -357,6 +371,8 @@ class FdtdRun(torch.autograd.Function):
         gmE = torch.zeros((3,) + grid.shape, dtype=grid.dtype, device=grid.device)
         grec = grec.to(grid.dtype).contiguous() if grec is not None else None
         nseg = len(ctx.checkpoints)
+        # cotangent of amp[t]: lam(D_src) after step t (source added to D, fdtd.py:122-124)
+        gamp = torch.zeros(n_steps, dtype=grid.dtype, device=grid.device) if ctx.needs_input_grad[1] else None
         # D history of one segment: Dhist[s] = D after step (seg_start + s)
         Dhist = torch.empty((ck + 1, 3) + grid.shape, dtype=grid.dtype, device=grid.device)
         for sgi in range(nseg - 1, -1, -1):
@@ -372,13 +388,22 @@ class FdtdRun(torch.autograd.Function):
                 Dhist[s + 1].copy_(st[3:6])
             for s in range(m - 1, -1, -1):
                 t = t0 + s
+                ga = None if gamp is None else gamp[t:t + 1]
                 if grec is not None:
                     gp = grec[t]
-                    grid.adjoint(lam, eps, Dhist[s], gmE, probes=probes, g_probe=gp, D_t=Dhist[s + 1])
+                    grid.adjoint(lam, eps, Dhist[s], gmE, probes=probes, g_probe=gp, D_t=Dhist[s + 1],
+                                 src=src, g_src=ga)
                 else:
-                    grid.adjoint(lam, eps, Dhist[s], gmE)
+                    grid.adjoint(lam, eps, Dhist[s], gmE, src=src, g_src=ga)
             del st
         geps = grid.eps_grad(eps, gmE, torch.zeros_like(eps)) if ctx.needs_input_grad[0] else None
+        g_amp = None
+        if gamp is not None:
+            shape, dtype, dev = ctx.amp_meta
+            g_amp = torch.zeros(shape, dtype=dtype, device=dev)
+            g_amp.view(-1)[:n_steps] = gamp.to(dev, dtype)
+        # lam now holds the cotangent of the state the run started from
+        gstate0 = lam if ctx.needs_input_grad[3] else None
         # the checkpoints stay with the graph: a retained graph (one backward
         # per output row of a Jacobian) runs this more than once
-        return geps, None, None, None, None, None, None, None, None
+        return geps, g_amp, None, gstate0, None, None, None, None, None

@@ -28,6 +28,7 @@ CV_POL_EZ = 0
 CV_POL_HZ = 1
 CV_FLAG_X_IDENTITY = 1
 CV_FLAG_Y_IDENTITY = 2
+CV_FLAG_CPLX_MAT = 4
 CV_TILE_CELLS = 256
 CV_NOALLOC_SCRATCH_BYTES = 32 << 20
 CV_NOALLOC_MAX_INST = 64
@@ -134,12 +135,14 @@ _SIGS = {
     'cv_fdtd_step_tangent': (I, [I, P, P, P, P, P, P, P, P, P, P, P, P, D, I, I64, P, P, I, P, P, I, P, P, P]),
     'cv_fdtd_efields_tangent': (I, [I, P, P, P, P, P, P, P, P]),
     'cv_fdtd_efields_grad': (I, [I, P, P, P, P, P, P, P, P]),
-    'cv_fdtd_adjoint_step': (I, [I, P, P, P, P, P, P, P, P, P, I, P, P, P]),
+    'cv_fdtd_adjoint_step': (I, [I, P, P, P, P, P, P, P, P, P, I, P, P, I, I64, P, P]),
     'cv_fdtd_eps_grad': (I, [I, P, P, P, P, P, P]),
     'cv_fdtd_run_tm': (I, [I, P, P, P, P, P, I64, P, I, I, P, P, I, P, I, I, P]),
     'cv_fdtd_tm_configs': (I, [P, P, P, I]),
     'cv_fdtd_run_fused': (I, [I, P, P, P, P, P, I, I64, P, I, I, P, P, I, P, I, P]),
     'cv_segsum': (I, [I64, P, P, P, P, P, I, I, D, D, P, I, P, P]),
+    'cv_csr_solve_workspace_bytes': (I64, [I64]),
+    'cv_csr_solve': (I, [I64, I, P, P, P, P, P, D, I, I, P, I64, P, P]),
     'cv_dense_lu': (I, [I, P, P, P, P]),
     'cv_dense_lu_solve': (I, [I, P, P, P, I, I, P]),
     'cv_cnn_pack_weights': (I, [P, P, I, I, P]),

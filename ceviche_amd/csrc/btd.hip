@@ -949,6 +949,7 @@ extern "C" int cv_fdfd_btd_factor(const cv_fdfd_desc* desc, const cv_fdfd_desc* 
     CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
     CV_CHECK(dh.ny <= kBtdMax, "the block-tridiagonal factorisation needs ny <= %d (ny = %d)", kBtdMax, dh.ny);
     CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported");
+    CV_CHECK(!(dh.flags & CV_FLAG_CPLX_MAT), "complex Hz materials are not supported by the direct factorisation");
     CV_CHECK(dh.pol == CV_POL_EZ || m1_32, "Hz operator needs m1");
     const int64_t need = cv_fdfd_btd_factor_bytes(dh.nx, dh.ny);
     CV_CHECK(factor_bytes >= need, "factor buffer too small: %lld < %lld", (long long)factor_bytes, (long long)need);
@@ -1209,6 +1210,7 @@ static int bcr_factor_batch(int nb, const cv_fdfd_desc* const* desc, const cv_fd
         const cv_fdfd_desc& d = desc_host[q];
         CV_CHECK(desc[q] && prof32[q] && m0_32[q] && factor[q], "null pointer argument (instance %d)", q);
         CV_CHECK(d.nx > 0 && d.ny > 0 && d.cell_off == 0, "one instance starting at cell 0 is required");
+        CV_CHECK(!(d.flags & CV_FLAG_CPLX_MAT), "complex Hz materials are not supported by the direct factorisation");
         CV_CHECK(d.nx == dh.nx && d.ny == dh.ny && d.pol == dh.pol, "instance %d differs in shape from instance 0", q);
         CV_CHECK(d.ny <= kBtdMax, "block cyclic reduction needs ny <= %d (ny = %d)", kBtdMax, d.ny);
         CV_CHECK(d.nblk == 1 && d.n_cpl == 0, "frequency-coupled systems are not supported");

@@ -86,6 +86,13 @@ __host__ __device__ __forceinline__ cx<double> cdiv(cx<double> a, cx<double> b) 
 
 __host__ __device__ __forceinline__ bool cfinite(cx<double> a) { return isfinite(a.re) && isfinite(a.im); }
 
+// 1 / z in the operand's precision
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> crecip(cx<R> z) {
+    const R d = z.re * z.re + z.im * z.im;
+    return mk<R>(z.re / d, -z.im / d);
+}
+
 // ------------------------------------------------------------ reductions
 constexpr int kBlock = 256;           // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;

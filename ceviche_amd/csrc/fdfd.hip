@@ -43,10 +43,19 @@ k_hz_materials(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict_
         const int im = (i == 0) ? d.nx - 1 : i - 1;
         const int jm = (j == 0) ? d.ny - 1 : j - 1;
         const int64_t k = d.cell_off + loc;
+        const int64_t kjm = d.cell_off + (int64_t)i * d.ny + jm, kim = d.cell_off + (int64_t)im * d.ny + j;
+        if (d.flags & CV_FLAG_CPLX_MAT) {   // complex eps (fdfd.py:200-215 on complex arrays)
+            const cx<R>* ce = reinterpret_cast<const cx<R>*>(eps);
+            const cx<R> exx = R(0.5) * (ce[k] + ce[kjm]) + mk<R>(R(1e-5), R(0));
+            const cx<R> eyy = R(0.5) * (ce[k] + ce[kim]) + mk<R>(R(1e-5), R(0));
+            reinterpret_cast<cx<R>*>(m0)[k] = crecip(exx);
+            reinterpret_cast<cx<R>*>(m1)[k] = crecip(eyy);
+            continue;
+        }
         const R e = eps[k];
         // fdfd.py:203-204: eps_xx rolls along axis 1 (y), eps_yy along axis 0 (x)
-        const R exx = R(0.5) * (e + eps[d.cell_off + (int64_t)i * d.ny + jm]);
-        const R eyy = R(0.5) * (e + eps[d.cell_off + (int64_t)im * d.ny + j]);
+        const R exx = R(0.5) * (e + eps[kjm]);
+        const R eyy = R(0.5) * (e + eps[kim]);
         m0[k] = R(1) / (exx + R(1e-5));
         m1[k] = R(1) / (eyy + R(1e-5));
     }
@@ -104,8 +113,17 @@ k_material_grad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict
             const cx<R> ly = deriv_at<R>(d, prof, lam, (const R*)nullptr, 3, 1, i, j);  // Dyb^T lam
             const cx<R> dy = deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, j);    // Dyf x
             const R s = R(scale * d.scale);
-            g1[k] += s * (lx * dx).re;
-            g0[k] += s * (ly * dy).re;
+            if (d.flags & CV_FLAG_CPLX_MAT) {
+                // complex materials (holomorphic in A): the torch-convention
+                // cotangent is conj(s lam^T dA/dm x)
+                cx<R>* c0 = reinterpret_cast<cx<R>*>(g0);
+                cx<R>* c1 = reinterpret_cast<cx<R>*>(g1);
+                c1[k] = c1[k] + conj(s * (lx * dx));
+                c0[k] = c0[k] + conj(s * (ly * dy));
+            } else {
+                g1[k] += s * (lx * dx).re;
+                g0[k] += s * (ly * dy).re;
+            }
         }
     }
 }
@@ -134,6 +152,18 @@ k_hz_materials_grad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __rest
         const int64_t k = d.cell_off + loc;
         const int64_t kj = d.cell_off + (int64_t)i * d.ny + jp;
         const int64_t ki = d.cell_off + (int64_t)ip * d.ny + j;
+        if (d.flags & CV_FLAG_CPLX_MAT) {
+            // m = 1/(e + 1e-5) is holomorphic: cotangent conj(dm/de) g = -conj(m)^2 g
+            const cx<R>* c0 = reinterpret_cast<const cx<R>*>(m0);
+            const cx<R>* c1 = reinterpret_cast<const cx<R>*>(m1);
+            const cx<R>* h0 = reinterpret_cast<const cx<R>*>(g0);
+            const cx<R>* h1 = reinterpret_cast<const cx<R>*>(g1);
+            auto cxx = [&](int64_t c) { return h0 ? -(conj(c0[c]) * conj(c0[c])) * h0[c] : czero<R>(); };
+            auto cyy = [&](int64_t c) { return h1 ? -(conj(c1[c]) * conj(c1[c])) * h1[c] : czero<R>(); };
+            cx<R>* ge = reinterpret_cast<cx<R>*>(geps);
+            ge[k] = ge[k] + R(0.5) * (cxx(k) + cxx(kj)) + R(0.5) * (cyy(k) + cyy(ki));
+            continue;
+        }
         auto gxx = [&](int64_t c) { return g0 ? -m0[c] * m0[c] * g0[c] : R(0); };
         auto gyy = [&](int64_t c) { return g1 ? -m1[c] * m1[c] * g1[c] : R(0); };
         geps[k] += R(0.5) * (gxx(k) + gxx(kj)) + R(0.5) * (gyy(k) + gyy(ki));
@@ -159,7 +189,12 @@ k_deriv_matgrad(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict
         const int64_t k = d.cell_off + loc;
         const cx<R> w = coef * deriv_at<R>(d, prof, u, (const R*)nullptr, which, 0, i, j);
         const cx<R> gk = g[k];
-        gmat[k] += w.re * gk.re + w.im * gk.im;
+        if (d.flags & CV_FLAG_CPLX_MAT) {   // complex material: conj(w) g
+            cx<R>* gm = reinterpret_cast<cx<R>*>(gmat);
+            gm[k] = gm[k] + conj(w) * gk;
+        } else {
+            gmat[k] += w.re * gk.re + w.im * gk.im;
+        }
     }
 }
 
@@ -193,21 +228,21 @@ k_material_apply(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restric
             const int64_t kim = d.cell_off + (int64_t)im * d.ny + j, kjm = d.cell_off + (int64_t)i * d.ny + jm;
             const cx<R>* kxb = prof + d.prof_off + d.nx;
             const cx<R>* kyb = prof + d.prof_off + 2 * d.nx + d.ny;
-            const cx<R> wx = dm1[k] * deriv_at<R>(d, prof, x, (const R*)nullptr, 0, 0, i, j);
+            const cx<R> wx = mat_times<R>(d, dm1, k, deriv_at<R>(d, prof, x, (const R*)nullptr, 0, 0, i, j), false);
             cx<R> vx;
             if (d.flags & CV_FLAG_X_IDENTITY) {
                 vx = kxb[i] * wx;
             } else {
-                cx<R> wxm = dm1[kim] * deriv_at<R>(d, prof, x, (const R*)nullptr, 0, 0, im, j);
+                cx<R> wxm = mat_times<R>(d, dm1, kim, deriv_at<R>(d, prof, x, (const R*)nullptr, 0, 0, im, j), false);
                 if (i == 0) wxm = wxm * conj(mk<R>(R(d.phx_re), R(d.phx_im)));
                 vx = kxb[i] * (wx - wxm);
             }
-            const cx<R> wy = dm0[k] * deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, j);
+            const cx<R> wy = mat_times<R>(d, dm0, k, deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, j), false);
             cx<R> vy;
             if (d.flags & CV_FLAG_Y_IDENTITY) {
                 vy = kyb[j] * wy;
             } else {
-                cx<R> wym = dm0[kjm] * deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, jm);
+                cx<R> wym = mat_times<R>(d, dm0, kjm, deriv_at<R>(d, prof, x, (const R*)nullptr, 2, 0, i, jm), false);
                 if (j == 0) wym = wym * conj(mk<R>(R(d.phy_re), R(d.phy_im)));
                 vy = kyb[j] * (wy - wym);
             }
@@ -234,8 +269,19 @@ k_hz_materials_jvp(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restr
         const int im = (i == 0) ? d.nx - 1 : i - 1;
         const int jm = (j == 0) ? d.ny - 1 : j - 1;
         const int64_t k = d.cell_off + loc;
-        const R dxx = R(0.5) * (de[k] + de[d.cell_off + (int64_t)i * d.ny + jm]);
-        const R dyy = R(0.5) * (de[k] + de[d.cell_off + (int64_t)im * d.ny + j]);
+        const int64_t kjm = d.cell_off + (int64_t)i * d.ny + jm, kim = d.cell_off + (int64_t)im * d.ny + j;
+        if (d.flags & CV_FLAG_CPLX_MAT) {   // holomorphic tangent dm = -m^2 de_avg
+            const cx<R>* c0 = reinterpret_cast<const cx<R>*>(m0);
+            const cx<R>* c1 = reinterpret_cast<const cx<R>*>(m1);
+            const cx<R>* ce = reinterpret_cast<const cx<R>*>(de);
+            const cx<R> dxx = R(0.5) * (ce[k] + ce[kjm]);
+            const cx<R> dyy = R(0.5) * (ce[k] + ce[kim]);
+            reinterpret_cast<cx<R>*>(dm0)[k] = -(c0[k] * c0[k]) * dxx;
+            reinterpret_cast<cx<R>*>(dm1)[k] = -(c1[k] * c1[k]) * dyy;
+            continue;
+        }
+        const R dxx = R(0.5) * (de[k] + de[kjm]);
+        const R dyy = R(0.5) * (de[k] + de[kim]);
         dm0[k] = -m0[k] * m0[k] * dxx;
         dm1[k] = -m1[k] * m1[k] * dyy;
     }

@@ -54,10 +54,11 @@ __host__ __device__ __forceinline__ Tri<R> axis_ez(const cx<R>* __restrict__ F, 
 }
 
 // Hz x-part (Dxb diag(u) Dxf) at row i; u_i = u[i], u_m = u[im] (material on the
-// axis' own rows).  F = kxf, B = kxb.
-template <typename R>
+// axis' own rows; U = R for real materials, cx<R> for complex ones).
+// F = kxf, B = kxb.
+template <typename R, typename U>
 __host__ __device__ __forceinline__ Tri<R> axis_hz(const cx<R>* __restrict__ F, const cx<R>* __restrict__ B,
-                                          int n, int i, cx<R> ph, R u_i, R u_m, bool identity) {
+                                          int n, int i, cx<R> ph, U u_i, U u_m, bool identity) {
     Tri<R> t;
     if (identity) {
         t.m = czero<R>();
@@ -113,10 +114,16 @@ __host__ __device__ __forceinline__ Row5<R> row_coefs(const cv_fdfd_desc& d, con
     } else {
         const int im = (i == 0) ? nx - 1 : i - 1;
         const int jm = (j == 0) ? ny - 1 : j - 1;
-        const R ux = m1[k], uxm = m1[d.cell_off + (int64_t)im * ny + j];
-        const R uy = m0[k], uym = m0[d.cell_off + (int64_t)i * ny + jm];
-        tx = axis_hz<R>(kxf, kxb, nx, i, phx, ux, uxm, idx);
-        ty = axis_hz<R>(kyf, kyb, ny, j, phy, uy, uym, idy);
+        const int64_t kxm = d.cell_off + (int64_t)im * ny + j, kym = d.cell_off + (int64_t)i * ny + jm;
+        if (d.flags & CV_FLAG_CPLX_MAT) {   // complex eps: m0, m1 hold interleaved complex values
+            const cx<R>* c0 = reinterpret_cast<const cx<R>*>(m0);
+            const cx<R>* c1 = reinterpret_cast<const cx<R>*>(m1);
+            tx = axis_hz<R, cx<R>>(kxf, kxb, nx, i, phx, c1[k], c1[kxm], idx);
+            ty = axis_hz<R, cx<R>>(kyf, kyb, ny, j, phy, c0[k], c0[kym], idy);
+        } else {
+            tx = axis_hz<R, R>(kxf, kxb, nx, i, phx, m1[k], m1[kxm], idx);
+            ty = axis_hz<R, R>(kyf, kyb, ny, j, phy, m0[k], m0[kym], idy);
+        }
         diag = sh;
     }
     Row5<R> r;
@@ -226,7 +233,18 @@ __host__ __device__ __forceinline__ cx<R> apply_any(const cv_fdfd_desc& d, const
 // ------------------------------------------------- single derivative ops
 // which: 0 xf, 1 xb, 2 yf, 3 yb.  mode: 0 out = coef * mat .* (D u)
 //                                      1 out = coef * D^T (mat .* u)
-//                                      2 out = conj(coef) * D^H (mat .* u)
+//                                      2 out = conj(coef) * D^H (conj(mat) .* u)
+// mat is real, or interleaved complex when the instance has CV_FLAG_CPLX_MAT.
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> mat_times(const cv_fdfd_desc& d, const R* __restrict__ mat, int64_t k,
+                                           cx<R> v, bool conj_mat) {
+    if (d.flags & CV_FLAG_CPLX_MAT) {
+        const cx<R> m = reinterpret_cast<const cx<R>*>(mat)[k];
+        return (conj_mat ? conj(m) : m) * v;
+    }
+    return mat[k] * v;
+}
+
 template <typename R>
 __host__ __device__ __forceinline__ cx<R> deriv_at(const cv_fdfd_desc& d, const cx<R>* __restrict__ prof,
                                           const cx<R>* __restrict__ u, const R* __restrict__ mat,
@@ -247,7 +265,7 @@ __host__ __device__ __forceinline__ cx<R> deriv_at(const cv_fdfd_desc& d, const 
     };
     auto val = [&](int aa) -> cx<R> {
         cx<R> v = u[idx(aa)];
-        if (mode != 0 && mat) v = mat[idx(aa)] * v;
+        if (mode != 0 && mat) v = mat_times<R>(d, mat, idx(aa), v, mode == 2);
         return v;
     };
     auto kk = [&](int aa) -> cx<R> { return herm ? conj(K[aa]) : K[aa]; };
@@ -279,7 +297,7 @@ __host__ __device__ __forceinline__ cx<R> deriv_at(const cv_fdfd_desc& d, const 
             out = kk(a) * val(a) - t;
         }
     }
-    if (mode == 0 && mat) out = mat[idx(a)] * out;
+    if (mode == 0 && mat) out = mat_times<R>(d, mat, idx(a), out, false);
     return out;
 }
 

@@ -1361,37 +1361,49 @@ extern "C" int cv_fdtd_efields(int dtype, const cv_fdtd_desc* d, const void* sig
     return CV_OK;
 }
 
+// g_src[0] = lam(D_comp, cell): the cotangent of a point source amplitude
+// added to D at this step (the lam after the step's probe injection).
+template <typename R>
+__global__ void k_gather_src(const R* __restrict__ lam, int64_t idx, R* __restrict__ g_src) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) g_src[0] = lam[idx];
+}
+
+template <typename R>
+static int adjoint_step_impl(const cv_fdtd_desc* d, const void* sig, const void* eps, void* lam_, void* tmp,
+                             const void* D_prev, void* g_mE, const int32_t* probe_comp, const int64_t* probe_cell,
+                             int n_probes, const void* g_probe, const void* D_t, int src_comp, int64_t src_cell,
+                             void* g_src, hipStream_t s) {
+    Fdtd<R> f;
+    int rc = make_fdtd<R>(d, sig, eps, lam_, f);
+    if (rc) return rc;
+    R* lam = (R*)lam_;
+    if (n_probes > 0)
+        k_inject<R><<<(n_probes + 255) / 256, 256, 0, s>>>(f, lam, probe_comp, probe_cell, n_probes,
+                                                           (const R*)g_probe, (const R*)D_t, (R*)g_mE);
+    if (g_src) {
+        CV_CHECK(src_comp >= 0 && src_comp < 3 && src_cell >= 0 && src_cell < f.n,
+                 "source gradient requested for an invalid source (%d, %lld)", src_comp, (long long)src_cell);
+        k_gather_src<R><<<1, 64, 0, s>>>(lam, (int64_t)(3 + src_comp) * f.n + src_cell, (R*)g_src);
+    }
+    k_adj_a<R><<<nblk(f.n), 256, 0, s>>>(f, lam, (R*)tmp);
+    k_adj_b<R><<<nblk(f.n), 256, 0, s>>>(f, lam, (const R*)tmp, (const R*)D_prev, (R*)g_mE);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
 extern "C" int cv_fdtd_adjoint_step(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
                                     void* lam, void* tmp, const void* D_prev, void* g_mE, const int32_t* probe_comp,
                                     const int64_t* probe_cell, int n_probes, const void* g_probe,
-                                    const void* D_t, void* stream) {
+                                    const void* D_t, int src_comp, int64_t src_cell, void* g_src, void* stream) {
     hipStream_t s = as_stream(stream);
     CV_CHECK(lam && tmp, "null adjoint buffers");
-    if (dtype == CV_F32) {
-        Fdtd<float> f;
-        int rc = make_fdtd<float>(d, sig, eps, lam, f);
-        if (rc) return rc;
-        if (n_probes > 0)
-            k_inject<float><<<(n_probes + 255) / 256, 256, 0, s>>>(f, (float*)lam, probe_comp, probe_cell, n_probes,
-                                             (const float*)g_probe, (const float*)D_t, (float*)g_mE);
-        k_adj_a<float><<<nblk(f.n), 256, 0, s>>>(f, (float*)lam, (float*)tmp);
-        k_adj_b<float><<<nblk(f.n), 256, 0, s>>>(f, (float*)lam, (const float*)tmp, (const float*)D_prev,
-                                                 (float*)g_mE);
-    } else if (dtype == CV_F64) {
-        Fdtd<double> f;
-        int rc = make_fdtd<double>(d, sig, eps, lam, f);
-        if (rc) return rc;
-        if (n_probes > 0)
-            k_inject<double><<<(n_probes + 255) / 256, 256, 0, s>>>(f, (double*)lam, probe_comp, probe_cell, n_probes,
-                                              (const double*)g_probe, (const double*)D_t, (double*)g_mE);
-        k_adj_a<double><<<nblk(f.n), 256, 0, s>>>(f, (double*)lam, (double*)tmp);
-        k_adj_b<double><<<nblk(f.n), 256, 0, s>>>(f, (double*)lam, (const double*)tmp, (const double*)D_prev,
-                                                  (double*)g_mE);
-    } else {
-        CV_CHECK(false, "unknown dtype %d", dtype);
-    }
-    CV_LAUNCH_CHECK();
-    return CV_OK;
+    if (dtype == CV_F32)
+        return adjoint_step_impl<float>(d, sig, eps, lam, tmp, D_prev, g_mE, probe_comp, probe_cell, n_probes,
+                                        g_probe, D_t, src_comp, src_cell, g_src, s);
+    if (dtype == CV_F64)
+        return adjoint_step_impl<double>(d, sig, eps, lam, tmp, D_prev, g_mE, probe_comp, probe_cell, n_probes,
+                                         g_probe, D_t, src_comp, src_cell, g_src, s);
+    CV_CHECK(false, "unknown dtype %d", dtype);
 }
 
 extern "C" int cv_fdtd_eps_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,

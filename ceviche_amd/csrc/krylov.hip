@@ -29,8 +29,12 @@ struct KState {
     double rr, bb, tol2, rhrh;
     double rr_true, scale;        // true residual norm^2 at the last check; rhs scale (mixed)
     int32_t iter, done, restart, max_iter;
-    int32_t restarts, pad0, pad1, pad2;
+    int32_t restarts, skip, pad1, pad2;   // skip: K2 / K3 pass over a restarted iteration
 };
+
+// An exact (rh, v) = 0 restarts the iteration with rh = r; after this many
+// restarts in one solve it is reported as a breakdown.
+constexpr int kMaxRestarts = 64;
 
 // Scratch of the two-level last-workgroup reductions: per-tile partials,
 // per-group partials and counters (indexed by the group's first tile) and one
@@ -103,6 +107,7 @@ k_init(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
     st->rho = mk(rr, 0.0);
     st->rhrh = rr;
     st->restart = 1;
+    st->skip = 0;
     if (!isfinite(rr)) {
         st->done = kNonFinite;
     } else if (rr <= st->tol2) {
@@ -170,9 +175,17 @@ k_bicg_p(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
         return;
     }
     if (rhv.re == 0.0 && rhv.im == 0.0) {
-        st->done = kBreakdown;
+        // (rh, v) = 0 exactly: restart with rh = p = r (a shadow residual with
+        // a single non-zero, as a point right-hand side gives, meets this)
+        st->skip = 1;
+        st->restart = 1;
+        st->rho = mk(st->rr, 0.0);
+        st->rhrh = st->rr;
+        st->restarts += 1;
+        if (st->restarts > kMaxRestarts) st->done = kBreakdown;
         return;
     }
+    st->skip = 0;
     st->alpha = cdiv(st->rho, rhv);
 }
 
@@ -188,7 +201,7 @@ k_bicg_s(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     const cv_fdfd_desc d = load_desc(desc, t.inst);
     const int sys = system_of(d, t.inst);
     KState* st = state + sys;
-    if (st->done != kRunning) return;
+    if (st->done != kRunning || st->skip) return;
     const cx<R> alpha = ccast<R>(st->alpha);
     const int n = d.nx * d.ny;
     auto fs = [&](int64_t k) -> cx<R> { return r[k] - alpha * v[k]; };
@@ -231,7 +244,7 @@ k_bicg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tile
     const cv_fdfd_desc d = load_desc(desc, t.inst);
     const int sys = system_of(d, t.inst);
     KState* st = state + sys;
-    if (st->done != kRunning) return;
+    if (st->done != kRunning || st->skip) return;
     const cx<R> alpha = ccast<R>(st->alpha), omega = ccast<R>(st->omega);
     const int n = d.nx * d.ny;
     double acc[3] = {0.0, 0.0, 0.0};

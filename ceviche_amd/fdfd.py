@@ -150,14 +150,33 @@ class fdfd_hz(fdfd):
     def __init__(self, omega, dL, eps_r, npml, bloch_phases=None, **kw):
         super().__init__(omega, dL, eps_r, npml, bloch_phases=bloch_phases, **kw)
 
+    def _complex_ops(self):
+        """Operators for complex eps_r: the Hz materials 1/(eps_xx + 1e-5) and
+        1/(eps_yy + 1e-5) are then complex (fdfd.py:200-215 on complex
+        arrays); built on first use, solved by BiCGSTAB."""
+        if getattr(self, '_cops', None) is None:
+            spec = dict(omega=self.omega, dL=self.dL, shape=self.shape, npml=self.npml,
+                        bloch=(self.bloch_x, self.bloch_y))
+            op = FdfdOperator('hz', [spec], dtype=torch.complex128, complex_materials=True)
+            op32 = (FdfdOperator('hz', [spec], dtype=torch.complex64, complex_materials=True)
+                    if self.precision == 'complex64' else None)
+            self._cops = (op, op32)
+        return self._cops
+
+    @property
+    def last_solve_info(self):
+        ops = self._cops if getattr(self, '_last_complex', False) else (self.op, self.op32)
+        op = ops[1] if ops[1] is not None else ops[0]
+        return op.last_info[0] if op.last_info else None
+
     def _solve_fn(self, eps_vec, Mz_vec):
-        if torch.is_complex(eps_vec):
-            raise NotImplementedError('fdfd_hz with complex eps_r is not supported')
-        m0, m1 = HzMaterials.apply(eps_vec, self.op)
+        self._last_complex = torch.is_complex(eps_vec)
+        op, op32 = self._complex_ops() if self._last_complex else (self.op, self.op32)
+        m0, m1 = HzMaterials.apply(eps_vec, op)
         b = 1j * self.omega * Mz_vec
-        Hz = self._solve_linear(m0, m1, b)
-        Ex = FdfdDeriv.apply(Hz, m0, self.op, 2, 1 / 1j / self.omega / EPSILON_0)
-        Ey = FdfdDeriv.apply(Hz, m1, self.op, 0, -1 / 1j / self.omega / EPSILON_0)
+        Hz = FdfdSolve.apply(m0, m1, b, op, op32, self.solver_options)
+        Ex = FdfdDeriv.apply(Hz, m0, op, 2, 1 / 1j / self.omega / EPSILON_0)
+        Ey = FdfdDeriv.apply(Hz, m1, op, 0, -1 / 1j / self.omega / EPSILON_0)
         return Ex, Ey, Hz
 
 

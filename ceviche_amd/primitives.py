@@ -13,14 +13,17 @@ the reference's defvjp/defjvp rules (primitives.py:74-98, 121-148, 178-257)
 written in torch's complex convention (cotangent dL/dRe + i dL/dIm, hence
 A^H where the reference has A^T).
 
-All numeric work runs in HIP kernels (segmented product sums, dense LU);
-the host only groups index sets (the symbolic analysis), once per index set.
+All numeric work runs in HIP kernels (segmented product sums, dense LU,
+CSR BiCGSTAB); the host only groups index sets (the symbolic analysis), once
+per index set.
 """
 import numpy as np
 import torch
 
 from . import _lib
 from ._lib import check, lib, ptr, stream_ptr
+from .utils import (make_sparse, transpose_indices, make_rand, make_rand_complex, make_rand_indeces,  # noqa: F401
+                    make_rand_sparse, der_num, grad_num, get_entries_indices, make_IO_matrices)
 
 DENSE_MAX_N = 8192
 
@@ -146,12 +149,40 @@ def _densify(P, e):
     return A
 
 
+KRYLOV_TOL = 1e-10
+KRYLOV_MAX_ITER = 200000
+
+
+class _Krylov:
+    """sp_solve factor for N > DENSE_MAX_N: no factorisation; each solve is the
+    device Jacobi-preconditioned BiCGSTAB (cv_csr_solve) on the CSR of A, or of
+    A^T / A^H (the entries gathered in column order) for the reverse mode, to a
+    relative FP64 residual of KRYLOV_TOL.  Non-convergence raises."""
+
+    def __init__(self, P, e):
+        self.P, self.e = P, e
+        self.info = []
+
+    def solve(self, b, trans=0):
+        from .solvers import csr_solve
+        P = self.P
+        if trans == 0:
+            indptr, col, val = P.by_row.ptr, P.cols_by_row, self.e[P.by_row.order]
+        else:
+            indptr, col, val = P.by_col.ptr, P.rows_by_col, self.e[P.by_col.order]
+            if trans == 2:
+                val = val.conj()
+        x, info = csr_solve(indptr, col, val.resolve_conj(), _c(b), tol=KRYLOV_TOL, max_iter=KRYLOV_MAX_ITER)
+        self.info.append(info)
+        return x
+
+
+def _factor(P, e):
+    return _LU(P, e) if P.N <= DENSE_MAX_N else _Krylov(P, e)
+
+
 class _LU:
     def __init__(self, P, e):
-        if P.N > DENSE_MAX_N:
-            raise NotImplementedError('sp_solve on a general %d x %d matrix: the dense LU path handles N <= %d; '
-                                      'use the FDFD classes (matrix-free Krylov) for large grids'
-                                      % (P.N, P.N, DENSE_MAX_N))
         self.n = P.N
         self.A = _densify(P, e)
         self.piv = torch.empty(P.N, dtype=torch.int32, device=_dev())
@@ -206,7 +237,7 @@ class SpSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, entries, b, P):
         e = _c(entries)
-        lu = _LU(P, e)
+        lu = _factor(P, e)
         x = lu.solve(b)
         ctx.P, ctx.lu = P, lu
         ctx.save_for_backward(e, x)
@@ -346,7 +377,10 @@ def sp_mult(entries, indices, x):
 
 
 def sp_solve(entries, indices, b):
-    """A^-1 b (dense LU with partial pivoting on the GPU); primitives.py:103-119."""
+    """A^-1 b; primitives.py:103-119.  N <= DENSE_MAX_N: dense LU with partial
+    pivoting on the GPU (exact, like spsolve); larger N: the device
+    Jacobi-preconditioned BiCGSTAB on the CSR of A to a relative FP64 residual
+    of KRYLOV_TOL (the VJP and JVP solves likewise on A^H and A)."""
     as_t = _is_t(entries, b)
     N = int(b.numel() if isinstance(b, torch.Tensor) else np.asarray(b).size)
     P = Pattern.get(indices, N)
@@ -365,11 +399,7 @@ def spsp_mult(entries_a, indices_a, entries_x, indices_x, N):
     return eb.detach().cpu().numpy(), ib
 
 
-def transpose_indices(indices):
-    """ceviche/utils.py:37-39."""
-    if isinstance(indices, torch.Tensor):
-        return torch.flip(indices, dims=(0,))
-    return np.flip(indices, axis=0)
-
-
-__all__ = ['sp_mult', 'sp_solve', 'spsp_mult', 'transpose_indices']
+__all__ = ['sp_mult', 'sp_solve', 'spsp_mult',
+           # the helpers the reference's primitives module imports from its utils (primitives.py:5-6)
+           'make_sparse', 'transpose_indices', 'make_rand', 'make_rand_complex', 'make_rand_indeces',
+           'make_rand_sparse', 'der_num', 'grad_num', 'get_entries_indices', 'make_IO_matrices']

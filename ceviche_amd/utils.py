@@ -1,5 +1,14 @@
-"""Probe and spectrum post-processing with the reference's API
-(ceviche/utils.py:316-332, 350-399):
+"""Utilities with the reference's API (ceviche/utils.py).
+
+Sparse-matrix helpers (utils.py:13-73): make_sparse, get_entries_indices,
+transpose_indices, block_4, make_IO_matrices; random test data
+(utils.py:78-103): make_rand, make_rand_complex, make_rand_indeces,
+make_rand_entries_indices, make_rand_sparse, make_rand_sparse_density;
+numerical derivatives (utils.py:108-125): der_num, grad_num.  These are host
+utilities like the reference's (scipy / numpy formats); torch inputs to the
+sparse helpers stay on their device.
+
+Probe and spectrum post-processing (utils.py:316-332, 350-399):
 
     measured = measure_fields(F, source, steps, probes, component='Ez')
     X = my_fft(x)
@@ -22,10 +31,121 @@ axis and the result is the Hamming-windowed series itself; this is
 reproduced as written.
 """
 import numpy as np
+import scipy.sparse as sp
 import torch
 
 from . import _lib
 from ._probes import ProbeWeights, weighted_sums
+
+
+# ------------------------------------------------ sparse matrix utilities
+def make_sparse(entries, indices, shape):
+    """CSR matrix of the COO triplets with duplicates summed, complex128
+    (utils.py:13-25).  numpy inputs give a scipy.sparse CSR matrix like the
+    reference; torch inputs give a torch sparse CSR tensor on their device,
+    coalesced (duplicates summed) by torch."""
+    if isinstance(entries, torch.Tensor) or isinstance(indices, torch.Tensor):
+        e = torch.as_tensor(entries).to(torch.complex128)
+        idx = torch.as_tensor(indices, device=e.device).to(torch.int64)
+        coo = torch.sparse_coo_tensor(idx, e, size=tuple(int(v) for v in shape)).coalesce()
+        return coo.to_sparse_csr()
+    coo = sp.coo_matrix((entries, indices), shape=shape, dtype=np.complex128)
+    return coo.tocsr()
+
+
+def transpose_indices(indices):
+    """Index set of the transposed matrix: rows and columns exchanged (utils.py:37-39)."""
+    if isinstance(indices, torch.Tensor):
+        return torch.flip(indices, dims=(0,))
+    return np.flip(indices, axis=0)
+
+
+def get_entries_indices(csr_matrix):
+    """(entries, indices) of a CSR matrix in storage order, indices = [rows;
+    cols] (utils.py:27-35); a torch sparse CSR tensor gives torch tensors."""
+    if isinstance(csr_matrix, torch.Tensor):
+        crow = csr_matrix.crow_indices()
+        cols = csr_matrix.col_indices()
+        counts = crow[1:] - crow[:-1]
+        rows = torch.repeat_interleave(torch.arange(counts.numel(), device=crow.device), counts)
+        return csr_matrix.values(), torch.stack((rows, cols))
+    coo = csr_matrix.tocoo()
+    return csr_matrix.data, np.vstack((coo.row, coo.col))
+
+
+def block_4(A, B, C, D):
+    """[[A, B], [C, D]] from four sparse blocks (utils.py:41-48)."""
+    return sp.hstack([sp.vstack([A, C]), sp.vstack([B, D])])
+
+
+def make_IO_matrices(indices, N):
+    """One-hot incidence matrices (I, O) of an index set (utils.py:50-73):
+    column k of I (N x M) selects entry k's row, O = J^T (M x N) its column,
+    so that A = I diag(a) O and a = diag(I^T A O^T)."""
+    idx = np.asarray(indices.detach().cpu().numpy() if isinstance(indices, torch.Tensor) else indices)
+    M = idx.shape[1]
+    ones = np.ones(M)
+    I = make_sparse(ones, np.vstack((idx[0], np.arange(M))), shape=(N, M))
+    J = make_sparse(ones, np.vstack((idx[1], np.arange(M))), shape=(N, M))
+    return I, J.T
+
+
+# --------------------------------------------------- data generation
+def make_rand(N):
+    """N uniform values in [-0.5, 0.5) from numpy's global generator (utils.py:78-80)."""
+    return np.random.random(N) - 0.5
+
+
+def make_rand_complex(N):
+    """Complex values with real and imaginary parts from make_rand (utils.py:82-84)."""
+    return make_rand(N) + 1j * make_rand(N)
+
+
+def make_rand_indeces(N, M):
+    """(2, M) random indices into an N x N matrix (utils.py:86-88; the reference's spelling)."""
+    return np.random.randint(low=0, high=N, size=(2, M))
+
+
+def make_rand_entries_indices(N, M):
+    """M random complex entries and their random indices (utils.py:90-94)."""
+    entries = make_rand_complex(M)
+    indices = make_rand_indeces(N, M)
+    return entries, indices
+
+
+def make_rand_sparse(N, M):
+    """Random N x N CSR matrix from M random triplets (utils.py:96-99)."""
+    entries, indices = make_rand_entries_indices(N, M)
+    return make_sparse(entries, indices, shape=(N, N))
+
+
+def make_rand_sparse_density(N, density=1):
+    """Random complex N x N sparse matrix of a given density (utils.py:101-103)."""
+    return sp.random(N, N, density=density) + 1j * sp.random(N, N, density=density)
+
+
+# ------------------------------------------------- numerical derivatives
+def der_num(fn, arg, index, delta):
+    """Central difference (fn(arg + delta/2 e_i) - fn(arg - delta/2 e_i)) / delta
+    along entry ``index`` of a numpy array (utils.py:108-115); a complex delta
+    steps the imaginary part (discarded, with numpy's ComplexWarning, when arg
+    is real, as in the reference)."""
+    arg_i_for = arg.copy()
+    arg_i_back = arg.copy()
+    arg_i_for[index] += delta / 2
+    arg_i_back[index] -= delta / 2
+    return (fn(arg_i_for) - fn(arg_i_back)) / delta
+
+
+def grad_num(fn, arg, delta=1e-6):
+    """Complex numerical gradient dfn/dRe - i dfn/dIm of every entry, complex128
+    (utils.py:117-125): autograd's cotangent convention."""
+    N = arg.size
+    grad = np.zeros((N,), dtype=np.complex128)
+    for i in range(N):
+        grad[i] = der_num(fn, arg, i, delta)
+        grad[i] += der_num(fn, arg, i, 1j * delta)
+    return grad
 
 
 def measure_fields(F, source, steps, probes, component='Ez'):

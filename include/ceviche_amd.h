@@ -49,6 +49,11 @@ extern "C" {
 #define CV_POL_HZ 1
 #define CV_FLAG_X_IDENTITY 1   /* Nx == 1: Dx* is the identity (derivatives.py:76-77) */
 #define CV_FLAG_Y_IDENTITY 2   /* Ny == 1: Dy* is the identity (derivatives.py:78-79) */
+#define CV_FLAG_CPLX_MAT   4   /* Hz with complex eps: m0, m1 (and the eps, material
+                                  tangent and gradient arrays of the Hz material
+                                  entry points) hold interleaved complex values
+                                  (fdfd.py:200-215 on complex eps); not accepted by
+                                  the direct factorisations */
 
 /* One FDFD instance of a batch.  Plain data, 128 bytes, device-resident array.
  *   pol == CV_POL_EZ:  A = scale*(Dxf Dxb + Dyf Dyb) + diag(dshift*(m0 + i m1))
@@ -309,11 +314,15 @@ int cv_probe_weights(int dtype, const void* field, int64_t field_stride, int T, 
  * lam: 18-array adjoint state after step t (updated in place to before step t);
  * tmp: 3-array scratch; D_prev: D after step t-1; D_t: D after step t (for E
  * probes); g_mE (3 arrays, may be NULL) accumulates dL/d(1/eps_aa);
- * g_probe[n_probes]: cotangent of the step-t probe record, injected first. */
+ * g_probe[n_probes]: cotangent of the step-t probe record, injected first.
+ * g_src (device, one real, may be NULL): receives lam(D_src_comp, src_cell)
+ * after the injection, the cotangent of a point-source amplitude added to D
+ * at this step (fdtd.py:122-124). */
 int cv_fdtd_adjoint_step(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps,
                          void* lam, void* tmp, const void* D_prev, void* g_mE,
                          const int32_t* probe_comp, const int64_t* probe_cell, int n_probes,
-                         const void* g_probe, const void* D_t, void* stream);
+                         const void* g_probe, const void* D_t, int src_comp, int64_t src_cell,
+                         void* g_src, void* stream);
 
 /* Reverse of cv_fdtd_efields for a dense cotangent gE[3][n]:
  * gD[3][n] += gE / eps_aa;  g_mE[3][n] += D * gE (g_mE may be NULL). */
@@ -367,6 +376,21 @@ int cv_fdtd_eps_grad(int dtype, const cv_fdtd_desc* d, const void* sig, const vo
 int cv_segsum(int64_t n_seg, const int64_t* ptr, const int64_t* iu, const int64_t* iv,
               const void* u, const void* v, int conj_u, int conj_v, double alpha_re, double alpha_im,
               const int64_t* oidx, int accumulate, void* out, void* stream);
+
+/* Solves A x = b for a general sparse complex128 matrix in CSR form
+ * (indptr[n+1], indices, data: the arrays of a scipy.sparse.csr_matrix, int32
+ * or int64 as index_bits says; duplicates in a row are summed) by
+ * Jacobi-preconditioned BiCGSTAB with FP64 reductions, to
+ * ||b - A x|| <= tol ||b|| checked on a freshly computed residual.  x holds
+ * the initial guess.  Replaces spsolve (solvers.py:50-62) and the Krylov
+ * branch (solvers.py:64-75) behind solve_linear, the empty hook
+ * _solve_cuda(A, b) (solvers.py:77-79), and the factorisation of sp_solve
+ * (primitives.py:103-148) for matrices above the dense-LU size.  info: one
+ * host entry.  CV_ERR_NOT_CONVERGED when max_iter is reached. */
+int64_t cv_csr_solve_workspace_bytes(int64_t n);
+int cv_csr_solve(int64_t n, int index_bits, const void* indptr, const void* indices, const void* data,
+                 const void* b, void* x, double tol, int max_iter, int check_every, void* workspace,
+                 int64_t workspace_bytes, cv_solve_info* info, void* stream);
 
 /* In-place LU with partial pivoting of a column-major n x n complex128
  * matrix (P A = L U, unit L), pivot rows in piv[n] (device int32); *info_dev

@@ -1,0 +1,116 @@
+"""Complex permittivity in both polarisations: fdfd_hz averages and inverts
+complex eps (ceviche/fdfd.py:200-244), and the reference's running
+tests/test_gradients_fdfd_complex.py (test_Hz_forward, test_Ez_forward:
+30x30, omega = 2 pi 200 THz, dL = 1 um, pml 10, eps = (1 + U[0,1)) +
+i (1 + U[0,1)), point source 1e-8 at the centre scaled by c eps, objective
+|E|^2 + |H|^2 per cell, jacobian(mode='forward') w.r.t. c) against central
+differences of the oracle; plus reverse-mode gradients w.r.t. complex eps
+and fields against spsolve."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fdfd_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _setup(seed=0, shape=(30, 30)):
+    rng = np.random.default_rng(seed)
+    eps = rng.random(shape) + 1 + 1j * (rng.random(shape) + 1)
+    src = np.zeros(shape)
+    src[shape[0] // 2, shape[1] // 2] = 1e-8
+    return dict(omega=2 * np.pi * 200e12, dL=1e-6, npml=[10, 10], eps=eps, src=src)
+
+
+@pytest.mark.parametrize('precision', ['complex128', 'complex64'])
+def test_hz_complex_eps_fields_match_spsolve(precision):
+    import ceviche_amd as cv
+    rng = np.random.default_rng(5)
+    omega = 2 * np.pi * R.C_0 / 1.55e-6
+    shape = (48, 40)
+    eps = rng.uniform(1, 6, shape) - 1j * rng.uniform(0, 0.8, shape)
+    src = np.zeros(shape)
+    src[20, 17] = 1.0
+    F = cv.fdfd_hz(omega, 50e-9, eps, [8, 6], bloch_phases=[0.0, 0.4], precision=precision)
+    Ex, Ey, Hz = F.solve(src)
+    print('hz complex info', precision, F.last_solve_info, flush=True)
+    ex, ey, hz = R.solve_hz(omega, 50e-9, eps, [8, 6], src, bloch_phases=(0.0, 0.4))
+    for a, b in ((Hz, hz), (Ex, ex), (Ey, ey)):
+        assert a.dtype == np.complex128 and a.shape == shape
+        assert _rel(a, b) <= 1e-6
+    assert F.last_solve_info['rel_residual'] <= (1e-10 if precision == 'complex128' else 1e-6)
+    # a real eps on the same object still takes the real-material operator
+    F.eps_r = eps.real
+    _, _, Hz2 = F.solve(src)
+    assert _rel(Hz2, R.solve_hz(omega, 50e-9, eps.real, [8, 6], src, bloch_phases=(0.0, 0.4))[2]) <= 1e-6
+
+
+@pytest.mark.parametrize('pol', ['hz', 'ez'])
+def test_reference_complex_forward_mode(pol):
+    import ceviche_amd as cv
+    s = _setup(pol == 'ez')
+    eps_r, src = s['eps'], s['src']
+    cls = cv.fdfd_hz if pol == 'hz' else cv.fdfd_ez
+    F = cls(s['omega'], s['dL'], eps_r, s['npml'])
+    eps_t, src_t = torch.from_numpy(eps_r), torch.from_numpy(src)
+
+    def J(c):
+        c = torch.as_tensor(c, dtype=torch.float64)
+        F.eps_r = c * eps_t
+        a, b, z = F.solve(c * eps_t * src_t)
+        return (z.abs() ** 2 + a.abs() ** 2 + b.abs() ** 2).reshape(-1)
+
+    jf = cv.jacobian(J, mode='forward')(1.0)
+    solver = R.solve_hz if pol == 'hz' else R.solve_ez
+
+    def Jr(c):
+        a, b, z = solver(s['omega'], s['dL'], c * eps_r, s['npml'], c * eps_r * src)
+        return (np.abs(z) ** 2 + np.abs(a) ** 2 + np.abs(b) ** 2).ravel()
+
+    h = 1e-6
+    num = ((Jr(1 + h) - Jr(1 - h)) / (2 * h))[:, None]
+    assert jf.shape == (eps_r.size, 1)
+    assert np.linalg.norm(jf - num) <= 1e-4 * np.linalg.norm(num)
+
+
+def test_hz_complex_eps_reverse_gradient():
+    """Reverse mode w.r.t. complex eps (torch convention dL/dRe + i dL/dIm),
+    every entry against central differences of the oracle in the real and
+    imaginary directions (the reference's disabled tes1t_Hz_reverse set-up,
+    on a smaller grid)."""
+    import ceviche_amd as cv
+    s = _setup(3, shape=(16, 14))
+    s['npml'] = [4, 4]
+    eps0, src = s['eps'], s['src']
+    eps = torch.tensor(eps0, requires_grad=True)
+    F = cv.fdfd_hz(s['omega'], s['dL'], eps, s['npml'])
+    a, b, z = F.solve(torch.from_numpy(src) * eps)
+    L = sum((v.abs() ** 2).sum() for v in (a, b, z))
+    L.backward()
+    g = eps.grad.numpy()
+
+    def Lr(e):
+        f = R.solve_hz(s['omega'], s['dL'], e, s['npml'], e * src)
+        return sum(np.sum(np.abs(v) ** 2) for v in f)
+
+    assert L.item() == pytest.approx(Lr(eps0), rel=1e-8)
+    h = 1e-6
+    num = np.zeros(eps0.shape, dtype=complex)
+    for idx in np.ndindex(eps0.shape):
+        d = np.zeros(eps0.shape)
+        d[idx] = h
+        num[idx] = (Lr(eps0 + d) - Lr(eps0 - d)) / (2 * h) + 1j * (Lr(eps0 + 1j * d) - Lr(eps0 - 1j * d)) / (2 * h)
+    assert np.linalg.norm(g - num) <= 1e-4 * np.linalg.norm(num)
+
+
+def test_hz_complex_eps_direct_method_is_rejected():
+    import ceviche_amd as cv
+    s = _setup(1, shape=(20, 20))
+    F = cv.fdfd_hz(s['omega'], s['dL'], s['eps'], [4, 4], solver_options=cv.SolverOptions(method='direct'))
+    with pytest.raises(ValueError):
+        F.solve(s['src'])

@@ -205,3 +205,36 @@ def test_direct_method_requires_a_narrow_grid():
         F = cls(OMEGA, 50e-9, np.ones(shape), [2, 2], solver_options=cv.SolverOptions(method='direct'))
         with pytest.raises(ValueError):
             F.solve(np.ones(shape))
+
+
+def test_direct_failure_falls_back_to_bicgstab():
+    """The direct path forced to fail from the host (its FGMRES limited to one
+    step, far from the 1e-12 tolerance): under method='auto' the solve, its
+    gradient and its forward-mode tangent fall back to BiCGSTAB with a
+    warning and still match spsolve; under method='direct' the failure
+    raises."""
+    import warnings
+    import ceviche_amd as cv
+    from ceviche_amd._lib import NotConvergedError
+    from tests._cfg import cfg1
+    c = cfg1()
+    opts = cv.SolverOptions(tol=1e-12, direct_max_iter=1)
+    eps = torch.tensor(c['eps'], requires_grad=True)
+    F = cv.fdfd_ez(c['omega'], c['dL'], eps, c['npml'], solver_options=opts)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter('always')
+        _, _, Ez = F.solve(torch.from_numpy(c['src']))
+        info = F.last_solve_info
+        (Ez[90, 60].abs() ** 2).backward()
+    assert any('falling back' in str(x.message) for x in w)
+    assert info.get('method', 'bicgstab') == 'bicgstab' and info['rel_residual'] <= 1e-12
+    hx, hy, ez = R.solve_ez(c['omega'], c['dL'], c['eps'], c['npml'], c['src'])
+    assert np.linalg.norm(Ez.detach().cpu().numpy() - ez) / np.linalg.norm(ez) <= 1e-6
+    cot = np.zeros_like(ez)
+    cot[90, 60] = 2 * ez[90, 60]
+    gref, _ = R.grad_eps_ez(c['omega'], c['dL'], c['eps'], c['npml'], c['src'], cot)
+    assert np.linalg.norm(eps.grad.cpu().numpy() - gref) / np.linalg.norm(gref) <= 1e-4
+    F2 = cv.fdfd_ez(c['omega'], c['dL'], c['eps'], c['npml'],
+                    solver_options=cv.SolverOptions(tol=1e-12, direct_max_iter=1, method='direct'))
+    with pytest.raises(NotConvergedError):
+        F2.solve(c['src'])

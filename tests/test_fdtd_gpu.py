@@ -167,6 +167,90 @@ def test_run_probes_and_checkpointed_adjoint_2d(prec):
         assert np.sum(g * d) == pytest.approx(dd, rel=gtol)
 
 
+def _oracle_forward_then_run(eps, dL, npml, Jpre, src_pos, amp, probes, dtype=np.float64):
+    """Probe energy of len(Jpre) reference steps with dense Jz sources (not
+    recorded) followed by a point-source run recording ``probes``."""
+    R = T.FDTD(eps, dL, npml, dtype=dtype)
+    for Jz in Jpre:
+        R.forward(Jz=Jz.astype(dtype))
+    J = np.zeros(R.shape, dtype=dtype)
+    L = 0.0
+    for t in range(len(amp)):
+        J[src_pos] = amp[t]
+        f = R.forward(Jz=J)
+        L = L + sum(f[c][p] ** 2 for c, p in probes)
+    return L
+
+
+@pytest.mark.parametrize('kernel', ['auto', 'generic'])
+def test_chained_runs_gradients_reach_eps_and_amplitudes(kernel):
+    """Two run() calls in a row: the second starts from the first's final
+    state, so eps and the amplitudes of the first run also get the second
+    run's cotangents through the state.  Against complex step of one
+    continuous oracle run."""
+    import ceviche_amd as cv
+    rng = np.random.default_rng(21)
+    shape = (30, 26, 1)
+    eps0 = np.where(rng.standard_normal(shape) > 0, 12.25, 1.0)
+    dL, npml, n1, n2 = 20e-9, [5, 5, 0], 70, 90
+    amp0 = np.exp(-(np.arange(n1 + n2) - 40) ** 2 / (2 * 12 ** 2))
+    probes = [('Ez', (8, 13, 0)), ('Hy', (22, 13, 0)), ('Ez', (15, 20, 0))]
+    src = (15, 13, 0)
+    eps = torch.tensor(eps0, requires_grad=True)
+    amp = torch.tensor(amp0, requires_grad=True)
+    F = cv.fdtd(eps, dL, npml)
+    r1 = F.run(n1, src, amp[:n1], probes, checkpoint_every=23, kernel=kernel)
+    r2 = F.run(n2, src, amp[n1:], probes, checkpoint_every=31, kernel=kernel)
+    L = (r1 ** 2).sum() + (r2 ** 2).sum()
+    L.backward()
+    Lref, tr = _oracle_probe_energy(eps0, dL, npml, src, amp0, probes)
+    assert float(L) == pytest.approx(Lref, rel=1e-10)
+    h = 1e-30
+    d = np.random.default_rng(5).standard_normal(shape)
+    Lc, _ = _oracle_probe_energy(eps0 + 1j * h * d, dL, npml, src, amp0.astype(complex), probes, dtype=np.complex128)
+    assert np.sum(eps.grad.cpu().numpy() * d) == pytest.approx(np.imag(Lc) / h, rel=1e-8)
+    da = np.random.default_rng(6).standard_normal(n1 + n2)
+    Lc, _ = _oracle_probe_energy(eps0.astype(complex), dL, npml, src, amp0 + 1j * h * da, probes,
+                                 dtype=np.complex128)
+    assert amp.grad is not None
+    assert np.sum(amp.grad.cpu().numpy() * da) == pytest.approx(np.imag(Lc) / h, rel=1e-8)
+    # the first run's amplitudes alone (they act through the carried state in run 2 as well)
+    da1 = np.concatenate([da[:n1], np.zeros(n2)])
+    Lc, _ = _oracle_probe_energy(eps0.astype(complex), dL, npml, src, amp0 + 1j * h * da1, probes,
+                                 dtype=np.complex128)
+    assert np.sum(amp.grad.cpu().numpy() * da1) == pytest.approx(np.imag(Lc) / h, rel=1e-8)
+
+
+def test_forward_steps_then_run_gradient_through_the_state():
+    """forward() steps (per-step adjoint) followed by run() (checkpointed
+    adjoint): the run's cotangent flows back through the carried state into
+    the per-step part.  3D grid, against complex step of the oracle."""
+    import ceviche_amd as cv
+    rng = np.random.default_rng(22)
+    shape = (9, 8, 7)
+    eps0 = rng.uniform(1, 4, shape)
+    dL, npml = 25e-9, [2, 2, 2]
+    Jpre = [rng.standard_normal(shape) * np.exp(-(t - 6) ** 2 / 8.0) for t in range(12)]
+    amp0 = np.exp(-(np.arange(50) - 10) ** 2 / (2 * 5 ** 2))
+    probes = [('Ex', (3, 4, 3)), ('Hz', (6, 2, 4)), ('Ez', (4, 4, 1))]
+    src = (4, 4, 3)
+    eps = torch.tensor(eps0, requires_grad=True)
+    F = cv.fdtd(eps, dL, npml)
+    for Jz in Jpre:
+        F.forward(Jz=torch.from_numpy(Jz))
+    rec = F.run(50, src, amp0, probes, checkpoint_every=13)
+    L = (rec ** 2).sum()
+    L.backward()
+    Lref = _oracle_forward_then_run(eps0, dL, npml, Jpre, src, amp0, probes)
+    assert float(L) == pytest.approx(Lref, rel=1e-10)
+    h = 1e-30
+    for seed in (0, 1):
+        d = np.random.default_rng(40 + seed).standard_normal(shape)
+        Lc = _oracle_forward_then_run(eps0 + 1j * h * d, dL, npml, Jpre, src, amp0.astype(complex), probes,
+                                      dtype=np.complex128)
+        assert np.sum(eps.grad.cpu().numpy() * d) == pytest.approx(np.imag(Lc) / h, rel=1e-8)
+
+
 def test_3d_checkpointed_adjoint_with_e_and_h_probes():
     import ceviche_amd as cv
     rng = np.random.default_rng(8)
