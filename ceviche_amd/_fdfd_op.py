@@ -31,16 +31,17 @@ class SolverOptions:
 
     method: 'auto' (the block-tridiagonal factorisation + FGMRES for a single
     instance with ny <= 128, or nx <= 128 on the transposed grid, BiCGSTAB
-    otherwise), 'direct' (require the former) or 'bicgstab'.  restart: FGMRES
+    otherwise), 'direct' (require the former), 'bicgstab', or 'cocg' (COCG
+    on the complex-symmetric form: zero Bloch phases, uncoupled instances).  restart: FGMRES
     cycle length of the direct path; direct_max_iter: its FGMRES step limit
     (max_iter when None).  Under 'auto' a direct solve that fails (singular
-    factor, FGMRES not converged, out of memory) falls back to BiCGSTAB with a
-    warning; under 'direct' it raises."""
+    factor, FGMRES not converged, out of memory) falls back to the Krylov
+    method with a warning; under 'direct' it raises."""
 
     def __init__(self, tol=None, max_iter=200000, check_every=32, inner_tol=1e-4, max_sweeps=12, method='auto',
                  restart=30, direct_max_iter=None):
-        if method not in ('auto', 'direct', 'bicgstab'):
-            raise ValueError("method must be 'auto', 'direct' or 'bicgstab'")
+        if method not in ('auto', 'direct', 'bicgstab', 'cocg'):
+            raise ValueError("method must be 'auto', 'direct', 'bicgstab' or 'cocg'")
         self.tol = tol
         self.max_iter = max_iter
         self.direct_max_iter = direct_max_iter
@@ -310,6 +311,28 @@ class FdfdOperator:
             self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
 
+    def cocg_applies(self):
+        """COCG needs the complex-symmetric form: zero Bloch phases, no
+        frequency-coupled blocks."""
+        d = self.desc_host
+        return (self.cpl_len == 0 and bool(np.all(d['nblk'] == 1)) and bool(np.all(d['phx_re'] == 1.0))
+                and bool(np.all(d['phx_im'] == 0.0)) and bool(np.all(d['phy_re'] == 1.0))
+                and bool(np.all(d['phy_im'] == 0.0)))
+
+    def krylov_method(self, opts):
+        """The Krylov method a solve with ``opts`` runs: 'cocg' only when asked
+        for (method='cocg'), else 'bicgstab'.  COCG needs about a third of
+        BiCGSTAB's operator applications on small and mid-size blob problems,
+        but its residual can swing by orders of magnitude on the way (complex64
+        rounding does not survive that, and on the held-out 512^2 and 1024^2
+        blob structures it diverges), so it is not the default."""
+        method = (opts or SolverOptions()).method
+        if method == 'cocg':
+            if not self.cocg_applies():
+                raise ValueError("method='cocg' needs zero Bloch phases and uncoupled instances")
+            return 'cocg'
+        return 'bicgstab'
+
     def solve_scaled(self, bs, m0, m1, transpose=False, x0=None, opts=None, mixed_ops=None, prof=None):
         """Solve (c A) x = bs (bs already multiplied by c).  Returns x.
 
@@ -321,6 +344,8 @@ class FdfdOperator:
         opts = opts or SolverOptions()
         info = (_lib.SolveInfo * self.n_inst)()
         pr, pr64 = (self.prof, None) if prof is None else prof
+        kmethod = self.krylov_method(opts)
+        kcode = _lib.CV_KRYLOV_COCG if kmethod == 'cocg' else _lib.CV_KRYLOV_BICGSTAB
         if mixed_ops is not None:
             op64, m0_64, m1_64 = mixed_ops
             pr64 = op64.prof if pr64 is None else pr64
@@ -331,20 +356,21 @@ class FdfdOperator:
             rc = lib().cv_fdfd_solve_mixed(
                 ptr(op64.desc), self.desc_host.ctypes.data_as(ctypes.c_void_p), self.n_inst, ptr(self.tiles),
                 self.n_tiles, ptr(pr64), ptr(m0_64), ptr(m1_64), ptr(pr), ptr(m0), ptr(m1),
-                ptr(b64), ptr(x), int(bool(transpose)), float(tol), float(opts.inner_tol), int(opts.max_iter),
+                ptr(b64), ptr(x), int(bool(transpose)), kcode, float(tol), float(opts.inner_tol), int(opts.max_iter),
                 int(opts.max_sweeps), int(opts.check_every), ptr(ws), ws.numel(), info, stream_ptr())
         else:
             tol = opts.tol if opts.tol is not None else DEFAULT_TOL[self.dtype]
             b = bs.to(self.dtype).contiguous()
             x = torch.zeros_like(b) if x0 is None else x0.to(self.dtype).clone().contiguous()
             ws = self._workspace(False)
-            rc = lib().cv_fdfd_solve(
+            entry = lib().cv_fdfd_cocg if kmethod == 'cocg' else lib().cv_fdfd_solve
+            rc = entry(
                 self.code, ptr(self.desc), self.desc_host.ctypes.data_as(ctypes.c_void_p), self.n_inst,
                 ptr(self.tiles), self.n_tiles, ptr(pr), ptr(m0), ptr(m1), ptr(b), ptr(x),
                 int(bool(transpose)), float(tol), int(opts.max_iter), int(opts.check_every), ptr(ws), ws.numel(),
                 info, stream_ptr())
         self.last_info = [dict(iterations=i.iterations, status=i.status, restarts=i.restarts,
-                               rel_residual=i.rel_residual) for i in info]
+                               rel_residual=i.rel_residual, method=kmethod) for i in info]
         check(rc, 'fdfd solve')
         return x
 
@@ -576,7 +602,7 @@ class BlockDirect:
 
 def _use_blockdiag(op, opts):
     method = (opts or SolverOptions()).method
-    if method == 'bicgstab':
+    if method in ('bicgstab', 'cocg'):
         return False
     if method == 'direct' and not (blockdiag_applies(op) and op.shapes[0][0] >= 8):
         raise ValueError("method='direct' needs one coupled system with ny <= %d and nx >= 8" % BTD_MAX_NY)
@@ -588,7 +614,7 @@ def _use_direct(op, opts):
     applies and its factor fits in DIRECT_MEMORY_FRACTION of the free device
     memory."""
     method = (opts or SolverOptions()).method
-    if method == 'bicgstab':
+    if method in ('bicgstab', 'cocg'):
         return False
     if method == 'direct':
         if not direct_applies(op):
@@ -612,7 +638,7 @@ def _direct_failed(err, opts):
         return False
     if isinstance(err, torch.cuda.OutOfMemoryError) or (isinstance(err, CevicheError)
                                                          and err.status in _DIRECT_FALLBACK):
-        warnings.warn('direct FDFD solve failed (%s); falling back to Jacobi-preconditioned BiCGSTAB' % err,
+        warnings.warn('direct FDFD solve failed (%s); falling back to Jacobi-preconditioned Krylov' % err,
                       RuntimeWarning)
         return True
     return False

@@ -32,6 +32,8 @@ CV_FLAG_CPLX_MAT = 4
 CV_TILE_CELLS = 256
 CV_NOALLOC_SCRATCH_BYTES = 32 << 20
 CV_NOALLOC_MAX_INST = 64
+CV_KRYLOV_BICGSTAB = 0
+CV_KRYLOV_COCG = 1
 CV_FACTOR_BTD = 1
 CV_FACTOR_BCR = 2
 
@@ -129,7 +131,8 @@ _SIGS = {
     'cv_noalloc_init': (I, []),
     'cv_fdfd_solve_noalloc': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, P]),
     'cv_fdfd_solve_mixed_workspace_bytes': (I64, [I64, I, I]),
-    'cv_fdfd_solve_mixed': (I, [P, P, I, P, I, P, P, P, P, P, P, P, P, I, D, D, I, I, I, P, I64, P, P]),
+    'cv_fdfd_cocg': (I, [I, P, P, I, P, I, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
+    'cv_fdfd_solve_mixed': (I, [P, P, I, P, I, P, P, P, P, P, P, P, P, I, I, D, D, I, I, I, P, I64, P, P]),
     'cv_fdtd_step': (I, [I, P, P, P, P, P, P, P, D, I, I64, P, I, P, P, I, P, P]),
     'cv_fdtd_efields': (I, [I, P, P, P, P, P, P]),
     'cv_fdtd_step_tangent': (I, [I, P, P, P, P, P, P, P, P, P, P, P, P, D, I, I64, P, P, I, P, P, I, P, P, P]),
@@ -183,6 +186,10 @@ def lib():
 
 def exported_symbols():
     return list(_SIGS)
+
+
+def status_name(rc):
+    return _STATUS_NAMES.get(rc, 'status %d' % rc)
 
 
 def check(rc, what=''):

@@ -394,6 +394,223 @@ k_mixed_resid(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__
         st->done = kRunning;
 }
 
+// ------------------------------------------------------------------ COCG
+// Conjugate orthogonal CG on the complex-symmetric form B = W A of the
+// operator (zero Bloch phases, no coupled blocks): with the forward SC-PML
+// factors s (kf = 1/(s dL)), W = 1/(kxf_i kyf_j) for Ez and 1/(kxb_i kyb_j)
+// for Hz makes W A symmetric (A = S^-1 D0 S^-1 D0 with D0b = -D0f^T).
+// A x = b is solved as B x = W b; A^T y = b (A^T = B W^-1) as B u = b with
+// y = W u.  Jacobi preconditioning on B, M = diag(B) = W_i A_ii; the
+// recurrences use the unconjugated bilinear form (x, y) = sum x_i y_i.  One
+// iteration is two fused kernels:
+//   C1  p' = z + beta p (p' = z on restart) at every stencil neighbour,
+//       q = B p' = W (A p'), partial (p', q)                  -> alpha
+//   C2  x += alpha p' (W p' for A^T), z -= alpha q / M, partials (r, z)
+//       with r = M z, and |rho|^2 of the true residual rho = r / W (A) or
+//       r (A^T)                                     -> beta, convergence
+// z = M^-1 r is the only residual vector kept.
+// W is normalised to the product of the two s factors (O(1)): the profiles
+// hold 1/(s dL) (1/s on an identity axis) and |scale| = dL^2 for both
+// polarisations, so float vectors stay far from the denormal range.
+template <typename R>
+__device__ __forceinline__ cx<R> sym_weight(const cv_fdfd_desc& d, const cx<R>* __restrict__ prof, int i, int j) {
+    const cx<R>* kxf = prof + d.prof_off;
+    const cx<R>* kxb = kxf + d.nx;
+    const cx<R>* kyf = kxb + d.nx;
+    const cx<R>* kyb = kyf + d.ny;
+    const R dL = R(sqrt(fabs(d.scale)));
+    cx<R> a = (d.pol == CV_POL_EZ) ? kxf[i] * kyf[j] : kxb[i] * kyb[j];
+    if (!(d.flags & CV_FLAG_X_IDENTITY)) a = dL * a;
+    if (!(d.flags & CV_FLAG_Y_IDENTITY)) a = dL * a;
+    return crecip(a);
+}
+
+// mode 0: fresh start from the initial guess x; mode 1: verification of a
+// converged instance with a freshly computed residual (restart when above
+// tolerance).  z = M^-1 r, r = W (b - A x) (A) or b - A^T x (A^T).
+template <typename R, bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_cocg_init(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+            const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
+            const cx<R>* __restrict__ b, const cx<R>* __restrict__ x, cx<R>* __restrict__ z, RedBuf rb,
+            KState* __restrict__ state, double tol, int max_iter, int mode) {
+    __shared__ double lds[kWaves * 4];
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    KState* st = state + t.inst;
+    const int done0 = (mode == 0) ? kRunning : st->done;
+    if (mode == 1 && done0 == kRunning) return;
+    const bool write_z = (mode == 0) || (done0 == kConverged);
+    const int n = d.nx * d.ny;
+    auto fx = [&](int64_t k) { return x[k]; };
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> bk = b[k];
+        const cx<R> rho = bk - apply_any<R, TRANS>(d, prof, m0, m1, i, j, fx, x[k]);
+        const cx<R> w = sym_weight<R>(d, prof, i, j);
+        const cx<R> mii = w * row_coefs<R>(d, prof, m0, m1, i, j).c;
+        const cx<R> r = TRANS ? rho : w * rho;
+        const cx<R> zk = r * crecip(mii);
+        if (write_z) z[k] = zk;
+        const cx<double> rz = ccast<double>(r) * ccast<double>(zk);   // products in FP64
+        acc[0] += rz.re;
+        acc[1] += rz.im;
+        acc[2] += norm2(ccast<double>(rho));
+        acc[3] += norm2(ccast<double>(bk));
+    }
+    block_sum<4>(acc, lds);
+    double tot[4];
+    if (!last_block_reduce<4>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x,
+                              d.tile_first, d.tile_count, lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        st->bb = tot[3];
+        st->tol2 = tol * tol * tot[3];
+        st->max_iter = max_iter;
+        st->iter = 0;
+        st->restarts = 0;
+    }
+    st->rr_true = tot[2];
+    if (mode == 1 && done0 != kConverged) return;  // report only
+    st->rr = tot[2];
+    st->rho = mk(tot[0], tot[1]);
+    st->restart = 1;
+    st->skip = 0;
+    if (!(isfinite(tot[0]) && isfinite(tot[1]) && isfinite(tot[2]))) {
+        st->done = kNonFinite;
+    } else if (tot[2] <= st->tol2) {
+        st->done = kConverged;
+    } else {
+        st->done = kRunning;
+        if (mode == 1) st->restarts += 1;
+        if (st->iter >= st->max_iter) st->done = kMaxIter;
+    }
+}
+
+// C1: p' = z + beta p (z on restart); q = W (A p'); (p', q) -> alpha.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+k_cocg_p(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+         const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
+         const cx<R>* __restrict__ z, const cx<R>* __restrict__ p_old, cx<R>* __restrict__ p_new,
+         cx<R>* __restrict__ qv, RedBuf rb, KState* __restrict__ state) {
+    __shared__ double lds[kWaves * 2];
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    KState* st = state + t.inst;
+    if (st->done != kRunning) return;
+    const bool restart = st->restart != 0;
+    const cx<R> beta = ccast<R>(st->beta);
+    const int n = d.nx * d.ny;
+    auto fp = [&](int64_t k) -> cx<R> {
+        if (restart) return z[k];
+        return z[k] + beta * p_old[k];
+    };
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> pk = fp(k);
+        p_new[k] = pk;
+        const cx<R> qk = sym_weight<R>(d, prof, i, j) * apply_row<R>(d, prof, m0, m1, i, j, fp, pk);
+        qv[k] = qk;
+        const cx<double> pq = ccast<double>(pk) * ccast<double>(qk);
+        acc[0] += pq.re;
+        acc[1] += pq.im;
+    }
+    block_sum<2>(acc, lds);
+    double tot[2];
+    if (!last_block_reduce<2>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x,
+                              d.tile_first, d.tile_count, lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    const cx<double> sigma = mk(tot[0], tot[1]);
+    st->restart = 0;
+    if (!cfinite(sigma)) {
+        st->done = kNonFinite;
+        return;
+    }
+    if (sigma.re == 0.0 && sigma.im == 0.0) {   // (p, B p) = 0: restart with p = z
+        st->skip = 1;
+        st->restart = 1;
+        st->restarts += 1;
+        if (st->restarts > kMaxRestarts) st->done = kBreakdown;
+        return;
+    }
+    st->skip = 0;
+    st->alpha = cdiv(st->rho, sigma);
+}
+
+// C2: x += alpha p' (W p' for A^T); z -= alpha q / M; (r, z), |rho|^2 -> beta.
+template <typename R, bool TRANS>
+__global__ void __launch_bounds__(kBlock)
+k_cocg_x(const cv_fdfd_desc* __restrict__ desc, const cv_tile* __restrict__ tiles,
+         const cx<R>* __restrict__ prof, const R* __restrict__ m0, const R* __restrict__ m1,
+         cx<R>* __restrict__ x, const cx<R>* __restrict__ p, cx<R>* __restrict__ z,
+         const cx<R>* __restrict__ qv, RedBuf rb, KState* __restrict__ state) {
+    __shared__ double lds[kWaves * 3];
+    const cv_tile t = tiles[blockIdx.x];
+    const cv_fdfd_desc d = load_desc(desc, t.inst);
+    KState* st = state + t.inst;
+    if (st->done != kRunning || st->skip) return;
+    const cx<R> alpha = ccast<R>(st->alpha);
+    const int n = d.nx * d.ny;
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+        const int loc = t.first + threadIdx.x + q * kBlock;
+        if (loc >= n) break;
+        const int i = loc / d.ny, j = loc - i * d.ny;
+        const int64_t k = d.cell_off + loc;
+        const cx<R> w = sym_weight<R>(d, prof, i, j);
+        const cx<R> mii = w * row_coefs<R>(d, prof, m0, m1, i, j).c;
+        const cx<R> pk = p[k];
+        x[k] = x[k] + alpha * (TRANS ? w * pk : pk);
+        const cx<R> zk = z[k] - alpha * (qv[k] * crecip(mii));
+        z[k] = zk;
+        const cx<double> r = ccast<double>(mii) * ccast<double>(zk);
+        const cx<double> rz = r * ccast<double>(zk);
+        const cx<double> rho = TRANS ? r : cdiv(r, ccast<double>(w));
+        acc[0] += rz.re;
+        acc[1] += rz.im;
+        acc[2] += norm2(rho);
+    }
+    block_sum<3>(acc, lds);
+    double tot[3];
+    if (!last_block_reduce<3>(acc, rb.partials, rb.gpartials, rb.counters + t.inst, rb.gcount, blockIdx.x,
+                              d.tile_first, d.tile_count, lds, tot))
+        return;
+    if (threadIdx.x != 0) return;
+    const cx<double> rz_new = mk(tot[0], tot[1]);
+    const double rr = tot[2];
+    st->iter += 1;
+    st->rr = rr;
+    if (!(cfinite(rz_new) && isfinite(rr))) {
+        st->done = kNonFinite;
+    } else if (rr <= st->tol2) {
+        st->done = kConverged;
+    } else if (st->iter >= st->max_iter) {
+        st->done = kMaxIter;
+    } else if (rz_new.re == 0.0 && rz_new.im == 0.0) {
+        st->restart = 1;   // (r, z) = 0 without convergence: restart with p = z
+        st->restarts += 1;
+        if (st->restarts > kMaxRestarts) st->done = kBreakdown;
+        st->rho = rz_new;
+    } else {
+        st->beta = cdiv(rz_new, st->rho);
+        st->rho = rz_new;
+    }
+}
+
 // --------------------------------------------------------------- host side
 struct Work {
     void* r;
@@ -529,6 +746,74 @@ static int solve_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, i
     return CV_OK;
 }
 
+// COCG over the batch with the same host protocol as solve_impl (the
+// iterations between host checks captured once into a graph, or launched
+// directly when use_graph is false).  Work: z in w.r, p in w.p[0..1], q in w.v[0].
+template <typename R, bool TRANS>
+static int cocg_impl(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst, const cv_tile* tiles,
+                     int n_tiles, const void* prof_, const void* m0_, const void* m1_, const void* b_, void* x_,
+                     double tol, int max_iter, int check_every, Work& w, KState* hs, hipStream_t s,
+                     bool use_graph = true) {
+    const cx<R>* prof = (const cx<R>*)prof_;
+    const R* m0 = (const R*)m0_;
+    const R* m1 = (const R*)m1_;
+    const cx<R>* b = (const cx<R>*)b_;
+    cx<R>* x = (cx<R>*)x_;
+    cx<R>* z = (cx<R>*)w.r;
+    cx<R>* p[2] = {(cx<R>*)w.p[0], (cx<R>*)w.p[1]};
+    cx<R>* qv = (cx<R>*)w.v[0];
+    k_zero_u32<<<(n_inst + n_tiles + 255) / 256, 256, 0, s>>>(w.rb.counters, n_inst + n_tiles);
+    k_cocg_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, z, w.rb, w.state, tol,
+                                                     max_iter, 0);
+    CV_LAUNCH_CHECK();
+    auto body = [&](hipStream_t q) {
+        int cur = 0;
+        for (int c = 0; c < check_every; ++c) {
+            const int nxt = cur ^ 1;
+            k_cocg_p<R><<<n_tiles, kBlock, 0, q>>>(desc, tiles, prof, m0, m1, z, p[cur], p[nxt], qv, w.rb, w.state);
+            k_cocg_x<R, TRANS><<<n_tiles, kBlock, 0, q>>>(desc, tiles, prof, m0, m1, x, p[nxt], z, qv, w.rb,
+                                                          w.state);
+            cur = nxt;
+        }
+    };
+    LoopGraph graph;
+    if (use_graph) CV_HIP(graph.capture(body));
+    int checks = 0;
+    while (true) {
+        if (use_graph) {
+            CV_HIP(hipGraphLaunch(graph.exec, s));
+        } else {
+            body(s);
+            CV_LAUNCH_CHECK();
+        }
+        CV_HIP(hipMemcpyAsync(hs, w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
+        CV_HIP(hipStreamSynchronize(s));
+        bool all_done = true;
+        for (int k = 0; k < n_inst; ++k) all_done &= (hs[k].done != kRunning);
+        if (!all_done) continue;
+        k_cocg_init<R, TRANS><<<n_tiles, kBlock, 0, s>>>(desc, tiles, prof, m0, m1, b, x, z, w.rb, w.state, tol,
+                                                         max_iter, 1);
+        CV_LAUNCH_CHECK();
+        CV_HIP(hipMemcpyAsync(hs, w.state, sizeof(KState) * n_inst, hipMemcpyDeviceToHost, s));
+        CV_HIP(hipStreamSynchronize(s));
+        bool again = false;
+        for (int k = 0; k < n_inst; ++k) again |= (hs[k].done == kRunning);
+        if (!again || ++checks >= 8) break;
+    }
+    return CV_OK;
+}
+
+// COCG applies to instances with zero Bloch phases and no coupled blocks.
+static int check_cocg(const cv_fdfd_desc* dh, int n_inst) {
+    for (int k = 0; k < n_inst; ++k) {
+        const cv_fdfd_desc& d = dh[k];
+        CV_CHECK(d.nblk == 1 && d.n_cpl == 0, "instance %d: COCG does not take frequency-coupled systems", k);
+        CV_CHECK(d.phx_re == 1.0 && d.phx_im == 0.0 && d.phy_re == 1.0 && d.phy_im == 0.0,
+                 "instance %d: COCG needs zero Bloch phases (the operator is complex symmetric only then)", k);
+    }
+    return CV_OK;
+}
+
 static int status_of(const KState& k) {
     switch (k.done) {
         case kConverged: return CV_OK;
@@ -593,6 +878,24 @@ extern "C" int64_t cv_fdfd_solve_workspace_bytes(int dtype, int64_t total_cells,
     return layout(cs, total_cells, n_inst, n_tiles, nullptr, nullptr);
 }
 
+// Dispatch of one batched Krylov solve on (method, precision, orientation).
+static int run_krylov(int method, int dtype, int transpose, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host,
+                      int n_inst, const cv_tile* tiles, int n_tiles, const void* prof, const void* m0, const void* m1,
+                      const void* b, void* x, double tol, int max_iter, int check_every, Work& w, KState* hs,
+                      hipStream_t s, bool use_graph) {
+#define CV_KRYLOV_ARGS desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol, max_iter, check_every, w, hs, \
+                       s, use_graph
+    if (method == CV_KRYLOV_COCG) {
+        if (dtype == CV_C128)
+            return transpose ? cocg_impl<double, true>(CV_KRYLOV_ARGS) : cocg_impl<double, false>(CV_KRYLOV_ARGS);
+        return transpose ? cocg_impl<float, true>(CV_KRYLOV_ARGS) : cocg_impl<float, false>(CV_KRYLOV_ARGS);
+    }
+    if (dtype == CV_C128)
+        return transpose ? solve_impl<double, true>(CV_KRYLOV_ARGS) : solve_impl<double, false>(CV_KRYLOV_ARGS);
+    return transpose ? solve_impl<float, true>(CV_KRYLOV_ARGS) : solve_impl<float, false>(CV_KRYLOV_ARGS);
+#undef CV_KRYLOV_ARGS
+}
+
 // Shared by cv_fdfd_solve and cv_fdfd_solve_noalloc: checks the arguments,
 // lays the workspace out at `workspace`, runs the batched BiCGSTAB and fills
 // `info` from the host copy `hs` of the n_inst solver states.
@@ -600,10 +903,15 @@ static int solve_entry(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* 
                        const cv_tile* tiles, int n_tiles, const void* prof, const void* m0, const void* m1,
                        const void* b, void* x, int transpose, double tol, int max_iter, int check_every,
                        void* workspace, int64_t workspace_bytes, cv_solve_info* info, KState* hs, bool use_graph,
-                       hipStream_t s) {
+                       hipStream_t s, int method = CV_KRYLOV_BICGSTAB) {
     int64_t cells = 0;
     {
         const int rc = check_batch(desc_host, n_inst, n_tiles, &cells);
+        if (rc != CV_OK) return rc;
+    }
+    CV_CHECK(method == CV_KRYLOV_BICGSTAB || method == CV_KRYLOV_COCG, "unknown Krylov method %d", method);
+    if (method == CV_KRYLOV_COCG) {
+        const int rc = check_cocg(desc_host, n_inst);
         if (rc != CV_OK) return rc;
     }
     for (int k = 0; k < n_inst; ++k)
@@ -615,17 +923,8 @@ static int solve_entry(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* 
              (long long)need);
     if (check_every <= 0) check_every = 16;
     check_every += check_every & 1;
-    int rc;
-    if (dtype == CV_C128)
-        rc = transpose ? solve_impl<double, true>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
-                                                  max_iter, check_every, w, hs, s, use_graph)
-                       : solve_impl<double, false>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x,
-                                                   tol, max_iter, check_every, w, hs, s, use_graph);
-    else
-        rc = transpose ? solve_impl<float, true>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
-                                                 max_iter, check_every, w, hs, s, use_graph)
-                       : solve_impl<float, false>(desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, tol,
-                                                  max_iter, check_every, w, hs, s, use_graph);
+    const int rc = run_krylov(method, dtype, transpose, desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x,
+                              tol, max_iter, check_every, w, hs, s, use_graph);
     if (rc != CV_OK) return rc;
     int worst;
     fill_info(desc_host, hs, info, n_inst, &worst);
@@ -643,6 +942,20 @@ extern "C" int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_
     std::vector<KState> hs(n_inst);
     return solve_entry(dtype, desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, transpose, tol, max_iter,
                        check_every, workspace, workspace_bytes, info, hs.data(), true, as_stream(stream));
+}
+
+extern "C" int cv_fdfd_cocg(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                            const cv_tile* tiles, int n_tiles, const void* prof, const void* m0, const void* m1,
+                            const void* b, void* x, int transpose, double tol, int max_iter, int check_every,
+                            void* workspace, int64_t workspace_bytes, cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && tiles && prof && m0 && b && x && workspace && info, "null pointer argument");
+    CV_CHECK(n_inst > 0 && n_tiles > 0, "empty batch");
+    CV_CHECK(dtype == CV_C64 || dtype == CV_C128, "unknown dtype %d", dtype);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    std::vector<KState> hs(n_inst);
+    return solve_entry(dtype, desc, desc_host, n_inst, tiles, n_tiles, prof, m0, m1, b, x, transpose, tol, max_iter,
+                       check_every, workspace, workspace_bytes, info, hs.data(), true, as_stream(stream),
+                       CV_KRYLOV_COCG);
 }
 
 // ------------------------------------------------ allocation-free solve
@@ -724,7 +1037,7 @@ extern "C" int64_t cv_fdfd_solve_mixed_workspace_bytes(int64_t total_cells, int 
 extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
                                    const cv_tile* tiles, int n_tiles, const void* prof64, const void* m0_64,
                                    const void* m1_64, const void* prof32, const void* m0_32, const void* m1_32,
-                                   const void* b64, void* x64, int transpose, double tol, double inner_tol,
+                                   const void* b64, void* x64, int transpose, int method, double tol, double inner_tol,
                                    int max_iter, int max_sweeps, int check_every, void* workspace,
                                    int64_t workspace_bytes, cv_solve_info* info, void* stream) {
     CV_CHECK(desc && desc_host && tiles && prof64 && m0_64 && prof32 && m0_32 && b64 && x64 && workspace && info,
@@ -738,6 +1051,11 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
     const int64_t need = cv_fdfd_solve_mixed_workspace_bytes(cells, n_inst, n_tiles);
     CV_CHECK(workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
              (long long)need);
+    CV_CHECK(method == CV_KRYLOV_BICGSTAB || method == CV_KRYLOV_COCG, "unknown Krylov method %d", method);
+    if (method == CV_KRYLOV_COCG) {
+        const int rc = check_cocg(desc_host, n_inst);
+        if (rc != CV_OK) return rc;
+    }
     if (check_every <= 0) check_every = 16;
     check_every += check_every & 1;
     if (max_sweeps <= 0) max_sweeps = 10;
@@ -783,12 +1101,8 @@ extern "C" int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc*
         CV_LAUNCH_CHECK();
         // Inner solve over all tiles: instances already converged get a zero rhs
         // (b32 left from their last sweep is irrelevant: their x64 is not updated).
-        if (transpose)
-            rc = solve_impl<float, true>(desc, desc_host, n_inst, tiles, n_tiles, prof32, m0_32, m1_32, b32, x32,
-                                         inner_tol, max_iter, check_every, w, hi.data(), s);
-        else
-            rc = solve_impl<float, false>(desc, desc_host, n_inst, tiles, n_tiles, prof32, m0_32, m1_32, b32, x32,
-                                          inner_tol, max_iter, check_every, w, hi.data(), s);
+        rc = run_krylov(method, CV_C64, transpose, desc, desc_host, n_inst, tiles, n_tiles, prof32, m0_32, m1_32,
+                        b32, x32, inner_tol, max_iter, check_every, w, hi.data(), s, true);
         if (rc != CV_OK) return rc;
         for (int k = 0; k < n_inst; ++k) {
             const int sys = k - desc_host[k].blk;

@@ -257,6 +257,21 @@ class Problem:
             check(rc, 'bicgstab')
         return x, dict(iterations=info[0].iterations, rel_residual=info[0].rel_residual, status=info[0].status)
 
+    def cocg(self, bs, tol=1e-6, max_iter=400000, x0=None, check_every=32):
+        """Plain Jacobi-preconditioned COCG on the complex-symmetric form
+        (complex128) -> (x, info); no exception on failure."""
+        op = self.op64
+        x = torch.zeros_like(bs) if x0 is None else x0.clone()
+        info = (_lib.SolveInfo * 1)()
+        ws = op._workspace(False)
+        rc = lib().cv_fdfd_cocg(op.code, ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), 1,
+                                ptr(op.tiles), op.n_tiles, ptr(op.prof), ptr(self.m0_64), ptr(self.m1_64), ptr(bs),
+                                ptr(x), 0, float(tol), int(max_iter), int(check_every), ptr(ws), ws.numel(), info,
+                                stream_ptr())
+        if rc not in (_lib.CV_OK, _lib.CV_ERR_NOT_CONVERGED):
+            check(rc, 'cocg')
+        return x, dict(iterations=info[0].iterations, rel_residual=info[0].rel_residual, status=info[0].status)
+
     def fgmres(self, bs, tol=1e-6, restart=30, max_iter=100000, inner_rtol=1e-3, precond='jacobi', net=None,
                gain=1.0, x0=None):
         """Flexible GMRES (complex64 Arnoldi, complex128 iterate and FP64 true

@@ -180,6 +180,7 @@ int cv_fdfd_coupling_grad(int dtype, const cv_fdfd_desc* desc, const cv_tile* ti
 
 /* ------------------------------------------------------------- Krylov solve */
 #define CV_KRYLOV_BICGSTAB 0
+#define CV_KRYLOV_COCG     1
 
 /* Per-instance solver telemetry written back to a host array. */
 typedef struct cv_solve_info {
@@ -211,6 +212,23 @@ int cv_fdfd_solve(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_
                   void* workspace, int64_t workspace_bytes,
                   cv_solve_info* info, void* stream);
 
+/* The same batched solve by Jacobi-preconditioned COCG (conjugate orthogonal
+ * CG) on the complex-symmetric form W A of the operator, W = 1/(kxf kyf)
+ * per cell for Ez and 1/(kxb kyb) for Hz (the forward / backward SC-PML
+ * factors, derivatives.py:34-59): one stencil pass and two fused kernels per
+ * iteration, FP64 reductions, convergence on the true FP64 residual of the
+ * original system.  A^T solves run as W A u = b, x = W u.  Needs zero Bloch
+ * phases and uncoupled instances (CV_ERR_INVALID otherwise).  Replaces the
+ * same reference solve as cv_fdfd_solve (solvers.py:37-75), the Krylov
+ * family being scipy's (solvers.py:20-30). */
+int cv_fdfd_cocg(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
+                 const cv_tile* tiles, int n_tiles,
+                 const void* prof, const void* m0, const void* m1,
+                 const void* b, void* x, int transpose,
+                 double tol, int max_iter, int check_every,
+                 void* workspace, int64_t workspace_bytes,
+                 cv_solve_info* info, void* stream);
+
 /* Allocation-free variant of cv_fdfd_solve.  No counterpart in the reference:
  * it is for callers that cannot allocate (code under HIP graph capture, tight
  * serving loops, harnesses that forbid allocation).  It calls no hipMalloc /
@@ -238,13 +256,14 @@ int cv_fdfd_solve_noalloc(int dtype, const cv_fdfd_desc* desc, const cv_fdfd_des
                           cv_solve_info* info, void* stream);
 
 /* Mixed precision: complex128 x/b/operator (prof64, m0_64, m1_64) for the
- * residual, complex64 copies (prof32, m0_32, m1_32) for the inner BiCGSTAB. */
+ * residual, complex64 copies (prof32, m0_32, m1_32) for the inner Krylov
+ * solve, method CV_KRYLOV_BICGSTAB or CV_KRYLOV_COCG. */
 int64_t cv_fdfd_solve_mixed_workspace_bytes(int64_t total_cells, int n_inst, int n_tiles);
 int cv_fdfd_solve_mixed(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, int n_inst,
                         const cv_tile* tiles, int n_tiles,
                         const void* prof64, const void* m0_64, const void* m1_64,
                         const void* prof32, const void* m0_32, const void* m1_32,
-                        const void* b64, void* x64, int transpose,
+                        const void* b64, void* x64, int transpose, int method,
                         double tol, double inner_tol, int max_iter, int max_sweeps, int check_every,
                         void* workspace, int64_t workspace_bytes,
                         cv_solve_info* info, void* stream);

@@ -209,10 +209,10 @@ def test_direct_method_requires_a_narrow_grid():
 
 def test_direct_failure_falls_back_to_bicgstab():
     """The direct path forced to fail from the host (its FGMRES limited to one
-    step, far from the 1e-12 tolerance): under method='auto' the solve, its
-    gradient and its forward-mode tangent fall back to BiCGSTAB with a
-    warning and still match spsolve; under method='direct' the failure
-    raises."""
+    step, far from the 1e-12 tolerance): under method='auto' the solve and its
+    gradient fall back to the Jacobi-preconditioned Krylov method (BiCGSTAB)
+    with a warning and still match spsolve; under
+    method='direct' the failure raises."""
     import warnings
     import ceviche_amd as cv
     from ceviche_amd._lib import NotConvergedError
@@ -227,7 +227,7 @@ def test_direct_failure_falls_back_to_bicgstab():
         info = F.last_solve_info
         (Ez[90, 60].abs() ** 2).backward()
     assert any('falling back' in str(x.message) for x in w)
-    assert info.get('method', 'bicgstab') == 'bicgstab' and info['rel_residual'] <= 1e-12
+    assert info['method'] == 'bicgstab' and info['rel_residual'] <= 1e-12
     hx, hy, ez = R.solve_ez(c['omega'], c['dL'], c['eps'], c['npml'], c['src'])
     assert np.linalg.norm(Ez.detach().cpu().numpy() - ez) / np.linalg.norm(ez) <= 1e-6
     cot = np.zeros_like(ez)

@@ -4,6 +4,7 @@ import os
 
 import numpy as np
 import pytest
+import scipy.sparse.linalg as spla
 import torch
 
 from oracle import fdfd_ref as R
@@ -62,7 +63,7 @@ def test_derivative_operators(which, mode):
     assert np.linalg.norm(out - ref) <= 1e-14 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize('method', ['auto', 'bicgstab'])
+@pytest.mark.parametrize('method', ['auto', 'bicgstab', 'cocg'])
 def test_cfg1_ez_fields_match_spsolve(method):
     import ceviche_amd as cv
     c = cfg1()
@@ -70,7 +71,7 @@ def test_cfg1_ez_fields_match_spsolve(method):
     Hx, Hy, Ez = F.solve(c['src'])
     info = F.last_solve_info
     print('cfg1 info', method, info, flush=True)
-    assert info.get('method', 'bicgstab') == ('direct' if method == 'auto' else 'bicgstab')
+    assert info['method'] == ('direct' if method == 'auto' else method)
     hx, hy, ez = R.solve_ez(c['omega'], c['dL'], c['eps'], c['npml'], c['src'])
     for a, b in ((Ez, ez), (Hx, hx), (Hy, hy)):
         assert isinstance(a, np.ndarray) and a.dtype == np.complex128 and a.shape == (120, 120)
@@ -78,7 +79,7 @@ def test_cfg1_ez_fields_match_spsolve(method):
     assert info['rel_residual'] <= 1e-10
 
 
-@pytest.mark.parametrize('method', ['auto', 'bicgstab'])
+@pytest.mark.parametrize('method', ['auto', 'bicgstab', 'cocg'])
 def test_cfg1_ez_gradient_matches_oracle_adjoint(method):
     import ceviche_amd as cv
     c = cfg1()
@@ -250,3 +251,44 @@ def test_reverse_jacobian_matches_reference_convention():
     g = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'waveguide_box_120.npz'))['grad_eps']
     assert jr.shape == (1, 120 * 120)
     assert np.linalg.norm(jr[0] - g.ravel()) <= 1e-4 * np.linalg.norm(g)
+
+
+@pytest.mark.parametrize('pol', ['ez', 'hz'])
+@pytest.mark.parametrize('dtype', [torch.complex128, torch.complex64])
+def test_cocg_forward_and_transposed_solves_match_spsolve(pol, dtype):
+    """COCG on W A (W = 1/(kxf kyf) for Ez, 1/(kxb kyb) for Hz) for A x = b and
+    A^T y = b on a batch of two uneven instances with PML, against scipy."""
+    from ceviche_amd._fdfd_op import FdfdOperator, SolverOptions
+    rng = np.random.default_rng(12)
+    specs, epss = [], []
+    for shape, lam, npml in [((70, 56), 1.5e-6, (8, 6)), ((48, 90), 1.6e-6, (6, 10))]:
+        specs.append(dict(omega=2 * np.pi * R.C_0 / lam, dL=40e-9, shape=shape, npml=npml))
+        epss.append(rng.uniform(1, 8, shape))
+    op = FdfdOperator(pol, specs, dtype=dtype)
+    assert op.cocg_applies()
+    eps = torch.from_numpy(np.concatenate([e.ravel() for e in epss])).cuda()
+    m0, m1 = op.materials(eps)
+    b = rng.standard_normal(op.total_cells) + 1j * rng.standard_normal(op.total_cells)
+    tol = 1e-10 if dtype == torch.complex128 else 1e-5
+    for tr in (False, True):
+        bs = torch.from_numpy(b).cuda().to(dtype) * op.scale_vector(dtype)
+        x = op.solve_scaled(bs, m0, m1, transpose=tr, opts=SolverOptions(tol=tol, method='cocg'))
+        x = x.cpu().numpy().astype(np.complex128)
+        for k, (sp, e) in enumerate(zip(specs, epss)):
+            A = (R.make_A_ez if pol == 'ez' else R.make_A_hz)(sp['omega'], sp['dL'], e, sp['npml'])
+            A = (A.T if tr else A).tocsc()
+            sl = slice(op.offsets[k], op.offsets[k] + e.size)
+            ref = spla.spsolve(A, b[sl])
+            assert op.last_info[k]['method'] == 'cocg'
+            assert op.last_info[k]['rel_residual'] <= tol
+            assert np.linalg.norm(x[sl] - ref) / np.linalg.norm(ref) <= (1e-7 if dtype == torch.complex128 else 5e-3)
+
+
+def test_cocg_is_refused_with_bloch_phases():
+    from ceviche_amd._fdfd_op import SolverOptions
+    op, _ = _op('ez', (20, 16), (3, 3), bloch=(0.3, 0.0))
+    assert not op.cocg_applies()
+    assert op.krylov_method(SolverOptions()) == 'bicgstab'
+    assert op.krylov_method(SolverOptions(method='auto')) == 'bicgstab'
+    with pytest.raises(ValueError):
+        op.krylov_method(SolverOptions(method='cocg'))
