@@ -1,17 +1,21 @@
 #!/usr/bin/env python
 """Benchmark of the MI355X FDTD / FDFD hot path.  Prints ONE JSON line.
 
-Main line (``value``): 2D TM FDTD, 400x400 cells, dL 20 nm, npml 20, binary
-blob permittivity, Gaussian-pulse Jz point source, 2000 time steps, 4 Ez
-probes recorded every step, float32 -- one "step" of this benchmark is one
-full 2000-time-step run from zero fields with inputs resident on the GPU.
-value = cells * time steps / second (Gcell/s).
+Main line (``value``): 3D FDTD, 256^3 cells, float32 (the precision the 3D
+workload is specified in), dL 25 nm, npml 10, binary blobs in a 2.1-index
+background, Gaussian-pulse z-dipole, 1000 time steps recording the monitor
+plane's Ex, Ey, Hx, Hy every step -- one "step" of this benchmark is one full
+1000-time-step run from zero fields with inputs resident on the GPU.
+value = cells * time steps / second (Gcell/s).  The 256^3 state (1.2 GB)
+does not fit the 256 MB last-level cache, so this is the HBM-bound FDTD case.
 
-roofline: the dominant kernel (the temporally blocked TM kernel) timed with
-HIP events on the stream it is launched on; algorithmic bytes per cell-step
-are (2 * 3 fields + eps) * 4 B = 28 B.  Extra measured fields are reported
-under "extras" (float64 error against the oracle's golden trace, adjoint
-gradient time, an HBM-sized 2D run, a 3D run, FDFD solves/s, a copy probe).
+roofline: one Yee step (the H and D phase kernels) timed with HIP events on
+the stream they are launched on; algorithmic bytes per cell-step are
+(2 * 6 fields + eps) * 4 B = 52 B.  Extra measured fields are reported under
+"extras": the 2D TM workload in float32 and float64 with its own rooflines
+and golden-trace errors, adjoint gradients, an HBM-sized 2D run, FDFD
+solves/s (120^2 direct path, 512^2 / 1024^2 held-out blob structures plain
+vs learned, the multi-frequency isolator, the batch), a copy probe.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--extras all|none|name,...]
 """
@@ -37,7 +41,76 @@ def _sync():
     torch.cuda.synchronize()
 
 
-def fdtd_tm_main(steps, warmup):
+def fdtd_3d_main(steps, warmup):
+    """256^3 float32 FDTD (the 3D workload: dL 25 nm, npml 10, binary blobs in a
+    2.1-index background, z-dipole Gaussian pulse), 1000 time steps recording
+    Ex, Ey, Hx, Hy on the 64 x 64 monitor plane every step; one step of the
+    benchmark is one full run from zero fields (flux formed from the records),
+    inputs resident on the GPU."""
+    import numpy as np
+    import torch
+    import ceviche_amd as cv
+    from ceviche_amd import workloads
+    N, T = 256, 1000
+    c = workloads.fdtd_3d_blobs(N, T)
+    F = cv.fdtd(torch.tensor(c['eps'], dtype=torch.float32), c['dL'], c['npml'], precision='float32')
+    amp = torch.tensor(c['amp']).cuda()
+    npl = len(c['probes']) // 4
+
+    def one_run():
+        F.initialize_fields()
+        r = F.run(T, c['src'], amp, c['probes']).double()
+        return (r[:, :npl] * r[:, 3 * npl:] - r[:, npl:2 * npl] * r[:, 2 * npl:3 * npl]).sum()
+
+    flux = one_run()
+    for _ in range(warmup):
+        one_run()
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        flux = one_run()
+    _sync()
+    dt = time.perf_counter() - t0
+    cells = N ** 3
+    gcells = cells * T * steps / dt / 1e9
+    # roofline: the H and D step kernels alone (cv_fdtd_step, no source or
+    # probes), HIP events on the launch stream; 52 B per cell-step
+    g = F._grid
+    st = g.new_state()
+    g.step(st, F._eps_dev, n_steps=2)
+    reps = 40
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    _sync()
+    e0.record()
+    g.step(st, F._eps_dev, n_steps=reps)
+    e1.record()
+    e1.synchronize()
+    step_ms = e0.elapsed_time(e1) / reps
+    bytes_per_step = 52.0 * cells
+    achieved = bytes_per_step / (step_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic_3d(),
+            "kernel": "k_h<float> + k_d<float> (one Yee step)", "avg_step_us": round(step_ms * 1e3, 2),
+            "algorithmic_bytes_per_launch": bytes_per_step,
+            "note": "52 B/cell-step ((2 x 6 fields + eps) x 4 B) x 256^3 cells per step, the H and D phase kernels "
+                    "of one step timed together; traffic = FETCH_SIZE + WRITE_SIZE of both kernels per step from "
+                    "profiles/fdtd_3d_256_traffic.md"}
+    return dict(value=gcells, ms=1e3 * dt / steps, flux=float(flux), roofline=roof)
+
+
+def committed_traffic_3d():
+    """HBM bytes of one 256^3 step (k_h + k_d) from the committed counter summary."""
+    path = os.path.join(ROOT, 'profiles', 'fdtd_3d_256_traffic.md')
+    try:
+        for line in open(path):
+            if line.startswith('traffic_bytes_per_step'):
+                return float(line.split('=')[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def fdtd_tm_main(steps, warmup, precision='float32'):
     import numpy as np
     import torch
     import ceviche_amd as cv
@@ -45,8 +118,10 @@ def fdtd_tm_main(steps, warmup):
     from ceviche_amd._fdtd_op import tm_configs
     c = workloads.fdtd_tm_blobs(400, 2000)
     eps = torch.tensor(c['eps'])
-    F = cv.fdtd(eps, c['dL'], c['npml'], precision='float32')
-    amp = torch.tensor(c['amp'])
+    F = cv.fdtd(eps, c['dL'], c['npml'], precision=precision)
+    amp = torch.tensor(c['amp']).cuda()
+    fb = 4 if precision == 'float32' else 8
+    tname = 'float' if precision == 'float32' else 'double'
 
     def one_run():
         F.initialize_fields()
@@ -68,7 +143,7 @@ def fdtd_tm_main(steps, warmup):
     from ceviche_amd._fdtd_op import autotune_tm
     cfg = autotune_tm(F._grid, F._eps_dev)
     wv, rpt, ts = tm_configs()[cfg]
-    tag = "k_tm_block<float,%d,%d,%d>" % (wv, rpt, ts)
+    tag = "k_tm_block<%s,%d,%d,%d>" % (tname, wv, rpt, ts)
     n_launch = (c['steps'] + ts - 1) // ts
     F.initialize_fields()
     amp_dev = amp.to(F._grid.device)
@@ -77,7 +152,7 @@ def fdtd_tm_main(steps, warmup):
     pc, pl = _probe_tensors(F._grid, c['probes'])
     si, sj, sk = c['src']
     src_cell = (si * 400 + sj) * 1 + sk
-    recb = torch.empty((c['steps'], 4), dtype=torch.float32, device=F._grid.device)
+    recb = torch.empty((c['steps'], 4), dtype=F._grid.dtype, device=F._grid.device)
     reps = 5
     _sync()
     e0.record()
@@ -86,15 +161,15 @@ def fdtd_tm_main(steps, warmup):
     e1.record()
     e1.synchronize()
     avg_launch_ms = e0.elapsed_time(e1) / (reps * n_launch)
-    bytes_per_launch = 28.0 * cells * ts
+    bytes_per_launch = 7.0 * fb * cells * ts
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": committed_traffic(tag),
             "kernel": tag, "avg_launch_us": round(avg_launch_ms * 1e3, 2),
             "algorithmic_bytes_per_launch": bytes_per_launch,
-            "note": "28 B/cell-step x 160000 cells x %d steps per launch; the 400x400 state (2.6 MB) is "
-                    "cache-resident, so the fraction counts compulsory bytes, not HBM traffic" % ts}
+            "note": "%d B/cell-step x 160000 cells x %d steps per launch; the 400x400 state is cache-resident, so "
+                    "the fraction counts compulsory bytes, not HBM traffic" % (7 * fb, ts)}
     return dict(value=gcells, ms=ms, rec=rec, roofline=roof, cfg=(wv, rpt, ts))
 
 
@@ -112,8 +187,38 @@ def committed_traffic(kernel_tag):
 
 
 def cpu_baseline_fdtd(seconds=12.0):
-    """The oracle (numpy restatement of fdtd.py, float64, one core) on the same
-    400x400 problem for as many steps as fit in ~``seconds``."""
+    """The oracle (numpy restatement of ceviche's fdtd.forward, float64 like the
+    reference, one core) on the same 256^3 problem for as many time steps as
+    fit in about ``seconds`` (the reference's per-step cost does not change
+    with the step index)."""
+    import numpy as np
+    from ceviche_amd import workloads
+    from oracle import fdtd_ref
+    try:
+        from threadpoolctl import threadpool_limits
+        limit = threadpool_limits(1)
+    except Exception:  # pragma: no cover
+        limit = None
+    N = 256
+    c = workloads.fdtd_3d_blobs(N, 1000)
+    F = fdtd_ref.FDTD(c['eps'], c['dL'], c['npml'])
+    J = np.zeros(F.shape)
+    t0 = time.perf_counter()
+    n = 0
+    while (time.perf_counter() - t0 < seconds or n < 1) and n < 1000:
+        J[c['src']] = c['amp'][n]
+        F.forward(Jz=J)
+        n += 1
+    dt = time.perf_counter() - t0
+    del F
+    return {"value": round(N ** 3 * n / dt / 1e9, 6), "unit": "Gcell/s", "cores": 1,
+            "host_cores": len(os.sched_getaffinity(0)), "kind": "port",
+            "sample": "256^3 float64 numpy restatement of ceviche fdtd.forward, the first %d of the 1000 time "
+                      "steps (%.1f s, one thread)" % (n, dt)}
+
+
+def cpu_baseline_fdtd_tm(seconds=8.0):
+    """The oracle on the 400x400 TM workload (float64, one core)."""
     import numpy as np
     from ceviche_amd import workloads
     from oracle import fdtd_ref
@@ -127,9 +232,9 @@ def cpu_baseline_fdtd(seconds=12.0):
         F.forward(Jz=J)
         n += 1
     dt = time.perf_counter() - t0
-    return {"value": round(400 * 400 * n / dt / 1e9, 6), "unit": "Gcell/s", "cores": 1, "kind": "port",
-            "sample": "400x400x1 float64 numpy restatement of ceviche fdtd.forward, first %d of 2000 steps "
-                      "(%.1f s)" % (n, dt)}
+    return {"value": round(400 * 400 * n / dt / 1e9, 6), "unit": "Gcell/s", "cores": 1,
+            "host_cores": len(os.sched_getaffinity(0)), "kind": "port",
+            "sample": "400x400x1 float64 numpy restatement, first %d of 2000 steps (%.1f s)" % (n, dt)}
 
 
 def extras_all(main, which='all'):
@@ -150,18 +255,21 @@ def extras_all(main, which='all'):
 
     gold = os.path.join(ROOT, 'tests', 'golden', 'tm_blobs_400_trace.npy')
 
-    def trace_errors():
+    def tm_400():
+        """The 2D TM workload (400x400, 2000 steps, 4 Ez probes) in float32 and
+        float64: Gcell/s of the run (median of 5 after tuning), the roofline of
+        the temporally blocked kernel, and the probe traces against the
+        committed float64 golden trace."""
         ref = np.load(gold)
-        c = workloads.fdtd_tm_blobs(400, 2000)
-        F = cv.fdtd(torch.tensor(c['eps']), c['dL'], c['npml'], precision='float64')
-        _sync()
-        t0 = time.perf_counter()
-        r64 = F.run(c['steps'], c['src'], torch.tensor(c['amp']), c['probes']).cpu().numpy()
-        t64 = time.perf_counter() - t0
-        r32 = main['rec'].double().cpu().numpy()
         rel = lambda a: float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
-        return {"fp64_trace_rel_err": rel(r64), "fp32_trace_rel_err": rel(r32),
-                "fp64_gcells": round(400 * 400 * 2000 / t64 / 1e9, 2)}
+        res = {}
+        for prec in ('float32', 'float64'):
+            r = fdtd_tm_main(5, 1, prec)
+            res[prec] = {"gcells": round(r['value'], 2), "ms_per_run": round(r['ms'], 3),
+                         "trace_rel_err_vs_golden": rel(r['rec'].double().cpu().numpy()),
+                         "roofline": r['roofline'], "tile": list(r['cfg'])}
+        res["cpu_baseline"] = cpu_baseline_fdtd_tm(4.0)
+        return res
 
     def adjoint():
         c = workloads.fdtd_tm_blobs(400, 2000)
@@ -584,7 +692,7 @@ def extras_all(main, which='all'):
         gbs = 2 * 4 * n * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
         return {"copy_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
 
-    table = [('fdtd_tm_400_trace', trace_errors), ('fdtd_tm_400_adjoint', adjoint),
+    table = [('fdtd_tm_400', tm_400), ('fdtd_tm_400_adjoint', adjoint),
              ('fdtd_tm_400_weighted_probes', weighted_probes), ('fdtd_tm_8192', tm_hbm),
              ('fdtd_3d_256_adjoint', adjoint_3d),
              ('fdtd_3d_256', fdtd_3d), ('fdfd_ez_120', fdfd_cfg1), ('fdfd_ez_1024_plain', fdfd_1024),
@@ -613,8 +721,8 @@ def main():
     if not torch.cuda.is_available():
         print('bench.py needs a GPU', file=sys.stderr)
         sys.exit(1)
-    main_res = fdtd_tm_main(args.steps, args.warmup)
-    log('main: %.2f Gcell/s, %.3f ms per 2000-step run' % (main_res['value'], main_res['ms']))
+    main_res = fdtd_3d_main(args.steps, args.warmup)
+    log('main: %.2f Gcell/s, %.1f ms per 1000-step run' % (main_res['value'], main_res['ms']))
     which = 'all' if args.extras == 'all' else set(args.extras.split(','))
     extras = extras_all(main_res, which) if args.extras != 'none' else {}
     cpu = cpu_baseline_fdtd(args.cpu_seconds)
@@ -625,16 +733,17 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(main_res['ms'], 4),
+        "ms_per_step": round(main_res['ms'], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "2D TM FDTD 400x400x1, dL 20 nm, npml 20, binary-blob eps {1, 12.25}, "
-                               "Gaussian-pulse Jz point source, 2000 time steps, 4 Ez probes per step",
-                   "grid": [400, 400, 1], "time_steps": 2000, "precision": "float32",
-                   "tile_waves_rows_per_thread_steps_per_launch": list(main_res['cfg'])},
+        "config": {"workload": "3D FDTD 256x256x256, dL 25 nm, npml 10, background eps 4.41 with a centred 64^3 "
+                               "design region of binary blobs {1, 12.25}, Gaussian-pulse z-dipole, 1000 time steps "
+                               "recording Ex, Ey, Hx, Hy on the 64x64 monitor plane (flux) every step",
+                   "grid": [256, 256, 256], "time_steps": 1000, "precision": "float32",
+                   "kernels": "two-phase H / D step kernels"},
         "roofline": main_res['roofline'],
         "cpu_baseline": cpu,
         "extras": extras,
