@@ -45,7 +45,8 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + '.tmp'
-    cmd = [HIPCC, '--offload-arch=' + ARCH, '-shared', '-fPIC', '-o', tmp] + objs
+    # rocBLAS / rocSOLVER: batched dense LU and products of the nested-dissection factor
+    cmd = [HIPCC, '--offload-arch=' + ARCH, '-shared', '-fPIC', '-o', tmp] + objs + ['-lrocsolver', '-lrocblas']
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError('link failed:\n%s\n%s' % (' '.join(cmd), p.stderr))

@@ -29,10 +29,13 @@ _CODE_OF = {torch.complex128: _lib.CV_C128, torch.complex64: _lib.CV_C64}
 class SolverOptions:
     """Solver settings.  tol is the relative FP64 residual ||b - A x|| / ||b||.
 
-    method: 'auto' (the block-tridiagonal factorisation + FGMRES for a single
-    instance with ny <= 128, or nx <= 128 on the transposed grid, BiCGSTAB
-    otherwise), 'direct' (require the former), 'bicgstab', or 'cocg' (COCG
-    on the complex-symmetric form: zero Bloch phases, uncoupled instances).  restart: FGMRES
+    method: 'auto' (a direct factorisation + FP64-refined FGMRES for a
+    single uncoupled instance: block cyclic reduction when ny <= 128, or
+    nx <= 128 on the transposed grid, with real materials, nested dissection
+    otherwise; BiCGSTAB for batches and coupled systems or when the factor
+    does not fit in half the free device memory), 'direct' (require a
+    factorisation), 'bicgstab', or 'cocg' (COCG on the complex-symmetric
+    form: zero Bloch phases, uncoupled instances).  restart: FGMRES
     cycle length of the direct path; direct_max_iter: its FGMRES step limit
     (max_iter when None).  Under 'auto' a direct solve that fails (singular
     factor, FGMRES not converged, out of memory) falls back to the Krylov
@@ -609,22 +612,140 @@ def _use_blockdiag(op, opts):
     return blockdiag_applies(op) and op.shapes[0][0] >= 8
 
 
+def nd_applies(op):
+    """True when the nested-dissection factorisation handles ``op``: one
+    uncoupled instance (any shape, real or complex materials)."""
+    return op.n_inst == 1 and op.cpl_len == 0 and op.shapes[0][0] * op.shapes[0][1] >= 4
+
+
+def _direct_kind(op):
+    """'bcr' (block cyclic reduction, an extent <= 128 with real materials),
+    'nd' (nested dissection) or None."""
+    if direct_applies(op):
+        return 'bcr'
+    if nd_applies(op):
+        return 'nd'
+    return None
+
+
+def _factor_bytes(op, kind):
+    if kind == 'bcr':
+        return direct_factor_bytes(op)
+    plan = nd_plan(op.shapes[0], op.device)
+    return int(lib().cv_nd_factor_bytes(plan.ptr))
+
+
 def _use_direct(op, opts):
-    """method 'direct' requires the direct path; 'auto' takes it when it
-    applies and its factor fits in DIRECT_MEMORY_FRACTION of the free device
-    memory."""
+    """The direct factorisation to use ('bcr', 'nd') or None.  method 'direct'
+    requires one; 'auto' takes it when it applies and its factor fits in
+    DIRECT_MEMORY_FRACTION of the free device memory."""
     method = (opts or SolverOptions()).method
     if method in ('bicgstab', 'cocg'):
-        return False
+        return None
+    kind = _direct_kind(op)
     if method == 'direct':
-        if not direct_applies(op):
-            raise ValueError("method='direct' needs one uncoupled instance with nx or ny <= %d and real Hz "
-                             "materials" % BTD_MAX_NY)
-        return True
-    if not direct_applies(op):
-        return False
+        if kind is None:
+            raise ValueError("method='direct' needs one uncoupled instance")
+        return kind
+    if kind is None:
+        return None
     free, _ = torch.cuda.mem_get_info(op.device)
-    return direct_factor_bytes(op) <= DIRECT_MEMORY_FRACTION * free
+    return kind if _factor_bytes(op, kind) <= DIRECT_MEMORY_FRACTION * free else None
+
+
+def make_direct(op, m0, m1, kind):
+    return Direct(op, m0, m1) if kind == 'bcr' else NdDirect(op, m0, m1)
+
+
+class NdPlan:
+    """Host symbolic analysis of an nx x ny grid and its device index arrays."""
+
+    def __init__(self, shape, device):
+        nx, ny = (int(v) for v in shape)
+        L = lib()
+        p = L.cv_nd_plan_create(nx, ny)
+        if not p:
+            raise CevicheError(_lib.CV_ERR_INVALID, L.cv_last_error().decode(errors='replace'))
+        self.ptr = ctypes.c_void_p(p)
+        nbytes = int(L.cv_nd_plan_bytes(self.ptr))
+        self.dev = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        check(L.cv_nd_plan_upload(self.ptr, ptr(self.dev), nbytes, stream_ptr()), 'nd plan upload')
+
+    def __del__(self):
+        try:
+            if _lib._lib is not None and self.ptr:
+                _lib._lib.cv_nd_plan_destroy(self.ptr)
+                self.ptr = None
+        except Exception:  # interpreter teardown
+            pass
+
+
+_ND_PLANS = {}
+
+
+def nd_plan(shape, device):
+    """The cached NdPlan of a grid shape on a device."""
+    key = (tuple(int(v) for v in shape), str(device))
+    plan = _ND_PLANS.get(key)
+    if plan is None:
+        if len(_ND_PLANS) > 16:
+            _ND_PLANS.clear()
+        plan = _ND_PLANS[key] = NdPlan(shape, device)
+    return plan
+
+
+class NdDirect:
+    """Nested-dissection factorisation of one FDFD instance (complex64, the
+    periodic wrap dropped) and FP64-refined FGMRES solves with it in either
+    orientation (csrc/nd.hip, cv_fdfd_fgmres_nd).  One factorisation serves
+    the forward, adjoint (A^T) and tangent solves of a linear system."""
+
+    kind = 'nd'
+
+    def __init__(self, op, m0, m1):
+        if not nd_applies(op):
+            raise ValueError('the nested-dissection factorisation needs one uncoupled instance')
+        self.user_op = self.op = op
+        self.plan = nd_plan(op.shapes[0], op.device)
+        lo = lambda m: None if m is None else m.to(torch.complex64 if m.is_complex() else torch.float32).contiguous()
+        self.prof32 = op.prof.to(torch.complex64).contiguous()
+        self.m0, self.m1 = lo(m0), lo(m1)
+        L = lib()
+        nbytes = int(L.cv_nd_factor_bytes(self.plan.ptr))
+        self.factor = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
+        check(L.cv_nd_factor(self.plan.ptr, ptr(self.plan.dev), ptr(op.desc),
+                             op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(self.prof32), ptr(self.m0),
+                             ptr(self.m1), ptr(self.factor), nbytes, stream_ptr()), 'nested dissection')
+
+    def apply(self, v, transpose=False):
+        """M^-1 v (complex64) with the factorisation alone."""
+        v = v.to(torch.complex64).contiguous()
+        z = torch.empty_like(v)
+        check(lib().cv_nd_apply(self.plan.ptr, ptr(self.plan.dev), ptr(self.factor), ptr(v), ptr(z),
+                                int(bool(transpose)), stream_ptr()), 'nd_apply')
+        return z
+
+    def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
+        """(c A) x = bs (or (c A)^T x = bs) to the FP64 tolerance."""
+        op = self.op
+        opts = opts or SolverOptions()
+        tol = opts.tol if opts.tol is not None else DEFAULT_TOL[torch.complex128]
+        nx, ny = op.shapes[0]
+        L = lib()
+        need = L.cv_fdfd_fgmres_workspace_bytes(nx, ny, int(opts.restart), _lib.CV_PRECOND_NONE, 0)
+        ws = torch.empty(need, dtype=torch.uint8, device=op.device)
+        b = bs.to(torch.complex128).contiguous()
+        x = torch.zeros_like(b)
+        info = (_lib.SolveInfo * 1)()
+        max_iter = opts.max_iter if opts.direct_max_iter is None else opts.direct_max_iter
+        rc = L.cv_fdfd_fgmres_nd(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(op.prof64),
+                                 ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1), self.plan.ptr,
+                                 ptr(self.plan.dev), ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)),
+                                 float(tol), int(opts.restart), int(max_iter), ptr(ws), need, info, stream_ptr())
+        op.last_info = [dict(iterations=info[0].iterations, status=info[0].status, restarts=info[0].restarts,
+                             rel_residual=info[0].rel_residual, method='direct', factor='nd')]
+        check(rc, 'fdfd direct solve (nested dissection)')
+        return x
 
 
 # status codes after which the direct path gives way to BiCGSTAB under 'auto'
@@ -679,9 +800,10 @@ class FdfdSolve(torch.autograd.Function):
         c = op.scale_vector(torch.float64)
         # one factorisation serves the forward, adjoint and tangent solves
         direct = None
-        if _use_direct(op, opts):
+        kind = _use_direct(op, opts)
+        if kind is not None:
             try:
-                direct = Direct(op, m0, m1)
+                direct = make_direct(op, m0, m1, kind)
             except (CevicheError, torch.cuda.OutOfMemoryError) as err:
                 if not _direct_failed(err, opts):
                     raise

@@ -146,6 +146,15 @@ _SIGS = {
     'cv_segsum': (I, [I64, P, P, P, P, P, I, I, D, D, P, I, P, P]),
     'cv_csr_solve_workspace_bytes': (I64, [I64]),
     'cv_csr_solve': (I, [I64, I, P, P, P, P, P, D, I, I, P, I64, P, P]),
+    'cv_nd_plan_create': (P, [I, I]),
+    'cv_nd_plan_destroy': (None, [P]),
+    'cv_nd_plan_bytes': (I64, [P]),
+    'cv_nd_factor_bytes': (I64, [P]),
+    'cv_nd_plan_stats': (I, [P, P, P, P, P, I]),
+    'cv_nd_plan_upload': (I, [P, P, I64, P]),
+    'cv_nd_factor': (I, [P, P, P, P, P, P, P, P, I64, P]),
+    'cv_nd_apply': (I, [P, P, P, P, P, I, P]),
+    'cv_fdfd_fgmres_nd': (I, [P, P, P, P, P, P, P, P, P, P, P, P, P, I, D, I, I, P, I64, P, P]),
     'cv_dense_lu': (I, [I, P, P, P, P]),
     'cv_dense_lu_solve': (I, [I, P, P, P, I, I, P]),
     'cv_cnn_pack_weights': (I, [P, P, I, I, P]),

@@ -469,6 +469,47 @@ int bcr_apply_batch(const cv_fdfd_desc& dh, int nb, const void* const* factors, 
                     int64_t stride, int transpose, hipStream_t s);
 }
 
+namespace cv {
+int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, const cx<float>* v, cx<float>* z,
+             int transpose, hipStream_t s);
+}
+
+extern "C" int cv_fdfd_fgmres_nd(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                                 const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                                 const void* m1_32, const cv_nd_plan* plan, const void* plan_dev, const void* factor,
+                                 const void* b, void* x, int transpose, double tol, int restart, int max_iter,
+                                 void* workspace, int64_t workspace_bytes, cv_solve_info* info, void* stream) {
+    CV_CHECK(desc && desc_host && prof64 && m0_64 && prof32 && m0_32 && plan && plan_dev && factor && b && x &&
+                 workspace && info,
+             "null pointer argument");
+    const cv_fdfd_desc dh = desc_host[0];
+    CV_CHECK(dh.nx > 0 && dh.ny > 0 && dh.cell_off == 0, "one instance starting at cell 0 is required");
+    CV_CHECK(dh.pol == CV_POL_EZ || (m1_64 && m1_32), "Hz operator needs m1");
+    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported by FGMRES");
+    CV_CHECK(restart >= 1 && restart <= kFgMaxM, "restart must be in [1, %d]", kFgMaxM);
+    CV_CHECK(tol > 0 && max_iter > 0, "tol and max_iter must be positive");
+    const int m = restart;
+    const int64_t need = cv_fdfd_fgmres_workspace_bytes(dh.nx, dh.ny, m, CV_PRECOND_NONE, 0);
+    CV_CHECK(need > 0 && workspace_bytes >= need, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+             (long long)need);
+    FgWork w;
+    fg_layout((int64_t)dh.nx * dh.ny, m, 0, (char*)workspace, &w);
+    hipStream_t s = as_stream(stream);
+    auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
+        return nd_apply(plan, plan_dev, factor, v, z, transpose, s);
+    };
+    auto after_cycle = [](double, double) {};
+    int status;
+    if (transpose)
+        status = fgmres_core<true>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
+                                   w, precondition, after_cycle, info, s);
+    else
+        status = fgmres_core<false>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
+                                    w, precondition, after_cycle, info, s);
+    info[0].pad = 0;
+    return status;
+}
+
 extern "C" int cv_fdfd_fgmres_blockdiag(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
                                         const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
                                         const void* m1_32, const cv_fdfd_desc* block_desc_host,

@@ -1,0 +1,641 @@
+// Nested-dissection direct factorisation of a 2D FDFD operator (complex64),
+// the preconditioner of FP64-refined FGMRES for grids too wide for the
+// block-tridiagonal path.  It replaces, for those grids, the sparse direct
+// factorisation of ceviche's default solve (solvers.py:50-62 behind
+// primitives.py:103-119, fdfd.py:172-192 / :211-244).
+//
+// The grid is bisected recursively (the longer side at its middle) down to
+// leaf boxes of at most kLeafCells cells.  A cell is on the boundary of a box
+// when one of its four stencil neighbours (without the periodic wrap, which
+// the preconditioner drops like the block-tridiagonal path drops the x-wrap)
+// lies outside the box.  Each node of the tree eliminates its interior I --
+// the whole box minus its boundary for a leaf, the boundary cells of its two
+// children that are not on its own boundary otherwise -- from a dense frontal
+// matrix F over [I; B]:
+//   leaf:     F = A restricted to the box (stencil rows of the cells),
+//   internal: F = the two children's Schur complements on their boundaries
+//             (placed at their positions) + the A entries that couple the
+//             two children across their shared edge,
+// then factorises F_II = P L U, forms X = F_II^-1 F_IB and passes
+// S = F_BB - F_BI X up to its parent.  Nodes of equal height (distance to the
+// leaves) are processed together as strided batches padded to the level's
+// largest |I| (kmax, identity padding) and |B| (bmax, zero padding): dense LU
+// (rocsolver getrf / getrs) and the Schur products (rocblas gemm) run on the
+// matrix cores.  The symbolic analysis (tree, index sets, maps) is host C++
+// done once per grid shape.
+//
+// Solve of A x = b (transpose 0) with the stored F_II LU, X and F_BI:
+//   up the tree:   y_I = F_II^-1 b_I,  b_B -= F_BI y_I
+//   down the tree: x_I = y_I - X x_B
+// and of A^T x = b (transpose 1):
+//   up:   b_B -= X^T b_I,  y_I = F_II^-T b_I
+//   down: x_I = y_I - F_II^-T (F_BI^T x_B)
+// (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]] and F_IB^T F_II^-T = X^T).
+#include "fdfd_stencil.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+namespace cv {
+namespace {
+
+constexpr int kLeafCells = 64;
+
+struct NdGroup {               // (parent, child) pairs whose child sits at one level
+    int child_level = 0;
+    std::vector<int32_t> parent, child, map;   // map: per pair, the child level's bmax entries
+    int64_t off_parent = 0, off_child = 0, off_map = 0;
+};
+
+struct NdLevel {
+    int nb = 0, kmax = 0, bmax = 0, m = 0;
+    bool leaf = false;
+    std::vector<int32_t> icells, bcells;   // nb * kmax, nb * bmax (-1 = padding)
+    std::vector<int32_t> lcell, lnbr;      // leaves: per position the cell and 4 neighbour positions
+    std::vector<NdGroup> groups;           // internal: children's Schur complements
+    std::vector<int32_t> xnode, xpu, xpv, xcell, xdir;   // internal: couplings across the shared edge
+    // byte offsets of the device copies (plan buffer) and of the numeric data (factor buffer)
+    int64_t off_icells = 0, off_bcells = 0, off_lcell = 0, off_lnbr = 0;
+    int64_t off_xnode = 0, off_xpu = 0, off_xpv = 0, off_xcell = 0, off_xdir = 0;
+    int64_t off_F = 0, off_piv = 0, info_first = 0;
+};
+
+}  // namespace
+}  // namespace cv
+
+struct cv_nd_plan {
+    int nx = 0, ny = 0;
+    std::vector<cv::NdLevel> L;
+    int64_t plan_bytes = 0, factor_bytes = 0;
+    int64_t off_w = 0, off_y = 0, off_t = 0, off_u = 0, off_info = 0;
+    int64_t n_nodes = 0;
+};
+
+namespace cv {
+namespace {
+
+int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+struct Box {
+    int x0, x1, y0, y1;
+    int c0 = -1, c1 = -1, height = 0;
+    std::vector<int32_t> I, B;
+};
+
+int build_tree(std::vector<Box>& nodes, int x0, int x1, int y0, int y1) {
+    const int idx = (int)nodes.size();
+    nodes.push_back(Box{x0, x1, y0, y1});
+    if ((int64_t)(x1 - x0) * (y1 - y0) <= kLeafCells || (x1 - x0 < 2 && y1 - y0 < 2)) return idx;
+    int c0, c1;
+    if (x1 - x0 >= y1 - y0) {
+        const int xm = (x0 + x1) / 2;
+        c0 = build_tree(nodes, x0, xm, y0, y1);
+        c1 = build_tree(nodes, xm, x1, y0, y1);
+    } else {
+        const int ym = (y0 + y1) / 2;
+        c0 = build_tree(nodes, x0, x1, y0, ym);
+        c1 = build_tree(nodes, x0, x1, ym, y1);
+    }
+    nodes[idx].c0 = c0;
+    nodes[idx].c1 = c1;
+    nodes[idx].height = 1 + std::max(nodes[c0].height, nodes[c1].height);
+    return idx;
+}
+
+// boundary cells of box b (a stencil neighbour inside the grid but outside the box), sorted
+void box_boundary(const Box& b, int nx, int ny, std::vector<int32_t>& out) {
+    out.clear();
+    for (int i = b.x0; i < b.x1; ++i)
+        for (int j = b.y0; j < b.y1; ++j) {
+            const bool on = (i == b.x0 && b.x0 > 0) || (i == b.x1 - 1 && b.x1 < nx) || (j == b.y0 && b.y0 > 0) ||
+                            (j == b.y1 - 1 && b.y1 < ny);
+            if (on) out.push_back((int32_t)((int64_t)i * ny + j));
+        }
+}
+
+// stencil neighbours of cell c in the order xm, xp, ym, yp (-1 outside the grid; no wrap)
+void neighbours(int32_t c, int nx, int ny, int32_t (&nb)[4]) {
+    const int i = c / ny, j = c % ny;
+    nb[0] = (i > 0) ? c - ny : -1;
+    nb[1] = (i < nx - 1) ? c + ny : -1;
+    nb[2] = (j > 0) ? c - 1 : -1;
+    nb[3] = (j < ny - 1) ? c + 1 : -1;
+}
+
+}  // namespace
+}  // namespace cv
+
+using namespace cv;
+
+extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
+    if (nx <= 0 || ny <= 0 || (int64_t)nx * ny >= (int64_t)0x7fffffff) {
+        set_error("nd plan: bad grid %d x %d", nx, ny);
+        return nullptr;
+    }
+    cv_nd_plan* P = new cv_nd_plan();
+    P->nx = nx;
+    P->ny = ny;
+    std::vector<Box> nodes;
+    nodes.reserve(4 * ((int64_t)nx * ny / kLeafCells + 4));
+    build_tree(nodes, 0, nx, 0, ny);
+    const int64_t N = (int64_t)nx * ny;
+    int H = 0;
+    for (auto& b : nodes) H = std::max(H, b.height);
+    // interior and boundary sets, bottom-up (children have larger indices than their parent)
+    for (int64_t q = (int64_t)nodes.size() - 1; q >= 0; --q) {
+        Box& b = nodes[q];
+        box_boundary(b, nx, ny, b.B);
+        if (b.c0 < 0) {
+            for (int i = b.x0; i < b.x1; ++i)
+                for (int j = b.y0; j < b.y1; ++j) {
+                    const int32_t c = (int32_t)((int64_t)i * ny + j);
+                    if (!std::binary_search(b.B.begin(), b.B.end(), c)) b.I.push_back(c);
+                }
+        } else {
+            std::vector<int32_t> U;
+            U.reserve(nodes[b.c0].B.size() + nodes[b.c1].B.size());
+            std::merge(nodes[b.c0].B.begin(), nodes[b.c0].B.end(), nodes[b.c1].B.begin(), nodes[b.c1].B.end(),
+                       std::back_inserter(U));
+            for (int32_t c : U)
+                if (!std::binary_search(b.B.begin(), b.B.end(), c)) b.I.push_back(c);
+        }
+    }
+    // group by height
+    P->L.resize(H + 1);
+    std::vector<int> level_index(nodes.size());
+    for (size_t q = 0; q < nodes.size(); ++q) {
+        NdLevel& L = P->L[nodes[q].height];
+        level_index[q] = L.nb++;
+        L.kmax = std::max<int>(L.kmax, (int)nodes[q].I.size());
+        L.bmax = std::max<int>(L.bmax, (int)nodes[q].B.size());
+        L.leaf = nodes[q].c0 < 0;
+    }
+    for (auto& L : P->L) L.m = L.kmax + L.bmax;
+    std::vector<std::vector<int>> members(H + 1);
+    for (size_t q = 0; q < nodes.size(); ++q) members[nodes[q].height].push_back((int)q);
+    // position of a cell in the current node's [I; B] ordering
+    std::vector<int32_t> pos(N, -1);
+    for (int h = 0; h <= H; ++h) {
+        NdLevel& L = P->L[h];
+        L.icells.assign((size_t)L.nb * L.kmax, -1);
+        L.bcells.assign((size_t)L.nb * L.bmax, -1);
+        if (L.leaf) {
+            L.lcell.assign((size_t)L.nb * L.m, -1);
+            L.lnbr.assign((size_t)L.nb * L.m * 4, -1);
+        }
+        for (int q : members[h]) {
+            const Box& b = nodes[q];
+            const int e = level_index[q];
+            for (size_t p = 0; p < b.I.size(); ++p) {
+                L.icells[(size_t)e * L.kmax + p] = b.I[p];
+                pos[b.I[p]] = (int32_t)p;
+            }
+            for (size_t p = 0; p < b.B.size(); ++p) {
+                L.bcells[(size_t)e * L.bmax + p] = b.B[p];
+                pos[b.B[p]] = (int32_t)(L.kmax + p);
+            }
+            auto inside = [&](int32_t c) {
+                if (c < 0) return false;
+                const int i = c / ny, j = c % ny;
+                return i >= b.x0 && i < b.x1 && j >= b.y0 && j < b.y1;
+            };
+            if (b.c0 < 0) {
+                auto put = [&](int32_t c) {
+                    const int p = pos[c];
+                    L.lcell[(size_t)e * L.m + p] = c;
+                    int32_t nbv[4];
+                    neighbours(c, nx, ny, nbv);
+                    for (int d = 0; d < 4; ++d)
+                        L.lnbr[((size_t)e * L.m + p) * 4 + d] = inside(nbv[d]) ? pos[nbv[d]] : -1;
+                };
+                for (int32_t c : b.I) put(c);
+                for (int32_t c : b.B) put(c);
+            } else {
+                for (int side = 0; side < 2; ++side) {
+                    const int cq = side == 0 ? b.c0 : b.c1;
+                    const int ch = nodes[cq].height;
+                    const NdLevel& CL = P->L[ch];
+                    NdGroup* g = nullptr;
+                    for (auto& gg : L.groups)
+                        if (gg.child_level == ch) g = &gg;
+                    if (!g) {
+                        L.groups.push_back(NdGroup());
+                        g = &L.groups.back();
+                        g->child_level = ch;
+                    }
+                    g->parent.push_back(e);
+                    g->child.push_back(level_index[cq]);
+                    const size_t base = g->map.size();
+                    g->map.resize(base + CL.bmax, -1);
+                    for (size_t p = 0; p < nodes[cq].B.size(); ++p) g->map[base + p] = pos[nodes[cq].B[p]];
+                }
+                // couplings between the two children: u in B(c0), v in B(c1) adjacent
+                const Box& b0 = nodes[b.c0];
+                const Box& b1 = nodes[b.c1];
+                auto in1 = [&](int32_t c) {
+                    if (c < 0) return false;
+                    const int i = c / ny, j = c % ny;
+                    return i >= b1.x0 && i < b1.x1 && j >= b1.y0 && j < b1.y1;
+                };
+                for (int32_t u : b0.B) {
+                    int32_t nbv[4];
+                    neighbours(u, nx, ny, nbv);
+                    for (int d = 0; d < 4; ++d) {
+                        const int32_t v = nbv[d];
+                        if (!in1(v)) continue;
+                        // A[u, v]: row u, direction d; A[v, u]: row v, the opposite direction
+                        L.xnode.push_back(e);
+                        L.xpu.push_back(pos[u]);
+                        L.xpv.push_back(pos[v]);
+                        L.xcell.push_back(u);
+                        L.xdir.push_back(d);
+                        L.xnode.push_back(e);
+                        L.xpu.push_back(pos[v]);
+                        L.xpv.push_back(pos[u]);
+                        L.xcell.push_back(v);
+                        L.xdir.push_back(d ^ 1);
+                    }
+                }
+            }
+            for (int32_t c : b.I) pos[c] = -1;
+            for (int32_t c : b.B) pos[c] = -1;
+        }
+    }
+    // device layouts: index arrays in the plan buffer, numeric data in the factor buffer
+    int64_t po = 0, fo = 0, ymax = 0, tmax = 0, info_first = 0;
+    auto take = [](int64_t& o, int64_t bytes) {
+        const int64_t r = o;
+        o += al256(bytes);
+        return r;
+    };
+    for (auto& L : P->L) {
+        L.off_icells = take(po, 4 * (int64_t)L.icells.size());
+        L.off_bcells = take(po, 4 * (int64_t)L.bcells.size());
+        L.off_lcell = take(po, 4 * (int64_t)L.lcell.size());
+        L.off_lnbr = take(po, 4 * (int64_t)L.lnbr.size());
+        for (auto& g : L.groups) {
+            g.off_parent = take(po, 4 * (int64_t)g.parent.size());
+            g.off_child = take(po, 4 * (int64_t)g.child.size());
+            g.off_map = take(po, 4 * (int64_t)g.map.size());
+        }
+        L.off_xnode = take(po, 4 * (int64_t)L.xnode.size());
+        L.off_xpu = take(po, 4 * (int64_t)L.xpu.size());
+        L.off_xpv = take(po, 4 * (int64_t)L.xpv.size());
+        L.off_xcell = take(po, 4 * (int64_t)L.xcell.size());
+        L.off_xdir = take(po, 4 * (int64_t)L.xdir.size());
+        L.off_F = take(fo, 8 * (int64_t)L.nb * L.m * L.m);
+        L.off_piv = take(fo, 4 * (int64_t)L.nb * std::max(1, L.kmax));
+        L.info_first = info_first;
+        info_first += L.nb;
+        ymax = std::max<int64_t>(ymax, (int64_t)L.nb * std::max(L.kmax, L.bmax));
+        tmax = std::max<int64_t>(tmax, (int64_t)L.nb * std::max(L.kmax, L.bmax));
+    }
+    P->n_nodes = info_first;
+    P->off_info = take(fo, 4 * info_first);
+    P->off_w = take(fo, 8 * N);
+    P->off_y = take(fo, 8 * std::max<int64_t>(1, ymax));
+    P->off_t = take(fo, 8 * std::max<int64_t>(1, tmax));
+    P->off_u = take(fo, 8 * std::max<int64_t>(1, tmax));
+    P->plan_bytes = std::max<int64_t>(po, 256);
+    P->factor_bytes = fo;
+    return P;
+}
+
+extern "C" void cv_nd_plan_destroy(cv_nd_plan* p) { delete p; }
+
+extern "C" int64_t cv_nd_plan_bytes(const cv_nd_plan* p) { return p ? p->plan_bytes : -1; }
+extern "C" int64_t cv_nd_factor_bytes(const cv_nd_plan* p) { return p ? p->factor_bytes : -1; }
+
+extern "C" int cv_nd_plan_stats(const cv_nd_plan* p, int* levels, int* nb, int* kmax, int* bmax, int max_levels) {
+    CV_CHECK(p, "null plan");
+    const int n = (int)p->L.size();
+    if (levels) *levels = n;
+    for (int h = 0; h < std::min(n, max_levels); ++h) {
+        if (nb) nb[h] = p->L[h].nb;
+        if (kmax) kmax[h] = p->L[h].kmax;
+        if (bmax) bmax[h] = p->L[h].bmax;
+    }
+    return CV_OK;
+}
+
+extern "C" int cv_nd_plan_upload(const cv_nd_plan* p, void* plan_dev, int64_t plan_bytes, void* stream) {
+    CV_CHECK(p && plan_dev, "null argument");
+    CV_CHECK(plan_bytes >= p->plan_bytes, "plan buffer too small: %lld < %lld", (long long)plan_bytes,
+             (long long)p->plan_bytes);
+    hipStream_t s = as_stream(stream);
+    char* base = (char*)plan_dev;
+    auto up = [&](int64_t off, const std::vector<int32_t>& v) -> int {
+        if (!v.empty()) CV_HIP(hipMemcpyAsync(base + off, v.data(), 4 * v.size(), hipMemcpyHostToDevice, s));
+        return CV_OK;
+    };
+    for (const auto& L : p->L) {
+        int rc = CV_OK;
+        if ((rc = up(L.off_icells, L.icells)) || (rc = up(L.off_bcells, L.bcells)) || (rc = up(L.off_lcell, L.lcell)) ||
+            (rc = up(L.off_lnbr, L.lnbr)) || (rc = up(L.off_xnode, L.xnode)) || (rc = up(L.off_xpu, L.xpu)) ||
+            (rc = up(L.off_xpv, L.xpv)) || (rc = up(L.off_xcell, L.xcell)) || (rc = up(L.off_xdir, L.xdir)))
+            return rc;
+        for (const auto& g : L.groups)
+            if ((rc = up(g.off_parent, g.parent)) || (rc = up(g.off_child, g.child)) || (rc = up(g.off_map, g.map)))
+                return rc;
+    }
+    // the host vectors are read by the copies above: wait for them before returning
+    CV_HIP(hipStreamSynchronize(s));
+    return CV_OK;
+}
+
+namespace cv {
+namespace {
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ cx<float> row_dir(const Row5<float>& r, int d) {
+    return d == 0 ? r.xm : d == 1 ? r.xp : d == 2 ? r.ym : r.yp;
+}
+
+// leaf frontal matrices: F[p, p] = A_uu, F[p, q] = A_uv for stencil
+// neighbours inside the box; identity on padded interior positions.
+__global__ void __launch_bounds__(256)
+k_nd_leaf(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
+          const float* __restrict__ m1, const int32_t* __restrict__ lcell, const int32_t* __restrict__ lnbr,
+          cx<float>* __restrict__ F, int nb, int m, int kmax) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * m) return;
+    const int e = (int)(t / m), p = (int)(t - (int64_t)e * m);
+    cx<float>* Fe = F + (int64_t)e * m * m;
+    const int32_t c = lcell[t];
+    if (c < 0) {
+        if (p < kmax) Fe[(int64_t)p * m + p] = mk<float>(1.f, 0.f);
+        return;
+    }
+    const cv_fdfd_desc d = desc[0];
+    const Row5<float> r = row_coefs<float>(d, prof, m0, m1, c / d.ny, c % d.ny);
+    Fe[(int64_t)p * m + p] = r.c;
+#pragma unroll
+    for (int dd = 0; dd < 4; ++dd) {
+        const int q = lnbr[t * 4 + dd];
+        if (q >= 0) Fe[(int64_t)q * m + p] = row_dir(r, dd);
+    }
+}
+
+// children's Schur complements (the bottom-right bc x bc block of their F)
+// into the parents' F at the mapped positions; one workgroup per pair
+__global__ void __launch_bounds__(256)
+k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int32_t* __restrict__ parent,
+            const int32_t* __restrict__ child, const int32_t* __restrict__ map, cx<float>* __restrict__ F, int m) {
+    const int pr = blockIdx.x;
+    const cx<float>* S = Fc + (int64_t)child[pr] * mc * mc + (int64_t)kc * mc + kc;
+    cx<float>* Fp = F + (int64_t)parent[pr] * m * m;
+    const int32_t* mp = map + (int64_t)pr * bc;
+    for (int idx = threadIdx.x; idx < bc * bc; idx += blockDim.x) {
+        const int jj = idx / bc, ii = idx - jj * bc;
+        const int pi = mp[ii], pj = mp[jj];
+        if (pi >= 0 && pj >= 0) Fp[(int64_t)pj * m + pi] = S[(int64_t)jj * mc + ii];
+    }
+}
+
+// couplings across the children's shared edge: F[pu, pv] = A[u, v]
+__global__ void __launch_bounds__(256)
+k_nd_cross(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
+           const float* __restrict__ m1, const int32_t* __restrict__ node, const int32_t* __restrict__ pu,
+           const int32_t* __restrict__ pv, const int32_t* __restrict__ cell, const int32_t* __restrict__ dir, int n,
+           cx<float>* __restrict__ F, int m) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const cv_fdfd_desc d = desc[0];
+    const int32_t c = cell[t];
+    const Row5<float> r = row_coefs<float>(d, prof, m0, m1, c / d.ny, c % d.ny);
+    F[(int64_t)node[t] * m * m + (int64_t)pv[t] * m + pu[t]] = row_dir(r, dir[t]);
+}
+
+// identity on the padded interior positions of internal nodes
+__global__ void __launch_bounds__(256)
+k_nd_pad(const int32_t* __restrict__ icells, int nb, int kmax, cx<float>* __restrict__ F, int m) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * kmax) return;
+    if (icells[t] >= 0) return;
+    const int e = (int)(t / kmax), p = (int)(t - (int64_t)e * kmax);
+    F[(int64_t)e * m * m + (int64_t)p * m + p] = mk<float>(1.f, 0.f);
+}
+
+__global__ void __launch_bounds__(256)
+k_nd_gather(const int32_t* __restrict__ cells, int64_t n, const cx<float>* __restrict__ w, cx<float>* __restrict__ y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int32_t c = cells[t];
+    y[t] = c >= 0 ? w[c] : czero<float>();
+}
+
+// w[cells] = y (sub = 0) or w[cells] -= y (sub = 1)
+__global__ void __launch_bounds__(256)
+k_nd_scatter(const int32_t* __restrict__ cells, int64_t n, const cx<float>* __restrict__ y, cx<float>* __restrict__ w,
+             int sub) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int32_t c = cells[t];
+    if (c < 0) return;
+    w[c] = sub ? w[c] - y[t] : y[t];
+}
+
+__global__ void __launch_bounds__(256) k_nd_copy(const cx<float>* __restrict__ a, cx<float>* __restrict__ b, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = a[t];
+}
+
+int nblk(int64_t n) { return (int)((n + 255) / 256); }
+
+// ------------------------------------------------------- library handle
+// One rocBLAS handle per device (rocSOLVER uses rocBLAS handles), created on
+// first use and kept for the process; the stream is set on every call.
+std::mutex g_handle_mu;
+rocblas_handle g_handle[64] = {};
+
+int get_handle(hipStream_t s, rocblas_handle* out) {
+    int dev = 0;
+    CV_HIP(hipGetDevice(&dev));
+    CV_CHECK(dev >= 0 && dev < 64, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_handle_mu);
+    if (!g_handle[dev]) {
+        rocblas_handle h = nullptr;
+        CV_CHECK(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
+        g_handle[dev] = h;
+    }
+    CV_CHECK(rocblas_set_stream(g_handle[dev], s) == rocblas_status_success, "rocblas_set_stream failed");
+    *out = g_handle[dev];
+    return CV_OK;
+}
+
+#define CV_RB(expr)                                                                     \
+    do {                                                                                \
+        const rocblas_status _st = (expr);                                              \
+        if (_st != rocblas_status_success) {                                            \
+            ::cv::set_error("%s:%d %s: rocblas status %d", __FILE__, __LINE__, #expr, (int)_st); \
+            return CV_ERR_HIP;                                                          \
+        }                                                                               \
+    } while (0)
+
+using rc_t = rocblas_float_complex;
+
+}  // namespace
+
+// M^-1 v (transpose 0) or M^-T v (transpose 1) with the nested-dissection
+// factor; v and z complex64 over the grid (may alias).
+int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, const cx<float>* v, cx<float>* z,
+             int transpose, hipStream_t s) {
+    const char* pb = (const char*)plan_dev;
+    char* fb = (char*)factor;
+    const int64_t N = (int64_t)P->nx * P->ny;
+    cx<float>* w = (cx<float>*)(fb + P->off_w);
+    cx<float>* Y = (cx<float>*)(fb + P->off_y);
+    cx<float>* T = (cx<float>*)(fb + P->off_t);
+    cx<float>* U = (cx<float>*)(fb + P->off_u);
+    rocblas_handle h;
+    {
+        const int rc = get_handle(s, &h);
+        if (rc != CV_OK) return rc;
+    }
+    const rc_t one{1.f, 0.f}, zero{0.f, 0.f};
+    k_nd_copy<<<nblk(N), 256, 0, s>>>(v, w, N);
+    const int H = (int)P->L.size();
+    const rocblas_operation op = transpose ? rocblas_operation_transpose : rocblas_operation_none;
+    for (int l = 0; l < H; ++l) {   // up the tree
+        const NdLevel& L = P->L[l];
+        if (L.kmax == 0) continue;
+        const int32_t* ic = (const int32_t*)(pb + L.off_icells);
+        const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
+        rc_t* F = (rc_t*)(fb + L.off_F);
+        const rocblas_int* piv = (const rocblas_int*)(fb + L.off_piv);
+        const int64_t nI = (int64_t)L.nb * L.kmax, nB = (int64_t)L.nb * L.bmax;
+        const rocblas_stride sF = (rocblas_stride)L.m * L.m;
+        k_nd_gather<<<nblk(nI), 256, 0, s>>>(ic, nI, w, Y);
+        CV_LAUNCH_CHECK();
+        if (transpose && L.bmax > 0) {   // b_B -= X^T b_I
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.kmax, L.bmax, &one,
+                                                F + (int64_t)L.kmax * L.m, L.m, sF, (const rc_t*)Y, 1, L.kmax, &zero,
+                                                (rc_t*)T, 1, L.bmax, L.nb));
+            k_nd_scatter<<<nblk(nB), 256, 0, s>>>(bc, nB, T, w, 1);
+        }
+        CV_RB(rocsolver_cgetrs_strided_batched(h, op, L.kmax, 1, F, L.m, sF, piv, L.kmax, (rc_t*)Y, L.kmax, L.kmax,
+                                               L.nb));
+        k_nd_scatter<<<nblk(nI), 256, 0, s>>>(ic, nI, Y, w, 0);
+        if (!transpose && L.bmax > 0) {   // b_B -= F_BI y_I
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.bmax, L.kmax, &one, F + L.kmax, L.m, sF,
+                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)T, 1, L.bmax, L.nb));
+            k_nd_scatter<<<nblk(nB), 256, 0, s>>>(bc, nB, T, w, 1);
+        }
+        CV_LAUNCH_CHECK();
+    }
+    for (int l = H - 1; l >= 0; --l) {   // down the tree
+        const NdLevel& L = P->L[l];
+        if (L.kmax == 0 || L.bmax == 0) continue;
+        const int32_t* ic = (const int32_t*)(pb + L.off_icells);
+        const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
+        rc_t* F = (rc_t*)(fb + L.off_F);
+        const rocblas_int* piv = (const rocblas_int*)(fb + L.off_piv);
+        const int64_t nI = (int64_t)L.nb * L.kmax, nB = (int64_t)L.nb * L.bmax;
+        const rocblas_stride sF = (rocblas_stride)L.m * L.m;
+        k_nd_gather<<<nblk(nB), 256, 0, s>>>(bc, nB, w, T);
+        CV_LAUNCH_CHECK();
+        if (!transpose) {   // x_I = y_I - X x_B
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.kmax, L.bmax, &one,
+                                                F + (int64_t)L.kmax * L.m, L.m, sF, (const rc_t*)T, 1, L.bmax, &zero,
+                                                (rc_t*)U, 1, L.kmax, L.nb));
+        } else {            // x_I = y_I - F_II^-T (F_BI^T x_B)
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.bmax, L.kmax, &one, F + L.kmax,
+                                                L.m, sF, (const rc_t*)T, 1, L.bmax, &zero, (rc_t*)U, 1, L.kmax,
+                                                L.nb));
+            CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_transpose, L.kmax, 1, F, L.m, sF, piv,
+                                                   L.kmax, (rc_t*)U, L.kmax, L.kmax, L.nb));
+        }
+        k_nd_scatter<<<nblk(nI), 256, 0, s>>>(ic, nI, U, w, 1);
+        CV_LAUNCH_CHECK();
+    }
+    k_nd_copy<<<nblk(N), 256, 0, s>>>(w, z, N);
+    CV_LAUNCH_CHECK();
+    return CV_OK;
+}
+
+}  // namespace cv
+
+extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_fdfd_desc* desc,
+                            const cv_fdfd_desc* desc_host, const void* prof32, const void* m0_32, const void* m1_32,
+                            void* factor, int64_t factor_bytes, void* stream) {
+    CV_CHECK(P && plan_dev && desc && desc_host && prof32 && m0_32 && factor, "null pointer argument");
+    const cv_fdfd_desc& dh = desc_host[0];
+    CV_CHECK(dh.nx == P->nx && dh.ny == P->ny && dh.cell_off == 0, "descriptor does not match the %d x %d plan",
+             P->nx, P->ny);
+    CV_CHECK(dh.nblk == 1 && dh.n_cpl == 0, "frequency-coupled systems are not supported");
+    CV_CHECK(dh.pol == CV_POL_EZ || m1_32, "Hz operator needs m1");
+    CV_CHECK(factor_bytes >= P->factor_bytes, "factor buffer too small: %lld < %lld", (long long)factor_bytes,
+             (long long)P->factor_bytes);
+    hipStream_t s = as_stream(stream);
+    rocblas_handle h;
+    {
+        const int rc = get_handle(s, &h);
+        if (rc != CV_OK) return rc;
+    }
+    const char* pb = (const char*)plan_dev;
+    char* fb = (char*)factor;
+    const cx<float>* prof = (const cx<float>*)prof32;
+    const float* m0 = (const float*)m0_32;
+    const float* m1 = (const float*)m1_32;
+    rocblas_int* info = (rocblas_int*)(fb + P->off_info);
+    const rc_t minus_one{-1.f, 0.f}, one{1.f, 0.f};
+    for (size_t l = 0; l < P->L.size(); ++l) {
+        const NdLevel& L = P->L[l];
+        rc_t* F = (rc_t*)(fb + L.off_F);
+        const rocblas_stride sF = (rocblas_stride)L.m * L.m;
+        CV_HIP(hipMemsetAsync(F, 0, 8 * (size_t)L.nb * L.m * L.m, s));
+        if (L.leaf) {
+            k_nd_leaf<<<nblk((int64_t)L.nb * L.m), 256, 0, s>>>(desc, prof, m0, m1, (const int32_t*)(pb + L.off_lcell),
+                                                                 (const int32_t*)(pb + L.off_lnbr), (cx<float>*)F,
+                                                                 L.nb, L.m, L.kmax);
+        } else {
+            for (const auto& g : L.groups) {
+                const NdLevel& C = P->L[g.child_level];
+                if (C.bmax == 0 || g.parent.empty()) continue;
+                k_nd_extend<<<(int)g.parent.size(), 256, 0, s>>>(
+                    (const cx<float>*)(fb + C.off_F), C.m, C.kmax, C.bmax, (const int32_t*)(pb + g.off_parent),
+                    (const int32_t*)(pb + g.off_child), (const int32_t*)(pb + g.off_map), (cx<float>*)F, L.m);
+            }
+            const int nx_ = (int)L.xnode.size();
+            if (nx_ > 0)
+                k_nd_cross<<<nblk(nx_), 256, 0, s>>>(desc, prof, m0, m1, (const int32_t*)(pb + L.off_xnode),
+                                                     (const int32_t*)(pb + L.off_xpu), (const int32_t*)(pb + L.off_xpv),
+                                                     (const int32_t*)(pb + L.off_xcell),
+                                                     (const int32_t*)(pb + L.off_xdir), nx_, (cx<float>*)F, L.m);
+            if (L.kmax > 0)
+                k_nd_pad<<<nblk((int64_t)L.nb * L.kmax), 256, 0, s>>>((const int32_t*)(pb + L.off_icells), L.nb,
+                                                                     L.kmax, (cx<float>*)F, L.m);
+        }
+        CV_LAUNCH_CHECK();
+        if (L.kmax == 0) continue;
+        rocblas_int* piv = (rocblas_int*)(fb + L.off_piv);
+        CV_RB(rocsolver_cgetrf_strided_batched(h, L.kmax, L.kmax, F, L.m, sF, piv, L.kmax, info + L.info_first, L.nb));
+        if (L.bmax == 0) continue;
+        // X = F_II^-1 F_IB in place, then S = F_BB - F_BI X
+        CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, L.kmax, L.bmax, F, L.m, sF, piv, L.kmax,
+                                               F + (int64_t)L.kmax * L.m, L.m, sF, L.nb));
+        CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, L.bmax, L.bmax, L.kmax,
+                                            &minus_one, F + L.kmax, L.m, sF, F + (int64_t)L.kmax * L.m, L.m, sF, &one,
+                                            F + (int64_t)L.kmax * L.m + L.kmax, L.m, sF, L.nb));
+    }
+    std::vector<rocblas_int> hinfo(P->n_nodes);
+    CV_HIP(hipMemcpyAsync(hinfo.data(), info, 4 * (size_t)P->n_nodes, hipMemcpyDeviceToHost, s));
+    CV_HIP(hipStreamSynchronize(s));
+    for (int64_t q = 0; q < P->n_nodes; ++q)
+        if (hinfo[q] != 0) {
+            set_error("nested dissection: zero pivot %d in the frontal matrix of node %lld", (int)hinfo[q],
+                      (long long)q);
+            return CV_ERR_SINGULAR;
+        }
+    return CV_OK;
+}
+
+extern "C" int cv_nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, const void* v, void* z,
+                           int transpose, void* stream) {
+    CV_CHECK(P && plan_dev && factor && v && z, "null pointer argument");
+    return nd_apply(P, plan_dev, factor, (const cx<float>*)v, (cx<float>*)z, transpose, as_stream(stream));
+}

@@ -561,6 +561,45 @@ int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_hos
                           int64_t workspace_bytes, cv_solve_info* info, void* stream);
 
 /* ------------------------------------------------------------------ misc */
+/* ------------------------------------- nested-dissection direct factor
+ * Direct factorisation of one FDFD instance of any shape (complex64, the
+ * periodic wrap dropped), the preconditioner of FP64-refined FGMRES for grids
+ * wider than the block-tridiagonal path takes: recursive bisection into
+ * leaf boxes of <= 64 cells, each tree node eliminating the cells its
+ * children share from a dense frontal matrix (LU with partial pivoting, Schur
+ * complement products), nodes of equal height in strided batches.  Replaces
+ * the sparse direct factorisation of solvers.py:50-62 (spsolve / pardiso)
+ * behind primitives.py:103-119 for those grids.
+ *   cv_nd_plan_create: symbolic analysis (host) of an nx x ny grid, NULL on
+ *     error (cv_last_error); cv_nd_plan_destroy frees it.
+ *   cv_nd_plan_upload: the plan's index arrays into a device buffer of
+ *     cv_nd_plan_bytes (once per plan; blocks until the copies are done).
+ *   cv_nd_factor: numeric factorisation of the complex64 operator (prof32,
+ *     m0_32, m1_32 as for cv_fdfd_bcr_factor) into a device buffer of
+ *     cv_nd_factor_bytes; CV_ERR_SINGULAR for a zero pivot.
+ *   cv_nd_apply: z = M^-1 v (transpose 0) or M^-T v (1), complex64, v and z
+ *     may alias.
+ *   cv_fdfd_fgmres_nd: cv_fdfd_fgmres_direct with this factor.
+ *   cv_nd_plan_stats: per level (height) the node count and the padded
+ *     interior / boundary sizes. */
+typedef struct cv_nd_plan cv_nd_plan;
+cv_nd_plan* cv_nd_plan_create(int nx, int ny);
+void cv_nd_plan_destroy(cv_nd_plan* plan);
+int64_t cv_nd_plan_bytes(const cv_nd_plan* plan);
+int64_t cv_nd_factor_bytes(const cv_nd_plan* plan);
+int cv_nd_plan_stats(const cv_nd_plan* plan, int* levels, int* nb, int* kmax, int* bmax, int max_levels);
+int cv_nd_plan_upload(const cv_nd_plan* plan, void* plan_dev, int64_t plan_bytes, void* stream);
+int cv_nd_factor(const cv_nd_plan* plan, const void* plan_dev, const cv_fdfd_desc* desc,
+                 const cv_fdfd_desc* desc_host, const void* prof32, const void* m0_32, const void* m1_32,
+                 void* factor, int64_t factor_bytes, void* stream);
+int cv_nd_apply(const cv_nd_plan* plan, const void* plan_dev, const void* factor, const void* v, void* z,
+                int transpose, void* stream);
+int cv_fdfd_fgmres_nd(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_host, const void* prof64,
+                      const void* m0_64, const void* m1_64, const void* prof32, const void* m0_32,
+                      const void* m1_32, const cv_nd_plan* plan, const void* plan_dev, const void* factor,
+                      const void* b, void* x, int transpose, double tol, int restart, int max_iter,
+                      void* workspace, int64_t workspace_bytes, cv_solve_info* info, void* stream);
+
 const char* cv_version(void);
 const char* cv_last_error(void);
 

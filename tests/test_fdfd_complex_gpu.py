@@ -37,13 +37,17 @@ def test_hz_complex_eps_fields_match_spsolve(precision, bloch):
     eps = rng.uniform(1, 6, shape) - 1j * rng.uniform(0, 0.8, shape)
     src = np.zeros(shape)
     src[20, 17] = 1.0
-    # COCG (asked for) where it applies: zero Bloch phases, complex128
+    # COCG (asked for) where it applies: zero Bloch phases, complex128; else
+    # 'auto': one instance with complex materials takes the nested-dissection
+    # factorisation (the block-cyclic-reduction path takes real ones only)
     cocg = bloch == (0.0, 0.0) and precision == 'complex128'
     F = cv.fdfd_hz(omega, 50e-9, eps, [8, 6], bloch_phases=list(bloch), precision=precision,
                    solver_options=cv.SolverOptions(method='cocg' if cocg else 'auto'))
     Ex, Ey, Hz = F.solve(src)
     print('hz complex info', precision, bloch, F.last_solve_info, flush=True)
-    assert F.last_solve_info['method'] == ('cocg' if cocg else 'bicgstab')
+    assert F.last_solve_info['method'] == ('cocg' if cocg else 'direct')
+    if not cocg:
+        assert F.last_solve_info['factor'] == 'nd'
     ex, ey, hz = R.solve_hz(omega, 50e-9, eps, [8, 6], src, bloch_phases=bloch)
     for a, b in ((Hz, hz), (Ex, ex), (Ey, ey)):
         assert a.dtype == np.complex128 and a.shape == shape
@@ -113,9 +117,12 @@ def test_hz_complex_eps_reverse_gradient():
     assert np.linalg.norm(g - num) <= 1e-4 * np.linalg.norm(num)
 
 
-def test_hz_complex_eps_direct_method_is_rejected():
+def test_hz_complex_eps_direct_method_uses_nested_dissection():
+    """method='direct' with complex Hz materials: the block-cyclic-reduction
+    factor takes real materials only, so the nested-dissection factor serves."""
     import ceviche_amd as cv
     s = _setup(1, shape=(20, 20))
     F = cv.fdfd_hz(s['omega'], s['dL'], s['eps'], [4, 4], solver_options=cv.SolverOptions(method='direct'))
-    with pytest.raises(ValueError):
-        F.solve(s['src'])
+    _, _, Hz = F.solve(s['src'])
+    assert F.last_solve_info['factor'] == 'nd'
+    assert _rel(Hz, R.solve_hz(s['omega'], s['dL'], s['eps'], [4, 4], s['src'])[2]) <= 1e-8

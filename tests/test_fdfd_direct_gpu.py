@@ -197,14 +197,66 @@ def test_direct_path_fields_gradient_and_forward_mode():
     assert np.linalg.norm(jf[:, 0] - num) <= 1e-5 * np.linalg.norm(num)
 
 
-def test_direct_method_requires_a_narrow_grid():
-    """method='direct' needs ny <= 128, or nx <= 128 (the transposed grid);
-    otherwise it is an error, not a silent fallback."""
+def _nowrap(A, nx, ny):
+    """A without its periodic wrap entries (what the nested-dissection factor inverts)."""
+    C = A.tocoo()
+    ri, rj, ci, cj = C.row // ny, C.row % ny, C.col // ny, C.col % ny
+    keep = (np.abs(ri - ci) + np.abs(rj - cj)) <= 1
+    return sp.coo_matrix((C.data[keep], (C.row[keep], C.col[keep])), shape=A.shape).tocsr()
+
+
+@pytest.mark.parametrize('pol', ['ez', 'hz'])
+@pytest.mark.parametrize('shape,npml', [((150, 140), (10, 12)), ((37, 210), (5, 9)), ((9, 7), (1, 1))])
+def test_nested_dissection_factor_inverts_the_unwrapped_operator(pol, shape, npml):
+    """cv_nd_apply with the complex64 factor against the scaled operator
+    without its periodic wrap, A x = b and A^T x = b, to complex64 accuracy."""
+    from ceviche_amd._fdfd_op import NdDirect
+    op = _op(pol, shape, npml, bloch=(0.3, 0.0))
+    rng = np.random.default_rng(17)
+    eps = rng.uniform(1, 12.25, shape)
+    m0, m1 = op.materials(torch.from_numpy(eps.ravel()))
+    nd = NdDirect(op, m0, m1)
+    An = _nowrap(_scaled_A(op, pol, eps, npml, (0.3, 0.0)), *shape)
+    x = rng.standard_normal(eps.size) + 1j * rng.standard_normal(eps.size)
+    for tr in (False, True):
+        b = (An.T if tr else An) @ x
+        y = nd.apply(torch.from_numpy(b).cuda(), transpose=tr).cpu().numpy().astype(np.complex128)
+        err = np.linalg.norm(y - x) / np.linalg.norm(x)
+        print('nd', pol, shape, 'transpose', tr, 'err %.2e' % err, flush=True)
+        assert err <= 2e-3
+
+
+@pytest.mark.parametrize('pol', ['ez', 'hz'])
+def test_wide_grids_take_the_nested_dissection_path(pol):
+    """A grid wider than 128 in both directions: method 'auto' (and
+    'direct') factorises it by nested dissection; fields against spsolve and
+    the adjoint gradient against the oracle / finite differences."""
     import ceviche_amd as cv
-    for cls, shape in ((cv.fdfd_ez, (130, 131)), (cv.fdfd_hz, (129, 140))):
-        F = cls(OMEGA, 50e-9, np.ones(shape), [2, 2], solver_options=cv.SolverOptions(method='direct'))
-        with pytest.raises(ValueError):
-            F.solve(np.ones(shape))
+    rng = np.random.default_rng(23)
+    shape, npml = (160, 150), (12, 10)
+    eps0 = rng.uniform(1, 6, shape)
+    src = np.zeros(shape)
+    src[40, 30:120] = 1.0
+    cls = cv.fdfd_ez if pol == 'ez' else cv.fdfd_hz
+    for method in ('auto', 'direct'):
+        F = cls(OMEGA, 50e-9, eps0, npml, solver_options=cv.SolverOptions(method=method))
+        out = F.solve(src)
+        info = F.last_solve_info
+        print('wide', pol, method, info, flush=True)
+        assert info['method'] == 'direct' and info['factor'] == 'nd' and info['rel_residual'] <= 1e-10
+        ref = (R.solve_ez if pol == 'ez' else R.solve_hz)(OMEGA, 50e-9, eps0, npml, src)
+        for a, b in zip(out, ref):
+            assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-8
+    if pol == 'ez':
+        eps = torch.tensor(eps0, requires_grad=True)
+        F = cls(OMEGA, 50e-9, eps, npml)
+        _, _, Ez = F.solve(torch.from_numpy(src))
+        (Ez[100, 75].abs() ** 2).backward()
+        ez = Ez.detach().cpu().numpy()
+        cot = np.zeros_like(ez)
+        cot[100, 75] = 2 * ez[100, 75]
+        gref, _ = R.grad_eps_ez(OMEGA, 50e-9, eps0, npml, src, cot)
+        assert np.linalg.norm(eps.grad.numpy() - gref) <= 1e-6 * np.linalg.norm(gref)
 
 
 def test_direct_failure_falls_back_to_bicgstab():
