@@ -16,21 +16,25 @@
 //   internal: F = the two children's Schur complements on their boundaries
 //             (placed at their positions) + the A entries that couple the
 //             two children across their shared edge,
-// then factorises F_II = P L U, forms X = F_II^-1 F_IB and passes
-// S = F_BB - F_BI X up to its parent.  Nodes of equal height (distance to the
-// leaves) are processed together as strided batches padded to the level's
-// largest |I| (kmax, identity padding) and |B| (bmax, zero padding): dense LU
-// (rocsolver getrf / getrs) and the Schur products (rocblas gemm) run on the
-// matrix cores.  The symbolic analysis (tree, index sets, maps) is host C++
-// done once per grid shape.
+// then factorises F_II = P L U, forms [Q X] = F_II^-1 [I F_IB] by two
+// triangular solves, passes S = F_BB - F_BI X up to its parent and keeps
+// Q = F_II^-1, X and G = F_BI Q in place of F's blocks:
+//   F = [[Q, X], [G, S]].
+// Nodes of equal height (distance to the leaves) are processed together as
+// strided batches padded to the level's largest |I| (kmax, identity padding)
+// and |B| (bmax, zero padding): dense LU (rocsolver getrf), the triangular
+// solves (rocblas trsm) and the products (rocblas gemm) run on the matrix
+// cores.  The symbolic analysis (tree, index sets, maps) is host C++ done
+// once per grid shape.
 //
-// Solve of A x = b (transpose 0) with the stored F_II LU, X and F_BI:
-//   up the tree:   y_I = F_II^-1 b_I,  b_B -= F_BI y_I
+// Solve of A x = b (transpose 0), one matrix-vector product per node and sweep:
+//   up the tree:   [y_I; t] = [Q; G] b_I,  b_B -= t
 //   down the tree: x_I = y_I - X x_B
 // and of A^T x = b (transpose 1):
-//   up:   b_B -= X^T b_I,  y_I = F_II^-T b_I
-//   down: x_I = y_I - F_II^-T (F_BI^T x_B)
-// (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]] and F_IB^T F_II^-T = X^T).
+//   up:   [y_I; t] = [Q X]^T b_I,  b_B -= t
+//   down: x_I = y_I - G^T x_B
+// (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]], F_IB^T F_II^-T = X^T and
+// F_II^-T F_BI^T = G^T).
 #include "fdfd_stencil.h"
 
 #include <rocblas/rocblas.h>
@@ -44,6 +48,7 @@ namespace cv {
 namespace {
 
 constexpr int kLeafCells = 64;
+constexpr int kNdTrsmMax = 64;   // interior sizes solved by rocBLAS batched trsm (see cv_nd_factor)
 
 struct NdGroup {               // (parent, child) pairs whose child sits at one level
     int child_level = 0;
@@ -61,7 +66,7 @@ struct NdLevel {
     // byte offsets of the device copies (plan buffer) and of the numeric data (factor buffer)
     int64_t off_icells = 0, off_bcells = 0, off_lcell = 0, off_lnbr = 0;
     int64_t off_xnode = 0, off_xpu = 0, off_xpv = 0, off_xcell = 0, off_xdir = 0;
-    int64_t off_F = 0, off_piv = 0, info_first = 0;
+    int64_t off_F = 0, off_piv = 0, off_perm = 0, info_first = 0;
 };
 
 }  // namespace
@@ -71,7 +76,7 @@ struct cv_nd_plan {
     int nx = 0, ny = 0;
     std::vector<cv::NdLevel> L;
     int64_t plan_bytes = 0, factor_bytes = 0;
-    int64_t off_w = 0, off_y = 0, off_t = 0, off_u = 0, off_info = 0;
+    int64_t off_w = 0, off_y = 0, off_t = 0, off_u = 0, off_info = 0, off_r = 0;
     int64_t n_nodes = 0;
 };
 
@@ -266,7 +271,7 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         }
     }
     // device layouts: index arrays in the plan buffer, numeric data in the factor buffer
-    int64_t po = 0, fo = 0, ymax = 0, tmax = 0, info_first = 0;
+    int64_t po = 0, fo = 0, ymax = 0, tmax = 0, rmax = 0, info_first = 0;
     auto take = [](int64_t& o, int64_t bytes) {
         const int64_t r = o;
         o += al256(bytes);
@@ -289,10 +294,12 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         L.off_xdir = take(po, 4 * (int64_t)L.xdir.size());
         L.off_F = take(fo, 8 * (int64_t)L.nb * L.m * L.m);
         L.off_piv = take(fo, 4 * (int64_t)L.nb * std::max(1, L.kmax));
+        L.off_perm = take(fo, 4 * (int64_t)L.nb * std::max(1, L.kmax));
         L.info_first = info_first;
         info_first += L.nb;
-        ymax = std::max<int64_t>(ymax, (int64_t)L.nb * std::max(L.kmax, L.bmax));
+        ymax = std::max<int64_t>(ymax, (int64_t)L.nb * L.m);
         tmax = std::max<int64_t>(tmax, (int64_t)L.nb * std::max(L.kmax, L.bmax));
+        rmax = std::max<int64_t>(rmax, (int64_t)L.nb * L.kmax * L.m);
     }
     P->n_nodes = info_first;
     P->off_info = take(fo, 4 * info_first);
@@ -300,6 +307,7 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
     P->off_y = take(fo, 8 * std::max<int64_t>(1, ymax));
     P->off_t = take(fo, 8 * std::max<int64_t>(1, tmax));
     P->off_u = take(fo, 8 * std::max<int64_t>(1, tmax));
+    P->off_r = take(fo, 8 * std::max<int64_t>(1, rmax));
     P->plan_bytes = std::max<int64_t>(po, 256);
     P->factor_bytes = fo;
     return P;
@@ -381,19 +389,73 @@ k_nd_leaf(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ p
 }
 
 // children's Schur complements (the bottom-right bc x bc block of their F)
-// into the parents' F at the mapped positions; one workgroup per pair
+// into the parents' F at the mapped positions; blockIdx.y = pair, blockIdx.x
+// = a slice of kNdCols child columns (one wave per column, rows over lanes)
+constexpr int kNdCols = 4;
 __global__ void __launch_bounds__(256)
 k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int32_t* __restrict__ parent,
             const int32_t* __restrict__ child, const int32_t* __restrict__ map, cx<float>* __restrict__ F, int m) {
-    const int pr = blockIdx.x;
-    const cx<float>* S = Fc + (int64_t)child[pr] * mc * mc + (int64_t)kc * mc + kc;
-    cx<float>* Fp = F + (int64_t)parent[pr] * m * m;
+    const int pr = blockIdx.y;
+    const int jj = blockIdx.x * kNdCols + (threadIdx.x >> 6);
+    if (jj >= bc) return;
     const int32_t* mp = map + (int64_t)pr * bc;
-    for (int idx = threadIdx.x; idx < bc * bc; idx += blockDim.x) {
-        const int jj = idx / bc, ii = idx - jj * bc;
-        const int pi = mp[ii], pj = mp[jj];
-        if (pi >= 0 && pj >= 0) Fp[(int64_t)pj * m + pi] = S[(int64_t)jj * mc + ii];
+    const int pj = mp[jj];
+    if (pj < 0) return;
+    const cx<float>* Scol = Fc + (int64_t)child[pr] * mc * mc + (int64_t)(kc + jj) * mc + kc;
+    cx<float>* Fcol = F + (int64_t)parent[pr] * m * m + (int64_t)pj * m;
+    for (int ii = threadIdx.x & 63; ii < bc; ii += 64) {
+        const int pi = mp[ii];
+        if (pi >= 0) Fcol[pi] = Scol[ii];
     }
+}
+
+// the row permutation of each node's LU (rocsolver's 1-based sequential row
+// interchanges ipiv) as perm[i] = the original row that ends in row i
+__global__ void k_nd_perm(const rocblas_int* __restrict__ ipiv, int32_t* __restrict__ perm, int nb, int k) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb) return;
+    int32_t* p = perm + (int64_t)e * k;
+    const rocblas_int* v = ipiv + (int64_t)e * k;
+    for (int i = 0; i < k; ++i) p[i] = i;
+    for (int i = 0; i < k; ++i) {
+        const int r = v[i] - 1;
+        if (r != i) {
+            const int32_t t = p[i];
+            p[i] = p[r];
+            p[r] = t;
+        }
+    }
+}
+
+// R = P [I F_IB] (k x m per node, column-major, ld k): R[i, j] = (perm[i] == j)
+// for j < k, F[perm[i], j] for j >= k; perm NULL: R = [I F_IB]
+__global__ void __launch_bounds__(256)
+k_nd_rhs(const cx<float>* __restrict__ F, const int32_t* __restrict__ perm, cx<float>* __restrict__ R, int nb, int k,
+         int m) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * k * m) return;
+    const int64_t km = (int64_t)k * m;
+    const int e = (int)(t / km);
+    const int64_t r = t - (int64_t)e * km;
+    const int j = (int)(r / k), i = (int)(r - (int64_t)j * k);
+    const int pi = perm ? perm[(int64_t)e * k + i] : i;
+    cx<float> v;
+    if (j < k) v = (pi == j) ? mk<float>(1.f, 0.f) : czero<float>();
+    else v = F[(int64_t)e * m * m + (int64_t)j * m + pi];
+    R[t] = v;
+}
+
+// copy a rows x cols block between per-node column-major matrices
+__global__ void __launch_bounds__(256)
+k_nd_block_copy(const cx<float>* __restrict__ src, int64_t src_stride, int src_ld, cx<float>* __restrict__ dst,
+                int64_t dst_stride, int dst_ld, int nb, int rows, int cols) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)rows * cols;
+    if (t >= (int64_t)nb * per) return;
+    const int e = (int)(t / per);
+    const int64_t r = t - (int64_t)e * per;
+    const int j = (int)(r / rows), i = (int)(r - (int64_t)j * rows);
+    dst[(int64_t)e * dst_stride + (int64_t)j * dst_ld + i] = src[(int64_t)e * src_stride + (int64_t)j * src_ld + i];
 }
 
 // couplings across the children's shared edge: F[pu, pv] = A[u, v]
@@ -437,6 +499,23 @@ k_nd_scatter(const int32_t* __restrict__ cells, int64_t n, const cx<float>* __re
     const int32_t c = cells[t];
     if (c < 0) return;
     w[c] = sub ? w[c] - y[t] : y[t];
+}
+
+// up-sweep result Z (nb x m per node): w[I] = Z[0:k], w[B] -= Z[k:m]
+__global__ void __launch_bounds__(256)
+k_nd_scatter_up(const int32_t* __restrict__ icells, const int32_t* __restrict__ bcells, int nb, int k, int b,
+                const cx<float>* __restrict__ Z, cx<float>* __restrict__ w) {
+    const int m = k + b;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * m) return;
+    const int e = (int)(t / m), p = (int)(t - (int64_t)e * m);
+    if (p < k) {
+        const int32_t c = icells[(int64_t)e * k + p];
+        if (c >= 0) w[c] = Z[t];
+    } else {
+        const int32_t c = bcells[(int64_t)e * b + (p - k)];
+        if (c >= 0) w[c] = w[c] - Z[t];
+    }
 }
 
 __global__ void __launch_bounds__(256) k_nd_copy(const cx<float>* __restrict__ a, cx<float>* __restrict__ b, int64_t n) {
@@ -491,6 +570,7 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
     cx<float>* Y = (cx<float>*)(fb + P->off_y);
     cx<float>* T = (cx<float>*)(fb + P->off_t);
     cx<float>* U = (cx<float>*)(fb + P->off_u);
+    cx<float>* Z = (cx<float>*)(fb + P->off_r);
     rocblas_handle h;
     {
         const int rc = get_handle(s, &h);
@@ -499,56 +579,43 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
     const rc_t one{1.f, 0.f}, zero{0.f, 0.f};
     k_nd_copy<<<nblk(N), 256, 0, s>>>(v, w, N);
     const int H = (int)P->L.size();
-    const rocblas_operation op = transpose ? rocblas_operation_transpose : rocblas_operation_none;
-    for (int l = 0; l < H; ++l) {   // up the tree
+    for (int l = 0; l < H; ++l) {   // up the tree: [y_I; t] = [Q; G] b_I (or [Q X]^T b_I)
         const NdLevel& L = P->L[l];
         if (L.kmax == 0) continue;
         const int32_t* ic = (const int32_t*)(pb + L.off_icells);
         const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
-        rc_t* F = (rc_t*)(fb + L.off_F);
-        const rocblas_int* piv = (const rocblas_int*)(fb + L.off_piv);
-        const int64_t nI = (int64_t)L.nb * L.kmax, nB = (int64_t)L.nb * L.bmax;
+        const rc_t* F = (const rc_t*)(fb + L.off_F);
+        const int64_t nI = (int64_t)L.nb * L.kmax;
         const rocblas_stride sF = (rocblas_stride)L.m * L.m;
         k_nd_gather<<<nblk(nI), 256, 0, s>>>(ic, nI, w, Y);
         CV_LAUNCH_CHECK();
-        if (transpose && L.bmax > 0) {   // b_B -= X^T b_I
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.kmax, L.bmax, &one,
-                                                F + (int64_t)L.kmax * L.m, L.m, sF, (const rc_t*)Y, 1, L.kmax, &zero,
-                                                (rc_t*)T, 1, L.bmax, L.nb));
-            k_nd_scatter<<<nblk(nB), 256, 0, s>>>(bc, nB, T, w, 1);
-        }
-        CV_RB(rocsolver_cgetrs_strided_batched(h, op, L.kmax, 1, F, L.m, sF, piv, L.kmax, (rc_t*)Y, L.kmax, L.kmax,
-                                               L.nb));
-        k_nd_scatter<<<nblk(nI), 256, 0, s>>>(ic, nI, Y, w, 0);
-        if (!transpose && L.bmax > 0) {   // b_B -= F_BI y_I
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.bmax, L.kmax, &one, F + L.kmax, L.m, sF,
-                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)T, 1, L.bmax, L.nb));
-            k_nd_scatter<<<nblk(nB), 256, 0, s>>>(bc, nB, T, w, 1);
-        }
+        if (!transpose)
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.m, L.kmax, &one, F, L.m, sF,
+                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)Z, 1, L.m, L.nb));
+        else
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.kmax, L.m, &one, F, L.m, sF,
+                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)Z, 1, L.m, L.nb));
+        k_nd_scatter_up<<<nblk((int64_t)L.nb * L.m), 256, 0, s>>>(ic, bc, L.nb, L.kmax, L.bmax, Z, w);
         CV_LAUNCH_CHECK();
     }
-    for (int l = H - 1; l >= 0; --l) {   // down the tree
+    for (int l = H - 1; l >= 0; --l) {   // down the tree: x_I -= X x_B (or G^T x_B)
         const NdLevel& L = P->L[l];
         if (L.kmax == 0 || L.bmax == 0) continue;
         const int32_t* ic = (const int32_t*)(pb + L.off_icells);
         const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
-        rc_t* F = (rc_t*)(fb + L.off_F);
-        const rocblas_int* piv = (const rocblas_int*)(fb + L.off_piv);
+        const rc_t* F = (const rc_t*)(fb + L.off_F);
         const int64_t nI = (int64_t)L.nb * L.kmax, nB = (int64_t)L.nb * L.bmax;
         const rocblas_stride sF = (rocblas_stride)L.m * L.m;
         k_nd_gather<<<nblk(nB), 256, 0, s>>>(bc, nB, w, T);
         CV_LAUNCH_CHECK();
-        if (!transpose) {   // x_I = y_I - X x_B
+        if (!transpose)
             CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.kmax, L.bmax, &one,
                                                 F + (int64_t)L.kmax * L.m, L.m, sF, (const rc_t*)T, 1, L.bmax, &zero,
                                                 (rc_t*)U, 1, L.kmax, L.nb));
-        } else {            // x_I = y_I - F_II^-T (F_BI^T x_B)
+        else
             CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.bmax, L.kmax, &one, F + L.kmax,
                                                 L.m, sF, (const rc_t*)T, 1, L.bmax, &zero, (rc_t*)U, 1, L.kmax,
                                                 L.nb));
-            CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_transpose, L.kmax, 1, F, L.m, sF, piv,
-                                                   L.kmax, (rc_t*)U, L.kmax, L.kmax, L.nb));
-        }
         k_nd_scatter<<<nblk(nI), 256, 0, s>>>(ic, nI, U, w, 1);
         CV_LAUNCH_CHECK();
     }
@@ -582,7 +649,7 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
     const float* m0 = (const float*)m0_32;
     const float* m1 = (const float*)m1_32;
     rocblas_int* info = (rocblas_int*)(fb + P->off_info);
-    const rc_t minus_one{-1.f, 0.f}, one{1.f, 0.f};
+    const rc_t minus_one{-1.f, 0.f}, one{1.f, 0.f}, zero_c{0.f, 0.f};
     for (size_t l = 0; l < P->L.size(); ++l) {
         const NdLevel& L = P->L[l];
         rc_t* F = (rc_t*)(fb + L.off_F);
@@ -596,7 +663,8 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             for (const auto& g : L.groups) {
                 const NdLevel& C = P->L[g.child_level];
                 if (C.bmax == 0 || g.parent.empty()) continue;
-                k_nd_extend<<<(int)g.parent.size(), 256, 0, s>>>(
+                const dim3 grid((C.bmax + kNdCols - 1) / kNdCols, (unsigned)g.parent.size());
+                k_nd_extend<<<grid, 256, 0, s>>>(
                     (const cx<float>*)(fb + C.off_F), C.m, C.kmax, C.bmax, (const int32_t*)(pb + g.off_parent),
                     (const int32_t*)(pb + g.off_child), (const int32_t*)(pb + g.off_map), (cx<float>*)F, L.m);
             }
@@ -612,15 +680,48 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
         }
         CV_LAUNCH_CHECK();
         if (L.kmax == 0) continue;
+        const int k = L.kmax, b = L.bmax, m = L.m;
         rocblas_int* piv = (rocblas_int*)(fb + L.off_piv);
-        CV_RB(rocsolver_cgetrf_strided_batched(h, L.kmax, L.kmax, F, L.m, sF, piv, L.kmax, info + L.info_first, L.nb));
-        if (L.bmax == 0) continue;
-        // X = F_II^-1 F_IB in place, then S = F_BB - F_BI X
-        CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, L.kmax, L.bmax, F, L.m, sF, piv, L.kmax,
-                                               F + (int64_t)L.kmax * L.m, L.m, sF, L.nb));
-        CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, L.bmax, L.bmax, L.kmax,
-                                            &minus_one, F + L.kmax, L.m, sF, F + (int64_t)L.kmax * L.m, L.m, sF, &one,
-                                            F + (int64_t)L.kmax * L.m + L.kmax, L.m, sF, L.nb));
+        int32_t* perm = (int32_t*)(fb + L.off_perm);
+        rc_t* R = (rc_t*)(fb + P->off_r);
+        const rocblas_stride sR = (rocblas_stride)k * m;
+        CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
+        // [Q X] = F_II^-1 [I F_IB].  Up to kNdTrsmMax interior cells the
+        // batched triangular solves (one launch for all nodes) are fastest:
+        // R = P [I F_IB], then L and U solves; above it rocBLAS's blocked
+        // triangular solve issues one product per node and block, so
+        // rocsolver's batched getrs (pivots applied inside) is used instead.
+        if (k <= kNdTrsmMax) {
+            k_nd_perm<<<nblk(L.nb), 256, 0, s>>>(piv, perm, L.nb, k);
+            k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, perm, (cx<float>*)R, L.nb, k,
+                                                                 m);
+            CV_LAUNCH_CHECK();
+            CV_RB(rocblas_ctrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                                rocblas_diagonal_unit, k, m, &one, F, m, sF, R, k, sR, L.nb));
+            CV_RB(rocblas_ctrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                                rocblas_diagonal_non_unit, k, m, &one, F, m, sF, R, k, sR, L.nb));
+        } else {
+            k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, nullptr, (cx<float>*)R, L.nb,
+                                                                 k, m);
+            CV_LAUNCH_CHECK();
+            CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, k, m, F, m, sF, piv, k, R, k, sR,
+                                                   L.nb));
+        }
+        if (b > 0)   // S = F_BB - F_BI X
+            CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
+                                                &minus_one, F + k, m, sF, R + (int64_t)k * k, k, sR, &one,
+                                                F + (int64_t)k * m + k, m, sF, L.nb));
+        // [Q X] into the top k rows of F
+        k_nd_block_copy<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)R, sR, k, (cx<float>*)F, sF, m,
+                                                                    L.nb, k, m);
+        CV_LAUNCH_CHECK();
+        if (b > 0) {   // G = F_BI Q (into R, then over F_BI)
+            CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, k, k, &one,
+                                                F + k, m, sF, F, m, sF, &zero_c, R, b, (rocblas_stride)b * k, L.nb));
+            k_nd_block_copy<<<nblk((int64_t)L.nb * b * k), 256, 0, s>>>((const cx<float>*)R, (int64_t)b * k, b,
+                                                                        (cx<float>*)F + k, sF, m, L.nb, b, k);
+            CV_LAUNCH_CHECK();
+        }
     }
     std::vector<rocblas_int> hinfo(P->n_nodes);
     CV_HIP(hipMemcpyAsync(hinfo.data(), info, 4 * (size_t)P->n_nodes, hipMemcpyDeviceToHost, s));
