@@ -150,6 +150,7 @@ _SIGS = {
     'cv_nd_plan_destroy': (None, [P]),
     'cv_nd_plan_bytes': (I64, [P]),
     'cv_nd_factor_bytes': (I64, [P]),
+    'cv_nd_plan_flops': (D, [P]),
     'cv_nd_plan_stats': (I, [P, P, P, P, P, I]),
     'cv_nd_plan_upload': (I, [P, P, I64, P]),
     'cv_nd_factor': (I, [P, P, P, P, P, P, P, P, I64, P]),

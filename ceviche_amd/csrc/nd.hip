@@ -78,6 +78,7 @@ struct cv_nd_plan {
     int64_t plan_bytes = 0, factor_bytes = 0;
     int64_t off_w = 0, off_y = 0, off_t = 0, off_u = 0, off_info = 0, off_r = 0;
     int64_t n_nodes = 0;
+    double flops = 0.0;   // real flops of one numeric factorisation (unpadded sizes)
 };
 
 namespace cv {
@@ -168,6 +169,13 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
             for (int32_t c : U)
                 if (!std::binary_search(b.B.begin(), b.B.end(), c)) b.I.push_back(c);
         }
+    }
+    // algorithmic work of one factorisation, real flops of complex arithmetic
+    // (a complex multiply-add is 8): LU k^3/3, [Q X] = F_II^-1 [I F_IB]
+    // k^2 (k + b), S b^2 k and G b k^2 complex multiply-adds per node
+    for (const auto& b : nodes) {
+        const double k = (double)b.I.size(), bb = (double)b.B.size();
+        P->flops += 8.0 * (k * k * k / 3.0 + k * k * (k + bb) + bb * bb * k + bb * k * k);
     }
     // group by height
     P->L.resize(H + 1);
@@ -317,6 +325,7 @@ extern "C" void cv_nd_plan_destroy(cv_nd_plan* p) { delete p; }
 
 extern "C" int64_t cv_nd_plan_bytes(const cv_nd_plan* p) { return p ? p->plan_bytes : -1; }
 extern "C" int64_t cv_nd_factor_bytes(const cv_nd_plan* p) { return p ? p->factor_bytes : -1; }
+extern "C" double cv_nd_plan_flops(const cv_nd_plan* p) { return p ? p->flops : -1.0; }
 
 extern "C" int cv_nd_plan_stats(const cv_nd_plan* p, int* levels, int* nb, int* kmax, int* bmax, int max_levels) {
     CV_CHECK(p, "null plan");

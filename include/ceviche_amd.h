@@ -587,6 +587,7 @@ cv_nd_plan* cv_nd_plan_create(int nx, int ny);
 void cv_nd_plan_destroy(cv_nd_plan* plan);
 int64_t cv_nd_plan_bytes(const cv_nd_plan* plan);
 int64_t cv_nd_factor_bytes(const cv_nd_plan* plan);
+double cv_nd_plan_flops(const cv_nd_plan* plan);   /* real flops of one cv_nd_factor (unpadded) */
 int cv_nd_plan_stats(const cv_nd_plan* plan, int* levels, int* nb, int* kmax, int* bmax, int max_levels);
 int cv_nd_plan_upload(const cv_nd_plan* plan, void* plan_dev, int64_t plan_bytes, void* stream);
 int cv_nd_factor(const cv_nd_plan* plan, const void* plan_dev, const cv_fdfd_desc* desc,
