@@ -16,25 +16,24 @@
 //   internal: F = the two children's Schur complements on their boundaries
 //             (placed at their positions) + the A entries that couple the
 //             two children across their shared edge,
-// then factorises F_II = P L U, forms [Q X] = F_II^-1 [I F_IB] by two
-// triangular solves, passes S = F_BB - F_BI X up to its parent and keeps
-// Q = F_II^-1, X and G = F_BI Q in place of F's blocks:
-//   F = [[Q, X], [G, S]].
+// then factorises F_II = P L U, inverts it in place (Q = F_II^-1 from the LU
+// factors), forms X = Q F_IB and passes S = F_BB - F_BI X up to its parent.
+// F keeps Q, X and the original F_BI in place of its blocks:
+//   F = [[Q, X], [F_BI, S]].
 // Nodes of equal height (distance to the leaves) are processed together as
 // strided batches padded to the level's largest |I| (kmax, identity padding)
-// and |B| (bmax, zero padding): dense LU (rocsolver getrf), the triangular
-// solves (rocblas trsm) and the products (rocblas gemm) run on the matrix
-// cores.  The symbolic analysis (tree, index sets, maps) is host C++ done
-// once per grid shape.
+// and |B| (bmax, zero padding): dense LU and inversion (rocsolver getrf /
+// getri) and the two products (rocblas gemm) run on the matrix cores, one
+// launch sequence per level.  The symbolic analysis (tree, index sets, maps)
+// is host C++ done once per grid shape.
 //
-// Solve of A x = b (transpose 0), one matrix-vector product per node and sweep:
-//   up the tree:   [y_I; t] = [Q; G] b_I,  b_B -= t
+// Solve of A x = b (transpose 0), matrix-vector products per node and sweep:
+//   up the tree:   y_I = Q b_I,  b_B -= F_BI y_I
 //   down the tree: x_I = y_I - X x_B
 // and of A^T x = b (transpose 1):
 //   up:   [y_I; t] = [Q X]^T b_I,  b_B -= t
-//   down: x_I = y_I - G^T x_B
-// (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]], F_IB^T F_II^-T = X^T and
-// F_II^-T F_BI^T = G^T).
+//   down: x_I = y_I - Q^T (F_BI^T x_B)
+// (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]], F_IB^T F_II^-T = X^T).
 #include "fdfd_stencil.h"
 
 #include <rocblas/rocblas.h>
@@ -48,7 +47,6 @@ namespace cv {
 namespace {
 
 constexpr int kLeafCells = 64;
-constexpr int kNdTrsmMax = 64;   // interior sizes solved by rocBLAS batched trsm (see cv_nd_factor)
 
 struct NdGroup {               // (parent, child) pairs whose child sits at one level
     int child_level = 0;
@@ -66,7 +64,7 @@ struct NdLevel {
     // byte offsets of the device copies (plan buffer) and of the numeric data (factor buffer)
     int64_t off_icells = 0, off_bcells = 0, off_lcell = 0, off_lnbr = 0;
     int64_t off_xnode = 0, off_xpu = 0, off_xpv = 0, off_xcell = 0, off_xdir = 0;
-    int64_t off_F = 0, off_piv = 0, off_perm = 0, info_first = 0;
+    int64_t off_F = 0, off_piv = 0, info_first = 0;
 };
 
 }  // namespace
@@ -171,11 +169,11 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         }
     }
     // algorithmic work of one factorisation, real flops of complex arithmetic
-    // (a complex multiply-add is 8): LU k^3/3, [Q X] = F_II^-1 [I F_IB]
-    // k^2 (k + b), S b^2 k and G b k^2 complex multiply-adds per node
+    // (a complex multiply-add is 8): LU k^3/3, inverse from the LU factors
+    // 2k^3/3, X = Q F_IB k^2 b and S b^2 k complex multiply-adds per node
     for (const auto& b : nodes) {
         const double k = (double)b.I.size(), bb = (double)b.B.size();
-        P->flops += 8.0 * (k * k * k / 3.0 + k * k * (k + bb) + bb * bb * k + bb * k * k);
+        P->flops += 8.0 * (k * k * k + k * k * bb + bb * bb * k);
     }
     // group by height
     P->L.resize(H + 1);
@@ -302,7 +300,6 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         L.off_xdir = take(po, 4 * (int64_t)L.xdir.size());
         L.off_F = take(fo, 8 * (int64_t)L.nb * L.m * L.m);
         L.off_piv = take(fo, 4 * (int64_t)L.nb * std::max(1, L.kmax));
-        L.off_perm = take(fo, 4 * (int64_t)L.nb * std::max(1, L.kmax));
         L.info_first = info_first;
         info_first += L.nb;
         ymax = std::max<int64_t>(ymax, (int64_t)L.nb * L.m);
@@ -416,42 +413,6 @@ k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int3
         const int pi = mp[ii];
         if (pi >= 0) Fcol[pi] = Scol[ii];
     }
-}
-
-// the row permutation of each node's LU (rocsolver's 1-based sequential row
-// interchanges ipiv) as perm[i] = the original row that ends in row i
-__global__ void k_nd_perm(const rocblas_int* __restrict__ ipiv, int32_t* __restrict__ perm, int nb, int k) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nb) return;
-    int32_t* p = perm + (int64_t)e * k;
-    const rocblas_int* v = ipiv + (int64_t)e * k;
-    for (int i = 0; i < k; ++i) p[i] = i;
-    for (int i = 0; i < k; ++i) {
-        const int r = v[i] - 1;
-        if (r != i) {
-            const int32_t t = p[i];
-            p[i] = p[r];
-            p[r] = t;
-        }
-    }
-}
-
-// R = P [I F_IB] (k x m per node, column-major, ld k): R[i, j] = (perm[i] == j)
-// for j < k, F[perm[i], j] for j >= k; perm NULL: R = [I F_IB]
-__global__ void __launch_bounds__(256)
-k_nd_rhs(const cx<float>* __restrict__ F, const int32_t* __restrict__ perm, cx<float>* __restrict__ R, int nb, int k,
-         int m) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nb * k * m) return;
-    const int64_t km = (int64_t)k * m;
-    const int e = (int)(t / km);
-    const int64_t r = t - (int64_t)e * km;
-    const int j = (int)(r / k), i = (int)(r - (int64_t)j * k);
-    const int pi = perm ? perm[(int64_t)e * k + i] : i;
-    cx<float> v;
-    if (j < k) v = (pi == j) ? mk<float>(1.f, 0.f) : czero<float>();
-    else v = F[(int64_t)e * m * m + (int64_t)j * m + pi];
-    R[t] = v;
 }
 
 // copy a rows x cols block between per-node column-major matrices
@@ -588,7 +549,7 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
     const rc_t one{1.f, 0.f}, zero{0.f, 0.f};
     k_nd_copy<<<nblk(N), 256, 0, s>>>(v, w, N);
     const int H = (int)P->L.size();
-    for (int l = 0; l < H; ++l) {   // up the tree: [y_I; t] = [Q; G] b_I (or [Q X]^T b_I)
+    for (int l = 0; l < H; ++l) {   // up the tree: [y_I; t] = [Q; F_BI Q] b_I (or [Q X]^T b_I)
         const NdLevel& L = P->L[l];
         if (L.kmax == 0) continue;
         const int32_t* ic = (const int32_t*)(pb + L.off_icells);
@@ -598,16 +559,20 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
         const rocblas_stride sF = (rocblas_stride)L.m * L.m;
         k_nd_gather<<<nblk(nI), 256, 0, s>>>(ic, nI, w, Y);
         CV_LAUNCH_CHECK();
-        if (!transpose)
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.m, L.kmax, &one, F, L.m, sF,
+        if (!transpose) {   // y_I = Q b_I into Z[0:k], t = F_BI y_I into Z[k:m]
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.kmax, L.kmax, &one, F, L.m, sF,
                                                 (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)Z, 1, L.m, L.nb));
-        else
+            if (L.bmax > 0)
+                CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.bmax, L.kmax, &one, F + L.kmax, L.m,
+                                                    sF, (const rc_t*)Z, 1, L.m, &zero, (rc_t*)Z + L.kmax, 1, L.m,
+                                                    L.nb));
+        } else
             CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.kmax, L.m, &one, F, L.m, sF,
                                                 (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)Z, 1, L.m, L.nb));
         k_nd_scatter_up<<<nblk((int64_t)L.nb * L.m), 256, 0, s>>>(ic, bc, L.nb, L.kmax, L.bmax, Z, w);
         CV_LAUNCH_CHECK();
     }
-    for (int l = H - 1; l >= 0; --l) {   // down the tree: x_I -= X x_B (or G^T x_B)
+    for (int l = H - 1; l >= 0; --l) {   // down the tree: x_I -= X x_B (or Q^T F_BI^T x_B)
         const NdLevel& L = P->L[l];
         if (L.kmax == 0 || L.bmax == 0) continue;
         const int32_t* ic = (const int32_t*)(pb + L.off_icells);
@@ -621,10 +586,13 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
             CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.kmax, L.bmax, &one,
                                                 F + (int64_t)L.kmax * L.m, L.m, sF, (const rc_t*)T, 1, L.bmax, &zero,
                                                 (rc_t*)U, 1, L.kmax, L.nb));
-        else
+        else {
             CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.bmax, L.kmax, &one, F + L.kmax,
-                                                L.m, sF, (const rc_t*)T, 1, L.bmax, &zero, (rc_t*)U, 1, L.kmax,
+                                                L.m, sF, (const rc_t*)T, 1, L.bmax, &zero, (rc_t*)Y, 1, L.kmax,
                                                 L.nb));
+            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.kmax, L.kmax, &one, F, L.m, sF,
+                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)U, 1, L.kmax, L.nb));
+        }
         k_nd_scatter<<<nblk(nI), 256, 0, s>>>(ic, nI, U, w, 1);
         CV_LAUNCH_CHECK();
     }
@@ -691,44 +659,24 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
         if (L.kmax == 0) continue;
         const int k = L.kmax, b = L.bmax, m = L.m;
         rocblas_int* piv = (rocblas_int*)(fb + L.off_piv);
-        int32_t* perm = (int32_t*)(fb + L.off_perm);
         rc_t* R = (rc_t*)(fb + P->off_r);
-        const rocblas_stride sR = (rocblas_stride)k * m;
+        const rocblas_stride sR = (rocblas_stride)k * std::max(b, 1);
+        // Q = F_II^-1 in place: LU with partial pivoting, then the inverse
+        // from the factors (rocsolver's batched getri works on blocks and
+        // products, where triangular solves with k or m right-hand sides
+        // issue one small launch per node and block)
         CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
-        // [Q X] = F_II^-1 [I F_IB].  Up to kNdTrsmMax interior cells the
-        // batched triangular solves (one launch for all nodes) are fastest:
-        // R = P [I F_IB], then L and U solves; above it rocBLAS's blocked
-        // triangular solve issues one product per node and block, so
-        // rocsolver's batched getrs (pivots applied inside) is used instead.
-        if (k <= kNdTrsmMax) {
-            k_nd_perm<<<nblk(L.nb), 256, 0, s>>>(piv, perm, L.nb, k);
-            k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, perm, (cx<float>*)R, L.nb, k,
-                                                                 m);
-            CV_LAUNCH_CHECK();
-            CV_RB(rocblas_ctrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                                rocblas_diagonal_unit, k, m, &one, F, m, sF, R, k, sR, L.nb));
-            CV_RB(rocblas_ctrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                                                rocblas_diagonal_non_unit, k, m, &one, F, m, sF, R, k, sR, L.nb));
-        } else {
-            k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, nullptr, (cx<float>*)R, L.nb,
-                                                                 k, m);
-            CV_LAUNCH_CHECK();
-            CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, k, m, F, m, sF, piv, k, R, k, sR,
-                                                   L.nb));
-        }
-        if (b > 0)   // S = F_BB - F_BI X
+        CV_RB(rocsolver_cgetri_strided_batched(h, k, F, m, sF, piv, k, info + L.info_first, L.nb));
+        if (b > 0) {
+            // X = Q F_IB (into R), S = F_BB - F_BI X, then X over F_IB
+            CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, k, b, k, &one, F,
+                                                m, sF, F + (int64_t)k * m, m, sF, &zero_c, R, k, sR, L.nb));
             CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
-                                                &minus_one, F + k, m, sF, R + (int64_t)k * k, k, sR, &one,
-                                                F + (int64_t)k * m + k, m, sF, L.nb));
-        // [Q X] into the top k rows of F
-        k_nd_block_copy<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)R, sR, k, (cx<float>*)F, sF, m,
-                                                                    L.nb, k, m);
-        CV_LAUNCH_CHECK();
-        if (b > 0) {   // G = F_BI Q (into R, then over F_BI)
-            CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, k, k, &one,
-                                                F + k, m, sF, F, m, sF, &zero_c, R, b, (rocblas_stride)b * k, L.nb));
-            k_nd_block_copy<<<nblk((int64_t)L.nb * b * k), 256, 0, s>>>((const cx<float>*)R, (int64_t)b * k, b,
-                                                                        (cx<float>*)F + k, sF, m, L.nb, b, k);
+                                                &minus_one, F + k, m, sF, R, k, sR, &one, F + (int64_t)k * m + k, m,
+                                                sF, L.nb));
+            k_nd_block_copy<<<nblk((int64_t)L.nb * k * b), 256, 0, s>>>((const cx<float>*)R, sR, k,
+                                                                        (cx<float>*)F + (int64_t)k * m, sF, m, L.nb,
+                                                                        k, b);
             CV_LAUNCH_CHECK();
         }
     }
