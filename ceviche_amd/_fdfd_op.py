@@ -29,11 +29,11 @@ _CODE_OF = {torch.complex128: _lib.CV_C128, torch.complex64: _lib.CV_C64}
 class SolverOptions:
     """Solver settings.  tol is the relative FP64 residual ||b - A x|| / ||b||.
 
-    method: 'auto' (a direct factorisation + FP64-refined FGMRES for a
-    single uncoupled instance: block cyclic reduction when ny <= 128, or
-    nx <= 128 on the transposed grid, with real materials, nested dissection
-    otherwise; BiCGSTAB for batches and coupled systems or when the factor
-    does not fit in half the free device memory), 'direct' (require a
+    method: 'auto' (a direct factorisation + FP64-refined FGMRES for each
+    uncoupled instance, one instance after another in a batch: block cyclic
+    reduction when ny <= 128, or nx <= 128 on the transposed grid, with real
+    materials, nested dissection otherwise; BiCGSTAB for coupled systems or
+    when a factor does not fit in half the free device memory), 'direct' (require a
     factorisation), 'bicgstab', or 'cocg' (COCG on the complex-symmetric
     form: zero Bloch phases, uncoupled instances).  restart: FGMRES
     cycle length of the direct path; direct_max_iter: its FGMRES step limit
@@ -635,17 +635,40 @@ def _factor_bytes(op, kind):
     return int(lib().cv_nd_factor_bytes(plan.ptr))
 
 
+def instance_operators(op):
+    """Single-instance operators (same polarisation, precision and specs) of
+    the instances of an uncoupled batch, cached on ``op``."""
+    if getattr(op, '_instances', None) is None:
+        op._instances = [FdfdOperator(op.pol, [sp], dtype=op.dtype, device=op.device, complex_materials=op.cmat)
+                         for sp in op.specs]
+    return op._instances
+
+
 def _use_direct(op, opts):
-    """The direct factorisation to use ('bcr', 'nd') or None.  method 'direct'
-    requires one; 'auto' takes it when it applies and its factor fits in
-    DIRECT_MEMORY_FRACTION of the free device memory."""
+    """The direct factorisation to use ('bcr', 'nd', 'instances' for an
+    uncoupled batch factorised instance by instance) or None.  method
+    'direct' requires one; 'auto' takes it when it applies and a factor (the
+    largest instance's, for a batch) fits in DIRECT_MEMORY_FRACTION of the
+    free device memory."""
     method = (opts or SolverOptions()).method
     if method in ('bicgstab', 'cocg'):
         return None
+    if op.n_inst > 1 and op.cpl_len == 0:
+        subs = instance_operators(op)
+        kinds = [_direct_kind(sub) for sub in subs]
+        if None in kinds:
+            if method == 'direct':
+                raise ValueError("method='direct' needs every instance to have a direct factorisation")
+            return None
+        if method == 'direct':
+            return 'instances'
+        free, _ = torch.cuda.mem_get_info(op.device)
+        need = max(_factor_bytes(sub, kd) for sub, kd in zip(subs, kinds))
+        return 'instances' if need <= DIRECT_MEMORY_FRACTION * free else None
     kind = _direct_kind(op)
     if method == 'direct':
         if kind is None:
-            raise ValueError("method='direct' needs one uncoupled instance")
+            raise ValueError("method='direct' needs uncoupled instances")
         return kind
     if kind is None:
         return None
@@ -654,7 +677,59 @@ def _use_direct(op, opts):
 
 
 def make_direct(op, m0, m1, kind):
+    if kind == 'instances':
+        return InstanceDirect(op, m0, m1)
     return Direct(op, m0, m1) if kind == 'bcr' else NdDirect(op, m0, m1)
+
+
+class InstanceDirect:
+    """Direct solves of an uncoupled batch, instance by instance: each
+    instance is factorised by its own kind (block cyclic reduction or nested
+    dissection) and solved by FP64-refined FGMRES.  The factors are kept for
+    the adjoint and tangent solves when all of them fit in
+    DIRECT_MEMORY_FRACTION of the free device memory; otherwise each solve
+    factorises its instance again and releases the factor."""
+
+    kind = 'instances'
+
+    def __init__(self, op, m0, m1):
+        self.op = op
+        self.subs = instance_operators(op)
+        self.kinds = [_direct_kind(sub) for sub in self.subs]
+        if None in self.kinds:
+            raise ValueError('every instance of the batch needs a direct factorisation')
+        self.m0, self.m1 = m0, m1
+        self.sizes = [nx * ny for nx, ny in op.shapes]
+        free, _ = torch.cuda.mem_get_info(op.device)
+        total = sum(_factor_bytes(sub, kd) for sub, kd in zip(self.subs, self.kinds))
+        self.keep = total <= DIRECT_MEMORY_FRACTION * free
+        self.factors = [None] * op.n_inst
+        if self.keep:   # factorise now, so a singular instance falls back before any solve
+            for k in range(op.n_inst):
+                self.factors[k] = self._make(k)
+
+    def _seg(self, v, k):
+        if v is None:
+            return None
+        o = self.op.offsets[k]
+        return v[o:o + self.sizes[k]]
+
+    def _make(self, k):
+        return make_direct(self.subs[k], self._seg(self.m0, k), self._seg(self.m1, k), self.kinds[k])
+
+    def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
+        x = torch.empty(bs.shape, dtype=torch.complex128, device=bs.device)
+        info = []
+        for k in range(self.op.n_inst):
+            f = self.factors[k] if self.factors[k] is not None else self._make(k)
+            sub = self.subs[k]
+            x[self.op.offsets[k]:self.op.offsets[k] + self.sizes[k]] = f.solve(
+                self._seg(bs, k).contiguous(), self._seg(m0_64, k).contiguous(),
+                None if m1_64 is None else self._seg(m1_64, k).contiguous(), transpose=transpose, opts=opts)
+            info += sub.last_info
+            del f
+        self.op.last_info = info
+        return x
 
 
 class NdPlan:
