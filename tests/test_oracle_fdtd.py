@@ -135,3 +135,21 @@ def test_complex_step_matches_central_differences_on_reference_setup():
         cs = np.imag(obj(eps + 1j * h * d, dtype=np.complex128)) / h
         fd = (obj(eps + 1e-6 * d) - obj(eps - 1e-6 * d)) / 2e-6
         assert cs == pytest.approx(fd, rel=1e-5, abs=1e-8 * abs(cs) + 1e-30)
+
+
+def test_tm_400_golden_fixtures_are_the_oracle_run():
+    """The committed 400x400 TM fixtures: the first 300 steps of the trace
+    recomputed by the oracle, and the adjoint fixture's probe energy equal to
+    the sum of squares of the committed trace."""
+    import os
+    from ceviche_amd.workloads import fdtd_tm_blobs
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+    tr = np.load(os.path.join(gold, 'tm_blobs_400_trace.npy'))
+    c = fdtd_tm_blobs()
+    assert tr.shape == (c['steps'], 4)
+    head = T.run_probes(c['eps'], c['dL'], c['npml'], c['src'], lambda t: c['amp'][t], 300,
+                        [p for _, p in c['probes']], component='Ez')
+    assert np.array_equal(head, tr[:300])
+    fx = np.load(os.path.join(gold, 'tm_blobs_400_adjoint.npz'))
+    assert float(fx['L']) == pytest.approx(float(np.sum(tr ** 2)), rel=1e-13)
+    assert fx['directional_derivatives'].shape == (len(fx['seeds']),)
