@@ -98,14 +98,17 @@ def fdfd_main(steps, warmup, N=1024):
     tflops = flops / (fact_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": 157.3, "unit": "TFLOP/s",
             "frac": round(tflops / 157.3, 4), "traffic": None,
-            "kernel": "cv_nd_factor (batched complex64 LU, triangular solves and Schur products per tree level)",
+            "kernel": "cv_nd_factor (per tree level: frontal assembly, batched complex64 LU and inverse of the "
+                      "interior block, two Schur GEMMs)",
             "avg_factor_ms": round(fact_ms, 2), "algorithmic_flops_per_factor": flops,
             "apply_ms": round(apply_ms, 3),
-            "note": "real flops of one nested-dissection factorisation at unpadded sizes (per node LU k^3/3, "
-                    "[Q X] = F_II^-1 [I F_IB] k^2 (k + b), S b^2 k, G b k^2 complex multiply-adds) over its time "
-                    "(HIP events, 3 factorisations); peak = MI355X dense FP32 matrix-core rate (complex64 products "
-                    "run on the FP32 matrix cores); one solve = one factorisation + %d FGMRES steps of one factor "
-                    "application each" % info['iterations']}
+            "note": "real flops of one nested-dissection factorisation at unpadded sizes (per node with k interior "
+                    "and b boundary cells: LU k^3/3, inverse from the LU factors 2k^3/3, X = Q F_IB k^2 b, "
+                    "S = F_BB - F_BI X b^2 k complex multiply-adds of 8 real flops) over its time (HIP events on "
+                    "the launch stream, 3 factorisations); the factorisation is a sequence of launches per level, "
+                    "so the time is the sum of its kernels and gaps; peak = MI355X dense FP32 matrix-core rate "
+                    "(complex64 products run on the FP32 matrix cores); one solve = one factorisation + %d FGMRES "
+                    "steps of one factor application each" % info['iterations']}
     return dict(value=steps / dt, ms=1e3 * dt / steps, info=info, true_res=true_res, roofline=roof, N=N,
                 lam=w['lam'])
 
@@ -641,41 +644,44 @@ def extras_all(main, which='all'):
                              "factorisation of each sideband block"}
 
     def fdfd_batch():
-        """Multi-frequency batch (4 blob designs x 4 wavelengths, sizes from
-        {200, 300, 400, 600}^2) solved in one batched launch sequence, in
-        complex128 and in complex64 with FP64 residual refinement, then the
-        per-instance adjoint gradients of a modal-overlap objective
-        |sum_y conj(m(y)) Ez[N - 25, y]|^2, m = the fundamental mode of the
-        output cross section (ceviche_amd.modes.get_modes)."""
+        """The multi-frequency blob batch at its full size: 64 designs with
+        sizes from {200, 300, 400, 600}^2 x 8 wavelengths over 1.50-1.60 um
+        (512 solves, workloads.fdfd_blob_batch(64, 8)) to a relative FP64
+        residual of 1e-6, then the per-instance adjoint gradients of the
+        modal-overlap objective |sum_y conj(m(y)) Ez[N - 25, y]|^2 summed over
+        the batch, m = the fundamental mode of the output cross section
+        (ceviche_amd.modes.get_modes).  The default path factorises every
+        instance in complex64 (nested dissection) and solves by FGMRES with the
+        FP64 residual; the batched one-launch BiCGSTAB (complex128) on a
+        16-solve subset (4 designs x 4 wavelengths) for comparison."""
         from ceviche_amd._fdfd_op import FdfdOperator, FdfdSolve, SolverOptions
         from ceviche_amd.modes import get_modes
-        insts = workloads.fdfd_blob_batch(n_inst=4, n_lambda=4)
-        specs = [dict(omega=d['omega'], dL=d['dL'], shape=d['eps'].shape, npml=d['npml']) for d in insts]
-        op = FdfdOperator('ez', specs)
-        op32 = FdfdOperator('ez', specs, torch.complex64)
-        eps = torch.from_numpy(np.concatenate([d['eps'].ravel() for d in insts])).cuda()
-        b = torch.from_numpy(np.concatenate([1j * d['omega'] * d['src'].ravel() for d in insts])).cuda()
-        res = {"instances": len(insts), "sizes": sorted(set(int(d['eps'].shape[0]) for d in insts))}
-        t0 = time.perf_counter()
+
+        def setup(insts):
+            specs = [dict(omega=d['omega'], dL=d['dL'], shape=d['eps'].shape, npml=d['npml']) for d in insts]
+            op = FdfdOperator('ez', specs)
+            eps = torch.from_numpy(np.concatenate([d['eps'].ravel() for d in insts])).cuda()
+            b = torch.from_numpy(np.concatenate([1j * d['omega'] * d['src'].ravel() for d in insts])).cuda()
+            return op, eps, b
+
         mode_of = {}
-        for d in insts:
-            N = d['eps'].shape[0]
-            key = (N, d['omega'])
-            if key not in mode_of:
-                _, v = get_modes(d['eps'][N - 25, :], d['omega'], d['dL'], d['npml'][1], m=1)
-                mode_of[key] = torch.from_numpy(np.conj(v[:, 0])).cuda()
-        res["mode_solves_s"] = round(time.perf_counter() - t0, 3)
-        for name, o32 in (('complex128', None), ('complex64_fp64_refined', op32)):
-            # complex64 BiCGSTAB stagnates near its rounding level: short inner solves, more refinement sweeps
-            opts = (SolverOptions(tol=1e-6, max_iter=400000) if o32 is None else
-                    SolverOptions(tol=1e-6, max_iter=20000, inner_tol=1e-3, max_sweeps=40))
-            log('  batch %s: solving' % name)
+
+        def modes_for(insts):
+            for d in insts:
+                N = d['eps'].shape[0]
+                key = (N, d['omega'])
+                if key not in mode_of:
+                    _, v = get_modes(d['eps'][N - 25, :], d['omega'], d['dL'], d['npml'][1], m=1)
+                    mode_of[key] = torch.from_numpy(np.conj(v[:, 0])).cuda()
+
+        def run(insts, opts):
+            op, eps, b = setup(insts)
             e = eps.clone().requires_grad_(True)
             _sync()
             t0 = time.perf_counter()
             m0, m1 = op.materials(e)
-            x = FdfdSolve.apply(m0, m1, b, op, o32, opts)
-            info = (o32 or op).last_info
+            x = FdfdSolve.apply(m0, m1, b, op, None, opts)
+            info = op.last_info
             L = 0.0
             for k, d in enumerate(insts):
                 N = d['eps'].shape[0]
@@ -683,27 +689,33 @@ def extras_all(main, which='all'):
                 L = L + torch.abs((f[N - 25, :] * mode_of[(N, d['omega'])].to(f.dtype)).sum()) ** 2
             _sync()
             t_fwd = time.perf_counter() - t0
-            log('  batch %s: forward %.2f s, adjoint' % (name, t_fwd))
             L.backward()
             _sync()
             t_all = time.perf_counter() - t0
             its = [i['iterations'] for i in info]
-            res[name] = {"forward_s": round(t_fwd, 3), "fwd_plus_adjoint_s": round(t_all, 3),
-                         "solves_per_s": round(len(insts) / t_fwd, 3),
-                         "iterations_min_max": [min(its), max(its)],
-                         "max_fp64_rel_residual": max(i['rel_residual'] for i in info),
-                         "grad_norm": float(e.grad.norm())}
-        # the same instances one at a time (complex128, forward only)
-        opts = SolverOptions(tol=1e-6, max_iter=400000)
-        log('  batch: one by one')
+            return {"solves": len(insts), "forward_s": round(t_fwd, 3), "fwd_plus_adjoint_s": round(t_all, 3),
+                    "solves_per_s": round(len(insts) / t_fwd, 3),
+                    "iterations_min_max": [min(its), max(its)],
+                    "max_fp64_rel_residual": max(i['rel_residual'] for i in info),
+                    "grad_norm": float(e.grad.norm())}
+
+        full = workloads.fdfd_blob_batch(n_inst=64, n_lambda=8)
+        res = {"designs": 64, "wavelengths": 8, "sizes": sorted(set(int(d['eps'].shape[0]) for d in full))}
         t0 = time.perf_counter()
-        for d in insts[:4]:
-            F = cv.fdfd_ez(d['omega'], d['dL'], d['eps'], d['npml'], solver_options=opts)
-            F.solve(d['src'])
-        _sync()
-        res["one_by_one_complex128_solves_per_s"] = round(4 / (time.perf_counter() - t0), 3)
-        res["note"] = ("BiCGSTAB with FP64 reductions; all instances advance in the same kernels; gradients by "
-                       "one batched adjoint solve (A^T) per precision")
+        modes_for(full)
+        res["mode_solves_s"] = round(time.perf_counter() - t0, 3)
+        log('  batch: 512 solves, default path')
+        res["default_complex64_factor_fp64_fgmres"] = run(full, SolverOptions(tol=1e-6))
+        sub = workloads.fdfd_blob_batch(n_inst=4, n_lambda=4)
+        modes_for(sub)
+        log('  batch: 16-solve subset, batched BiCGSTAB')
+        res["subset_16_batched_bicgstab_complex128"] = run(sub, SolverOptions(tol=1e-6, max_iter=400000,
+                                                                               method='bicgstab'))
+        res["subset_16_default"] = run(sub, SolverOptions(tol=1e-6))
+        res["note"] = ("default: every instance factorised (complex64 nested dissection; factors rebuilt for the "
+                       "adjoint when they do not all fit in half the free memory) and solved by FGMRES with FP64 "
+                       "residual and inner products; batched BiCGSTAB: all 16 instances advance in the same "
+                       "kernels (hipGraph), FP64 reductions; gradients by the adjoint (A^T) solves")
         return res
 
     def fdfd_learned():
