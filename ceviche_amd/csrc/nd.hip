@@ -17,9 +17,9 @@
 //             (placed at their positions) + the A entries that couple the
 //             two children across their shared edge,
 // then factorises F_II = P L U, inverts it in place (Q = F_II^-1 from the LU
-// factors), forms X = Q F_IB and passes S = F_BB - F_BI X up to its parent.
-// F keeps Q, X and the original F_BI in place of its blocks:
-//   F = [[Q, X], [F_BI, S]].
+// factors), forms X = Q F_IB, passes S = F_BB - F_BI X up to its parent and
+// G = F_BI Q.  F keeps Q, X and G in place of its blocks:
+//   F = [[Q, X], [G, S]].
 // Nodes of equal height (distance to the leaves) are processed together as
 // strided batches padded to the level's largest |I| (kmax, identity padding)
 // and |B| (bmax, zero padding): dense LU and inversion (rocsolver getrf /
@@ -27,13 +27,16 @@
 // launch sequence per level.  The symbolic analysis (tree, index sets, maps)
 // is host C++ done once per grid shape.
 //
-// Solve of A x = b (transpose 0), matrix-vector products per node and sweep:
-//   up the tree:   y_I = Q b_I,  b_B -= F_BI y_I
+// Solve of A x = b (transpose 0), one matrix-vector product per node and sweep:
+//   up the tree:   [y_I; t] = [Q; G] b_I,  b_B -= t
 //   down the tree: x_I = y_I - X x_B
 // and of A^T x = b (transpose 1):
 //   up:   [y_I; t] = [Q X]^T b_I,  b_B -= t
-//   down: x_I = y_I - Q^T (F_BI^T x_B)
-// (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]], F_IB^T F_II^-T = X^T).
+//   down: x_I = y_I - G^T x_B
+// (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]], F_IB^T F_II^-T = X^T and
+// F_II^-T F_BI^T = G^T).  Each product is one launch over 64 x 64 tiles of
+// all nodes of a level, the input gathered from the work vector, with the
+// per-tile partial sums added and scattered back by a second launch.
 #include "fdfd_stencil.h"
 
 #include <rocblas/rocblas.h>
@@ -47,6 +50,7 @@ namespace cv {
 namespace {
 
 constexpr int kLeafCells = 64;
+constexpr int kMvTile = 64;        // rows and columns of one tile of the application products
 
 struct NdGroup {               // (parent, child) pairs whose child sits at one level
     int child_level = 0;
@@ -74,7 +78,7 @@ struct cv_nd_plan {
     int nx = 0, ny = 0;
     std::vector<cv::NdLevel> L;
     int64_t plan_bytes = 0, factor_bytes = 0;
-    int64_t off_w = 0, off_y = 0, off_t = 0, off_u = 0, off_info = 0, off_r = 0;
+    int64_t off_w = 0, off_part = 0, off_info = 0, off_r = 0;
     int64_t n_nodes = 0;
     double flops = 0.0;   // real flops of one numeric factorisation (unpadded sizes)
 };
@@ -170,10 +174,11 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
     }
     // algorithmic work of one factorisation, real flops of complex arithmetic
     // (a complex multiply-add is 8): LU k^3/3, inverse from the LU factors
-    // 2k^3/3, X = Q F_IB k^2 b and S b^2 k complex multiply-adds per node
+    // 2k^3/3, X = Q F_IB k^2 b, S b^2 k and G = F_BI Q b k^2 complex
+    // multiply-adds per node
     for (const auto& b : nodes) {
         const double k = (double)b.I.size(), bb = (double)b.B.size();
-        P->flops += 8.0 * (k * k * k + k * k * bb + bb * bb * k);
+        P->flops += 8.0 * (k * k * k + 2.0 * k * k * bb + bb * bb * k);
     }
     // group by height
     P->L.resize(H + 1);
@@ -277,7 +282,7 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         }
     }
     // device layouts: index arrays in the plan buffer, numeric data in the factor buffer
-    int64_t po = 0, fo = 0, ymax = 0, tmax = 0, rmax = 0, info_first = 0;
+    int64_t po = 0, fo = 0, pmax = 0, rmax = 0, info_first = 0;
     auto take = [](int64_t& o, int64_t bytes) {
         const int64_t r = o;
         o += al256(bytes);
@@ -302,16 +307,16 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         L.off_piv = take(fo, 4 * (int64_t)L.nb * std::max(1, L.kmax));
         L.info_first = info_first;
         info_first += L.nb;
-        ymax = std::max<int64_t>(ymax, (int64_t)L.nb * L.m);
-        tmax = std::max<int64_t>(tmax, (int64_t)L.nb * std::max(L.kmax, L.bmax));
+        // partial sums of the tiled products: per node ceil(k/64) tiles of m
+        // outputs (up sweeps) or ceil(b/64) tiles of k outputs (down sweeps)
+        pmax = std::max<int64_t>(pmax, (int64_t)L.nb * std::max<int64_t>((int64_t)((L.kmax + kMvTile - 1) / kMvTile) * L.m,
+                                                                        (int64_t)((L.bmax + kMvTile - 1) / kMvTile) * L.kmax));
         rmax = std::max<int64_t>(rmax, (int64_t)L.nb * L.kmax * L.m);
     }
     P->n_nodes = info_first;
     P->off_info = take(fo, 4 * info_first);
     P->off_w = take(fo, 8 * N);
-    P->off_y = take(fo, 8 * std::max<int64_t>(1, ymax));
-    P->off_t = take(fo, 8 * std::max<int64_t>(1, tmax));
-    P->off_u = take(fo, 8 * std::max<int64_t>(1, tmax));
+    P->off_part = take(fo, 8 * std::max<int64_t>(1, pmax));
     P->off_r = take(fo, 8 * std::max<int64_t>(1, rmax));
     P->plan_bytes = std::max<int64_t>(po, 256);
     P->factor_bytes = fo;
@@ -452,39 +457,139 @@ k_nd_pad(const int32_t* __restrict__ icells, int nb, int kmax, cx<float>* __rest
     F[(int64_t)e * m * m + (int64_t)p * m + p] = mk<float>(1.f, 0.f);
 }
 
+// One 64 x 64 tile of a per-node product over a block of F (rows row0..,
+// columns col0.., M x N, column-major with leading dimension m).  The input
+// vector is gathered from the work vector w through xcells (xn entries per
+// node, -1 = padding).  TRANS = 0: y = A x, the four waves take 16 columns
+// each, lanes the rows, the wave sums combined through LDS; partial sums per
+// column tile in part[(e * tn + tj) * M + i].  TRANS = 1: y = A^T x, lanes
+// the rows, each wave 16 output columns reduced over the lanes; partial sums
+// per row tile in part[(e * tm + ti) * N + j].  Workgroups are node-major.
+template <bool TRANS>
 __global__ void __launch_bounds__(256)
-k_nd_gather(const int32_t* __restrict__ cells, int64_t n, const cx<float>* __restrict__ w, cx<float>* __restrict__ y) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int32_t c = cells[t];
-    y[t] = c >= 0 ? w[c] : czero<float>();
-}
-
-// w[cells] = y (sub = 0) or w[cells] -= y (sub = 1)
-__global__ void __launch_bounds__(256)
-k_nd_scatter(const int32_t* __restrict__ cells, int64_t n, const cx<float>* __restrict__ y, cx<float>* __restrict__ w,
-             int sub) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int32_t c = cells[t];
-    if (c < 0) return;
-    w[c] = sub ? w[c] - y[t] : y[t];
-}
-
-// up-sweep result Z (nb x m per node): w[I] = Z[0:k], w[B] -= Z[k:m]
-__global__ void __launch_bounds__(256)
-k_nd_scatter_up(const int32_t* __restrict__ icells, const int32_t* __restrict__ bcells, int nb, int k, int b,
-                const cx<float>* __restrict__ Z, cx<float>* __restrict__ w) {
-    const int m = k + b;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nb * m) return;
-    const int e = (int)(t / m), p = (int)(t - (int64_t)e * m);
-    if (p < k) {
-        const int32_t c = icells[(int64_t)e * k + p];
-        if (c >= 0) w[c] = Z[t];
+k_nd_mv(const cx<float>* __restrict__ F, int m, int row0, int col0, int M, int N, int tm, int tn,
+        const int32_t* __restrict__ xcells, int xn, const cx<float>* __restrict__ w, cx<float>* __restrict__ part) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int per = tm * tn;
+    const int e = blockIdx.x / per;
+    const int rem = blockIdx.x - e * per;
+    const int ti = rem / tn, tj = rem - ti * tn;
+    const int i = ti * kMvTile + lane;
+    const cx<float>* A = F + (int64_t)e * m * m + row0 + (int64_t)col0 * m;
+    const int32_t* xc = xcells + (int64_t)e * xn;
+    const int j0 = tj * kMvTile + wid * 16;
+    if (!TRANS) {
+        __shared__ cx<float> red[4][kMvTile];
+        cx<float> acc = czero<float>();
+        if (i < M) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const int j = j0 + c;
+                if (j < N) {
+                    const int32_t cell = xc[j];
+                    const cx<float> xj = cell >= 0 ? w[cell] : czero<float>();
+                    acc = acc + A[i + (int64_t)j * m] * xj;
+                }
+            }
+        }
+        red[wid][lane] = acc;
+        __syncthreads();
+        if (wid == 0 && i < M) {
+            const cx<float> v = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+            part[((int64_t)e * tn + tj) * M + i] = v;
+        }
     } else {
-        const int32_t c = bcells[(int64_t)e * b + (p - k)];
-        if (c >= 0) w[c] = w[c] - Z[t];
+        cx<float> xi = czero<float>();
+        if (i < M) {
+            const int32_t cell = xc[i];
+            if (cell >= 0) xi = w[cell];
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int j = j0 + c;
+            cx<float> v = (i < M && j < N) ? A[i + (int64_t)j * m] * xi : czero<float>();
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                v.re += __shfl_xor(v.re, off);
+                v.im += __shfl_xor(v.im, off);
+            }
+            if (lane == 0 && j < N) part[((int64_t)e * tm + ti) * N + j] = v;
+        }
+    }
+}
+
+// The same product for nodes of at most kMvNode rows and columns, one
+// workgroup per node, fused with the gather and the scatter (a node's input
+// and output cells belong to that node alone within a level): the input is
+// gathered into LDS, then one thread per output, 16 independent loads in
+// flight: TRANS = 0 an output row (the loads of a wave are coalesced), TRANS
+// = 1 an output column (each thread reads its column contiguously).
+// Outputs o < k go to the interior cells (w = y when set_interior, else
+// w -= y), o >= k to the boundary cells (w -= y).
+constexpr int kMvNode = 256;
+template <bool TRANS>
+__global__ void __launch_bounds__(256)
+k_nd_mv_node(const cx<float>* __restrict__ F, int m, int row0, int col0, int M, int N,
+             const int32_t* __restrict__ xcells, int xn, const int32_t* __restrict__ icells, int k,
+             const int32_t* __restrict__ bcells, int b, cx<float>* __restrict__ w, int set_interior) {
+    __shared__ cx<float> xs[kMvNode];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const int32_t* xc = xcells + (int64_t)e * xn;
+    for (int q = tid; q < xn; q += blockDim.x) {
+        const int32_t c = xc[q];
+        xs[q] = c >= 0 ? w[c] : czero<float>();
+    }
+    __syncthreads();
+    const int o = tid;
+    const int nout = TRANS ? N : M, nin = TRANS ? M : N;
+    if (o >= nout) return;
+    const cx<float>* A = F + (int64_t)e * m * m + row0 + (int64_t)col0 * m;
+    // element q of output o's row (TRANS = 0) or column (TRANS = 1)
+    const cx<float>* base = TRANS ? A + (int64_t)o * m : A + o;
+    const int64_t step = TRANS ? 1 : m;
+    cx<float> acc = czero<float>();
+    int q = 0;
+    for (; q + 16 <= nin; q += 16) {
+        cx<float> a[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u] = base[(int64_t)(q + u) * step];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc = acc + a[u] * xs[q + u];
+    }
+    for (; q + 4 <= nin; q += 4) {
+        cx<float> a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = base[(int64_t)(q + u) * step];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = acc + a[u] * xs[q + u];
+    }
+    for (; q < nin; ++q) acc = acc + base[(int64_t)q * step] * xs[q];
+    if (o < k) {
+        const int32_t c = icells[(int64_t)e * k + o];
+        if (c >= 0) w[c] = set_interior ? acc : w[c] - acc;
+    } else {
+        const int32_t c = bcells[(int64_t)e * b + (o - k)];
+        if (c >= 0) w[c] = w[c] - acc;
+    }
+}
+
+// Sum of the nt partial-sum tiles of each output o (Mout per node) and the
+// scatter into the work vector: o < k to the interior cells (w = sum when
+// set_interior, else w -= sum), o >= k to the boundary cells (w -= sum).
+__global__ void __launch_bounds__(256)
+k_nd_mv_reduce(const cx<float>* __restrict__ part, int nt, int Mout, int nb, const int32_t* __restrict__ icells,
+               int k, const int32_t* __restrict__ bcells, int b, cx<float>* __restrict__ w, int set_interior) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * Mout) return;
+    const int e = (int)(t / Mout), o = (int)(t - (int64_t)e * Mout);
+    cx<float> sum = czero<float>();
+    for (int q = 0; q < nt; ++q) sum = sum + part[((int64_t)e * nt + q) * Mout + o];
+    if (o < k) {
+        const int32_t c = icells[(int64_t)e * k + o];
+        if (c >= 0) w[c] = set_interior ? sum : w[c] - sum;
+    } else {
+        const int32_t c = bcells[(int64_t)e * b + (o - k)];
+        if (c >= 0) w[c] = w[c] - sum;
     }
 }
 
@@ -537,64 +642,55 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
     char* fb = (char*)factor;
     const int64_t N = (int64_t)P->nx * P->ny;
     cx<float>* w = (cx<float>*)(fb + P->off_w);
-    cx<float>* Y = (cx<float>*)(fb + P->off_y);
-    cx<float>* T = (cx<float>*)(fb + P->off_t);
-    cx<float>* U = (cx<float>*)(fb + P->off_u);
-    cx<float>* Z = (cx<float>*)(fb + P->off_r);
-    rocblas_handle h;
-    {
-        const int rc = get_handle(s, &h);
-        if (rc != CV_OK) return rc;
-    }
-    const rc_t one{1.f, 0.f}, zero{0.f, 0.f};
+    cx<float>* part = (cx<float>*)(fb + P->off_part);
     k_nd_copy<<<nblk(N), 256, 0, s>>>(v, w, N);
+    CV_LAUNCH_CHECK();
+    auto tiles = [](int n) { return (n + kMvTile - 1) / kMvTile; };
+    // one product per level and sweep: tiles (launch 1), sum + scatter (launch 2)
+    auto product = [&](const NdLevel& L, bool trans, int row0, int col0, int M, int N, const int32_t* xcells,
+                       int xn, int Mout, int set_interior) -> int {
+        const int32_t* ic = (const int32_t*)(pb + L.off_icells);
+        const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
+        const cx<float>* F = (const cx<float>*)(fb + L.off_F);
+        if (L.m <= kMvNode) {   // small nodes: one fused launch, one workgroup per node
+            if (!trans)
+                k_nd_mv_node<false><<<L.nb, 64 * ((M + 63) / 64), 0, s>>>(F, L.m, row0, col0, M, N, xcells, xn, ic,
+                                                                         L.kmax, bc, L.bmax, w, set_interior);
+            else
+                k_nd_mv_node<true><<<L.nb, 64 * ((N + 63) / 64), 0, s>>>(F, L.m, row0, col0, M, N, xcells, xn, ic,
+                                                                        L.kmax, bc, L.bmax, w, set_interior);
+            CV_LAUNCH_CHECK();
+            return CV_OK;
+        }
+        const int tm = tiles(M), tn = tiles(N);
+        const int64_t nwg = (int64_t)L.nb * tm * tn;
+        if (!trans)
+            k_nd_mv<false><<<(unsigned)nwg, 256, 0, s>>>(F, L.m, row0, col0, M, N, tm, tn, xcells, xn, w, part);
+        else
+            k_nd_mv<true><<<(unsigned)nwg, 256, 0, s>>>(F, L.m, row0, col0, M, N, tm, tn, xcells, xn, w, part);
+        CV_LAUNCH_CHECK();
+        const int nt = trans ? tm : tn;
+        k_nd_mv_reduce<<<nblk((int64_t)L.nb * Mout), 256, 0, s>>>(part, nt, Mout, L.nb, ic, L.kmax, bc, L.bmax, w,
+                                                                   set_interior);
+        CV_LAUNCH_CHECK();
+        return CV_OK;
+    };
     const int H = (int)P->L.size();
-    for (int l = 0; l < H; ++l) {   // up the tree: [y_I; t] = [Q; F_BI Q] b_I (or [Q X]^T b_I)
+    for (int l = 0; l < H; ++l) {   // up the tree: [y_I; t] = [Q; G] b_I (or [Q X]^T b_I), b_B -= t
         const NdLevel& L = P->L[l];
         if (L.kmax == 0) continue;
         const int32_t* ic = (const int32_t*)(pb + L.off_icells);
-        const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
-        const rc_t* F = (const rc_t*)(fb + L.off_F);
-        const int64_t nI = (int64_t)L.nb * L.kmax;
-        const rocblas_stride sF = (rocblas_stride)L.m * L.m;
-        k_nd_gather<<<nblk(nI), 256, 0, s>>>(ic, nI, w, Y);
-        CV_LAUNCH_CHECK();
-        if (!transpose) {   // y_I = Q b_I into Z[0:k], t = F_BI y_I into Z[k:m]
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.kmax, L.kmax, &one, F, L.m, sF,
-                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)Z, 1, L.m, L.nb));
-            if (L.bmax > 0)
-                CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.bmax, L.kmax, &one, F + L.kmax, L.m,
-                                                    sF, (const rc_t*)Z, 1, L.m, &zero, (rc_t*)Z + L.kmax, 1, L.m,
-                                                    L.nb));
-        } else
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.kmax, L.m, &one, F, L.m, sF,
-                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)Z, 1, L.m, L.nb));
-        k_nd_scatter_up<<<nblk((int64_t)L.nb * L.m), 256, 0, s>>>(ic, bc, L.nb, L.kmax, L.bmax, Z, w);
-        CV_LAUNCH_CHECK();
+        const int rc = !transpose ? product(L, false, 0, 0, L.m, L.kmax, ic, L.kmax, L.m, 1)
+                                  : product(L, true, 0, 0, L.kmax, L.m, ic, L.kmax, L.m, 1);
+        if (rc != CV_OK) return rc;
     }
-    for (int l = H - 1; l >= 0; --l) {   // down the tree: x_I -= X x_B (or Q^T F_BI^T x_B)
+    for (int l = H - 1; l >= 0; --l) {   // down the tree: x_I -= X x_B (or G^T x_B)
         const NdLevel& L = P->L[l];
         if (L.kmax == 0 || L.bmax == 0) continue;
-        const int32_t* ic = (const int32_t*)(pb + L.off_icells);
         const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
-        const rc_t* F = (const rc_t*)(fb + L.off_F);
-        const int64_t nI = (int64_t)L.nb * L.kmax, nB = (int64_t)L.nb * L.bmax;
-        const rocblas_stride sF = (rocblas_stride)L.m * L.m;
-        k_nd_gather<<<nblk(nB), 256, 0, s>>>(bc, nB, w, T);
-        CV_LAUNCH_CHECK();
-        if (!transpose)
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_none, L.kmax, L.bmax, &one,
-                                                F + (int64_t)L.kmax * L.m, L.m, sF, (const rc_t*)T, 1, L.bmax, &zero,
-                                                (rc_t*)U, 1, L.kmax, L.nb));
-        else {
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.bmax, L.kmax, &one, F + L.kmax,
-                                                L.m, sF, (const rc_t*)T, 1, L.bmax, &zero, (rc_t*)Y, 1, L.kmax,
-                                                L.nb));
-            CV_RB(rocblas_cgemv_strided_batched(h, rocblas_operation_transpose, L.kmax, L.kmax, &one, F, L.m, sF,
-                                                (const rc_t*)Y, 1, L.kmax, &zero, (rc_t*)U, 1, L.kmax, L.nb));
-        }
-        k_nd_scatter<<<nblk(nI), 256, 0, s>>>(ic, nI, U, w, 1);
-        CV_LAUNCH_CHECK();
+        const int rc = !transpose ? product(L, false, 0, L.kmax, L.kmax, L.bmax, bc, L.bmax, L.kmax, 0)
+                                  : product(L, true, L.kmax, 0, L.bmax, L.kmax, bc, L.bmax, L.kmax, 0);
+        if (rc != CV_OK) return rc;
     }
     k_nd_copy<<<nblk(N), 256, 0, s>>>(w, z, N);
     CV_LAUNCH_CHECK();
@@ -677,6 +773,13 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             k_nd_block_copy<<<nblk((int64_t)L.nb * k * b), 256, 0, s>>>((const cx<float>*)R, sR, k,
                                                                         (cx<float>*)F + (int64_t)k * m, sF, m, L.nb,
                                                                         k, b);
+            CV_LAUNCH_CHECK();
+            // G = F_BI Q (into R), then G over F_BI
+            const rocblas_stride sG = (rocblas_stride)b * k;
+            CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, k, k, &one,
+                                                F + k, m, sF, F, m, sF, &zero_c, R, b, sG, L.nb));
+            k_nd_block_copy<<<nblk((int64_t)L.nb * b * k), 256, 0, s>>>((const cx<float>*)R, sG, b,
+                                                                        (cx<float>*)F + k, sF, m, L.nb, b, k);
             CV_LAUNCH_CHECK();
         }
     }
