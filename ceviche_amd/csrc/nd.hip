@@ -16,15 +16,18 @@
 //   internal: F = the two children's Schur complements on their boundaries
 //             (placed at their positions) + the A entries that couple the
 //             two children across their shared edge,
-// then factorises F_II = P L U, inverts it in place (Q = F_II^-1 from the LU
-// factors), forms X = Q F_IB, passes S = F_BB - F_BI X up to its parent and
-// G = F_BI Q.  F keeps Q, X and G in place of its blocks:
+// then factorises F_II = P L U and forms Q = F_II^-1 and X = Q F_IB -- for
+// interior blocks of at most kNdSolveMin cells by inverting in place from the
+// LU factors and one product, above that by the triangular solves of
+// [I F_IB] (fewer, larger launches in rocSOLVER for these sizes) -- passes
+// S = F_BB - F_BI X up to its parent and forms G = F_BI Q.  F keeps Q, X and
+// G in place of its blocks:
 //   F = [[Q, X], [G, S]].
 // Nodes of equal height (distance to the leaves) are processed together as
 // strided batches padded to the level's largest |I| (kmax, identity padding)
-// and |B| (bmax, zero padding): dense LU and inversion (rocsolver getrf /
-// getri) and the two products (rocblas gemm) run on the matrix cores, one
-// launch sequence per level.  The symbolic analysis (tree, index sets, maps)
+// and |B| (bmax, zero padding): dense LU, inversion or triangular solves
+// (rocsolver getrf, getri / getrs) and the products (rocblas gemm) run on the
+// matrix cores, one launch sequence per level.  The symbolic analysis (tree, index sets, maps)
 // is host C++ done once per grid shape.
 //
 // Solve of A x = b (transpose 0), one matrix-vector product per node and sweep:
@@ -51,6 +54,7 @@ namespace {
 
 constexpr int kLeafCells = 64;
 constexpr int kMvTile = 64;        // rows and columns of one tile of the application products
+constexpr int kNdSolveMin = 128;   // interior blocks above this size: triangular solves instead of the inverse
 
 struct NdGroup {               // (parent, child) pairs whose child sits at one level
     int child_level = 0;
@@ -175,7 +179,8 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
     // algorithmic work of one factorisation, real flops of complex arithmetic
     // (a complex multiply-add is 8): LU k^3/3, inverse from the LU factors
     // 2k^3/3, X = Q F_IB k^2 b, S b^2 k and G = F_BI Q b k^2 complex
-    // multiply-adds per node
+    // multiply-adds per node (the triangular-solve route for large interior
+    // blocks does k^3/3 more)
     for (const auto& b : nodes) {
         const double k = (double)b.I.size(), bb = (double)b.B.size();
         P->flops += 8.0 * (k * k * k + 2.0 * k * k * bb + bb * bb * k);
@@ -418,6 +423,18 @@ k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int3
         const int pi = mp[ii];
         if (pi >= 0) Fcol[pi] = Scol[ii];
     }
+}
+
+// R = [I F_IB] (k x m per node, column-major, leading dimension k)
+__global__ void __launch_bounds__(256)
+k_nd_rhs(const cx<float>* __restrict__ F, cx<float>* __restrict__ R, int nb, int k, int m) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * k * m) return;
+    const int64_t km = (int64_t)k * m;
+    const int e = (int)(t / km);
+    const int64_t r = t - (int64_t)e * km;
+    const int j = (int)(r / k), i = (int)(r - (int64_t)j * k);
+    R[t] = j < k ? (i == j ? mk<float>(1.f, 0.f) : czero<float>()) : F[(int64_t)e * m * m + (int64_t)j * m + i];
 }
 
 // copy a rows x cols block between per-node column-major matrices
@@ -756,25 +773,42 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
         const int k = L.kmax, b = L.bmax, m = L.m;
         rocblas_int* piv = (rocblas_int*)(fb + L.off_piv);
         rc_t* R = (rc_t*)(fb + P->off_r);
-        const rocblas_stride sR = (rocblas_stride)k * std::max(b, 1);
-        // Q = F_II^-1 in place: LU with partial pivoting, then the inverse
-        // from the factors (rocsolver's batched getri works on blocks and
-        // products, where triangular solves with k or m right-hand sides
-        // issue one small launch per node and block)
         CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
-        CV_RB(rocsolver_cgetri_strided_batched(h, k, F, m, sF, piv, k, info + L.info_first, L.nb));
-        if (b > 0) {
-            // X = Q F_IB (into R), S = F_BB - F_BI X, then X over F_IB
-            CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, k, b, k, &one, F,
-                                                m, sF, F + (int64_t)k * m, m, sF, &zero_c, R, k, sR, L.nb));
-            CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
-                                                &minus_one, F + k, m, sF, R, k, sR, &one, F + (int64_t)k * m + k, m,
-                                                sF, L.nb));
-            k_nd_block_copy<<<nblk((int64_t)L.nb * k * b), 256, 0, s>>>((const cx<float>*)R, sR, k,
-                                                                        (cx<float>*)F + (int64_t)k * m, sF, m, L.nb,
-                                                                        k, b);
+        if (k > kNdSolveMin) {
+            // [Q X] = F_II^-1 [I F_IB] by the LU triangular solves (faster
+            // than the in-place inverse plus a product for these k), into the
+            // top k rows of F; then S = F_BB - F_BI X and G = F_BI Q
+            const rocblas_stride sR = (rocblas_stride)k * m;
+            k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, (cx<float>*)R, L.nb, k, m);
             CV_LAUNCH_CHECK();
-            // G = F_BI Q (into R), then G over F_BI
+            CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, k, m, F, m, sF, piv, k, R, k, sR,
+                                                   L.nb));
+            if (b > 0)
+                CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
+                                                    &minus_one, F + k, m, sF, R + (int64_t)k * k, k, sR, &one,
+                                                    F + (int64_t)k * m + k, m, sF, L.nb));
+            k_nd_block_copy<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)R, sR, k, (cx<float>*)F, sF,
+                                                                        m, L.nb, k, m);
+            CV_LAUNCH_CHECK();
+        } else {
+            // Q = F_II^-1 in place: LU with partial pivoting, then the inverse
+            // from the factors; X = Q F_IB (into R), S = F_BB - F_BI X, then X
+            // over F_IB
+            CV_RB(rocsolver_cgetri_strided_batched(h, k, F, m, sF, piv, k, info + L.info_first, L.nb));
+            if (b > 0) {
+                const rocblas_stride sR = (rocblas_stride)k * b;
+                CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, k, b, k, &one,
+                                                    F, m, sF, F + (int64_t)k * m, m, sF, &zero_c, R, k, sR, L.nb));
+                CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
+                                                    &minus_one, F + k, m, sF, R, k, sR, &one,
+                                                    F + (int64_t)k * m + k, m, sF, L.nb));
+                k_nd_block_copy<<<nblk((int64_t)L.nb * k * b), 256, 0, s>>>((const cx<float>*)R, sR, k,
+                                                                            (cx<float>*)F + (int64_t)k * m, sF, m,
+                                                                            L.nb, k, b);
+                CV_LAUNCH_CHECK();
+            }
+        }
+        if (b > 0) {   // G = F_BI Q (into R), then G over F_BI
             const rocblas_stride sG = (rocblas_stride)b * k;
             CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, k, k, &one,
                                                 F + k, m, sF, F, m, sF, &zero_c, R, b, sG, L.nb));
