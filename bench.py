@@ -98,13 +98,15 @@ def fdfd_main(steps, warmup, N=1024):
     tflops = flops / (fact_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": 157.3, "unit": "TFLOP/s",
             "frac": round(tflops / 157.3, 4), "traffic": None,
-            "kernel": "cv_nd_factor (per tree level: frontal assembly, batched complex64 LU and inverse of the "
-                      "interior block, two Schur GEMMs)",
+            "kernel": "cv_nd_factor (per tree level: frontal assembly, batched complex64 LU of the interior "
+                      "block, its inverse (<= 128 interior cells) or the triangular solves of [I F_IB] (above), "
+                      "the X, S and G products)",
             "avg_factor_ms": round(fact_ms, 2), "algorithmic_flops_per_factor": flops,
             "apply_ms": round(apply_ms, 3),
             "note": "real flops of one nested-dissection factorisation at unpadded sizes (per node with k interior "
                     "and b boundary cells: LU k^3/3, inverse from the LU factors 2k^3/3, X = Q F_IB k^2 b, "
-                    "S = F_BB - F_BI X b^2 k complex multiply-adds of 8 real flops) over its time (HIP events on "
+                    "S = F_BB - F_BI X b^2 k, G = F_BI Q b k^2 complex multiply-adds of 8 real flops) over its "
+                    "time (HIP events on "
                     "the launch stream, 3 factorisations); the factorisation is a sequence of launches per level, "
                     "so the time is the sum of its kernels and gaps; peak = MI355X dense FP32 matrix-core rate "
                     "(complex64 products run on the FP32 matrix cores); one solve = one factorisation + %d FGMRES "
