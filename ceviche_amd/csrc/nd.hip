@@ -425,6 +425,183 @@ k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int3
     }
 }
 
+// ---- blocked LU with partial pivoting of the large interior blocks
+// (kLuMin < k <= kLuMaxRows): per panel of kLuPanel columns one panel factorisation, one
+// interchange + triangular-solve launch and one rocBLAS GEMM, where
+// rocsolver's getrf spends several short launches on every column.  The
+// result is the LAPACK form getrs takes: L (unit, strictly lower) and U in
+// place of F_II, ipiv 1-based sequential row interchanges.
+constexpr int kLuPanel = 16;
+constexpr int kLuMin = 256;
+constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
+
+// LU of the panel rows j0..k-1, columns j0..j0+nbw-1 of every node
+// (blockIdx.x), NT threads holding RPT rows each in registers: per column the
+// pivot (largest |a|^2, ties to the lower row) by a workgroup argmax, the
+// two rows exchanged through LDS, then the rank-1 update of each thread's
+// rows.  Writes the factored panel back, ipiv and the first zero pivot.
+template <int NT, int RPT>
+__global__ void __launch_bounds__(NT)
+k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int* __restrict__ ipiv,
+           rocblas_int* __restrict__ info) {
+    constexpr int NW = NT / 64;
+    __shared__ float redv[NW];
+    __shared__ int redi[NW];
+    __shared__ cx<float> prow[kLuPanel];   // the pivot row (the new row c)
+    __shared__ cx<float> crow[kLuPanel];   // the old row c (moves to the pivot's place)
+    __shared__ int s_piv;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const int nr = k - j0;
+    cx<float>* Fe = F + (int64_t)e * m * m;
+    cx<float> a[RPT][kLuPanel];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = tid + q * NT;
+#pragma unroll
+        for (int c = 0; c < kLuPanel; ++c)
+            a[q][c] = (r < nr && c < nbw) ? Fe[(int64_t)(j0 + c) * m + j0 + r] : czero<float>();
+    }
+    int zero_at = 0;
+#pragma unroll
+    for (int c = 0; c < kLuPanel; ++c) {
+        if (c < nbw) {
+            float bv = -1.f;
+            int bi = nr;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = tid + q * NT;
+                if (r >= c && r < nr) {
+                    const float v = norm2(a[q][c]);
+                    if (v > bv || (v == bv && r < bi)) {
+                        bv = v;
+                        bi = r;
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const float ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if ((tid & 63) == 0) {
+                redv[tid >> 6] = bv;
+                redi[tid >> 6] = bi;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                float v = redv[0];
+                int i = redi[0];
+                for (int w = 1; w < NW; ++w)
+                    if (redv[w] > v || (redv[w] == v && redi[w] < i)) {
+                        v = redv[w];
+                        i = redi[w];
+                    }
+                if (i >= nr) i = c;   // no comparable entry (NaN column): keep the row; the zero test flags it
+                s_piv = i;
+                ipiv[(int64_t)e * k + j0 + c] = j0 + i + 1;
+            }
+            __syncthreads();
+            const int p = s_piv;
+            // the owners of rows p and c publish them
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = tid + q * NT;
+                if (r == p)
+#pragma unroll
+                    for (int c2 = 0; c2 < kLuPanel; ++c2) prow[c2] = a[q][c2];
+                if (r == c)
+#pragma unroll
+                    for (int c2 = 0; c2 < kLuPanel; ++c2) crow[c2] = a[q][c2];
+            }
+            __syncthreads();
+            if (p != c) {
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int r = tid + q * NT;
+                    if (r == c)
+#pragma unroll
+                        for (int c2 = 0; c2 < kLuPanel; ++c2) a[q][c2] = prow[c2];
+                    if (r == p)
+#pragma unroll
+                        for (int c2 = 0; c2 < kLuPanel; ++c2) a[q][c2] = crow[c2];
+                }
+            }
+            const cx<float> d = prow[c];
+            if (norm2(d) > 0.f) {
+                const cx<float> inv = crecip(d);
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int r = tid + q * NT;
+                    if (r > c && r < nr) {
+                        const cx<float> l = a[q][c] * inv;
+                        a[q][c] = l;
+#pragma unroll
+                        for (int c2 = c + 1; c2 < kLuPanel; ++c2) a[q][c2] = a[q][c2] - l * prow[c2];
+                    }
+                }
+            } else if (zero_at == 0) {
+                zero_at = j0 + c + 1;
+            }
+            __syncthreads();   // prow / crow / the reduction slots are reused by the next column
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = tid + q * NT;
+        if (r < nr)
+#pragma unroll
+            for (int c = 0; c < kLuPanel; ++c)
+                if (c < nbw) Fe[(int64_t)(j0 + c) * m + j0 + r] = a[q][c];
+    }
+    if (tid == 0 && zero_at && info[e] == 0) info[e] = zero_at;
+}
+
+// The panel's row interchanges in the other columns of F_II (j < j0 and
+// j >= j0 + nbw, rows j0..k-1), then U12 = L11^-1 A12 in the columns to the
+// right (L11 = the panel's unit lower nbw x nbw block).  One thread per
+// column; workgroups node-major, ceil(k / 256) per node.
+__global__ void __launch_bounds__(256)
+k_lu_rows(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, const rocblas_int* __restrict__ ipiv) {
+    __shared__ cx<float> L11[kLuPanel * kLuPanel];
+    __shared__ int pv[kLuPanel];
+    const int bpn = (k + 255) / 256;
+    const int e = blockIdx.x / bpn;
+    cx<float>* Fe = F + (int64_t)e * m * m;
+    for (int i = threadIdx.x; i < nbw * nbw; i += 256) {
+        const int c = i / nbw, r = i - c * nbw;   // L11[c * kLuPanel + r] = F[j0 + r, j0 + c]
+        L11[c * kLuPanel + r] = Fe[(int64_t)(j0 + c) * m + j0 + r];
+    }
+    if (threadIdx.x < nbw) pv[threadIdx.x] = ipiv[(int64_t)e * k + j0 + threadIdx.x] - 1;
+    __syncthreads();
+    const int j = (blockIdx.x - e * bpn) * 256 + threadIdx.x;
+    if (j >= k || (j >= j0 && j < j0 + nbw)) return;
+    cx<float>* col = Fe + (int64_t)j * m;
+    for (int t = 0; t < nbw; ++t) {
+        const int r = pv[t];
+        if (r != j0 + t) {
+            const cx<float> v = col[j0 + t];
+            col[j0 + t] = col[r];
+            col[r] = v;
+        }
+    }
+    if (j < j0) return;
+    cx<float> x[kLuPanel];
+#pragma unroll
+    for (int t = 0; t < kLuPanel; ++t) x[t] = t < nbw ? col[j0 + t] : czero<float>();
+#pragma unroll
+    for (int t = 0; t < kLuPanel; ++t)
+#pragma unroll
+        for (int u = 0; u < kLuPanel; ++u)
+            if (u < t && t < nbw) x[t] = x[t] - L11[u * kLuPanel + t] * x[u];
+#pragma unroll
+    for (int t = 0; t < kLuPanel; ++t)
+        if (t < nbw) col[j0 + t] = x[t];
+}
+
 // R = [I F_IB] (k x m per node, column-major, leading dimension k)
 __global__ void __launch_bounds__(256)
 k_nd_rhs(const cx<float>* __restrict__ F, cx<float>* __restrict__ R, int nb, int k, int m) {
@@ -740,6 +917,7 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
     const float* m1 = (const float*)m1_32;
     rocblas_int* info = (rocblas_int*)(fb + P->off_info);
     const rc_t minus_one{-1.f, 0.f}, one{1.f, 0.f}, zero_c{0.f, 0.f};
+    CV_HIP(hipMemsetAsync(info, 0, 4 * (size_t)P->n_nodes, s));
     for (size_t l = 0; l < P->L.size(); ++l) {
         const NdLevel& L = P->L[l];
         rc_t* F = (rc_t*)(fb + L.off_F);
@@ -773,7 +951,31 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
         const int k = L.kmax, b = L.bmax, m = L.m;
         rocblas_int* piv = (rocblas_int*)(fb + L.off_piv);
         rc_t* R = (rc_t*)(fb + P->off_r);
-        CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
+        if (k > kLuMin && k <= kLuMaxRows) {
+            const unsigned g_rows = (unsigned)(((k + 255) / 256) * (int64_t)L.nb);
+            for (int j0 = 0; j0 < k; j0 += kLuPanel) {
+                const int nbw = std::min(kLuPanel, k - j0);
+                const int nr = k - j0;   // panel rows: 512 threads x 1 or 2 rows, 1024 x 2 above 1024 rows
+                if (nr <= 512)
+                    k_lu_panel<512, 1><<<L.nb, 512, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv, info + L.info_first);
+                else if (nr <= 1024)
+                    k_lu_panel<512, 2><<<L.nb, 512, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv, info + L.info_first);
+                else
+                    k_lu_panel<1024, 2><<<L.nb, 1024, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv,
+                                                             info + L.info_first);
+                CV_LAUNCH_CHECK();
+                k_lu_rows<<<g_rows, 256, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv);
+                CV_LAUNCH_CHECK();
+                const int rest = k - j0 - nbw;
+                if (rest > 0)   // A22 -= L21 U12
+                    CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, rest, rest,
+                                                        nbw, &minus_one, F + (int64_t)j0 * m + j0 + nbw, m, sF,
+                                                        F + (int64_t)(j0 + nbw) * m + j0, m, sF, &one,
+                                                        F + (int64_t)(j0 + nbw) * m + j0 + nbw, m, sF, L.nb));
+            }
+        } else {
+            CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
+        }
         if (k > kNdSolveMin) {
             // [Q X] = F_II^-1 [I F_IB] by the LU triangular solves (faster
             // than the in-place inverse plus a product for these k), into the
