@@ -432,7 +432,7 @@ k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int3
 // result is the LAPACK form getrs takes: L (unit, strictly lower) and U in
 // place of F_II, ipiv 1-based sequential row interchanges.
 constexpr int kLuPanel = 16;
-constexpr int kLuMin = 256;
+constexpr int kLuMin = 128;
 constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
 
 // LU of the panel rows j0..k-1, columns j0..j0+nbw-1 of every node
