@@ -4,18 +4,18 @@
 // factorisation of ceviche's default solve (solvers.py:50-62 behind
 // primitives.py:103-119, fdfd.py:172-192 / :211-244).
 //
-// The grid is bisected recursively (the longer side at its middle) down to
-// leaf boxes of at most kLeafCells cells.  A cell is on the boundary of a box
-// when one of its four stencil neighbours (without the periodic wrap, which
-// the preconditioner drops like the block-tridiagonal path drops the x-wrap)
-// lies outside the box.  Each node of the tree eliminates its interior I --
-// the whole box minus its boundary for a leaf, the boundary cells of its two
-// children that are not on its own boundary otherwise -- from a dense frontal
-// matrix F over [I; B]:
-//   leaf:     F = A restricted to the box (stencil rows of the cells),
-//   internal: F = the two children's Schur complements on their boundaries
-//             (placed at their positions) + the A entries that couple the
-//             two children across their shared edge,
+// The grid is bisected recursively (nested dissection): a box of more than
+// kLeafCells cells is cut by the middle line of its longer side, which is
+// the node's separator; the two halves are its children.  A node's
+// variables I are all cells of a leaf box or the separator cells of an
+// internal node; its boundary B is the ring of cells around its box (the
+// stencil neighbours outside it, without the periodic wrap, which the
+// preconditioner drops like the block-tridiagonal path drops the x-wrap),
+// all of them on separators of its ancestors.  Each node eliminates I from a
+// dense frontal matrix F over [I; B], assembled as the sum of its children's
+// Schur complements (extend-add) and the couplings of A that involve its own
+// variables (each entry of A added once, at the first node that eliminates
+// one of its two cells),
 // then factorises F_II = P L U and forms Q = F_II^-1 and X = Q F_IB -- for
 // interior blocks of at most kNdSolveMin cells by inverting in place from the
 // LU factors and one product, above that by the triangular solves of
@@ -27,8 +27,10 @@
 // strided batches padded to the level's largest |I| (kmax, identity padding)
 // and |B| (bmax, zero padding): dense LU, inversion or triangular solves
 // (rocsolver getrf, getri / getrs) and the products (rocblas gemm) run on the
-// matrix cores, one launch sequence per level.  The symbolic analysis (tree, index sets, maps)
-// is host C++ done once per grid shape.
+// matrix cores, one launch sequence per level; the Schur complements of a
+// level's large interior blocks (kLuMin < k <= kLuMaxRows) take an own
+// blocked LU.  The symbolic analysis (tree, index sets, maps) is host C++
+// done once per grid shape.
 //
 // Solve of A x = b (transpose 0), one matrix-vector product per node and sweep:
 //   up the tree:   [y_I; t] = [Q; G] b_I,  b_B -= t
@@ -37,9 +39,12 @@
 //   up:   [y_I; t] = [Q X]^T b_I,  b_B -= t
 //   down: x_I = y_I - G^T x_B
 // (A^T = [[F_II^T, F_BI^T], [F_IB^T, F_BB^T]], F_IB^T F_II^-T = X^T and
-// F_II^-T F_BI^T = G^T).  Each product is one launch over 64 x 64 tiles of
-// all nodes of a level, the input gathered from the work vector, with the
-// per-tile partial sums added and scattered back by a second launch.
+// F_II^-T F_BI^T = G^T).  Each product is one launch for all nodes of a
+// level (a workgroup per node up to 256 rows and columns, else 64 x 64 tiles
+// with partial sums added by a second launch), the input gathered from the
+// work vector.  Nodes of a level share boundary cells (the two sides of a
+// separator), so their b_B contributions are summed per cell in a fixed
+// order by one more launch.
 #include "fdfd_stencil.h"
 
 #include <rocblas/rocblas.h>
@@ -56,22 +61,25 @@ constexpr int kLeafCells = 64;
 constexpr int kMvTile = 64;        // rows and columns of one tile of the application products
 constexpr int kNdSolveMin = 128;   // interior blocks above this size: triangular solves instead of the inverse
 
-struct NdGroup {               // (parent, child) pairs whose child sits at one level
-    int child_level = 0;
+struct NdGroup {               // (parent, child) pairs whose child sits at one level, one child slot
+    int child_level = 0, slot = 0;
     std::vector<int32_t> parent, child, map;   // map: per pair, the child level's bmax entries
     int64_t off_parent = 0, off_child = 0, off_map = 0;
 };
 
 struct NdLevel {
     int nb = 0, kmax = 0, bmax = 0, m = 0;
-    bool leaf = false;
     std::vector<int32_t> icells, bcells;   // nb * kmax, nb * bmax (-1 = padding)
-    std::vector<int32_t> lcell, lnbr;      // leaves: per position the cell and 4 neighbour positions
-    std::vector<NdGroup> groups;           // internal: children's Schur complements
-    std::vector<int32_t> xnode, xpu, xpv, xcell, xdir;   // internal: couplings across the shared edge
+    // per interior position the [I; B] positions of its 4 stencil neighbours
+    // whose couplings this node assembles (-1: outside the grid or eliminated
+    // below this node)
+    std::vector<int32_t> anbr;
+    std::vector<NdGroup> groups;           // children's Schur complements, extend-added slot by slot
+    // the level's boundary cells, each with its contributions e * bmax + j
+    // (several nodes of a level can border the same separator cell)
+    std::vector<int32_t> bs_cells, bs_ptr, bs_idx;
     // byte offsets of the device copies (plan buffer) and of the numeric data (factor buffer)
-    int64_t off_icells = 0, off_bcells = 0, off_lcell = 0, off_lnbr = 0;
-    int64_t off_xnode = 0, off_xpu = 0, off_xpv = 0, off_xcell = 0, off_xdir = 0;
+    int64_t off_icells = 0, off_bcells = 0, off_anbr = 0, off_bs_cells = 0, off_bs_ptr = 0, off_bs_idx = 0;
     int64_t off_F = 0, off_piv = 0, info_first = 0;
 };
 
@@ -82,7 +90,7 @@ struct cv_nd_plan {
     int nx = 0, ny = 0;
     std::vector<cv::NdLevel> L;
     int64_t plan_bytes = 0, factor_bytes = 0;
-    int64_t off_w = 0, off_part = 0, off_info = 0, off_r = 0;
+    int64_t off_w = 0, off_part = 0, off_tb = 0, off_info = 0, off_r = 0;
     int64_t n_nodes = 0;
     double flops = 0.0;   // real flops of one numeric factorisation (unpadded sizes)
 };
@@ -95,22 +103,29 @@ int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
 struct Box {
     int x0, x1, y0, y1;
     int c0 = -1, c1 = -1, height = 0;
+    int sx = -1, sy = -1;   // the separator: column x = sx or row y = sy (internal nodes)
     std::vector<int32_t> I, B;
 };
 
+// Recursive bisection: a box of more than kLeafCells cells (and a side of
+// at least 3) is cut by the middle line of its longer side, which becomes
+// the node's separator; the two remaining halves are its children.
 int build_tree(std::vector<Box>& nodes, int x0, int x1, int y0, int y1) {
     const int idx = (int)nodes.size();
     nodes.push_back(Box{x0, x1, y0, y1});
-    if ((int64_t)(x1 - x0) * (y1 - y0) <= kLeafCells || (x1 - x0 < 2 && y1 - y0 < 2)) return idx;
+    const int w = x1 - x0, h = y1 - y0;
+    if ((int64_t)w * h <= kLeafCells || (w < 3 && h < 3)) return idx;
     int c0, c1;
-    if (x1 - x0 >= y1 - y0) {
+    if (w >= h) {
         const int xm = (x0 + x1) / 2;
+        nodes[idx].sx = xm;
         c0 = build_tree(nodes, x0, xm, y0, y1);
-        c1 = build_tree(nodes, xm, x1, y0, y1);
+        c1 = build_tree(nodes, xm + 1, x1, y0, y1);
     } else {
         const int ym = (y0 + y1) / 2;
+        nodes[idx].sy = ym;
         c0 = build_tree(nodes, x0, x1, y0, ym);
-        c1 = build_tree(nodes, x0, x1, ym, y1);
+        c1 = build_tree(nodes, x0, x1, ym + 1, y1);
     }
     nodes[idx].c0 = c0;
     nodes[idx].c1 = c1;
@@ -118,15 +133,21 @@ int build_tree(std::vector<Box>& nodes, int x0, int x1, int y0, int y1) {
     return idx;
 }
 
-// boundary cells of box b (a stencil neighbour inside the grid but outside the box), sorted
-void box_boundary(const Box& b, int nx, int ny, std::vector<int32_t>& out) {
+// the cells outside box b that are stencil neighbours of its cells (its
+// 5-point ring, clipped to the grid): all of them lie on separators of
+// b's ancestors; sorted
+void box_ring(const Box& b, int nx, int ny, std::vector<int32_t>& out) {
     out.clear();
-    for (int i = b.x0; i < b.x1; ++i)
-        for (int j = b.y0; j < b.y1; ++j) {
-            const bool on = (i == b.x0 && b.x0 > 0) || (i == b.x1 - 1 && b.x1 < nx) || (j == b.y0 && b.y0 > 0) ||
-                            (j == b.y1 - 1 && b.y1 < ny);
-            if (on) out.push_back((int32_t)((int64_t)i * ny + j));
-        }
+    auto cell = [&](int i, int j) { return (int32_t)((int64_t)i * ny + j); };
+    if (b.x0 > 0)
+        for (int j = b.y0; j < b.y1; ++j) out.push_back(cell(b.x0 - 1, j));
+    if (b.y0 > 0)
+        for (int i = b.x0; i < b.x1; ++i) out.push_back(cell(i, b.y0 - 1));
+    if (b.y1 < ny)
+        for (int i = b.x0; i < b.x1; ++i) out.push_back(cell(i, b.y1));
+    if (b.x1 < nx)
+        for (int j = b.y0; j < b.y1; ++j) out.push_back(cell(b.x1, j));
+    std::sort(out.begin(), out.end());
 }
 
 // stencil neighbours of cell c in the order xm, xp, ym, yp (-1 outside the grid; no wrap)
@@ -157,23 +178,17 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
     const int64_t N = (int64_t)nx * ny;
     int H = 0;
     for (auto& b : nodes) H = std::max(H, b.height);
-    // interior and boundary sets, bottom-up (children have larger indices than their parent)
-    for (int64_t q = (int64_t)nodes.size() - 1; q >= 0; --q) {
-        Box& b = nodes[q];
-        box_boundary(b, nx, ny, b.B);
+    // variables I (all cells of a leaf, the separator of an internal node)
+    // and boundary B (the box's ring)
+    for (auto& b : nodes) {
+        box_ring(b, nx, ny, b.B);
         if (b.c0 < 0) {
             for (int i = b.x0; i < b.x1; ++i)
-                for (int j = b.y0; j < b.y1; ++j) {
-                    const int32_t c = (int32_t)((int64_t)i * ny + j);
-                    if (!std::binary_search(b.B.begin(), b.B.end(), c)) b.I.push_back(c);
-                }
+                for (int j = b.y0; j < b.y1; ++j) b.I.push_back((int32_t)((int64_t)i * ny + j));
+        } else if (b.sx >= 0) {
+            for (int j = b.y0; j < b.y1; ++j) b.I.push_back((int32_t)((int64_t)b.sx * ny + j));
         } else {
-            std::vector<int32_t> U;
-            U.reserve(nodes[b.c0].B.size() + nodes[b.c1].B.size());
-            std::merge(nodes[b.c0].B.begin(), nodes[b.c0].B.end(), nodes[b.c1].B.begin(), nodes[b.c1].B.end(),
-                       std::back_inserter(U));
-            for (int32_t c : U)
-                if (!std::binary_search(b.B.begin(), b.B.end(), c)) b.I.push_back(c);
+            for (int i = b.x0; i < b.x1; ++i) b.I.push_back((int32_t)((int64_t)i * ny + b.sy));
         }
     }
     // algorithmic work of one factorisation, real flops of complex arithmetic
@@ -193,9 +208,15 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         level_index[q] = L.nb++;
         L.kmax = std::max<int>(L.kmax, (int)nodes[q].I.size());
         L.bmax = std::max<int>(L.bmax, (int)nodes[q].B.size());
-        L.leaf = nodes[q].c0 < 0;
     }
-    for (auto& L : P->L) L.m = L.kmax + L.bmax;
+    for (auto& L : P->L) {
+        L.m = L.kmax + L.bmax;
+        if ((int64_t)L.nb * L.m * L.m > ((int64_t)1 << 40)) {
+            set_error("nd plan: %d x %d is too large", nx, ny);
+            delete P;
+            return nullptr;
+        }
+    }
     std::vector<std::vector<int>> members(H + 1);
     for (size_t q = 0; q < nodes.size(); ++q) members[nodes[q].height].push_back((int)q);
     // position of a cell in the current node's [I; B] ordering
@@ -204,10 +225,8 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         NdLevel& L = P->L[h];
         L.icells.assign((size_t)L.nb * L.kmax, -1);
         L.bcells.assign((size_t)L.nb * L.bmax, -1);
-        if (L.leaf) {
-            L.lcell.assign((size_t)L.nb * L.m, -1);
-            L.lnbr.assign((size_t)L.nb * L.m * 4, -1);
-        }
+        L.anbr.assign((size_t)L.nb * L.kmax * 4, -1);
+        std::vector<std::pair<int32_t, int32_t>> contrib;   // (boundary cell, e * bmax + j)
         for (int q : members[h]) {
             const Box& b = nodes[q];
             const int e = level_index[q];
@@ -218,76 +237,63 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
             for (size_t p = 0; p < b.B.size(); ++p) {
                 L.bcells[(size_t)e * L.bmax + p] = b.B[p];
                 pos[b.B[p]] = (int32_t)(L.kmax + p);
+                contrib.emplace_back(b.B[p], (int32_t)((int64_t)e * L.bmax + p));
             }
-            auto inside = [&](int32_t c) {
-                if (c < 0) return false;
-                const int i = c / ny, j = c % ny;
-                return i >= b.x0 && i < b.x1 && j >= b.y0 && j < b.y1;
-            };
-            if (b.c0 < 0) {
-                auto put = [&](int32_t c) {
-                    const int p = pos[c];
-                    L.lcell[(size_t)e * L.m + p] = c;
-                    int32_t nbv[4];
-                    neighbours(c, nx, ny, nbv);
-                    for (int d = 0; d < 4; ++d)
-                        L.lnbr[((size_t)e * L.m + p) * 4 + d] = inside(nbv[d]) ? pos[nbv[d]] : -1;
-                };
-                for (int32_t c : b.I) put(c);
-                for (int32_t c : b.B) put(c);
-            } else {
+            for (size_t p = 0; p < b.I.size(); ++p) {
+                int32_t nbv[4];
+                neighbours(b.I[p], nx, ny, nbv);
+                for (int d = 0; d < 4; ++d)
+                    L.anbr[((size_t)e * L.kmax + p) * 4 + d] = nbv[d] >= 0 ? pos[nbv[d]] : -1;
+            }
+            if (b.c0 >= 0) {
                 for (int side = 0; side < 2; ++side) {
                     const int cq = side == 0 ? b.c0 : b.c1;
                     const int ch = nodes[cq].height;
                     const NdLevel& CL = P->L[ch];
                     NdGroup* g = nullptr;
                     for (auto& gg : L.groups)
-                        if (gg.child_level == ch) g = &gg;
+                        if (gg.child_level == ch && gg.slot == side) g = &gg;
                     if (!g) {
                         L.groups.push_back(NdGroup());
                         g = &L.groups.back();
                         g->child_level = ch;
+                        g->slot = side;
                     }
                     g->parent.push_back(e);
                     g->child.push_back(level_index[cq]);
                     const size_t base = g->map.size();
                     g->map.resize(base + CL.bmax, -1);
-                    for (size_t p = 0; p < nodes[cq].B.size(); ++p) g->map[base + p] = pos[nodes[cq].B[p]];
-                }
-                // couplings between the two children: u in B(c0), v in B(c1) adjacent
-                const Box& b0 = nodes[b.c0];
-                const Box& b1 = nodes[b.c1];
-                auto in1 = [&](int32_t c) {
-                    if (c < 0) return false;
-                    const int i = c / ny, j = c % ny;
-                    return i >= b1.x0 && i < b1.x1 && j >= b1.y0 && j < b1.y1;
-                };
-                for (int32_t u : b0.B) {
-                    int32_t nbv[4];
-                    neighbours(u, nx, ny, nbv);
-                    for (int d = 0; d < 4; ++d) {
-                        const int32_t v = nbv[d];
-                        if (!in1(v)) continue;
-                        // A[u, v]: row u, direction d; A[v, u]: row v, the opposite direction
-                        L.xnode.push_back(e);
-                        L.xpu.push_back(pos[u]);
-                        L.xpv.push_back(pos[v]);
-                        L.xcell.push_back(u);
-                        L.xdir.push_back(d);
-                        L.xnode.push_back(e);
-                        L.xpu.push_back(pos[v]);
-                        L.xpv.push_back(pos[u]);
-                        L.xcell.push_back(v);
-                        L.xdir.push_back(d ^ 1);
+                    for (size_t p = 0; p < nodes[cq].B.size(); ++p) {
+                        const int32_t ps = pos[nodes[cq].B[p]];
+                        if (ps < 0) {   // a child's ring lies on its parent's separator or ring
+                            set_error("nd plan: inconsistent tree at node %d", q);
+                            delete P;
+                            return nullptr;
+                        }
+                        g->map[base + p] = ps;
                     }
                 }
             }
             for (int32_t c : b.I) pos[c] = -1;
             for (int32_t c : b.B) pos[c] = -1;
         }
+        // the groups of one child level: slot 0 before slot 1 (extend-add order)
+        std::stable_sort(L.groups.begin(), L.groups.end(), [](const NdGroup& x, const NdGroup& y) {
+            return x.child_level != y.child_level ? x.child_level < y.child_level : x.slot < y.slot;
+        });
+        std::sort(contrib.begin(), contrib.end());
+        L.bs_ptr.push_back(0);
+        for (size_t i = 0; i < contrib.size(); ++i) {
+            if (i == 0 || contrib[i].first != contrib[i - 1].first) {
+                if (i > 0) L.bs_ptr.push_back((int32_t)i);
+                L.bs_cells.push_back(contrib[i].first);
+            }
+            L.bs_idx.push_back(contrib[i].second);
+        }
+        if (!contrib.empty()) L.bs_ptr.push_back((int32_t)contrib.size());
     }
     // device layouts: index arrays in the plan buffer, numeric data in the factor buffer
-    int64_t po = 0, fo = 0, pmax = 0, rmax = 0, info_first = 0;
+    int64_t po = 0, fo = 0, pmax = 0, rmax = 0, tbmax = 0, info_first = 0;
     auto take = [](int64_t& o, int64_t bytes) {
         const int64_t r = o;
         o += al256(bytes);
@@ -296,18 +302,15 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
     for (auto& L : P->L) {
         L.off_icells = take(po, 4 * (int64_t)L.icells.size());
         L.off_bcells = take(po, 4 * (int64_t)L.bcells.size());
-        L.off_lcell = take(po, 4 * (int64_t)L.lcell.size());
-        L.off_lnbr = take(po, 4 * (int64_t)L.lnbr.size());
+        L.off_anbr = take(po, 4 * (int64_t)L.anbr.size());
         for (auto& g : L.groups) {
             g.off_parent = take(po, 4 * (int64_t)g.parent.size());
             g.off_child = take(po, 4 * (int64_t)g.child.size());
             g.off_map = take(po, 4 * (int64_t)g.map.size());
         }
-        L.off_xnode = take(po, 4 * (int64_t)L.xnode.size());
-        L.off_xpu = take(po, 4 * (int64_t)L.xpu.size());
-        L.off_xpv = take(po, 4 * (int64_t)L.xpv.size());
-        L.off_xcell = take(po, 4 * (int64_t)L.xcell.size());
-        L.off_xdir = take(po, 4 * (int64_t)L.xdir.size());
+        L.off_bs_cells = take(po, 4 * (int64_t)L.bs_cells.size());
+        L.off_bs_ptr = take(po, 4 * (int64_t)L.bs_ptr.size());
+        L.off_bs_idx = take(po, 4 * (int64_t)L.bs_idx.size());
         L.off_F = take(fo, 8 * (int64_t)L.nb * L.m * L.m);
         L.off_piv = take(fo, 4 * (int64_t)L.nb * std::max(1, L.kmax));
         L.info_first = info_first;
@@ -317,11 +320,13 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         pmax = std::max<int64_t>(pmax, (int64_t)L.nb * std::max<int64_t>((int64_t)((L.kmax + kMvTile - 1) / kMvTile) * L.m,
                                                                         (int64_t)((L.bmax + kMvTile - 1) / kMvTile) * L.kmax));
         rmax = std::max<int64_t>(rmax, (int64_t)L.nb * L.kmax * L.m);
+        tbmax = std::max<int64_t>(tbmax, (int64_t)L.nb * L.bmax);
     }
     P->n_nodes = info_first;
     P->off_info = take(fo, 4 * info_first);
     P->off_w = take(fo, 8 * N);
     P->off_part = take(fo, 8 * std::max<int64_t>(1, pmax));
+    P->off_tb = take(fo, 8 * std::max<int64_t>(1, tbmax));
     P->off_r = take(fo, 8 * std::max<int64_t>(1, rmax));
     P->plan_bytes = std::max<int64_t>(po, 256);
     P->factor_bytes = fo;
@@ -358,9 +363,9 @@ extern "C" int cv_nd_plan_upload(const cv_nd_plan* p, void* plan_dev, int64_t pl
     };
     for (const auto& L : p->L) {
         int rc = CV_OK;
-        if ((rc = up(L.off_icells, L.icells)) || (rc = up(L.off_bcells, L.bcells)) || (rc = up(L.off_lcell, L.lcell)) ||
-            (rc = up(L.off_lnbr, L.lnbr)) || (rc = up(L.off_xnode, L.xnode)) || (rc = up(L.off_xpu, L.xpu)) ||
-            (rc = up(L.off_xpv, L.xpv)) || (rc = up(L.off_xcell, L.xcell)) || (rc = up(L.off_xdir, L.xdir)))
+        if ((rc = up(L.off_icells, L.icells)) || (rc = up(L.off_bcells, L.bcells)) || (rc = up(L.off_anbr, L.anbr)) ||
+            (rc = up(L.off_bs_cells, L.bs_cells)) || (rc = up(L.off_bs_ptr, L.bs_ptr)) ||
+            (rc = up(L.off_bs_idx, L.bs_idx)))
             return rc;
         for (const auto& g : L.groups)
             if ((rc = up(g.off_parent, g.parent)) || (rc = up(g.off_child, g.child)) || (rc = up(g.off_map, g.map)))
@@ -379,40 +384,53 @@ __device__ __forceinline__ cx<float> row_dir(const Row5<float>& r, int d) {
     return d == 0 ? r.xm : d == 1 ? r.xp : d == 2 ? r.ym : r.yp;
 }
 
-// leaf frontal matrices: F[p, p] = A_uu, F[p, q] = A_uv for stencil
-// neighbours inside the box; identity on padded interior positions.
+// The couplings of A a node adds to its frontal matrix (after its
+// children's Schur complements): for every variable s (row p) the diagonal
+// and, for each stencil neighbour t at position q >= 0 of the node's
+// [I; B] ordering, A[s, t] at (p, q) and, when t is a boundary cell
+// (q >= kmax), A[t, s] at (q, p).  Couplings to cells eliminated below the
+// node (q < 0) were added there.  Each entry has a single writer.  Padded
+// interior positions take the identity.
 __global__ void __launch_bounds__(256)
-k_nd_leaf(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
-          const float* __restrict__ m1, const int32_t* __restrict__ lcell, const int32_t* __restrict__ lnbr,
-          cx<float>* __restrict__ F, int nb, int m, int kmax) {
+k_nd_assemble(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof,
+              const float* __restrict__ m0, const float* __restrict__ m1, const int32_t* __restrict__ icells,
+              const int32_t* __restrict__ anbr, cx<float>* __restrict__ F, int nb, int kmax, int m) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nb * m) return;
-    const int e = (int)(t / m), p = (int)(t - (int64_t)e * m);
+    if (t >= (int64_t)nb * kmax) return;
+    const int e = (int)(t / kmax), p = (int)(t - (int64_t)e * kmax);
     cx<float>* Fe = F + (int64_t)e * m * m;
-    const int32_t c = lcell[t];
+    const int32_t c = icells[t];
     if (c < 0) {
-        if (p < kmax) Fe[(int64_t)p * m + p] = mk<float>(1.f, 0.f);
+        Fe[(int64_t)p * m + p] = Fe[(int64_t)p * m + p] + mk<float>(1.f, 0.f);
         return;
     }
     const cv_fdfd_desc d = desc[0];
     const Row5<float> r = row_coefs<float>(d, prof, m0, m1, c / d.ny, c % d.ny);
-    Fe[(int64_t)p * m + p] = r.c;
+    Fe[(int64_t)p * m + p] = Fe[(int64_t)p * m + p] + r.c;
 #pragma unroll
     for (int dd = 0; dd < 4; ++dd) {
-        const int q = lnbr[t * 4 + dd];
-        if (q >= 0) Fe[(int64_t)q * m + p] = row_dir(r, dd);
+        const int q = anbr[t * 4 + dd];
+        if (q < 0) continue;
+        Fe[(int64_t)q * m + p] = Fe[(int64_t)q * m + p] + row_dir(r, dd);
+        if (q >= kmax) {
+            const int32_t tc = dd == 0 ? c - d.ny : dd == 1 ? c + d.ny : dd == 2 ? c - 1 : c + 1;
+            const Row5<float> rt = row_coefs<float>(d, prof, m0, m1, tc / d.ny, tc % d.ny);
+            Fe[(int64_t)p * m + q] = Fe[(int64_t)p * m + q] + row_dir(rt, dd ^ 1);
+        }
     }
 }
 
 // children's Schur complements (the bottom-right bc x bc block of their F)
-// into the parents' F at the mapped positions; blockIdx.y = pair, blockIdx.x
-// = a slice of kNdCols child columns (one wave per column, rows over lanes)
+// added into the parents' F at the mapped positions (one child slot per
+// launch, so every parent entry has one writer); blockIdx.x = pair,
+// blockIdx.y = a slice of kNdCols child columns (one wave per column, rows
+// over lanes)
 constexpr int kNdCols = 4;
 __global__ void __launch_bounds__(256)
 k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int32_t* __restrict__ parent,
             const int32_t* __restrict__ child, const int32_t* __restrict__ map, cx<float>* __restrict__ F, int m) {
-    const int pr = blockIdx.y;
-    const int jj = blockIdx.x * kNdCols + (threadIdx.x >> 6);
+    const int pr = blockIdx.x;
+    const int jj = blockIdx.y * kNdCols + (threadIdx.x >> 6);
     if (jj >= bc) return;
     const int32_t* mp = map + (int64_t)pr * bc;
     const int pj = mp[jj];
@@ -421,7 +439,7 @@ k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int3
     cx<float>* Fcol = F + (int64_t)parent[pr] * m * m + (int64_t)pj * m;
     for (int ii = threadIdx.x & 63; ii < bc; ii += 64) {
         const int pi = mp[ii];
-        if (pi >= 0) Fcol[pi] = Scol[ii];
+        if (pi >= 0) Fcol[pi] = Fcol[pi] + Scol[ii];
     }
 }
 
@@ -627,30 +645,6 @@ k_nd_block_copy(const cx<float>* __restrict__ src, int64_t src_stride, int src_l
     dst[(int64_t)e * dst_stride + (int64_t)j * dst_ld + i] = src[(int64_t)e * src_stride + (int64_t)j * src_ld + i];
 }
 
-// couplings across the children's shared edge: F[pu, pv] = A[u, v]
-__global__ void __launch_bounds__(256)
-k_nd_cross(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict__ prof, const float* __restrict__ m0,
-           const float* __restrict__ m1, const int32_t* __restrict__ node, const int32_t* __restrict__ pu,
-           const int32_t* __restrict__ pv, const int32_t* __restrict__ cell, const int32_t* __restrict__ dir, int n,
-           cx<float>* __restrict__ F, int m) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const cv_fdfd_desc d = desc[0];
-    const int32_t c = cell[t];
-    const Row5<float> r = row_coefs<float>(d, prof, m0, m1, c / d.ny, c % d.ny);
-    F[(int64_t)node[t] * m * m + (int64_t)pv[t] * m + pu[t]] = row_dir(r, dir[t]);
-}
-
-// identity on the padded interior positions of internal nodes
-__global__ void __launch_bounds__(256)
-k_nd_pad(const int32_t* __restrict__ icells, int nb, int kmax, cx<float>* __restrict__ F, int m) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nb * kmax) return;
-    if (icells[t] >= 0) return;
-    const int e = (int)(t / kmax), p = (int)(t - (int64_t)e * kmax);
-    F[(int64_t)e * m * m + (int64_t)p * m + p] = mk<float>(1.f, 0.f);
-}
-
 // One 64 x 64 tile of a per-node product over a block of F (rows row0..,
 // columns col0.., M x N, column-major with leading dimension m).  The input
 // vector is gathered from the work vector w through xcells (xn entries per
@@ -719,13 +713,14 @@ k_nd_mv(const cx<float>* __restrict__ F, int m, int row0, int col0, int M, int N
 // flight: TRANS = 0 an output row (the loads of a wave are coalesced), TRANS
 // = 1 an output column (each thread reads its column contiguously).
 // Outputs o < k go to the interior cells (w = y when set_interior, else
-// w -= y), o >= k to the boundary cells (w -= y).
+// w -= y), o >= k to the contribution buffer tb (e * b + o - k), which
+// k_nd_bsum subtracts from the boundary cells.
 constexpr int kMvNode = 256;
 template <bool TRANS>
 __global__ void __launch_bounds__(256)
 k_nd_mv_node(const cx<float>* __restrict__ F, int m, int row0, int col0, int M, int N,
-             const int32_t* __restrict__ xcells, int xn, const int32_t* __restrict__ icells, int k,
-             const int32_t* __restrict__ bcells, int b, cx<float>* __restrict__ w, int set_interior) {
+             const int32_t* __restrict__ xcells, int xn, const int32_t* __restrict__ icells, int k, int b,
+             cx<float>* __restrict__ w, cx<float>* __restrict__ tb, int set_interior) {
     __shared__ cx<float> xs[kMvNode];
     const int e = blockIdx.x, tid = threadIdx.x;
     const int32_t* xc = xcells + (int64_t)e * xn;
@@ -762,17 +757,16 @@ k_nd_mv_node(const cx<float>* __restrict__ F, int m, int row0, int col0, int M, 
         const int32_t c = icells[(int64_t)e * k + o];
         if (c >= 0) w[c] = set_interior ? acc : w[c] - acc;
     } else {
-        const int32_t c = bcells[(int64_t)e * b + (o - k)];
-        if (c >= 0) w[c] = w[c] - acc;
+        tb[(int64_t)e * b + (o - k)] = acc;
     }
 }
 
-// Sum of the nt partial-sum tiles of each output o (Mout per node) and the
-// scatter into the work vector: o < k to the interior cells (w = sum when
-// set_interior, else w -= sum), o >= k to the boundary cells (w -= sum).
+// Sum of the nt partial-sum tiles of each output o (Mout per node): o < k
+// to the interior cells (w = sum when set_interior, else w -= sum), o >= k
+// to the contribution buffer tb (e * b + o - k).
 __global__ void __launch_bounds__(256)
 k_nd_mv_reduce(const cx<float>* __restrict__ part, int nt, int Mout, int nb, const int32_t* __restrict__ icells,
-               int k, const int32_t* __restrict__ bcells, int b, cx<float>* __restrict__ w, int set_interior) {
+               int k, int b, cx<float>* __restrict__ w, cx<float>* __restrict__ tb, int set_interior) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nb * Mout) return;
     const int e = (int)(t / Mout), o = (int)(t - (int64_t)e * Mout);
@@ -782,9 +776,21 @@ k_nd_mv_reduce(const cx<float>* __restrict__ part, int nt, int Mout, int nb, con
         const int32_t c = icells[(int64_t)e * k + o];
         if (c >= 0) w[c] = set_interior ? sum : w[c] - sum;
     } else {
-        const int32_t c = bcells[(int64_t)e * b + (o - k)];
-        if (c >= 0) w[c] = w[c] - sum;
+        tb[(int64_t)e * b + (o - k)] = sum;
     }
+}
+
+// w[cell] -= the sum of its contributions, for each boundary cell of a level
+// (CSR over the level's nodes, fixed order)
+__global__ void __launch_bounds__(256)
+k_nd_bsum(const int32_t* __restrict__ cells, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx, int n,
+          const cx<float>* __restrict__ tb, cx<float>* __restrict__ w) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    cx<float> sum = czero<float>();
+    for (int q = ptr[t]; q < ptr[t + 1]; ++q) sum = sum + tb[idx[q]];
+    const int32_t c = cells[t];
+    w[c] = w[c] - sum;
 }
 
 __global__ void __launch_bounds__(256) k_nd_copy(const cx<float>* __restrict__ a, cx<float>* __restrict__ b, int64_t n) {
@@ -837,6 +843,7 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
     const int64_t N = (int64_t)P->nx * P->ny;
     cx<float>* w = (cx<float>*)(fb + P->off_w);
     cx<float>* part = (cx<float>*)(fb + P->off_part);
+    cx<float>* tb = (cx<float>*)(fb + P->off_tb);
     k_nd_copy<<<nblk(N), 256, 0, s>>>(v, w, N);
     CV_LAUNCH_CHECK();
     auto tiles = [](int n) { return (n + kMvTile - 1) / kMvTile; };
@@ -844,15 +851,14 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
     auto product = [&](const NdLevel& L, bool trans, int row0, int col0, int M, int N, const int32_t* xcells,
                        int xn, int Mout, int set_interior) -> int {
         const int32_t* ic = (const int32_t*)(pb + L.off_icells);
-        const int32_t* bc = (const int32_t*)(pb + L.off_bcells);
         const cx<float>* F = (const cx<float>*)(fb + L.off_F);
         if (L.m <= kMvNode) {   // small nodes: one fused launch, one workgroup per node
             if (!trans)
                 k_nd_mv_node<false><<<L.nb, 64 * ((M + 63) / 64), 0, s>>>(F, L.m, row0, col0, M, N, xcells, xn, ic,
-                                                                         L.kmax, bc, L.bmax, w, set_interior);
+                                                                         L.kmax, L.bmax, w, tb, set_interior);
             else
                 k_nd_mv_node<true><<<L.nb, 64 * ((N + 63) / 64), 0, s>>>(F, L.m, row0, col0, M, N, xcells, xn, ic,
-                                                                        L.kmax, bc, L.bmax, w, set_interior);
+                                                                        L.kmax, L.bmax, w, tb, set_interior);
             CV_LAUNCH_CHECK();
             return CV_OK;
         }
@@ -864,7 +870,7 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
             k_nd_mv<true><<<(unsigned)nwg, 256, 0, s>>>(F, L.m, row0, col0, M, N, tm, tn, xcells, xn, w, part);
         CV_LAUNCH_CHECK();
         const int nt = trans ? tm : tn;
-        k_nd_mv_reduce<<<nblk((int64_t)L.nb * Mout), 256, 0, s>>>(part, nt, Mout, L.nb, ic, L.kmax, bc, L.bmax, w,
+        k_nd_mv_reduce<<<nblk((int64_t)L.nb * Mout), 256, 0, s>>>(part, nt, Mout, L.nb, ic, L.kmax, L.bmax, w, tb,
                                                                    set_interior);
         CV_LAUNCH_CHECK();
         return CV_OK;
@@ -877,6 +883,13 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
         const int rc = !transpose ? product(L, false, 0, 0, L.m, L.kmax, ic, L.kmax, L.m, 1)
                                   : product(L, true, 0, 0, L.kmax, L.m, ic, L.kmax, L.m, 1);
         if (rc != CV_OK) return rc;
+        const int nbs = (int)L.bs_cells.size();
+        if (nbs > 0) {   // b_B -= t, the contributions of the level's nodes summed per boundary cell
+            k_nd_bsum<<<nblk(nbs), 256, 0, s>>>((const int32_t*)(pb + L.off_bs_cells),
+                                                (const int32_t*)(pb + L.off_bs_ptr), (const int32_t*)(pb + L.off_bs_idx),
+                                                nbs, tb, w);
+            CV_LAUNCH_CHECK();
+        }
     }
     for (int l = H - 1; l >= 0; --l) {   // down the tree: x_I -= X x_B (or G^T x_B)
         const NdLevel& L = P->L[l];
@@ -923,29 +936,21 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
         rc_t* F = (rc_t*)(fb + L.off_F);
         const rocblas_stride sF = (rocblas_stride)L.m * L.m;
         CV_HIP(hipMemsetAsync(F, 0, 8 * (size_t)L.nb * L.m * L.m, s));
-        if (L.leaf) {
-            k_nd_leaf<<<nblk((int64_t)L.nb * L.m), 256, 0, s>>>(desc, prof, m0, m1, (const int32_t*)(pb + L.off_lcell),
-                                                                 (const int32_t*)(pb + L.off_lnbr), (cx<float>*)F,
-                                                                 L.nb, L.m, L.kmax);
-        } else {
-            for (const auto& g : L.groups) {
-                const NdLevel& C = P->L[g.child_level];
-                if (C.bmax == 0 || g.parent.empty()) continue;
-                const dim3 grid((C.bmax + kNdCols - 1) / kNdCols, (unsigned)g.parent.size());
-                k_nd_extend<<<grid, 256, 0, s>>>(
-                    (const cx<float>*)(fb + C.off_F), C.m, C.kmax, C.bmax, (const int32_t*)(pb + g.off_parent),
-                    (const int32_t*)(pb + g.off_child), (const int32_t*)(pb + g.off_map), (cx<float>*)F, L.m);
-            }
-            const int nx_ = (int)L.xnode.size();
-            if (nx_ > 0)
-                k_nd_cross<<<nblk(nx_), 256, 0, s>>>(desc, prof, m0, m1, (const int32_t*)(pb + L.off_xnode),
-                                                     (const int32_t*)(pb + L.off_xpu), (const int32_t*)(pb + L.off_xpv),
-                                                     (const int32_t*)(pb + L.off_xcell),
-                                                     (const int32_t*)(pb + L.off_xdir), nx_, (cx<float>*)F, L.m);
-            if (L.kmax > 0)
-                k_nd_pad<<<nblk((int64_t)L.nb * L.kmax), 256, 0, s>>>((const int32_t*)(pb + L.off_icells), L.nb,
-                                                                     L.kmax, (cx<float>*)F, L.m);
+        // frontal matrix: the children's Schur complements (extend-add, one
+        // child slot per launch), then the node's own couplings of A
+        for (const auto& g : L.groups) {
+            const NdLevel& C = P->L[g.child_level];
+            if (C.bmax == 0 || g.parent.empty()) continue;
+            const dim3 grid((unsigned)g.parent.size(), (C.bmax + kNdCols - 1) / kNdCols);
+            k_nd_extend<<<grid, 256, 0, s>>>((const cx<float>*)(fb + C.off_F), C.m, C.kmax, C.bmax,
+                                             (const int32_t*)(pb + g.off_parent), (const int32_t*)(pb + g.off_child),
+                                             (const int32_t*)(pb + g.off_map), (cx<float>*)F, L.m);
+            CV_LAUNCH_CHECK();
         }
+        if (L.kmax > 0)
+            k_nd_assemble<<<nblk((int64_t)L.nb * L.kmax), 256, 0, s>>>(
+                desc, prof, m0, m1, (const int32_t*)(pb + L.off_icells), (const int32_t*)(pb + L.off_anbr),
+                (cx<float>*)F, L.nb, L.kmax, L.m);
         CV_LAUNCH_CHECK();
         if (L.kmax == 0) continue;
         const int k = L.kmax, b = L.bmax, m = L.m;
