@@ -37,7 +37,9 @@ class SolverOptions:
     factorisation), 'bicgstab', or 'cocg' (COCG on the complex-symmetric
     form: zero Bloch phases, uncoupled instances).  restart: FGMRES
     cycle length of the direct path; direct_max_iter: its FGMRES step limit
-    (max_iter when None).  Under 'auto' a direct solve that fails (singular
+    (min(max_iter, DIRECT_MAX_ITER) when None: with a factorisation the
+    steps number a few to a few tens, and a factor that does not converge
+    by then gives way to the fallback).  Under 'auto' a direct solve that fails (singular
     factor, FGMRES not converged, out of memory) falls back to the Krylov
     method with a warning; under 'direct' it raises."""
 
@@ -56,6 +58,11 @@ class SolverOptions:
 
 
 DEFAULT_TOL = {torch.complex128: 1e-10, torch.complex64: 1e-6}
+DIRECT_MAX_ITER = 600
+
+
+def _direct_max_iter(opts):
+    return min(opts.max_iter, DIRECT_MAX_ITER) if opts.direct_max_iter is None else opts.direct_max_iter
 
 
 class FdfdOperator:
@@ -527,7 +534,7 @@ class Direct:
         b = bs.to(torch.complex128).contiguous()
         x = torch.zeros_like(b)
         info = (_lib.SolveInfo * 1)()
-        max_iter = opts.max_iter if opts.direct_max_iter is None else opts.direct_max_iter
+        max_iter = _direct_max_iter(opts)
         rc = L.cv_fdfd_fgmres_direct(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(op.prof64),
                                      ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1),
                                      self.code, ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)), float(tol),
@@ -688,7 +695,10 @@ class InstanceDirect:
     dissection) and solved by FP64-refined FGMRES.  The factors are kept for
     the adjoint and tangent solves when all of them fit in
     DIRECT_MEMORY_FRACTION of the free device memory; otherwise each solve
-    factorises its instance again and releases the factor."""
+    factorises its instance again and releases the factor.  Under method
+    'auto' an instance whose factorisation or FGMRES fails is solved by
+    Jacobi BiCGSTAB alone (and stays on it for the adjoint and tangent
+    solves); the other instances keep their factors."""
 
     kind = 'instances'
 
@@ -704,9 +714,7 @@ class InstanceDirect:
         total = sum(_factor_bytes(sub, kd) for sub, kd in zip(self.subs, self.kinds))
         self.keep = total <= DIRECT_MEMORY_FRACTION * free
         self.factors = [None] * op.n_inst
-        if self.keep:   # factorise now, so a singular instance falls back before any solve
-            for k in range(op.n_inst):
-                self.factors[k] = self._make(k)
+        self.failed = set()   # instances on the Krylov fallback
 
     def _seg(self, v, k):
         if v is None:
@@ -718,16 +726,34 @@ class InstanceDirect:
         return make_direct(self.subs[k], self._seg(self.m0, k), self._seg(self.m1, k), self.kinds[k])
 
     def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
+        """Every instance by its direct factorisation; under method 'auto' an
+        instance whose factorisation or FGMRES fails (and its later adjoint /
+        tangent solves) falls back to Jacobi BiCGSTAB alone."""
         x = torch.empty(bs.shape, dtype=torch.complex128, device=bs.device)
         info = []
         for k in range(self.op.n_inst):
-            f = self.factors[k] if self.factors[k] is not None else self._make(k)
             sub = self.subs[k]
-            x[self.op.offsets[k]:self.op.offsets[k] + self.sizes[k]] = f.solve(
-                self._seg(bs, k).contiguous(), self._seg(m0_64, k).contiguous(),
-                None if m1_64 is None else self._seg(m1_64, k).contiguous(), transpose=transpose, opts=opts)
+            seg = (self._seg(bs, k).contiguous(), self._seg(m0_64, k).contiguous(),
+                   None if m1_64 is None else self._seg(m1_64, k).contiguous())
+            out = None
+            if k not in self.failed:
+                try:
+                    f = self.factors[k]
+                    if f is None:
+                        f = self._make(k)
+                        if self.keep:   # kept for the adjoint and tangent solves
+                            self.factors[k] = f
+                    out = f.solve(*seg, transpose=transpose, opts=opts)
+                    del f
+                except (CevicheError, torch.cuda.OutOfMemoryError) as err:
+                    if not _direct_failed(err, opts):
+                        raise
+                    self.failed.add(k)
+                    self.factors[k] = None
+            if out is None:
+                out = sub.solve_scaled(seg[0], seg[1], seg[2], transpose=transpose, opts=opts)
+            x[self.op.offsets[k]:self.op.offsets[k] + self.sizes[k]] = out
             info += sub.last_info
-            del f
         self.op.last_info = info
         return x
 
@@ -812,7 +838,7 @@ class NdDirect:
         b = bs.to(torch.complex128).contiguous()
         x = torch.zeros_like(b)
         info = (_lib.SolveInfo * 1)()
-        max_iter = opts.max_iter if opts.direct_max_iter is None else opts.direct_max_iter
+        max_iter = _direct_max_iter(opts)
         rc = L.cv_fdfd_fgmres_nd(ptr(op.desc), op.desc_host.ctypes.data_as(ctypes.c_void_p), ptr(op.prof64),
                                  ptr(m0_64), ptr(m1_64), ptr(self.prof32), ptr(self.m0), ptr(self.m1), self.plan.ptr,
                                  ptr(self.plan.dev), ptr(self.factor), ptr(b), ptr(x), int(bool(transpose)),

@@ -239,8 +239,23 @@ extern "C" int64_t cv_fdfd_fgmres_workspace_bytes(int nx, int ny, int restart, i
 
 namespace cv {
 
+// With a direct factorisation as the preconditioner a solve takes a few
+// restart cycles; four consecutive full cycles (all m steps taken) that reduce
+// the true residual norm by less than 10 % end it as not converged (the
+// caller's fallback takes over).  A cycle that stops early has met its
+// estimated target, so it neither counts nor resets the count.
+struct StallStop {
+    int stalls = 0;
+    bool operator()(double rr, double rr_prev, bool full) {
+        if (full) stalls = rr > 0.81 * rr_prev ? stalls + 1 : 0;
+        return stalls >= 4;
+    }
+};
+
 // The restarted flexible GMRES loop shared by every preconditioner.
-// precondition(v, z) writes z ~ M^-1 v; after_cycle(rr, rr_prev) may change it.
+// precondition(v, z) writes z ~ M^-1 v; after_cycle(rr, rr_prev, full) may
+// change it (full: the cycle took all m steps), and returning true ends the
+// solve as not converged (unless the tolerance was just met).
 template <bool TRANS, typename Precond, typename AfterCycle>
 static int fgmres_core(const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const void* prof64, const void* m0_64,
                        const void* m1_64, const void* prof32, const void* m0_32, const void* m1_32, const void* b,
@@ -349,7 +364,10 @@ static int fgmres_core(const cv_fdfd_desc* desc, const cv_fdfd_desc& dh, const v
         CV_LAUNCH_CHECK();
         const double rr_prev = rr;
         if ((rc = true_resid(&rr, &bb)) != CV_OK) return rc;
-        after_cycle(rr, rr_prev);
+        if (after_cycle(rr, rr_prev, k == m) && sqrt(rr) > target) {
+            status = CV_ERR_NOT_CONVERGED;
+            break;
+        }
     }
     info[0].iterations = iters;
     info[0].restarts = cycles;
@@ -407,7 +425,7 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
         return CV_OK;
     };
 
-    auto after_cycle = [&](double rr, double rr_prev) {
+    auto after_cycle = [&](double rr, double rr_prev, bool) {
         if (use == CV_PRECOND_UNET) {
             stalls = (rr > 0.9801 * rr_prev) ? stalls + 1 : 0;
             if (stalls >= 3) {
@@ -415,6 +433,7 @@ extern "C" int cv_fdfd_fgmres(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc
                 fell_back = 1;
             }
         }
+        return false;
     };
     const int status = fgmres_core<false>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter,
                                           inner_rtol, w, precondition, after_cycle, info, s);
@@ -452,7 +471,7 @@ extern "C" int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_des
     auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
         return direct_apply(factor_kind, desc, dh, prof32, m0_32, m1_32, factor, v, z, transpose, s);
     };
-    auto after_cycle = [](double, double) {};
+    StallStop after_cycle;
     int status;
     if (transpose)
         status = fgmres_core<true>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
@@ -498,7 +517,7 @@ extern "C" int cv_fdfd_fgmres_nd(const cv_fdfd_desc* desc, const cv_fdfd_desc* d
     auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
         return nd_apply(plan, plan_dev, factor, v, z, transpose, s);
     };
-    auto after_cycle = [](double, double) {};
+    StallStop after_cycle;
     int status;
     if (transpose)
         status = fgmres_core<true>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
@@ -547,7 +566,8 @@ extern "C" int cv_fdfd_fgmres_blockdiag(const cv_fdfd_desc* desc, const cv_fdfd_
     auto precondition = [&](const cx<float>* v, cx<float>* z) -> int {
         return bcr_apply_batch(block_desc_host[0], nb, factors, v, z, n0, transpose, s);
     };
-    auto after_cycle = [](double, double) {};
+    // no fallback behind this solve: it runs to max_iter
+    auto after_cycle = [](double, double, bool) { return false; };
     int status;
     if (transpose)
         status = fgmres_core<true>(desc, dh, prof64, m0_64, m1_64, prof32, m0_32, m1_32, b, x, tol, m, max_iter, 0.0,
