@@ -1176,6 +1176,532 @@ __global__ void __launch_bounds__(64 * kMRows / kMRpt) k_march(Fdtd<R> f, const 
     }
 }
 
+// ==================================== slab-marching single-pass 3D step
+// A workgroup of nz threads (one z line each, nz a multiple of 64 up to 256)
+// owns a band of TJ whole (y, z) rows and marches along x over a chunk of
+// slices, so z needs no halo and y one row on each side.  Per slice xi it
+// holds E(xi) and E(xi+1) of rows -1..TJ (E = D / eps_aa formed once from D
+// and eps of the band), forms the new H(xi) of rows -1..TJ-1 (row -1 only
+// Hx, Hz, the y-1 values the D update needs), then the new D(xi) of the band
+// from H(xi) and the H(xi-1) carried from the previous slice.  The z+1
+// neighbours of E and the z-1 neighbours of H go through LDS.  All reads come
+// from the old state A and all writes go to B (ping-pong), so neighbouring
+// bands may recompute each other's halo rows in any order; a chunk starts by
+// recomputing H(i0-1).  HBM traffic per cell-step: H, D, eps read, H, D
+// written (52 B in float32, plus the PML integrals inside the PML); the halo
+// rows are re-read by the neighbouring band on the same XCD (consecutive
+// bands of a chunk are mapped to one XCD).  Same arithmetic as h_cell /
+// d_cell, term by term.
+constexpr int kSlabMaxNz = 256;
+
+// Load / store at a 32-bit byte offset through a raw buffer resource over
+// one slice of one array (base and size uniform, in scalar registers; one
+// vector register per address).  An offset outside the slice reads 0 and
+// drops the store.  Resource word 3 = 0x00020000, the gfx9 raw-buffer format.
+template <typename R>
+__device__ __forceinline__ R ld_at(const R* base, uint32_t byte_off, uint32_t bytes) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+    if constexpr (sizeof(R) == 4)
+        return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
+    else
+        return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, 0, 0));
+}
+template <typename R>
+__device__ __forceinline__ void st_at(R* base, uint32_t byte_off, uint32_t bytes, R v) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+    if constexpr (sizeof(R) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)byte_off, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                              r, (int)byte_off, 0, 0);
+}
+
+template <typename R, int TJ>
+__global__ void __launch_bounds__(kSlabMaxNz) k_slab(Fdtd<R> f, const R* __restrict__ A, R* __restrict__ B,
+                                                     const R* __restrict__ sig, const R* __restrict__ epsv,
+                                                     PointSrc ps, const double* __restrict__ amp_dev, int t,
+                                                     int tiles_y, int chunk, int map_bands) {
+    __shared__ R sE[2][TJ + 1][kSlabMaxNz];   // Ex of rows 0..TJ-1 at [0][r], Ey of rows -1..TJ-1 at [1][r+1]
+    __shared__ R sH[2][TJ][kSlabMaxNz];       // new Hx, Hy of rows 0..TJ-1
+    // Workgroup b runs on XCD b % 8; consecutive bands of a chunk go to one
+    // XCD in consecutive dispatch order, so halo rows are re-read from the
+    // same L2.  map_bands: when the bands split evenly over the 8 XCDs, XCD x
+    // takes bands [x T/8, (x+1) T/8) of every chunk (T = bands), which
+    // spreads the x-PML chunks over all XCDs; otherwise XCD x takes a range
+    // of whole chunks.
+    const int b = blockIdx.x;
+    int ty, ch;
+    if (!map_bands) {   // XCD x takes a contiguous range of whole chunks
+        const int nb = gridDim.x;
+        const int l = ((nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+        ty = l % tiles_y;
+        ch = l / tiles_y;
+    } else if ((tiles_y & 7) == 0) {
+        const int per = tiles_y >> 3, j = b >> 3;
+        ty = (b & 7) * per + j % per;
+        ch = j / per;
+    } else {
+        ty = b % tiles_y;
+        ch = b / tiles_y;
+    }
+    const int i0 = ch * chunk;
+    const int i1 = min(f.nx, i0 + chunk);
+    if (i0 >= i1) return;   // uniform over the workgroup
+    const int64_t n = f.n;
+    const int nx = f.nx, ny = f.ny, nz = f.nz;
+    const int64_t plane = (int64_t)ny * nz;
+    const uint32_t sbytes = (uint32_t)(plane * (int64_t)sizeof(R));   // bytes of one slice
+    // the point source as (slice, offset in the slice); slice -1 = none
+    int src_x = -1;
+    uint32_t src_off = 0;
+    R src_amp = R(0);
+    if (ps.comp >= 0) {
+        src_x = (int)(ps.cell / plane);
+        src_off = (uint32_t)(ps.cell - (int64_t)src_x * plane) * (uint32_t)sizeof(R);
+        src_amp = R(amp_dev[t]);
+    }
+    // sigma with workgroup-uniform indices through scalar loads (constant
+    // address space), which a vector-memory wait does not wait for
+    using csig_t = const __attribute__((address_space(4))) R*;
+    const csig_t csig = (csig_t)sig;
+    const int k = threadIdx.x;
+    const int kp = (k + 1 == nz) ? 0 : k + 1;
+    const int km = (k == 0) ? nz - 1 : k - 1;
+    // byte offsets in a slice of rows -2..TJ of the band (wrapped) at index r + 2
+    uint32_t off[TJ + 3], offm[TJ + 3];
+#pragma unroll
+    for (int q = 0; q < TJ + 3; ++q) {
+        const uint32_t row = (uint32_t)pmod(ty * TJ + q - 2, ny) * (uint32_t)nz;
+        off[q] = (row + (uint32_t)k) * (uint32_t)sizeof(R);
+        offm[q] = (row + (uint32_t)km) * (uint32_t)sizeof(R);
+    }
+    R sHyj[TJ + 1], sDyj[TJ];   // sigma_y of rows -1..TJ-1 (H) and 0..TJ-1 (D)
+#pragma unroll
+    for (int q = 0; q < TJ + 1; ++q) sHyj[q] = csig[nx + pmod(ty * TJ + q - 1, ny)];
+#pragma unroll
+    for (int q = 0; q < TJ; ++q) sDyj[q] = csig[nx + ny + nz + nx + ty * TJ + q];
+    const R sHzk = sig[nx + ny + k], sDzk = sig[2 * (nx + ny) + nz + k];
+
+    // D of rows -1..TJ (Dy of row TJ is not needed), eps of rows -2..TJ, eps at
+    // z-1 of rows -1..TJ, all of slice xi
+    struct Raw {
+        R d[3][TJ + 2];
+        R e[TJ + 3];
+        R ek[TJ + 2];
+    };
+    auto load_raw = [&](int xi, Raw& rw) {
+        const R* ep = epsv + (int64_t)xi * plane;
+        const R* dp = A + 3 * n + (int64_t)xi * plane;
+#pragma unroll
+        for (int q = 0; q < TJ + 3; ++q) rw.e[q] = ld_at(ep, off[q], sbytes);
+#pragma unroll
+        for (int q = 0; q < TJ + 2; ++q) {
+            rw.ek[q] = ld_at(ep, offm[q + 1], sbytes);
+            rw.d[0][q] = ld_at(dp, off[q + 1], sbytes);
+            if (q < TJ + 1) rw.d[1][q] = ld_at(dp + n, off[q + 1], sbytes);
+            rw.d[2][q] = ld_at(dp + 2 * n, off[q + 1], sbytes);
+        }
+    };
+    // eps of the previous slice, rows -1..TJ, for eps_xx
+    auto to_E = [&](const Raw& rw, const R (&ep)[TJ + 2], R (&e)[3][TJ + 2]) {
+#pragma unroll
+        for (int q = 0; q < TJ + 2; ++q) {
+            const R e0 = rw.e[q + 1];
+            const R exx = (e0 + ep[q]) / R(2);
+            const R eyy = (e0 + rw.e[q]) / R(2);
+            const R ezz = (e0 + rw.ek[q]) / R(2);
+            e[0][q] = (R(1) / exx) * rw.d[0][q];
+            e[1][q] = (q < TJ + 1) ? (R(1) / eyy) * rw.d[1][q] : R(0);
+            e[2][q] = (R(1) / ezz) * rw.d[2][q];
+        }
+    };
+    // old H of rows -1..TJ-1 (Hy of row -1 is not needed)
+    auto load_H = [&](int xi, R (&h)[3][TJ + 1]) {
+        const R* hp = A + (int64_t)xi * plane;
+#pragma unroll
+        for (int q = 0; q < TJ + 1; ++q) {
+            h[0][q] = ld_at(hp, off[q + 1], sbytes);
+            h[1][q] = (q > 0) ? ld_at(hp + n, off[q + 1], sbytes) : R(0);
+            h[2][q] = ld_at(hp + 2 * n, off[q + 1], sbytes);
+        }
+    };
+    auto put_E = [&](const R (&e)[3][TJ + 2]) {
+#pragma unroll
+        for (int q = 0; q < TJ + 1; ++q) {
+            if (q < TJ) sE[0][q][k] = e[0][q + 1];
+            sE[1][q][k] = e[1][q];
+        }
+    };
+    // PML integrals the general H update of slice xi reads (ICE where the
+    // component's own sigma is non-zero, IH where both others are), all
+    // issued together; an integral that is not needed gets an offset past
+    // the slice (reads 0, no memory access).
+    auto h_aux_load = [&](int xi, R (&ice)[3][TJ + 1], R (&ih)[3][TJ + 1]) {
+        const R* ap = A + (int64_t)xi * plane;
+        const R sHx = csig[xi];
+#pragma unroll
+        for (int q = 0; q < TJ + 1; ++q) {
+            const R sg[3] = {sHx, sHyj[q], sHzk};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const R sa = sg[a], sb = sg[(a == 0) ? 1 : 0], sc = sg[(a == 2) ? 1 : 2];
+                ice[a][q] = ld_at(ap + (6 + a) * n, sa != R(0) ? off[q + 1] : sbytes, sbytes);
+                ih[a][q] = ld_at(ap + (9 + a) * n, (sb != R(0) && sc != R(0)) ? off[q + 1] : sbytes, sbytes);
+            }
+        }
+    };
+    // new H of slice xi, rows -1..TJ-1 (index r + 1), from E(xi) = eC (z+1
+    // values in sE), E(xi+1) = eN, the old H and the integrals of
+    // h_aux_load; `store` writes the band's rows and their PML integrals to
+    // B.  Components: row -1 Hx, Hz only, or (hyz_only) Hy, Hz of rows
+    // 0..TJ-1.
+    auto h_new = [&](int xi, const R (&eC)[3][TJ + 2], const R (&eN)[3][TJ + 2], const R (&hO)[3][TJ + 1],
+                     const R (&ice0)[3][TJ + 1], const R (&ih0)[3][TJ + 1], R (&h)[3][TJ + 1], bool store,
+                     bool hyz_only) {
+        R* bp = B + (int64_t)xi * plane;
+        const R sHx = csig[xi];
+#pragma unroll
+        for (int q = 0; q < TJ + 1; ++q) {
+            const int r = q - 1;
+            const R ex = eC[0][q], ey = eC[1][q], ez = eC[2][q];
+            R ce[3];
+            // curl_E, ceviche/derivatives.py:16-22 (forward differences, periodic)
+            ce[0] = ((eC[2][q + 1] - ez) - (sE[1][q][kp] - ey)) * f.inv_dL;
+            ce[1] = (r >= 0) ? ((sE[0][r < 0 ? 0 : r][kp] - ex) - (eN[2][q] - ez)) * f.inv_dL : R(0);
+            ce[2] = ((eN[1][q] - ey) - (eC[0][q + 1] - ex)) * f.inv_dL;
+            const uint32_t o = off[q + 1];
+            const R sg[3] = {sHx, sHyj[q], sHzk};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if ((r < 0 && a == 1) || (hyz_only && (r < 0 || a == 0))) {
+                    h[a][q] = R(0);
+                    continue;
+                }
+                const int ax_b = (a == 0) ? 1 : 0;
+                const int ax_c = (a == 2) ? 1 : 2;
+                const R sa = sg[a], sb = sg[ax_b], sc = sg[ax_c];
+                const int64_t ca = a * n;
+                const R hv = hO[a][q];
+                R hn;
+                if (sa == R(0) && sb == R(0) && sc == R(0)) {
+                    hn = f.h1 * hv + f.h2 * ce[a];
+                } else {
+                    R m1, m2, m3, m4;
+                    coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(-1), m1, m2, m3, m4);
+                    hn = m1 * hv + m2 * ce[a];
+                    if (sa != R(0)) {
+                        const R ice = ice0[a][q] + ce[a];
+                        if (store && r >= 0) st_at(bp + 6 * n + ca, o, sbytes, ice);
+                        hn += m3 * ice;
+                    }
+                    if (sb != R(0) && sc != R(0)) {
+                        const R ih = ih0[a][q] + hv;
+                        if (store && r >= 0) st_at(bp + 9 * n + ca, o, sbytes, ih);
+                        hn += m4 * ih;
+                    }
+                }
+                h[a][q] = hn;
+                if (store && r >= 0) st_at(bp + ca, o, sbytes, hn);
+            }
+        }
+    };
+
+    // Fast path: a slice outside the x PML in a band whose rows are outside
+    // the y PML.  There the only live PML integrals are those of the z axis
+    // (ICE of Hz, ICH of Dz, on z-PML lanes), their values are prefetched a
+    // slice ahead with the other loads (lanes outside the z PML pass an
+    // offset past the slice: no memory access), and the coefficients depend
+    // on the lane only: coef4(0, 0, s_z) for the x and y components and
+    // coef4(s_z, 0, 0) for z, the same calls with the same arguments as the
+    // general update, which the remaining slices use.
+    bool band_ypml = false;
+#pragma unroll
+    for (int q = 0; q < TJ + 1; ++q) band_ypml |= sHyj[q] != R(0);
+#pragma unroll
+    for (int q = 0; q < TJ; ++q) band_ypml |= sDyj[q] != R(0);
+    R cHxy1 = f.h1, cHxy2 = f.h2, cHz1 = f.h1, cHz2 = f.h2, cHz3 = R(0);
+    R cDxy1 = f.d1, cDxy2 = f.d2, cDz1 = f.d1, cDz2 = f.d2, cDz3 = R(0);
+    if (sHzk != R(0)) {
+        R m4;
+        coef4<R>(R(0), R(0), sHzk, R(f.dt), R(f.c0), R(f.eps0), R(-1), cHxy1, cHxy2, cHz3, m4);
+        coef4<R>(sHzk, R(0), R(0), R(f.dt), R(f.c0), R(f.eps0), R(-1), cHz1, cHz2, cHz3, m4);
+    }
+    if (sDzk != R(0)) {
+        R m4;
+        coef4<R>(R(0), R(0), sDzk, R(f.dt), R(f.c0), R(f.eps0), R(1), cDxy1, cDxy2, cDz3, m4);
+        coef4<R>(sDzk, R(0), R(0), R(f.dt), R(f.c0), R(f.eps0), R(1), cDz1, cDz2, cDz3, m4);
+    }
+    // ICE of Hz (rows -1..TJ-1) and ICH of Dz (rows 0..TJ-1) of slice xi
+    auto load_zaux = [&](int xi, R (&ice)[TJ + 1], R (&ich)[TJ]) {
+        const R* ap = A + (int64_t)xi * plane;
+#pragma unroll
+        for (int q = 0; q < TJ + 1; ++q) ice[q] = ld_at(ap + 8 * n, sHzk != R(0) ? off[q + 1] : sbytes, sbytes);
+#pragma unroll
+        for (int q = 0; q < TJ; ++q) ich[q] = ld_at(ap + 14 * n, sDzk != R(0) ? off[q + 2] : sbytes, sbytes);
+    };
+    // new H of slice xi on the fast path (same rows and components as h_new)
+    auto h_fast = [&](int xi, const R (&eC)[3][TJ + 2], const R (&eN)[3][TJ + 2], const R (&hO)[3][TJ + 1],
+                      const R (&ice)[TJ + 1], R (&h)[3][TJ + 1]) {
+        R* bp = B + (int64_t)xi * plane;
+#pragma unroll
+        for (int q = 0; q < TJ + 1; ++q) {
+            const int r = q - 1;
+            const R ex = eC[0][q], ey = eC[1][q], ez = eC[2][q];
+            const uint32_t o = off[q + 1];
+            const R ce0 = ((eC[2][q + 1] - ez) - (sE[1][q][kp] - ey)) * f.inv_dL;
+            const R ce2 = ((eN[1][q] - ey) - (eC[0][q + 1] - ex)) * f.inv_dL;
+            h[0][q] = cHxy1 * hO[0][q] + cHxy2 * ce0;
+            if (r >= 0) {
+                const R ce1 = ((sE[0][r < 0 ? 0 : r][kp] - ex) - (eN[2][q] - ez)) * f.inv_dL;
+                h[1][q] = cHxy1 * hO[1][q] + cHxy2 * ce1;
+            } else {
+                h[1][q] = R(0);
+            }
+            R hz = cHz1 * hO[2][q] + cHz2 * ce2;
+            if (sHzk != R(0)) {
+                const R ice_n = ice[q] + ce2;
+                if (r >= 0) st_at(bp + 8 * n, o, sbytes, ice_n);
+                hz += cHz3 * ice_n;
+            }
+            h[2][q] = hz;
+            if (r >= 0) {
+                st_at(bp, o, sbytes, h[0][q]);
+                st_at(bp + n, o, sbytes, h[1][q]);
+                st_at(bp + 2 * n, o, sbytes, hz);
+            }
+        }
+    };
+    R zIce[TJ + 1], zIch[TJ];        // z-PML integrals of slice xi (fast path)
+
+    R eP[TJ + 2];                    // eps of the slice before the one last loaded, rows -1..TJ
+    R eC[3][TJ + 2], eN[3][TJ + 2];  // E(xi), E(xi+1)
+    R dC[3][TJ], dN[3][TJ];          // old D(xi), D(xi+1) of the band
+    R hC[3][TJ + 1];                 // new H(xi)
+    R hPy[TJ], hPz[TJ];              // new Hy, Hz of H(xi-1), rows 0..TJ-1
+    R hO[3][TJ + 1];                 // old H(xi)
+    Raw rw;
+    const int xm1 = wrapm(i0, nx), xm2 = wrapm(xm1, nx);
+    {
+        const R* ep = epsv + (int64_t)xm2 * plane;
+#pragma unroll
+        for (int q = 0; q < TJ + 2; ++q) eP[q] = ld_at(ep, off[q + 1], sbytes);
+    }
+    // H(i0-1) from E(i0-1), E(i0) and the old H(i0-1)
+    load_raw(xm1, rw);
+    to_E(rw, eP, eC);
+#pragma unroll
+    for (int q = 0; q < TJ + 2; ++q) eP[q] = rw.e[q + 1];
+    load_raw(i0, rw);
+    to_E(rw, eP, eN);
+#pragma unroll
+    for (int q = 0; q < TJ + 2; ++q) eP[q] = rw.e[q + 1];
+#pragma unroll
+    for (int q = 0; q < TJ; ++q)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) dC[a][q] = rw.d[a][q + 1];
+    load_H(xm1, hO);
+    put_E(eC);
+    {
+        R ice[3][TJ + 1], ih[3][TJ + 1];
+        h_aux_load(xm1, ice, ih);
+        __syncthreads();
+        h_new(xm1, eC, eN, hO, ice, ih, hC, false, true);
+    }
+#pragma unroll
+    for (int q = 0; q < TJ; ++q) {
+        hPy[q] = hC[1][q + 1];
+        hPz[q] = hC[2][q + 1];
+    }
+    // E(i0), E(i0+1), old H(i0)
+    load_raw(wrapp(i0, nx), rw);
+    load_H(i0, hO);
+#pragma unroll
+    for (int q = 0; q < TJ + 2; ++q)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) eC[a][q] = eN[a][q];
+    to_E(rw, eP, eN);
+#pragma unroll
+    for (int q = 0; q < TJ + 2; ++q) eP[q] = rw.e[q + 1];
+#pragma unroll
+    for (int q = 0; q < TJ; ++q)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) dN[a][q] = rw.d[a][q + 1];
+    load_zaux(i0, zIce, zIch);
+    __syncthreads();   // every read of sE holding E(i0-1) is done
+    put_E(eC);
+    __syncthreads();
+
+    for (int xi = i0; xi < i1; ++xi) {
+        const bool more = xi + 1 < i1;   // uniform
+        const bool fast = !band_ypml && csig[xi] == R(0) && csig[nx + ny + nz + xi] == R(0);   // uniform
+        R zIceN[TJ + 1], zIchN[TJ];
+        if (more) load_zaux(wrapp(xi, nx), zIceN, zIchN);    // issued before the slice xi + 2 loads
+        R ice[3][TJ + 1], ih[3][TJ + 1];
+        if (!fast) h_aux_load(xi, ice, ih);                  // likewise
+        if (more) load_raw(wrapp(wrapp(xi, nx), nx), rw);   // slice xi + 2 in flight
+        if (fast) h_fast(xi, eC, eN, hO, zIce, hC);
+        else h_new(xi, eC, eN, hO, ice, ih, hC, true, false);
+#pragma unroll
+        for (int q = 0; q < TJ; ++q) {
+            sH[0][q][k] = hC[0][q + 1];
+            sH[1][q][k] = hC[1][q + 1];
+        }
+        if (more) load_H(wrapp(xi, nx), hO);   // old H(xi + 1) in flight
+        __syncthreads();
+        const int64_t p = (int64_t)xi * plane;
+        R* bp = B + p;
+        const R* ap = A + p;
+        const R sDx = csig[nx + ny + nz + xi];
+        R ich0[3][TJ], id0[3][TJ];   // PML integrals of the general D update, loaded together
+        if (!fast) {
+#pragma unroll
+            for (int q = 0; q < TJ; ++q) {
+                const R sg[3] = {sDx, sDyj[q], sDzk};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const R sa = sg[a], sb = sg[(a == 0) ? 1 : 0], sc = sg[(a == 2) ? 1 : 2];
+                    ich0[a][q] = ld_at(ap + (12 + a) * n, sa != R(0) ? off[q + 2] : sbytes, sbytes);
+                    id0[a][q] = ld_at(ap + (15 + a) * n, (sb != R(0) && sc != R(0)) ? off[q + 2] : sbytes, sbytes);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TJ; ++q) {
+            const int qh = q + 1;   // index of row q in hC
+            R ch[3];
+            // curl_H, ceviche/derivatives.py:24-30 (backward differences, periodic)
+            ch[0] = ((hC[2][qh] - hC[2][qh - 1]) - (hC[1][qh] - sH[1][q][km])) * f.inv_dL;
+            ch[1] = ((hC[0][qh] - sH[0][q][km]) - (hC[2][qh] - hPz[q])) * f.inv_dL;
+            ch[2] = ((hC[1][qh] - hPy[q]) - (hC[0][qh] - hC[0][qh - 1])) * f.inv_dL;
+            const uint32_t o = off[q + 2];
+            if (fast) {
+                R dn[3];
+                dn[0] = cDxy1 * dC[0][q] + cDxy2 * ch[0];
+                dn[1] = cDxy1 * dC[1][q] + cDxy2 * ch[1];
+                dn[2] = cDz1 * dC[2][q] + cDz2 * ch[2];
+                if (sDzk != R(0)) {
+                    const R ich = zIch[q] + ch[2];
+                    st_at(bp + 14 * n, o, sbytes, ich);
+                    dn[2] += cDz3 * ich;
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    if (ps.comp == a && xi == src_x && o == src_off) dn[a] += src_amp;
+                    st_at(bp + 3 * n + a * n, o, sbytes, dn[a]);
+                }
+                continue;
+            }
+            const R sg[3] = {sDx, sDyj[q], sDzk};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ax_b = (a == 0) ? 1 : 0;
+                const int ax_c = (a == 2) ? 1 : 2;
+                const R sa = sg[a], sb = sg[ax_b], sc = sg[ax_c];
+                const int64_t ca = a * n;
+                const R dv = dC[a][q];
+                R dn;
+                if (sa == R(0) && sb == R(0) && sc == R(0)) {
+                    dn = f.d1 * dv + f.d2 * ch[a];
+                } else {
+                    R m1, m2, m3, m4;
+                    coef4<R>(sa, sb, sc, R(f.dt), R(f.c0), R(f.eps0), R(1), m1, m2, m3, m4);
+                    dn = m1 * dv + m2 * ch[a];
+                    if (sa != R(0)) {
+                        const R ich = ich0[a][q] + ch[a];
+                        st_at(bp + 12 * n + ca, o, sbytes, ich);
+                        dn += m3 * ich;
+                    }
+                    if (sb != R(0) && sc != R(0)) {
+                        const R idv = id0[a][q] + dv;
+                        st_at(bp + 15 * n + ca, o, sbytes, idv);
+                        dn += m4 * idv;
+                    }
+                }
+                if (ps.comp == a && xi == src_x && o == src_off) dn += src_amp;
+                st_at(bp + 3 * n + ca, o, sbytes, dn);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < TJ; ++q) {
+                hPy[q] = hC[1][q + 1];
+                hPz[q] = hC[2][q + 1];
+            }
+#pragma unroll
+            for (int q = 0; q < TJ + 2; ++q)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) eC[a][q] = eN[a][q];
+#pragma unroll
+            for (int q = 0; q < TJ; ++q)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) dC[a][q] = dN[a][q];
+            to_E(rw, eP, eN);   // E(xi + 2)
+#pragma unroll
+            for (int q = 0; q < TJ + 2; ++q) eP[q] = rw.e[q + 1];
+#pragma unroll
+            for (int q = 0; q < TJ; ++q)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) dN[a][q] = rw.d[a][q + 1];
+#pragma unroll
+            for (int q = 0; q < TJ + 1; ++q) zIce[q] = zIceN[q];
+#pragma unroll
+            for (int q = 0; q < TJ; ++q) zIch[q] = zIchN[q];
+            put_E(eC);
+            __syncthreads();
+        }
+    }
+}
+
+// slab kernel applicability and launch shape: nz a multiple of 64 up to
+// kSlabMaxNz; bands of 2 rows when ny is even, else 1 (4-row bands hold
+// more registers: fewer waves, slower at 256^3, and float64 spills); chunks
+// of 8 slices (the fastest of 8, 16, 32, 64 measured at 256^3), or the
+// tuning override (g_slab_target then sets the chunk from a target
+// workgroup count)
+static int g_slab_rows = 0, g_slab_chunk = 0, g_slab_target = 0, g_slab_map = 0;
+
+static bool slab_shape(int nx, int ny, int nz, int elem, int& tj, int& chunk, int& nb) {
+    if (nz % 64 != 0 || nz > kSlabMaxNz) return false;
+    if ((int64_t)ny * nz * elem >= ((int64_t)1 << 32)) return false;   // byte offsets in a slice fit 32 bits
+    tj = (ny % 2 == 0) ? 2 : 1;
+    if (g_slab_rows > 0 && ny % g_slab_rows == 0) tj = g_slab_rows;
+    const int tiles_y = ny / tj;
+    int nch = (nx + 7) / 8;
+    if (g_slab_target > 0) nch = std::max(1, std::min((g_slab_target + tiles_y - 1) / tiles_y, nx / 8));
+    chunk = (nx + nch - 1) / nch;
+    if (g_slab_chunk > 0) chunk = std::min(nx, g_slab_chunk);
+    nch = (nx + chunk - 1) / chunk;
+    nb = tiles_y * nch;
+    return true;
+}
+
+template <typename R>
+static void launch_slab(const Fdtd<R>& f, const R* A, R* B, PointSrc ps, const double* amp_dev, int t, int tj,
+                        int chunk, int nb, hipStream_t s) {
+    const int tiles_y = f.ny / tj;
+    if (tj == 4) k_slab<R, 4><<<nb, f.nz, 0, s>>>(f, A, B, f.sig, f.eps, ps, amp_dev, t, tiles_y, chunk, g_slab_map);
+    else if (tj == 2) k_slab<R, 2><<<nb, f.nz, 0, s>>>(f, A, B, f.sig, f.eps, ps, amp_dev, t, tiles_y, chunk, g_slab_map);
+    else k_slab<R, 1><<<nb, f.nz, 0, s>>>(f, A, B, f.sig, f.eps, ps, amp_dev, t, tiles_y, chunk, g_slab_map);
+}
+
+}  // namespace cv
+
+// Launch-shape override of the slab-marching 3D step (band rows 1, 2 or 4,
+// slices per chunk, target workgroup count; 0 keeps the automatic choice;
+// map_bands 1 gives each XCD a range of bands of every chunk, 0 a range of
+// whole chunks), process-wide, for tuning sweeps.
+extern "C" int cv_fdtd_slab_tuning(int band_rows, int chunk, int target_workgroups, int map_bands) {
+    CV_CHECK(band_rows == 0 || band_rows == 1 || band_rows == 2 || band_rows == 4, "band rows must be 0, 1, 2 or 4");
+    CV_CHECK(chunk >= 0 && target_workgroups >= 0, "chunk and target must be >= 0");
+    cv::g_slab_rows = band_rows;
+    cv::g_slab_chunk = chunk;
+    cv::g_slab_target = target_workgroups;
+    cv::g_slab_map = map_bands ? 1 : 0;
+    return CV_OK;
+}
+
+namespace cv {
+
 // ------------------------------------------------------------ host glue
 template <typename R>
 static int make_fdtd(const cv_fdtd_desc* d, const void* sig, const void* eps, void* state, Fdtd<R>& f) {
@@ -1512,7 +2038,10 @@ extern "C" int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* s
         float* B = (float*)state_b;
         for (int t = 0; t < n_steps; ++t) {
             PointSrc ps{comp, src_cell, 0.0};
-            if (f.nz > 1) {
+            int tj, chunk_s, nb_s;
+            if (f.nz > 1 && slab_shape(f.nx, f.ny, f.nz, 4, tj, chunk_s, nb_s)) {
+                launch_slab<float>(f, A, B, ps, amp_dev, t0 + t, tj, chunk_s, nb_s, s);
+            } else if (f.nz > 1) {
                 const int ty = (f.ny + kMOwnY - 1) / kMOwnY, tz = (f.nz + kMOwnZ - 1) / kMOwnZ;
                 const int nch = std::max(1, std::min(f.nx / 4 > 0 ? f.nx / 4 : 1, (2048 + ty * tz - 1) / (ty * tz)));
                 const int chunk = (f.nx + nch - 1) / nch;
@@ -1540,7 +2069,10 @@ extern "C" int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* s
         double* B = (double*)state_b;
         for (int t = 0; t < n_steps; ++t) {
             PointSrc ps{comp, src_cell, 0.0};
-            if (f.nz > 1) {
+            int tj, chunk_s, nb_s;
+            if (f.nz > 1 && slab_shape(f.nx, f.ny, f.nz, 8, tj, chunk_s, nb_s)) {
+                launch_slab<double>(f, A, B, ps, amp_dev, t0 + t, tj, chunk_s, nb_s, s);
+            } else if (f.nz > 1) {
                 const int ty = (f.ny + kMOwnY - 1) / kMOwnY, tz = (f.nz + kMOwnZ - 1) / kMOwnZ;
                 const int nch = std::max(1, std::min(f.nx / 4 > 0 ? f.nx / 4 : 1, (2048 + ty * tz - 1) / (ty * tz)));
                 const int chunk = (f.nx + nch - 1) / nch;

@@ -172,9 +172,11 @@ class fdtd:
         kernel: 'auto' uses the temporally blocked 2D TM kernel when it applies
         (nz == 1, Jz source, TE fields zero, Ez/Hx/Hy probes, <= 256 probes;
         tile shape from the autotuner) and the two-phase per-step kernels
-        otherwise (measured faster than the single-pass kernel at 256^3);
+        otherwise (measured faster than the single-pass kernels at 256^3:
+        375 us per step against 420 us for the slab-marching kernel);
         'generic' forces the two-phase kernels, 'fused' the single-pass
-        kernel, an int a TM tile configuration."""
+        kernel (slab-marching when nz is a multiple of 64 up to 256,
+        x-marching tiles otherwise in 3D), an int a TM tile configuration."""
         g = self._grid
         g.desc.dL = self.dL
         comp = {'Jx': 0, 'Jy': 1, 'Jz': 2}[source_component]

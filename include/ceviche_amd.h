@@ -367,11 +367,23 @@ int cv_fdtd_run_tm(int dtype, const cv_fdtd_desc* d, const void* sig, const void
  * comp = -1 for none): each step reads state_a and writes state_b (18-array
  * layout), then the roles swap; after an odd n_steps the result is in state_b.
  * Probes as cv_fdtd_step, rec row t - rec_t0.  Same arithmetic as
- * cv_fdtd_step; one kernel per step instead of two. */
+ * cv_fdtd_step; one kernel per step instead of two.  3D grids with nz a
+ * multiple of 64 up to 256 use the slab-marching kernel (whole z lines per
+ * workgroup, bands of rows marched along x), other 3D grids the x-marching
+ * tile kernel.  Replaces the per-step loop of ceviche/fdtd.py:74-144 (the
+ * forward() calls of a run). */
 int cv_fdtd_run_fused(int dtype, const cv_fdtd_desc* d, const void* sig, const void* eps, void* state_a,
                       void* state_b, int comp, int64_t src_cell, const double* amp_dev, int t0,
                       int n_steps, const int32_t* probe_comp, const int64_t* probe_cell, int n_probes,
                       void* rec, int rec_t0, void* stream);
+
+/* Launch-shape override of the slab-marching kernel of cv_fdtd_run_fused,
+ * process-wide, for tuning sweeps: band_rows 1, 2 or 4 (0 = automatic: 2
+ * when ny is even, else 1), slices per chunk (0 = 8), target workgroup
+ * count that sets the chunk instead (0 = unused), map_bands 1 gives each
+ * XCD a range of bands of every chunk, 0 a range of whole chunks.  No
+ * counterpart in the reference (a tuning knob of this kernel). */
+int cv_fdtd_slab_tuning(int band_rows, int chunk, int target_workgroups, int map_bands);
 
 /* Tile configurations of cv_fdtd_run_tm: fills rpt[i], ts[i], waves[i]
  * (rows per thread, steps per launch, waves per workgroup; the region is

@@ -325,11 +325,19 @@ def test_tm_blocked_gradient_matches_generic():
 
 @pytest.mark.parametrize('shape,npml,comp', [((12, 10, 9), [3, 2, 2], 'Jz'), ((15, 13, 1), [4, 3, 0], 'Jx'),
                                              ((7, 9, 11), [0, 2, 3], 'Jy'), ((9, 20, 70), [2, 4, 5], 'Jz'),
-                                             ((5, 3, 2), [1, 1, 0], 'Jx'), ((12, 33, 130), [3, 3, 3], 'Jy')])
+                                             ((5, 3, 2), [1, 1, 0], 'Jx'), ((12, 33, 130), [3, 3, 3], 'Jy'),
+                                             # slab kernel: nz a multiple of 64 up to 256, bands of 4, 2, 1 rows
+                                             ((11, 8, 64), [3, 2, 4], 'Jz'), ((9, 6, 128), [2, 2, 5], 'Jx'),
+                                             ((10, 7, 64), [0, 3, 3], 'Jy'), ((5, 12, 192), [1, 2, 6], 'Jz'),
+                                             ((3, 2, 64), [1, 0, 2], 'Jx'), ((1, 4, 64), [0, 1, 3], 'Jy'),
+                                             ((20, 3, 256), [4, 1, 8], 'Jz'),
+                                             # interior bands and slices take the z-PML-only path
+                                             ((12, 16, 64), [3, 3, 4], 'Jz'), ((14, 12, 128), [2, 2, 0], 'Jy')])
 def test_fused_single_pass_kernel_matches_oracle(shape, npml, comp):
-    """The single-pass kernels (x-marching tiles in 3D, per-cell otherwise),
-    including (y, z) extents that are not multiples of the 16 x 64 tile and
-    grids smaller than one tile."""
+    """The single-pass kernels (slab marching when nz is a multiple of 64 up
+    to 256, x-marching tiles otherwise in 3D, per-cell in 2D), including
+    (y, z) extents that are not multiples of the tiles, grids smaller than one
+    tile and x extents of one slice."""
     print('fused', shape, npml, comp, flush=True)
     import ceviche_amd as cv
     rng = np.random.default_rng(13)
@@ -337,8 +345,8 @@ def test_fused_single_pass_kernel_matches_oracle(shape, npml, comp):
     steps, dL = 37, 30e-9
     amp = np.exp(-(np.arange(steps) - 12) ** 2 / (2 * 4 ** 2))
     src = tuple(s // 2 for s in shape)
-    probes = [('Ex', (1, min(2, shape[1] - 1), 0)), ('Hy', (shape[0] - 1, 1, shape[2] - 1)), ('Dz', src),
-              ('Hz', (2, min(3, shape[1] - 1), shape[2] // 2))]
+    probes = [('Ex', (min(1, shape[0] - 1), min(2, shape[1] - 1), 0)), ('Hy', (shape[0] - 1, 1, shape[2] - 1)), ('Dz', src),
+              ('Hz', (min(2, shape[0] - 1), min(3, shape[1] - 1), shape[2] // 2))]
     F = cv.fdtd(torch.tensor(eps0), dL, npml)
     rec = F.run(steps, src, amp, probes, source_component=comp, kernel='fused').cpu().numpy()
     fields = {k: v.detach().cpu().numpy() for k, v in F.fields.items()}
@@ -352,6 +360,31 @@ def test_fused_single_pass_kernel_matches_oracle(shape, npml, comp):
     assert _rel(rec, np.array(tr)) <= 1e-11
     for k in FIELDS:
         assert _rel(fields[k], r[k]) <= 1e-11, k
+
+
+def test_fused_and_two_phase_runs_agree_at_64_cubed():
+    """float32 64^3 with PML on every face, 120 steps: the slab-marching
+    single-pass run and the two-phase step kernels give the same probe
+    records and final fields (same arithmetic term by term; the float32 bound
+    allows for multiply-adds contracted differently, which over 120 steps
+    reach 1e-5 relative on the weak Hz of a Jz source; an index or halo
+    mistake gives O(1) differences)."""
+    print('fused vs two-phase 64^3 float32', flush=True)
+    import ceviche_amd as cv
+    rng = np.random.default_rng(21)
+    shape = (64, 64, 64)
+    eps0 = rng.uniform(1, 6, shape).astype(np.float32)
+    steps = 120
+    amp = np.exp(-(np.arange(steps) - 40) ** 2 / (2 * 10 ** 2))
+    probes = [('Ex', (30, 31, 32)), ('Hy', (5, 60, 2)), ('Ez', (63, 0, 63)), ('Hz', (32, 32, 10))]
+    out = []
+    for kernel in ('fused', 'generic'):
+        F = cv.fdtd(torch.tensor(eps0), 25e-9, [8, 8, 8], precision='float32')
+        rec = F.run(steps, (32, 32, 20), amp, probes, kernel=kernel).cpu().numpy().astype(np.float64)
+        out.append((rec, {k: v.detach().cpu().numpy().astype(np.float64) for k, v in F.fields.items()}))
+    assert _rel(out[0][0], out[1][0]) <= 1e-4
+    for k in FIELDS:
+        assert _rel(out[0][1][k], out[1][1][k]) <= 1e-4, k
 
 
 def test_forward_mode_run_matches_complex_step():
