@@ -825,12 +825,45 @@ def extras_all(main, which='all'):
         gbs = 2 * 4 * n * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
         return {"copy_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
 
+    def unet_conv():
+        """One 3x3 bf16 convolution layer of the U-Net preconditioner (32 -> 32
+        channels, NHWC, ReLU) at its finest level for a training batch of 16 x
+        256 x 256: the forward, backward-data and backward-weight kernels
+        (cv_conv3x3_*_bf16, MFMA 16x16x32 bf16), HIP events on the launch
+        stream, against the 2.5 PFLOP/s dense bf16 matrix-core peak."""
+        from ceviche_amd import learned
+        net = learned.UNet(levels=6)
+        B, H, W, C = 16, 256, 256, learned.CH
+        gen = torch.Generator(device='cuda').manual_seed(0)
+        x = torch.randn(B, H, W, C, device='cuda', generator=gen).to(torch.bfloat16)
+        gy = torch.randn(B, H, W, C, device='cuda', generator=gen).to(torch.bfloat16)
+        dW = torch.zeros_like(net.W)
+        db = torch.zeros_like(net.b)
+        y = net._conv(1, x, True)
+        flops = 2.0 * B * H * W * 9 * C * C
+        res = {"layer": "3x3, %d -> %d channels, batch %d x %d x %d, bf16 NHWC, fp32 accumulate" % (C, C, B, H, W),
+               "gflop_per_call": round(flops / 1e9, 2)}
+        for name, fn in (('forward', lambda: net._conv(1, x, True)), ('backward_data', lambda: net._dgrad(1, gy, y)),
+                         ('backward_weight', lambda: net._wgrad(1, x, gy, y, dW, db))):
+            fn()
+            _sync()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(10):
+                fn()
+            e1.record()
+            e1.synchronize()
+            us = e0.elapsed_time(e1) / 10 * 1e3
+            tf = flops / (us * 1e-6) / 1e12
+            res[name] = {"us": round(us, 1), "tflops": round(tf, 1), "frac_of_bf16_peak": round(tf / 2500.0, 4)}
+        return res
+
     table = [('fdtd_tm_400', tm_400), ('fdtd_tm_400_adjoint', adjoint),
              ('fdtd_tm_400_weighted_probes', weighted_probes), ('fdtd_tm_8192', tm_hbm),
              ('fdtd_3d_256_adjoint', adjoint_3d),
              ('fdtd_3d_256', fdtd_3d), ('fdfd_ez_120', fdfd_cfg1), ('fdfd_ez_1024_waveguide', fdfd_1024),
              ('fdfd_mf_isolator', fdfd_mf), ('fdfd_ez_batch', fdfd_batch), ('fdfd_learned', fdfd_learned),
-             ('sp_mult_1M', sp_mult_1m),
+             ('sp_mult_1M', sp_mult_1m), ('unet_conv', unet_conv),
              ('stream_copy', copy_probe)]
     for name, fn in table:
         if which == 'all' or name in which:

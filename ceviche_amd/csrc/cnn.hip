@@ -272,6 +272,22 @@ __global__ void __launch_bounds__(kBlock) k_conv3x3_wgrad(const uint16_t* __rest
     }
 }
 
+// First stage of the partial sum: segment blockIdx.y adds the partials of
+// workgroups [kWgSeg y, kWgSeg (y+1)) in workgroup order, one thread per
+// element, into seg[y].  The second stage (k_wgrad_reduce over the segments)
+// adds the segments in order, so the total is summed in a fixed order.
+constexpr int kWgSeg = 32;
+
+__global__ void __launch_bounds__(kBlock) k_wgrad_segsum(const float* __restrict__ partial, int n_wg,
+                                                         float* __restrict__ seg) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= kWgPartial) return;
+    const int g0 = blockIdx.y * kWgSeg, g1 = min(n_wg, g0 + kWgSeg);
+    float s = 0.f;
+    for (int g = g0; g < g1; ++g) s += partial[(size_t)g * kWgPartial + e];
+    seg[(size_t)blockIdx.y * kWgPartial + e] = s;
+}
+
 // dw (+)= sum over workgroups of partial[:, :kWElems], db (+)= ... [kWElems:]
 __global__ void __launch_bounds__(kBlock) k_wgrad_reduce(const float* __restrict__ partial, int n_wg,
                                                          float* __restrict__ dw, float* __restrict__ db,
@@ -542,9 +558,12 @@ extern "C" int cv_conv3x3_bwd_data_bf16(const void* gy, const void* ymask, const
     return conv_launch(gy, ymask, wpack_flip, nullptr, gx, B, H, W, 0, accumulate, as_stream(stream));
 }
 
+// workspace: one partial per workgroup, then one per segment of kWgSeg partials
 extern "C" int64_t cv_conv3x3_bwd_weight_workspace_bytes(int B, int H, int W) {
     if (B <= 0 || H <= 0 || W <= 0) return 0;
-    return (int64_t)wgrad_blocks(B, H, W) * kWgPartial * (int64_t)sizeof(float);
+    const int64_t nwg = wgrad_blocks(B, H, W);
+    const int64_t nseg = (nwg + kWgSeg - 1) / kWgSeg;
+    return (nwg + nseg) * kWgPartial * (int64_t)sizeof(float);
 }
 
 extern "C" int cv_conv3x3_bwd_weight_bf16(const void* x, const void* gy, const void* ymask, float* dw, float* db,
@@ -560,7 +579,11 @@ extern "C" int cv_conv3x3_bwd_weight_bf16(const void* x, const void* gy, const v
     k_conv3x3_wgrad<<<nwg, kBlock, 0, s>>>((const uint16_t*)x, (const uint16_t*)gy, (const uint16_t*)ymask,
                                            (float*)workspace, B, H, W, (W + kTW - 1) / kTW, (H + kTH - 1) / kTH);
     CV_LAUNCH_CHECK();
-    k_wgrad_reduce<<<grid_of(kWgPartial), kBlock, 0, s>>>((const float*)workspace, nwg, dw, db, accumulate);
+    const int nseg = (nwg + kWgSeg - 1) / kWgSeg;
+    float* seg = (float*)workspace + (size_t)nwg * kWgPartial;
+    k_wgrad_segsum<<<dim3(grid_of(kWgPartial), nseg), kBlock, 0, s>>>((const float*)workspace, nwg, seg);
+    CV_LAUNCH_CHECK();
+    k_wgrad_reduce<<<grid_of(kWgPartial), kBlock, 0, s>>>(seg, nseg, dw, db, accumulate);
     CV_LAUNCH_CHECK();
     return CV_OK;
 }
