@@ -443,6 +443,126 @@ k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int3
     }
 }
 
+// ---- small fronts (k <= 64 interior cells, m <= 16 MB rows): the whole
+// elimination of the interior block in one kernel, one workgroup per node.
+// A swap-free Gauss-Jordan sweep over the k interior pivots (partial
+// pivoting among the interior rows not yet used: largest |a|^2, ties to the
+// lower row) turns F in place into the sweep [[Q, X], [-G, S]] with the
+// interior rows in pivot order: pivot p, taken in column p at row r_p,
+// leaves in row r_p what row p of a row-exchanging sweep holds, and that
+// sweep's Q and -G columns are those of the exchanged rows.  The store
+// undoes both: element (x, y) goes to row pi(x) (the pivot that took row x)
+// for x < k, to column r_y for y < k, negated in the G block.  The result is
+// what the LU + inverse + products path stores: Q = F_II^-1, X = Q F_IB,
+// G = F_BI Q, S = F_BB - F_BI X.  Thread t holds rows r0 + 16a and columns
+// c0 + 16c (r0 = t % 16, c0 = t / 16) in registers; per pivot the pivot
+// column goes through LDS (its owners write it, every wave finds the pivot
+// row by the same butterfly argmax), then the scaled pivot row, then every
+// thread updates its MB x MB entries: a - col_i row_j, -col_i / a_pp in the
+// pivot column, the scaled row in the pivot row.  Two barriers per pivot,
+// the LDS vectors double-buffered.  info: first zero pivot (1-based), the
+// rocSOLVER convention.
+constexpr int kGjMaxK = 64;
+constexpr int kGjMaxM = 112;   // fronts of up to 7 x 16 rows (registers: 49 complex per thread)
+
+template <int MB>
+__global__ void __launch_bounds__(256) k_nd_gj(cx<float>* __restrict__ F, int m, int k,
+                                               rocblas_int* __restrict__ info) {
+    __shared__ cx<float> colp[2][16 * MB];
+    __shared__ cx<float> rowp[2][16 * MB];
+    __shared__ int s_row_of[kGjMaxK];   // pivot p -> its row r_p
+    __shared__ int s_pi[kGjMaxK];       // row x -> the pivot that took it
+    const int t = threadIdx.x, r0 = t & 15, c0 = t >> 4, lane = t & 63;
+    cx<float>* Fe = F + (size_t)blockIdx.x * m * m;
+    cx<float> v[MB][MB];
+#pragma unroll
+    for (int a = 0; a < MB; ++a)
+#pragma unroll
+        for (int c = 0; c < MB; ++c) {
+            const int i = r0 + 16 * a, j = c0 + 16 * c;
+            v[a][c] = (i < m && j < m) ? Fe[(size_t)j * m + i] : czero<float>();
+        }
+    uint64_t used = 0;   // rows taken as pivots (uniform)
+    int bad = 0;         // first zero pivot, 1-based (uniform)
+#pragma unroll
+    for (int pc = 0; pc < kGjMaxK / 16; ++pc) {
+        if (pc * 16 >= k) break;
+        for (int pp = 0; pp < 16; ++pp) {
+            const int p = pc * 16 + pp;
+            if (p >= k) break;
+            const int buf = p & 1;
+            if (c0 == pp) {   // owners of column p
+#pragma unroll
+                for (int a = 0; a < MB; ++a) colp[buf][r0 + 16 * a] = v[a][pc];
+            }
+            __syncthreads();
+            // pivot row: the same butterfly argmax in every wave
+            float bv = -1.f;
+            int bi = lane;
+            if (lane < k && !((used >> lane) & 1ull)) bv = norm2(colp[buf][lane]);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const float ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            const int r = __builtin_amdgcn_readfirstlane(bi);
+            const cx<float> apiv = colp[buf][r];
+            const bool zero_piv = norm2(apiv) == 0.f;
+            const cx<float> inv = zero_piv ? czero<float>() : crecip(apiv);
+            if (zero_piv && bad == 0) bad = p + 1;
+            const int rhi = r >> 4, rlo = r & 15;
+            if (r0 == rlo) {   // owners of row r: the scaled pivot row
+#pragma unroll
+                for (int c = 0; c < MB; ++c) {
+                    cx<float> x = v[0][c];
+#pragma unroll
+                    for (int a = 1; a < MB; ++a)
+                        if (a == rhi) x = v[a][c];
+                    rowp[buf][c0 + 16 * c] = (c == pc && c0 == pp) ? inv : x * inv;
+                }
+            }
+            if (t == 0) s_row_of[p] = r;
+            used |= 1ull << r;
+            __syncthreads();
+            cx<float> cr[MB], rr[MB];
+#pragma unroll
+            for (int a = 0; a < MB; ++a) cr[a] = colp[buf][r0 + 16 * a];
+#pragma unroll
+            for (int c = 0; c < MB; ++c) rr[c] = rowp[buf][c0 + 16 * c];
+#pragma unroll
+            for (int a = 0; a < MB; ++a) {
+                const bool in_r = (a == rhi) && (r0 == rlo);
+#pragma unroll
+                for (int c = 0; c < MB; ++c) {
+                    cx<float> u = v[a][c] - cr[a] * rr[c];
+                    if (c == pc && c0 == pp) u = -(cr[a] * inv);
+                    if (in_r) u = rr[c];
+                    v[a][c] = u;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t < k) s_pi[s_row_of[t]] = t;
+    if (t == 0) info[blockIdx.x] = bad;
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < MB; ++a)
+#pragma unroll
+        for (int c = 0; c < MB; ++c) {
+            const int x = r0 + 16 * a, y = c0 + 16 * c;
+            if (x < m && y < m) {
+                const int orow = x < k ? s_pi[x] : x;
+                const int ocol = y < k ? s_row_of[y] : y;
+                Fe[(size_t)ocol * m + orow] = (x >= k && y < k) ? -v[a][c] : v[a][c];
+            }
+        }
+}
+
 // ---- blocked LU with partial pivoting of the large interior blocks
 // (kLuMin < k <= kLuMaxRows): per panel of kLuPanel columns one panel factorisation, one
 // interchange + triangular-solve launch and one rocBLAS GEMM, where
@@ -956,6 +1076,16 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
         const int k = L.kmax, b = L.bmax, m = L.m;
         rocblas_int* piv = (rocblas_int*)(fb + L.off_piv);
         rc_t* R = (rc_t*)(fb + P->off_r);
+        if (k <= kGjMaxK && m <= kGjMaxM) {   // the whole interior elimination in one launch
+            const int mb = (m + 15) / 16;
+            cx<float>* Fc = (cx<float>*)F;
+            if (mb <= 4) k_nd_gj<4><<<L.nb, 256, 0, s>>>(Fc, m, k, info + L.info_first);
+            else if (mb == 5) k_nd_gj<5><<<L.nb, 256, 0, s>>>(Fc, m, k, info + L.info_first);
+            else if (mb == 6) k_nd_gj<6><<<L.nb, 256, 0, s>>>(Fc, m, k, info + L.info_first);
+            else k_nd_gj<7><<<L.nb, 256, 0, s>>>(Fc, m, k, info + L.info_first);
+            CV_LAUNCH_CHECK();
+            continue;
+        }
         if (k > kLuMin && k <= kLuMaxRows) {
             const unsigned g_rows = (unsigned)(((k + 255) / 256) * (int64_t)L.nb);
             for (int j0 = 0; j0 < k; j0 += kLuPanel) {
