@@ -57,7 +57,7 @@
 namespace cv {
 namespace {
 
-constexpr int kLeafCells = 64;
+constexpr int kLeafCells = 32;
 constexpr int kMvTile = 64;        // rows and columns of one tile of the application products
 constexpr int kNdSolveMin = 128;   // interior blocks above this size: triangular solves instead of the inverse
 
