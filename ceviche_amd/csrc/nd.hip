@@ -563,6 +563,106 @@ __global__ void __launch_bounds__(256) k_nd_gj(cx<float>* __restrict__ F, int m,
         }
 }
 
+// ---- fronts with at most 64 interior cells and up to 448 rows: the same
+// swap-free sweep on the left m x k block only (the sweep's first k columns
+// evolve on their own), which gives Q (rows in pivot order) over F_II and
+// -G over F_BI; the store undoes the permutations and the sign as k_nd_gj's
+// does, and X = Q F_IB, S = F_BB - G F_IB follow by two GEMMs on the
+// untouched F_IB.  1024 threads: thread t holds rows r0 + 64a (r0 = t % 64)
+// and columns c0 + 16c (c0 = t / 64) of the block.
+constexpr int kGjPanelMaxM = 448;
+
+template <int MR>
+__global__ void __launch_bounds__(1024) k_nd_gj_panel(cx<float>* __restrict__ F, int m, int k,
+                                                      rocblas_int* __restrict__ info) {
+    constexpr int MC = kGjMaxK / 16;
+    __shared__ cx<float> colp[2][64 * MR];
+    __shared__ cx<float> rowp[2][kGjMaxK];
+    __shared__ int s_row_of[kGjMaxK];
+    __shared__ int s_pi[kGjMaxK];
+    const int t = threadIdx.x, r0 = t & 63, c0 = t >> 6, lane = t & 63;
+    cx<float>* Fe = F + (size_t)blockIdx.x * m * m;
+    cx<float> v[MR][MC];
+#pragma unroll
+    for (int a = 0; a < MR; ++a)
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            const int i = r0 + 64 * a, j = c0 + 16 * c;
+            v[a][c] = (i < m && j < k) ? Fe[(size_t)j * m + i] : czero<float>();
+        }
+    uint64_t used = 0;
+    int bad = 0;
+#pragma unroll
+    for (int pc = 0; pc < MC; ++pc) {
+        if (pc * 16 >= k) break;
+        for (int pp = 0; pp < 16; ++pp) {
+            const int p = pc * 16 + pp;
+            if (p >= k) break;
+            const int buf = p & 1;
+            if (c0 == pp) {   // owners of column p
+#pragma unroll
+                for (int a = 0; a < MR; ++a) colp[buf][r0 + 64 * a] = v[a][pc];
+            }
+            __syncthreads();
+            float bv = -1.f;
+            int bi = lane;
+            if (lane < k && !((used >> lane) & 1ull)) bv = norm2(colp[buf][lane]);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const float ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            const int r = __builtin_amdgcn_readfirstlane(bi);   // < k <= 64: row r sits at a = 0
+            const cx<float> apiv = colp[buf][r];
+            const bool zero_piv = norm2(apiv) == 0.f;
+            const cx<float> inv = zero_piv ? czero<float>() : crecip(apiv);
+            if (zero_piv && bad == 0) bad = p + 1;
+            if (r0 == r) {   // owners of row r (its entries are v[0][*])
+#pragma unroll
+                for (int c = 0; c < MC; ++c)
+                    rowp[buf][c0 + 16 * c] = (c == pc && c0 == pp) ? inv : v[0][c] * inv;
+            }
+            if (t == 0) s_row_of[p] = r;
+            used |= 1ull << r;
+            __syncthreads();
+            cx<float> cr[MR], rr[MC];
+#pragma unroll
+            for (int a = 0; a < MR; ++a) cr[a] = colp[buf][r0 + 64 * a];
+#pragma unroll
+            for (int c = 0; c < MC; ++c) rr[c] = rowp[buf][c0 + 16 * c];
+#pragma unroll
+            for (int a = 0; a < MR; ++a) {
+                const bool in_r = (a == 0) && (r0 == r);
+#pragma unroll
+                for (int c = 0; c < MC; ++c) {
+                    cx<float> u = v[a][c] - cr[a] * rr[c];
+                    if (c == pc && c0 == pp) u = -(cr[a] * inv);
+                    if (in_r) u = rr[c];
+                    v[a][c] = u;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t < k) s_pi[s_row_of[t]] = t;
+    if (t == 0) info[blockIdx.x] = bad;
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < MR; ++a)
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            const int x = r0 + 64 * a, y = c0 + 16 * c;
+            if (x < m && y < k) {
+                const int orow = x < k ? s_pi[x] : x;
+                Fe[(size_t)s_row_of[y] * m + orow] = (x >= k) ? -v[a][c] : v[a][c];
+            }
+        }
+}
+
 // ---- blocked LU with partial pivoting of the large interior blocks
 // (kLuMin < k <= kLuMaxRows): per panel of kLuPanel columns one panel factorisation, one
 // interchange + triangular-solve launch and one rocBLAS GEMM, where
@@ -1084,6 +1184,29 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             else if (mb == 6) k_nd_gj<6><<<L.nb, 256, 0, s>>>(Fc, m, k, info + L.info_first);
             else k_nd_gj<7><<<L.nb, 256, 0, s>>>(Fc, m, k, info + L.info_first);
             CV_LAUNCH_CHECK();
+            continue;
+        }
+        if (k <= kGjMaxK && m <= kGjPanelMaxM) {   // the left block by the sweep, X and S by two GEMMs
+            const int mr = (m + 63) / 64;
+            cx<float>* Fc = (cx<float>*)F;
+            if (mr <= 3) k_nd_gj_panel<3><<<L.nb, 1024, 0, s>>>(Fc, m, k, info + L.info_first);
+            else if (mr == 4) k_nd_gj_panel<4><<<L.nb, 1024, 0, s>>>(Fc, m, k, info + L.info_first);
+            else if (mr == 5) k_nd_gj_panel<5><<<L.nb, 1024, 0, s>>>(Fc, m, k, info + L.info_first);
+            else if (mr == 6) k_nd_gj_panel<6><<<L.nb, 1024, 0, s>>>(Fc, m, k, info + L.info_first);
+            else k_nd_gj_panel<7><<<L.nb, 1024, 0, s>>>(Fc, m, k, info + L.info_first);
+            CV_LAUNCH_CHECK();
+            if (b > 0) {   // X = Q F_IB (into R), S = F_BB - G F_IB, then X over F_IB
+                const rocblas_stride sR = (rocblas_stride)k * b;
+                CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, k, b, k, &one,
+                                                    F, m, sF, F + (int64_t)k * m, m, sF, &zero_c, R, k, sR, L.nb));
+                CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
+                                                    &minus_one, F + k, m, sF, F + (int64_t)k * m, m, sF, &one,
+                                                    F + (int64_t)k * m + k, m, sF, L.nb));
+                k_nd_block_copy<<<nblk((int64_t)L.nb * k * b), 256, 0, s>>>((const cx<float>*)R, sR, k,
+                                                                            (cx<float>*)F + (int64_t)k * m, sF, m,
+                                                                            L.nb, k, b);
+                CV_LAUNCH_CHECK();
+            }
             continue;
         }
         if (k > kLuMin && k <= kLuMaxRows) {
