@@ -7,9 +7,12 @@
  * workspace is passed in by the caller, sized by the *_workspace_bytes
  * helpers.  The iterative solvers do ask the HIP runtime for resources of
  * their own on each call: cv_fdfd_solve captures its loop body on a private
- * stream and instantiates a HIP graph (both released before it returns), and
- * the FGMRES entry points keep host-side vectors.  cv_fdfd_solve_noalloc is
- * the variant that allocates nothing.  No torch types cross this boundary.
+ * stream and instantiates a HIP graph (both released before it returns), the
+ * FGMRES entry points keep host-side vectors, and the nested-dissection
+ * factorisation calls rocSOLVER / rocBLAS through one rocBLAS handle per
+ * device, created on first use, whose device workspace the libraries manage.
+ * cv_fdfd_solve_noalloc is the variant that allocates nothing.  No torch
+ * types cross this boundary.
  *
  * Layouts
  *   FDFD fields: complex, C order over (nx, ny) (y contiguous), packed over a
