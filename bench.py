@@ -100,9 +100,8 @@ def fdfd_main(steps, warmup, N=1024):
             "frac": round(tflops / 157.3, 4), "traffic": None,
             "kernel": "cv_nd_factor (per tree level: frontal assembly, then the interior elimination: a "
                       "Gauss-Jordan sweep in one kernel for fronts of <= 64 interior cells (the whole front up to "
-                      "112 rows, the left block up to 448 rows plus the X and S products), batched LU + inverse "
-                      "for <= 128 cells, own panel LU + triangular solves of [I F_IB] above; the X, S and G "
-                      "products)",
+                      "112 rows, the left block up to 448 rows plus the X and S products), own panel LU + "
+                      "triangular solves of [I F_IB] above 64 cells; the X, S and G products)",
             "avg_factor_ms": round(fact_ms, 2), "algorithmic_flops_per_factor": flops,
             "apply_ms": round(apply_ms, 3),
             "note": "real flops of one nested-dissection factorisation at unpadded sizes (per node with k interior "

@@ -16,12 +16,13 @@
 // Schur complements (extend-add) and the couplings of A that involve its own
 // variables (each entry of A added once, at the first node that eliminates
 // one of its two cells),
-// then factorises F_II = P L U and forms Q = F_II^-1 and X = Q F_IB -- for
-// interior blocks of at most kNdSolveMin cells by inverting in place from the
-// LU factors and one product, above that by the triangular solves of
-// [I F_IB] (fewer, larger launches in rocSOLVER for these sizes) -- passes
-// S = F_BB - F_BI X up to its parent and forms G = F_BI Q.  F keeps Q, X and
-// G in place of its blocks:
+// then forms Q = F_II^-1 and X = Q F_IB, passes S = F_BB - F_BI X up to its
+// parent and forms G = F_BI Q.  Fronts of at most 64 interior cells do this
+// by a Gauss-Jordan sweep (k_nd_gj, k_nd_gj_panel); larger interior blocks
+// are factorised F_II = P L U and give [Q X] by the triangular solves of
+// [I F_IB] (the LU of at most kNdSolveMin cells, on fronts the sweep kernels
+// do not hold, by inverting in place from the LU factors and one product).
+// F keeps Q, X and G in place of its blocks:
 //   F = [[Q, X], [G, S]].
 // Nodes of equal height (distance to the leaves) are processed together as
 // strided batches padded to the level's largest |I| (kmax, identity padding)
@@ -59,7 +60,7 @@ namespace {
 
 constexpr int kLeafCells = 32;
 constexpr int kMvTile = 64;        // rows and columns of one tile of the application products
-constexpr int kNdSolveMin = 128;   // interior blocks above this size: triangular solves instead of the inverse
+constexpr int kNdSolveMin = 64;   // interior blocks above this size: triangular solves instead of the inverse
 
 struct NdGroup {               // (parent, child) pairs whose child sits at one level, one child slot
     int child_level = 0, slot = 0;
@@ -670,7 +671,7 @@ __global__ void __launch_bounds__(1024) k_nd_gj_panel(cx<float>* __restrict__ F,
 // result is the LAPACK form getrs takes: L (unit, strictly lower) and U in
 // place of F_II, ipiv 1-based sequential row interchanges.
 constexpr int kLuPanel = 16;
-constexpr int kLuMin = 128;
+constexpr int kLuMin = 64;
 constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
 
 // LU of the panel rows j0..k-1, columns j0..j0+nbw-1 of every node
