@@ -423,25 +423,23 @@ k_nd_assemble(const cv_fdfd_desc* __restrict__ desc, const cx<float>* __restrict
 
 // children's Schur complements (the bottom-right bc x bc block of their F)
 // added into the parents' F at the mapped positions (one child slot per
-// launch, so every parent entry has one writer); blockIdx.x = pair,
-// blockIdx.y = a slice of kNdCols child columns (one wave per column, rows
-// over lanes)
-constexpr int kNdCols = 4;
+// launch, so every parent entry has one writer); one thread per entry of a
+// child's block, rows fastest (the reads of a wave are contiguous)
 __global__ void __launch_bounds__(256)
-k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, const int32_t* __restrict__ parent,
+k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, int npairs, const int32_t* __restrict__ parent,
             const int32_t* __restrict__ child, const int32_t* __restrict__ map, cx<float>* __restrict__ F, int m) {
-    const int pr = blockIdx.x;
-    const int jj = blockIdx.y * kNdCols + (threadIdx.x >> 6);
-    if (jj >= bc) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)bc * bc;
+    if (t >= (int64_t)npairs * per) return;
+    const int pr = (int)(t / per);
+    const int r = (int)(t - (int64_t)pr * per);
+    const int jj = r / bc, ii = r - jj * bc;
     const int32_t* mp = map + (int64_t)pr * bc;
-    const int pj = mp[jj];
-    if (pj < 0) return;
-    const cx<float>* Scol = Fc + (int64_t)child[pr] * mc * mc + (int64_t)(kc + jj) * mc + kc;
-    cx<float>* Fcol = F + (int64_t)parent[pr] * m * m + (int64_t)pj * m;
-    for (int ii = threadIdx.x & 63; ii < bc; ii += 64) {
-        const int pi = mp[ii];
-        if (pi >= 0) Fcol[pi] = Fcol[pi] + Scol[ii];
-    }
+    const int pj = mp[jj], pi = mp[ii];
+    if (pj < 0 || pi < 0) return;
+    const cx<float> v = Fc[(int64_t)child[pr] * mc * mc + (int64_t)(kc + jj) * mc + kc + ii];
+    cx<float>* dst = F + (int64_t)parent[pr] * m * m + (int64_t)pj * m + pi;
+    *dst = *dst + v;
 }
 
 // ---- small fronts (k <= 64 interior cells, m <= 16 MB rows): the whole
@@ -1162,10 +1160,10 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
         for (const auto& g : L.groups) {
             const NdLevel& C = P->L[g.child_level];
             if (C.bmax == 0 || g.parent.empty()) continue;
-            const dim3 grid((unsigned)g.parent.size(), (C.bmax + kNdCols - 1) / kNdCols);
-            k_nd_extend<<<grid, 256, 0, s>>>((const cx<float>*)(fb + C.off_F), C.m, C.kmax, C.bmax,
-                                             (const int32_t*)(pb + g.off_parent), (const int32_t*)(pb + g.off_child),
-                                             (const int32_t*)(pb + g.off_map), (cx<float>*)F, L.m);
+            const int npairs = (int)g.parent.size();
+            k_nd_extend<<<nblk((int64_t)npairs * C.bmax * C.bmax), 256, 0, s>>>(
+                (const cx<float>*)(fb + C.off_F), C.m, C.kmax, C.bmax, npairs, (const int32_t*)(pb + g.off_parent),
+                (const int32_t*)(pb + g.off_child), (const int32_t*)(pb + g.off_map), (cx<float>*)F, L.m);
             CV_LAUNCH_CHECK();
         }
         if (L.kmax > 0)
