@@ -12,7 +12,9 @@ the cotangent of a complex z is dL/dRe z + i dL/dIm z, so reverse mode uses
 A^H = conj(A)^T where the reference (autograd convention) uses A^T.
 """
 import ctypes
+import threading
 import warnings
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -408,6 +410,21 @@ def direct_factor_bytes(op):
 
 
 DIRECT_MEMORY_FRACTION = 0.5
+# Instances of an uncoupled batch solved at once on their own streams: a
+# small instance's factorisation and FGMRES are chains of short launches
+# that leave most of the GPU idle, so several run side by side.
+INSTANCE_WORKERS = 4
+_WORKER_STREAMS = {}
+
+
+def _worker_streams(device, n):
+    """n long-lived streams of a device (default priority), created once."""
+    d = torch.device(device)
+    key = d.index if d.index is not None else torch.cuda.current_device()
+    lst = _WORKER_STREAMS.setdefault(key, [])
+    while len(lst) < n:
+        lst.append(torch.cuda.Stream(device=d))
+    return lst[:n]
 
 
 def transposed_operator(op):
@@ -725,33 +742,77 @@ class InstanceDirect:
     def _make(self, k):
         return make_direct(self.subs[k], self._seg(self.m0, k), self._seg(self.m1, k), self.kinds[k])
 
+    def _direct_one(self, k, bs, m0_64, m1_64, transpose, opts):
+        """Instance k by its factorisation on the current stream; None when it
+        is (or now goes) on the Krylov fallback."""
+        if k in self.failed:
+            return None
+        seg = (self._seg(bs, k).contiguous(), self._seg(m0_64, k).contiguous(),
+               None if m1_64 is None else self._seg(m1_64, k).contiguous())
+        try:
+            f = self.factors[k]
+            if f is None:
+                f = self._make(k)
+                if self.keep:   # kept for the adjoint and tangent solves
+                    self.factors[k] = f
+            return f.solve(*seg, transpose=transpose, opts=opts)
+        except (CevicheError, torch.cuda.OutOfMemoryError) as err:
+            if not _direct_failed(err, opts):
+                raise
+            self.failed.add(k)
+            self.factors[k] = None
+            return None
+
     def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
-        """Every instance by its direct factorisation; under method 'auto' an
+        """Every instance by its direct factorisation, up to INSTANCE_WORKERS
+        at once on their own streams (largest first); under method 'auto' an
         instance whose factorisation or FGMRES fails (and its later adjoint /
-        tangent solves) falls back to Jacobi BiCGSTAB alone."""
+        tangent solves) falls back to Jacobi BiCGSTAB alone, on the caller's
+        stream after the others."""
+        n = self.op.n_inst
         x = torch.empty(bs.shape, dtype=torch.complex128, device=bs.device)
+        outs = [None] * n
+        order = sorted(range(n), key=lambda k: -self.sizes[k])
+        workers = min(INSTANCE_WORKERS, n)
+        if workers > 1:
+            main = torch.cuda.current_stream(bs.device)
+            ready = torch.cuda.Event()
+            ready.record(main)
+            streams = _worker_streams(bs.device, workers)
+            it = iter(order)
+            lock = threading.Lock()
+
+            def run(ws):
+                with torch.cuda.stream(ws):
+                    ws.wait_event(ready)
+                    while True:
+                        with lock:
+                            k = next(it, None)
+                        if k is None:
+                            return
+                        out = self._direct_one(k, bs, m0_64, m1_64, transpose, opts)
+                        if out is not None:
+                            out.record_stream(main)
+                        outs[k] = out
+
+            try:
+                with ThreadPoolExecutor(workers) as ex:
+                    for f in [ex.submit(run, ws) for ws in streams]:
+                        f.result()
+            finally:
+                for ws in streams:
+                    main.wait_stream(ws)
+        else:
+            for k in order:
+                outs[k] = self._direct_one(k, bs, m0_64, m1_64, transpose, opts)
         info = []
-        for k in range(self.op.n_inst):
+        for k in range(n):
             sub = self.subs[k]
-            seg = (self._seg(bs, k).contiguous(), self._seg(m0_64, k).contiguous(),
-                   None if m1_64 is None else self._seg(m1_64, k).contiguous())
-            out = None
-            if k not in self.failed:
-                try:
-                    f = self.factors[k]
-                    if f is None:
-                        f = self._make(k)
-                        if self.keep:   # kept for the adjoint and tangent solves
-                            self.factors[k] = f
-                    out = f.solve(*seg, transpose=transpose, opts=opts)
-                    del f
-                except (CevicheError, torch.cuda.OutOfMemoryError) as err:
-                    if not _direct_failed(err, opts):
-                        raise
-                    self.failed.add(k)
-                    self.factors[k] = None
+            out = outs[k]
             if out is None:
-                out = sub.solve_scaled(seg[0], seg[1], seg[2], transpose=transpose, opts=opts)
+                out = sub.solve_scaled(self._seg(bs, k).contiguous(), self._seg(m0_64, k).contiguous(),
+                                       None if m1_64 is None else self._seg(m1_64, k).contiguous(),
+                                       transpose=transpose, opts=opts)
             x[self.op.offsets[k]:self.op.offsets[k] + self.sizes[k]] = out
             info += sub.last_info
         self.op.last_info = info

@@ -1020,23 +1020,34 @@ __global__ void __launch_bounds__(256) k_nd_copy(const cx<float>* __restrict__ a
 int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
 // ------------------------------------------------------- library handle
-// One rocBLAS handle per device (rocSOLVER uses rocBLAS handles), created on
-// first use and kept for the process; the stream is set on every call.
+// One rocBLAS handle (rocSOLVER uses rocBLAS handles) per (device, stream),
+// created on first use, bound to its stream once and kept for the process:
+// threads factorising on different streams never share a handle, so a
+// handle's stream cannot be switched under another thread's calls.  Callers
+// use a few long-lived streams (the per-instance workers keep theirs).
 std::mutex g_handle_mu;
-rocblas_handle g_handle[64] = {};
+struct HandleSlot {
+    int dev;
+    hipStream_t stream;
+    rocblas_handle h;
+};
+std::vector<HandleSlot> g_handles;
 
 int get_handle(hipStream_t s, rocblas_handle* out) {
     int dev = 0;
     CV_HIP(hipGetDevice(&dev));
-    CV_CHECK(dev >= 0 && dev < 64, "device index %d out of range", dev);
     std::lock_guard<std::mutex> lock(g_handle_mu);
-    if (!g_handle[dev]) {
-        rocblas_handle h = nullptr;
-        CV_CHECK(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
-        g_handle[dev] = h;
-    }
-    CV_CHECK(rocblas_set_stream(g_handle[dev], s) == rocblas_status_success, "rocblas_set_stream failed");
-    *out = g_handle[dev];
+    for (const auto& slot : g_handles)
+        if (slot.dev == dev && slot.stream == s) {
+            *out = slot.h;
+            return CV_OK;
+        }
+    CV_CHECK(g_handles.size() < 256, "too many streams with rocBLAS handles");
+    rocblas_handle h = nullptr;
+    CV_CHECK(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
+    CV_CHECK(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
+    g_handles.push_back({dev, s, h});
+    *out = h;
     return CV_OK;
 }
 

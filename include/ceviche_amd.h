@@ -10,7 +10,9 @@
  * stream and instantiates a HIP graph (both released before it returns), the
  * FGMRES entry points keep host-side vectors, and the nested-dissection
  * factorisation calls rocSOLVER / rocBLAS through one rocBLAS handle per
- * device, created on first use, whose device workspace the libraries manage.
+ * (device, stream), created on first use and kept for the process (at most
+ * 256 pairs; CV_ERR_INVALID beyond), whose device workspace the libraries
+ * manage.
  * cv_fdfd_solve_noalloc is the variant that allocates nothing.  No torch
  * types cross this boundary.
  *
@@ -580,9 +582,12 @@ int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_hos
  * Direct factorisation of one FDFD instance of any shape (complex64, the
  * periodic wrap dropped), the preconditioner of FP64-refined FGMRES for grids
  * wider than the block-tridiagonal path takes: recursive bisection into
- * leaf boxes of <= 64 cells, each tree node eliminating the cells its
- * children share from a dense frontal matrix (LU with partial pivoting, Schur
- * complement products), nodes of equal height in strided batches.  Replaces
+ * leaf boxes of <= 32 cells, each tree node eliminating the cells its
+ * children share from a dense frontal matrix, nodes of equal height in
+ * strided batches: fronts of <= 64 eliminated cells by one Gauss-Jordan
+ * sweep with partial pivoting per node (whole front up to 112 rows, else the
+ * pivot panel and two Schur-complement GEMMs), larger ones by a blocked LU
+ * with partial pivoting and the same complement products.  Replaces
  * the sparse direct factorisation of solvers.py:50-62 (spsolve / pardiso)
  * behind primitives.py:103-119 for those grids.
  *   cv_nd_plan_create: symbolic analysis (host) of an nx x ny grid, NULL on
