@@ -413,7 +413,7 @@ DIRECT_MEMORY_FRACTION = 0.5
 # Instances of an uncoupled batch solved at once on their own streams: a
 # small instance's factorisation and FGMRES are chains of short launches
 # that leave most of the GPU idle, so several run side by side.
-INSTANCE_WORKERS = 4
+INSTANCE_WORKERS = 8
 _WORKER_STREAMS = {}
 
 

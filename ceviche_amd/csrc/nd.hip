@@ -673,170 +673,223 @@ constexpr int kLuMin = 64;
 constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
 
 // LU of the panel rows j0..k-1, columns j0..j0+nbw-1 of every node
-// (blockIdx.x), NT threads holding RPT rows each in registers: per column the
-// pivot (largest |a|^2, ties to the lower row) by a workgroup argmax, the
-// two rows exchanged through LDS, then the rank-1 update of each thread's
-// rows.  Writes the factored panel back, ipiv and the first zero pivot.
+// (blockIdx.x), NT threads holding RPT rows each in registers.  Rows are not
+// moved while the panel is factorised: each thread keeps the current
+// (interchanged) position of its rows, and per column every wave publishes
+// its candidate pivot (largest |a|^2, ties to the lower position) with the
+// candidate row to LDS; after one barrier all waves pick the same pivot,
+// exchange the two positions and apply the rank-1 update to the rows below
+// it.  The LDS slots alternate between even and odd columns, so one barrier
+// per column suffices: a wave writes column c + 2's slots only after the
+// barrier of column c + 1, which every wave reaches after reading column
+// c's.  The rows are stored at their final positions, then ipiv (the
+// sequential interchanges) and the first zero pivot -- the same values,
+// operation for operation, as exchanging the rows at every column.
 template <int NT, int RPT>
 __global__ void __launch_bounds__(NT)
 k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int* __restrict__ ipiv,
            rocblas_int* __restrict__ info) {
     constexpr int NW = NT / 64;
-    __shared__ float redv[NW];
-    __shared__ int redi[NW];
-    __shared__ cx<float> prow[kLuPanel];   // the pivot row (the new row c)
-    __shared__ cx<float> crow[kLuPanel];   // the old row c (moves to the pivot's place)
-    __shared__ int s_piv;
-    const int e = blockIdx.x, tid = threadIdx.x;
+    constexpr int kNone = 0x7fffffff;
+    __shared__ float s_v[2][NW];
+    __shared__ int s_pos[2][NW];
+    __shared__ cx<float> s_row[2][NW][kLuPanel];
+    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nr = k - j0;
     cx<float>* Fe = F + (int64_t)e * m * m;
     cx<float> a[RPT][kLuPanel];
+    int pos[RPT];   // current position of each held row (-1: no row)
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         const int r = tid + q * NT;
+        const int rr = min(r, nr - 1);
+        pos[q] = r < nr ? r : -1;
 #pragma unroll
-        for (int c = 0; c < kLuPanel; ++c)
-            a[q][c] = (r < nr && c < nbw) ? Fe[(int64_t)(j0 + c) * m + j0 + r] : czero<float>();
+        for (int c = 0; c < kLuPanel; ++c) {
+            const cx<float> v = Fe[(int64_t)(j0 + min(c, nbw - 1)) * m + j0 + rr];
+            a[q][c] = (r < nr && c < nbw) ? v : czero<float>();
+        }
     }
     int zero_at = 0;
+    int pv[kLuPanel];   // position exchanged with c at column c (thread 0's record)
 #pragma unroll
     for (int c = 0; c < kLuPanel; ++c) {
         if (c < nbw) {
+            const int par = c & 1;
             float bv = -1.f;
-            int bi = nr;
+            int bp = kNone;
 #pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                const int r = tid + q * NT;
-                if (r >= c && r < nr) {
+            for (int q = 0; q < RPT; ++q)
+                if (pos[q] >= c) {
                     const float v = norm2(a[q][c]);
-                    if (v > bv || (v == bv && r < bi)) {
+                    if (v > bv || (v == bv && pos[q] < bp)) {
                         bv = v;
-                        bi = r;
+                        bp = pos[q];
                     }
                 }
-            }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 const float ov = __shfl_xor(bv, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ov > bv || (ov == bv && oi < bi)) {
+                const int op = __shfl_xor(bp, off);
+                if (ov > bv || (ov == bv && op < bp)) {
                     bv = ov;
-                    bi = oi;
+                    bp = op;
                 }
             }
-            if ((tid & 63) == 0) {
-                redv[tid >> 6] = bv;
-                redi[tid >> 6] = bi;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q)
+                if (pos[q] == bp)
+#pragma unroll
+                    for (int c2 = c; c2 < kLuPanel; ++c2) s_row[par][w][c2] = a[q][c2];
+            if (lane == 0) {
+                s_v[par][w] = bv;
+                s_pos[par][w] = bp;
             }
             __syncthreads();
-            if (tid == 0) {
-                float v = redv[0];
-                int i = redi[0];
-                for (int w = 1; w < NW; ++w)
-                    if (redv[w] > v || (redv[w] == v && redi[w] < i)) {
-                        v = redv[w];
-                        i = redi[w];
-                    }
-                if (i >= nr) i = c;   // no comparable entry (NaN column): keep the row; the zero test flags it
-                s_piv = i;
-                ipiv[(int64_t)e * k + j0 + c] = j0 + i + 1;
+            // the workgroup's pivot from the NW candidates (ties: lower position, then lower wave)
+            float v = lane < NW ? s_v[par][lane] : -1.f;
+            int p = lane < NW ? s_pos[par][lane] : kNone;
+            int ws = lane;
+#pragma unroll
+            for (int off = 1; off < NW; off <<= 1) {
+                const float ov = __shfl_xor(v, off);
+                const int op = __shfl_xor(p, off);
+                const int ow = __shfl_xor(ws, off);
+                if (ov > v || (ov == v && (op < p || (op == p && ow < ws)))) {
+                    v = ov;
+                    p = op;
+                    ws = ow;
+                }
             }
-            __syncthreads();
-            const int p = s_piv;
-            // the owners of rows p and c publish them
+            // lanes 0..NW-1 hold the result; the other lanes reduced padding
+            p = __builtin_amdgcn_readlane(p, 0);
+            ws = __builtin_amdgcn_readlane(ws, 0);
+            if (p == kNone) {
+                // no comparable entry (a NaN column): keep the row at position
+                // c; the zero-pivot test flags it.  Uniform: every wave read
+                // the same slots.
+#pragma unroll
+                for (int q = 0; q < RPT; ++q)
+                    if (pos[q] == c)
+#pragma unroll
+                        for (int c2 = c; c2 < kLuPanel; ++c2) s_row[par][0][c2] = a[q][c2];
+                __syncthreads();
+                p = c;
+                ws = 0;
+            }
+            cx<float> pr[kLuPanel];
+#pragma unroll
+            for (int c2 = c; c2 < kLuPanel; ++c2) pr[c2] = s_row[par][ws][c2];
+            pv[c] = p;
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
-                const int r = tid + q * NT;
-                if (r == p)
-#pragma unroll
-                    for (int c2 = 0; c2 < kLuPanel; ++c2) prow[c2] = a[q][c2];
-                if (r == c)
-#pragma unroll
-                    for (int c2 = 0; c2 < kLuPanel; ++c2) crow[c2] = a[q][c2];
+                if (pos[q] == p)
+                    pos[q] = c;
+                else if (pos[q] == c)
+                    pos[q] = p;
             }
-            __syncthreads();
-            if (p != c) {
-#pragma unroll
-                for (int q = 0; q < RPT; ++q) {
-                    const int r = tid + q * NT;
-                    if (r == c)
-#pragma unroll
-                        for (int c2 = 0; c2 < kLuPanel; ++c2) a[q][c2] = prow[c2];
-                    if (r == p)
-#pragma unroll
-                        for (int c2 = 0; c2 < kLuPanel; ++c2) a[q][c2] = crow[c2];
-                }
-            }
-            const cx<float> d = prow[c];
+            const cx<float> d = pr[c];
             if (norm2(d) > 0.f) {
                 const cx<float> inv = crecip(d);
 #pragma unroll
-                for (int q = 0; q < RPT; ++q) {
-                    const int r = tid + q * NT;
-                    if (r > c && r < nr) {
+                for (int q = 0; q < RPT; ++q)
+                    if (pos[q] > c) {
                         const cx<float> l = a[q][c] * inv;
                         a[q][c] = l;
 #pragma unroll
-                        for (int c2 = c + 1; c2 < kLuPanel; ++c2) a[q][c2] = a[q][c2] - l * prow[c2];
+                        for (int c2 = c + 1; c2 < kLuPanel; ++c2) a[q][c2] = a[q][c2] - l * pr[c2];
                     }
-                }
             } else if (zero_at == 0) {
                 zero_at = j0 + c + 1;
             }
-            __syncthreads();   // prow / crow / the reduction slots are reused by the next column
         }
     }
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int r = tid + q * NT;
-        if (r < nr)
+    for (int q = 0; q < RPT; ++q)
+        if (pos[q] >= 0)
 #pragma unroll
             for (int c = 0; c < kLuPanel; ++c)
-                if (c < nbw) Fe[(int64_t)(j0 + c) * m + j0 + r] = a[q][c];
+                if (c < nbw) Fe[(int64_t)(j0 + c) * m + j0 + pos[q]] = a[q][c];
+    if (tid == 0) {
+#pragma unroll
+        for (int c = 0; c < kLuPanel; ++c)
+            if (c < nbw) ipiv[(int64_t)e * k + j0 + c] = j0 + pv[c] + 1;
+        if (zero_at && info[e] == 0) info[e] = zero_at;
     }
-    if (tid == 0 && zero_at && info[e] == 0) info[e] = zero_at;
 }
 
 // The panel's row interchanges in the other columns of F_II (j < j0 and
 // j >= j0 + nbw, rows j0..k-1), then U12 = L11^-1 A12 in the columns to the
 // right (L11 = the panel's unit lower nbw x nbw block).  One thread per
-// column; workgroups node-major, ceil(k / 256) per node.
+// column; workgroups node-major, ceil(k / 256) per node.  The interchanges
+// touch at most 2 nbw positions: lane i of every wave traces position
+// j0 + i (i < nbw) or the target of interchange i - nbw (when below the
+// panel and not an earlier target) back through the interchanges to the
+// position its final value comes from.  Each thread then loads every source
+// before it stores, so a column costs one round trip instead of nbw
+// dependent exchanges.
 __global__ void __launch_bounds__(256)
 k_lu_rows(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, const rocblas_int* __restrict__ ipiv) {
     __shared__ cx<float> L11[kLuPanel * kLuPanel];
-    __shared__ int pv[kLuPanel];
     const int bpn = (k + 255) / 256;
     const int e = blockIdx.x / bpn;
+    const int lane = threadIdx.x & 63;
     cx<float>* Fe = F + (int64_t)e * m * m;
     for (int i = threadIdx.x; i < nbw * nbw; i += 256) {
         const int c = i / nbw, r = i - c * nbw;   // L11[c * kLuPanel + r] = F[j0 + r, j0 + c]
         L11[c * kLuPanel + r] = Fe[(int64_t)(j0 + c) * m + j0 + r];
     }
-    if (threadIdx.x < nbw) pv[threadIdx.x] = ipiv[(int64_t)e * k + j0 + threadIdx.x] - 1;
+    const int tl = lane < kLuPanel ? lane : lane - kLuPanel;
+    const int my_pv = (tl < nbw) ? ipiv[(int64_t)e * k + j0 + tl] - 1 : -1;
+    int pvu[kLuPanel];   // the interchange targets, uniform
+#pragma unroll
+    for (int u = 0; u < kLuPanel; ++u) pvu[u] = __builtin_amdgcn_readlane(my_pv, u);
+    int dst = -1, src = -1;
+    if (lane < 2 * kLuPanel && tl < nbw) {
+        const int x = lane < kLuPanel ? j0 + lane : my_pv;
+        bool keep = lane < kLuPanel || x >= j0 + nbw;
+        int y = x;
+#pragma unroll
+        for (int u = kLuPanel - 1; u >= 0; --u)
+            if (u < nbw) {
+                if (lane >= kLuPanel && u < tl && pvu[u] == x) keep = false;
+                if (y == j0 + u)
+                    y = pvu[u];
+                else if (y == pvu[u])
+                    y = j0 + u;
+            }
+        if (keep) {
+            dst = x;
+            src = y;
+        }
+    }
+    int ds[2 * kLuPanel], sr[2 * kLuPanel];   // (destination, source) positions, uniform; -1: none
+#pragma unroll
+    for (int i = 0; i < 2 * kLuPanel; ++i) {
+        ds[i] = __builtin_amdgcn_readlane(dst, i);
+        sr[i] = __builtin_amdgcn_readlane(src, i);
+    }
     __syncthreads();
     const int j = (blockIdx.x - e * bpn) * 256 + threadIdx.x;
     if (j >= k || (j >= j0 && j < j0 + nbw)) return;
     cx<float>* col = Fe + (int64_t)j * m;
-    for (int t = 0; t < nbw; ++t) {
-        const int r = pv[t];
-        if (r != j0 + t) {
-            const cx<float> v = col[j0 + t];
-            col[j0 + t] = col[r];
-            col[r] = v;
-        }
-    }
-    if (j < j0) return;
-    cx<float> x[kLuPanel];
+    cx<float> val[2 * kLuPanel];
 #pragma unroll
-    for (int t = 0; t < kLuPanel; ++t) x[t] = t < nbw ? col[j0 + t] : czero<float>();
+    for (int i = 0; i < 2 * kLuPanel; ++i) val[i] = ds[i] >= 0 ? col[sr[i]] : czero<float>();
+    if (j < j0) {
+#pragma unroll
+        for (int i = 0; i < 2 * kLuPanel; ++i)
+            if (ds[i] >= 0 && ds[i] != sr[i]) col[ds[i]] = val[i];
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < kLuPanel; ++t)
 #pragma unroll
         for (int u = 0; u < kLuPanel; ++u)
-            if (u < t && t < nbw) x[t] = x[t] - L11[u * kLuPanel + t] * x[u];
+            if (u < t && t < nbw) val[t] = val[t] - L11[u * kLuPanel + t] * val[u];
 #pragma unroll
-    for (int t = 0; t < kLuPanel; ++t)
-        if (t < nbw) col[j0 + t] = x[t];
+    for (int i = 0; i < 2 * kLuPanel; ++i)
+        if (ds[i] >= 0) col[ds[i]] = val[i];
 }
 
 // R = [I F_IB] (k x m per node, column-major, leading dimension k)

@@ -296,3 +296,28 @@ def test_direct_failure_falls_back_to_bicgstab():
                     solver_options=cv.SolverOptions(tol=1e-12, direct_max_iter=1, method='direct'))
     with pytest.raises(NotConvergedError):
         F2.solve(c['src'])
+
+
+@pytest.mark.parametrize('N', [200, 300, 400, 600])
+def test_nested_dissection_pivots_on_blob_designs(N):
+    """One blob design per size class of the multi-frequency batch (dL 40 nm,
+    npml 20, lambda 1.50 um): their separator fronts need row interchanges in
+    the blocked LU of the fronts above 64 eliminated cells, where an
+    interchange applied wrongly leaves a zero pivot or an inaccurate factor.
+    Under method='direct' (no fallback) the factorisation must succeed and
+    FGMRES with it reach 1e-10 in at most 40 steps; at 200^2 the field
+    matches spsolve."""
+    import ceviche_amd as cv
+    from ceviche_amd import workloads
+    d = workloads.fdfd_blob_batch(n_inst=1, n_lambda=1, sizes=(N,))[0]
+    print('nd blob', N, flush=True)
+    F = cv.fdfd_ez(d['omega'], d['dL'], d['eps'], d['npml'],
+                   solver_options=cv.SolverOptions(tol=1e-10, method='direct'))
+    _, _, Ez = F.solve(torch.from_numpy(d['src']))
+    info = F.last_solve_info
+    print('  method', info['method'], 'iterations', info['iterations'], 'residual %.2e' % info['rel_residual'],
+          flush=True)
+    assert info.get('factor') == 'nd' and info['rel_residual'] <= 1e-10 and info['iterations'] <= 40
+    if N == 200:
+        ez = R.solve_ez(d['omega'], d['dL'], d['eps'], d['npml'], d['src'])[2]
+        assert np.linalg.norm(Ez.cpu().numpy() - ez) / np.linalg.norm(ez) <= 1e-8
