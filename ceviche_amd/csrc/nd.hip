@@ -672,6 +672,47 @@ constexpr int kLuPanel = 16;
 constexpr int kLuMin = 64;
 constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
 
+// Pivot search keys: one 64-bit word per candidate whose maximum is the
+// pivot, largest |a|^2 first and ties to the lower position.  High word: the
+// bits of |a|^2 plus one (non-negative floats order as their bit patterns;
+// 0 means no candidate), low word: the complement of the position.  A NaN
+// magnitude never becomes a candidate.
+__device__ __forceinline__ uint64_t piv_key(float v, int pos) {
+    return (v == v) ? ((uint64_t)(__float_as_uint(v) + 1u) << 32) | (uint32_t)(~pos) : 0ull;
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Maximum over each row of 16 lanes (every lane of the row gets it) by DPP
+// lane exchanges within the row: neighbours, pairs, half-row and row mirrors.
+// All lanes must be active.
+__device__ __forceinline__ uint64_t row16_max(uint64_t k) {
+    k = umax64(k, dpp_u64<0xB1>(k));    // quad_perm [1,0,3,2]
+    k = umax64(k, dpp_u64<0x4E>(k));    // quad_perm [2,3,0,1]
+    k = umax64(k, dpp_u64<0x141>(k));   // row_half_mirror
+    k = umax64(k, dpp_u64<0x140>(k));   // row_mirror
+    return k;
+}
+
+// Maximum over the wave, uniform.
+__device__ __forceinline__ uint64_t wave_max64(uint64_t k) {
+    k = row16_max(k);
+    return umax64(umax64(readlane64(k, 0), readlane64(k, 16)), umax64(readlane64(k, 32), readlane64(k, 48)));
+}
+
 // LU of the panel rows j0..k-1, columns j0..j0+nbw-1 of every node
 // (blockIdx.x), NT threads holding RPT rows each in registers.  Rows are not
 // moved while the panel is factorised: each thread keeps the current
@@ -690,9 +731,8 @@ __global__ void __launch_bounds__(NT)
 k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int* __restrict__ ipiv,
            rocblas_int* __restrict__ info) {
     constexpr int NW = NT / 64;
-    constexpr int kNone = 0x7fffffff;
-    __shared__ float s_v[2][NW];
-    __shared__ int s_pos[2][NW];
+    static_assert(NW <= 16, "the wave keys are reduced within one row of 16 lanes");
+    __shared__ uint64_t s_key[2][NW];
     __shared__ cx<float> s_row[2][NW][kLuPanel];
     const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nr = k - j0;
@@ -716,58 +756,30 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
     for (int c = 0; c < kLuPanel; ++c) {
         if (c < nbw) {
             const int par = c & 1;
-            float bv = -1.f;
-            int bp = kNone;
+            uint64_t key = 0;
 #pragma unroll
             for (int q = 0; q < RPT; ++q)
-                if (pos[q] >= c) {
-                    const float v = norm2(a[q][c]);
-                    if (v > bv || (v == bv && pos[q] < bp)) {
-                        bv = v;
-                        bp = pos[q];
-                    }
-                }
+                if (pos[q] >= c) key = umax64(key, piv_key(norm2(a[q][c]), pos[q]));
+            const uint64_t wkey = wave_max64(key);
+            if (wkey != 0) {
+                const int wpos = (int)~(uint32_t)wkey;
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const float ov = __shfl_xor(bv, off);
-                const int op = __shfl_xor(bp, off);
-                if (ov > bv || (ov == bv && op < bp)) {
-                    bv = ov;
-                    bp = op;
-                }
+                for (int q = 0; q < RPT; ++q)
+                    if (pos[q] == wpos)
+#pragma unroll
+                        for (int c2 = c; c2 < kLuPanel; ++c2) s_row[par][w][c2] = a[q][c2];
             }
-#pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (pos[q] == bp)
-#pragma unroll
-                    for (int c2 = c; c2 < kLuPanel; ++c2) s_row[par][w][c2] = a[q][c2];
-            if (lane == 0) {
-                s_v[par][w] = bv;
-                s_pos[par][w] = bp;
-            }
+            if (lane == 0) s_key[par][w] = wkey;
             __syncthreads();
-            // the workgroup's pivot from the NW candidates (ties: lower position, then lower wave)
-            float v = lane < NW ? s_v[par][lane] : -1.f;
-            int p = lane < NW ? s_pos[par][lane] : kNone;
-            int ws = lane;
-#pragma unroll
-            for (int off = 1; off < NW; off <<= 1) {
-                const float ov = __shfl_xor(v, off);
-                const int op = __shfl_xor(p, off);
-                const int ow = __shfl_xor(ws, off);
-                if (ov > v || (ov == v && (op < p || (op == p && ow < ws)))) {
-                    v = ov;
-                    p = op;
-                    ws = ow;
-                }
-            }
-            // lanes 0..NW-1 hold the result; the other lanes reduced padding
-            p = __builtin_amdgcn_readlane(p, 0);
-            ws = __builtin_amdgcn_readlane(ws, 0);
-            if (p == kNone) {
+            // the workgroup's pivot: the largest of the NW wave keys (unique
+            // unless zero: positions differ), found by every wave
+            const uint64_t k0 = lane < NW ? s_key[par][lane] : 0ull;
+            const uint64_t gkey = readlane64(row16_max(k0), 0);
+            int p, ws;
+            if (gkey == 0) {
                 // no comparable entry (a NaN column): keep the row at position
                 // c; the zero-pivot test flags it.  Uniform: every wave read
-                // the same slots.
+                // the same keys.
 #pragma unroll
                 for (int q = 0; q < RPT; ++q)
                     if (pos[q] == c)
@@ -776,6 +788,9 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
                 __syncthreads();
                 p = c;
                 ws = 0;
+            } else {
+                p = (int)~(uint32_t)gkey;
+                ws = __builtin_ctzll(__ballot(k0 == gkey));
             }
             cx<float> pr[kLuPanel];
 #pragma unroll
