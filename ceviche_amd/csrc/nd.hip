@@ -442,6 +442,47 @@ k_nd_extend(const cx<float>* __restrict__ Fc, int mc, int kc, int bc, int npairs
     *dst = *dst + v;
 }
 
+// Pivot search keys: one 64-bit word per candidate whose maximum is the
+// pivot, largest |a|^2 first and ties to the lower position.  High word: the
+// bits of |a|^2 plus one (non-negative floats order as their bit patterns;
+// 0 means no candidate), low word: the complement of the position.  A NaN
+// magnitude never becomes a candidate.
+__device__ __forceinline__ uint64_t piv_key(float v, int pos) {
+    return (v == v) ? ((uint64_t)(__float_as_uint(v) + 1u) << 32) | (uint32_t)(~pos) : 0ull;
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Maximum over each row of 16 lanes (every lane of the row gets it) by DPP
+// lane exchanges within the row: neighbours, pairs, half-row and row mirrors.
+// All lanes must be active.
+__device__ __forceinline__ uint64_t row16_max(uint64_t k) {
+    k = umax64(k, dpp_u64<0xB1>(k));    // quad_perm [1,0,3,2]
+    k = umax64(k, dpp_u64<0x4E>(k));    // quad_perm [2,3,0,1]
+    k = umax64(k, dpp_u64<0x141>(k));   // row_half_mirror
+    k = umax64(k, dpp_u64<0x140>(k));   // row_mirror
+    return k;
+}
+
+// Maximum over the wave, uniform.
+__device__ __forceinline__ uint64_t wave_max64(uint64_t k) {
+    k = row16_max(k);
+    return umax64(umax64(readlane64(k, 0), readlane64(k, 16)), umax64(readlane64(k, 32), readlane64(k, 48)));
+}
+
 // ---- small fronts (k <= 64 interior cells, m <= 16 MB rows): the whole
 // elimination of the interior block in one kernel, one workgroup per node.
 // A swap-free Gauss-Jordan sweep over the k interior pivots (partial
@@ -496,21 +537,14 @@ __global__ void __launch_bounds__(256) k_nd_gj(cx<float>* __restrict__ F, int m,
             }
             __syncthreads();
             // pivot row: the same butterfly argmax in every wave
-            float bv = -1.f;
-            int bi = lane;
-            if (lane < k && !((used >> lane) & 1ull)) bv = norm2(colp[buf][lane]);
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                const float ov = __shfl_xor(bv, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ov > bv || (ov == bv && oi < bi)) {
-                    bv = ov;
-                    bi = oi;
-                }
-            }
-            const int r = __builtin_amdgcn_readfirstlane(bi);
+            // the largest unused |a|^2 of column p, ties to the lower row: one
+            // key reduction, the same in every wave.  No comparable entry (a
+            // NaN column): the first unused row, reported as a zero pivot.
+            const uint64_t best = wave_max64((lane < k && !((used >> lane) & 1ull))
+                                                 ? piv_key(norm2(colp[buf][lane]), lane) : 0ull);
+            const int r = best ? (int)~(uint32_t)best : __builtin_ctzll(~used);
             const cx<float> apiv = colp[buf][r];
-            const bool zero_piv = norm2(apiv) == 0.f;
+            const bool zero_piv = best == 0 || norm2(apiv) == 0.f;
             const cx<float> inv = zero_piv ? czero<float>() : crecip(apiv);
             if (zero_piv && bad == 0) bad = p + 1;
             const int rhi = r >> 4, rlo = r & 15;
@@ -603,21 +637,14 @@ __global__ void __launch_bounds__(1024) k_nd_gj_panel(cx<float>* __restrict__ F,
                 for (int a = 0; a < MR; ++a) colp[buf][r0 + 64 * a] = v[a][pc];
             }
             __syncthreads();
-            float bv = -1.f;
-            int bi = lane;
-            if (lane < k && !((used >> lane) & 1ull)) bv = norm2(colp[buf][lane]);
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                const float ov = __shfl_xor(bv, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ov > bv || (ov == bv && oi < bi)) {
-                    bv = ov;
-                    bi = oi;
-                }
-            }
-            const int r = __builtin_amdgcn_readfirstlane(bi);   // < k <= 64: row r sits at a = 0
+            // the largest unused |a|^2 of column p, ties to the lower row: one
+            // key reduction, the same in every wave.  No comparable entry (a
+            // NaN column): the first unused row, reported as a zero pivot.
+            const uint64_t best = wave_max64((lane < k && !((used >> lane) & 1ull))
+                                                 ? piv_key(norm2(colp[buf][lane]), lane) : 0ull);
+            const int r = best ? (int)~(uint32_t)best : __builtin_ctzll(~used);   // < k <= 64: row r sits at a = 0
             const cx<float> apiv = colp[buf][r];
-            const bool zero_piv = norm2(apiv) == 0.f;
+            const bool zero_piv = best == 0 || norm2(apiv) == 0.f;
             const cx<float> inv = zero_piv ? czero<float>() : crecip(apiv);
             if (zero_piv && bad == 0) bad = p + 1;
             if (r0 == r) {   // owners of row r (its entries are v[0][*])
@@ -671,47 +698,6 @@ __global__ void __launch_bounds__(1024) k_nd_gj_panel(cx<float>* __restrict__ F,
 constexpr int kLuPanel = 16;
 constexpr int kLuMin = 64;
 constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
-
-// Pivot search keys: one 64-bit word per candidate whose maximum is the
-// pivot, largest |a|^2 first and ties to the lower position.  High word: the
-// bits of |a|^2 plus one (non-negative floats order as their bit patterns;
-// 0 means no candidate), low word: the complement of the position.  A NaN
-// magnitude never becomes a candidate.
-__device__ __forceinline__ uint64_t piv_key(float v, int pos) {
-    return (v == v) ? ((uint64_t)(__float_as_uint(v) + 1u) << 32) | (uint32_t)(~pos) : 0ull;
-}
-
-template <int CTRL>
-__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
-}
-
-__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
-
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
-    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Maximum over each row of 16 lanes (every lane of the row gets it) by DPP
-// lane exchanges within the row: neighbours, pairs, half-row and row mirrors.
-// All lanes must be active.
-__device__ __forceinline__ uint64_t row16_max(uint64_t k) {
-    k = umax64(k, dpp_u64<0xB1>(k));    // quad_perm [1,0,3,2]
-    k = umax64(k, dpp_u64<0x4E>(k));    // quad_perm [2,3,0,1]
-    k = umax64(k, dpp_u64<0x141>(k));   // row_half_mirror
-    k = umax64(k, dpp_u64<0x140>(k));   // row_mirror
-    return k;
-}
-
-// Maximum over the wave, uniform.
-__device__ __forceinline__ uint64_t wave_max64(uint64_t k) {
-    k = row16_max(k);
-    return umax64(umax64(readlane64(k, 0), readlane64(k, 16)), umax64(readlane64(k, 32), readlane64(k, 48)));
-}
 
 // LU of the panel rows j0..k-1, columns j0..j0+nbw-1 of every node
 // (blockIdx.x), NT threads holding RPT rows each in registers.  Rows are not
