@@ -821,39 +821,43 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
 
 // The panel's row interchanges in the other columns of F_II (j < j0 and
 // j >= j0 + nbw, rows j0..k-1), then U12 = L11^-1 A12 in the columns to the
-// right (L11 = the panel's unit lower nbw x nbw block).  One thread per
-// column; workgroups node-major, ceil(k / 256) per node.  The interchanges
-// touch at most 2 nbw positions: lane i of every wave traces position
-// j0 + i (i < nbw) or the target of interchange i - nbw (when below the
-// panel and not an earlier target) back through the interchanges to the
-// position its final value comes from.  Each thread then loads every source
-// before it stores, so a column costs one round trip instead of nbw
-// dependent exchanges.
+// right (L11 = the panel's unit lower nbw x nbw block).  64 columns per
+// workgroup, workgroups node-major (ceil(k / 64) per node).  The
+// interchanges touch at most 2 nbw positions: each lane traces entry
+// i = lane mod 32 -- position j0 + i (i < nbw) or the target of interchange
+// i - nbw (when below the panel and not an earlier target) -- back through
+// the interchanges to the position its final value comes from.  Lanes then
+// gather their entry of 8 columns (the 16 panel rows of a column are one
+// contiguous run), one thread per column solves with L11 in LDS, and the
+// values go back with the same mapping.
+constexpr int kRowsCols = 64;
 __global__ void __launch_bounds__(256)
 k_lu_rows(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, const rocblas_int* __restrict__ ipiv) {
     __shared__ cx<float> L11[kLuPanel * kLuPanel];
-    const int bpn = (k + 255) / 256;
+    __shared__ cx<float> vals[2 * kLuPanel][kRowsCols + 1];
+    const int bpn = (k + kRowsCols - 1) / kRowsCols;
     const int e = blockIdx.x / bpn;
-    const int lane = threadIdx.x & 63;
+    const int jb = (blockIdx.x - e * bpn) * kRowsCols;
+    const int lane = threadIdx.x & 63, li = lane & 31, cg = threadIdx.x >> 5;
     cx<float>* Fe = F + (int64_t)e * m * m;
     for (int i = threadIdx.x; i < nbw * nbw; i += 256) {
         const int c = i / nbw, r = i - c * nbw;   // L11[c * kLuPanel + r] = F[j0 + r, j0 + c]
         L11[c * kLuPanel + r] = Fe[(int64_t)(j0 + c) * m + j0 + r];
     }
-    const int tl = lane < kLuPanel ? lane : lane - kLuPanel;
+    const int tl = li < kLuPanel ? li : li - kLuPanel;
     const int my_pv = (tl < nbw) ? ipiv[(int64_t)e * k + j0 + tl] - 1 : -1;
-    int pvu[kLuPanel];   // the interchange targets, uniform
+    int pvu[kLuPanel];   // the interchange targets, uniform (lanes 0..15 hold them)
 #pragma unroll
     for (int u = 0; u < kLuPanel; ++u) pvu[u] = __builtin_amdgcn_readlane(my_pv, u);
     int dst = -1, src = -1;
-    if (lane < 2 * kLuPanel && tl < nbw) {
-        const int x = lane < kLuPanel ? j0 + lane : my_pv;
-        bool keep = lane < kLuPanel || x >= j0 + nbw;
+    if (tl < nbw) {
+        const int x = li < kLuPanel ? j0 + li : my_pv;
+        bool keep = li < kLuPanel || x >= j0 + nbw;
         int y = x;
 #pragma unroll
         for (int u = kLuPanel - 1; u >= 0; --u)
             if (u < nbw) {
-                if (lane >= kLuPanel && u < tl && pvu[u] == x) keep = false;
+                if (li >= kLuPanel && u < tl && pvu[u] == x) keep = false;
                 if (y == j0 + u)
                     y = pvu[u];
                 else if (y == pvu[u])
@@ -864,33 +868,101 @@ k_lu_rows(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, const rocbla
             src = y;
         }
     }
-    int ds[2 * kLuPanel], sr[2 * kLuPanel];   // (destination, source) positions, uniform; -1: none
+    // gather: lane entry li of columns jb + cg + 8q
 #pragma unroll
-    for (int i = 0; i < 2 * kLuPanel; ++i) {
-        ds[i] = __builtin_amdgcn_readlane(dst, i);
-        sr[i] = __builtin_amdgcn_readlane(src, i);
+    for (int q = 0; q < kRowsCols / 8; ++q) {
+        const int j = jb + cg + 8 * q;
+        const bool act = dst >= 0 && j < k && (j < j0 || j >= j0 + nbw);
+        vals[li][cg + 8 * q] = act ? Fe[(int64_t)j * m + src] : czero<float>();
     }
     __syncthreads();
-    const int j = (blockIdx.x - e * bpn) * 256 + threadIdx.x;
-    if (j >= k || (j >= j0 && j < j0 + nbw)) return;
-    cx<float>* col = Fe + (int64_t)j * m;
-    cx<float> val[2 * kLuPanel];
+    if (threadIdx.x < kRowsCols) {
+        const int j = jb + threadIdx.x;
+        if (j >= j0 + nbw && j < k) {
+            cx<float> x[kLuPanel];
 #pragma unroll
-    for (int i = 0; i < 2 * kLuPanel; ++i) val[i] = ds[i] >= 0 ? col[sr[i]] : czero<float>();
-    if (j < j0) {
+            for (int t = 0; t < kLuPanel; ++t) x[t] = vals[t][threadIdx.x];
 #pragma unroll
-        for (int i = 0; i < 2 * kLuPanel; ++i)
-            if (ds[i] >= 0 && ds[i] != sr[i]) col[ds[i]] = val[i];
-        return;
+            for (int t = 0; t < kLuPanel; ++t)
+#pragma unroll
+                for (int u = 0; u < kLuPanel; ++u)
+                    if (u < t && t < nbw) x[t] = x[t] - L11[u * kLuPanel + t] * x[u];
+#pragma unroll
+            for (int t = 0; t < kLuPanel; ++t) vals[t][threadIdx.x] = x[t];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kRowsCols / 8; ++q) {
+        const int j = jb + cg + 8 * q;
+        if (dst >= 0 && j < k && ((j < j0 && dst != src) || j >= j0 + nbw))
+            Fe[(int64_t)j * m + dst] = vals[li][cg + 8 * q];
+    }
+}
+
+// The trailing update of the blocked LU after a full 16-column panel,
+// A22 -= L21 U12 (A22 the rest x rest block below and right of the panel,
+// L21 rest x 16 under it, U12 16 x rest beside it; column-major, leading
+// dimension m, node e at F + e m^2).  One workgroup per 64 x 64 tile of A22
+// (grid.x row tiles, grid.y column tiles, grid.z nodes): L21's and U12's
+// slices staged in LDS, each thread accumulates a 4 x 4 block (rows tr + 16i,
+// columns tc + 16j) and reads and writes its A22 entries once, a wave
+// covering 4 columns x 16 consecutive rows per access.  Memory-bound.
+__global__ void __launch_bounds__(256)
+k_lu_update(cx<float>* __restrict__ F, int m, int j0, int rest) {
+    __shared__ cx<float> Ls[kLuPanel][64];   // Ls[kk][r] = L21[r0 + r, kk]
+    __shared__ cx<float> Us[kLuPanel][64];   // Us[kk][c] = U12[kk, c0 + c]
+    const int t = threadIdx.x, tr = t & 15, tc = t >> 4;
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    cx<float>* Fe = F + (int64_t)blockIdx.z * m * m;
+    const int b0 = j0 + kLuPanel;   // first row / column of A22
+    // this thread's A22 entries first, so their loads overlap the staging and the products
+    cx<float> cv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = min(c0 + tc + 16 * j, rest - 1);
+        const cx<float>* col = Fe + (int64_t)(b0 + c) * m + b0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cv[i][j] = col[min(r0 + tr + 16 * i, rest - 1)];
     }
 #pragma unroll
-    for (int t = 0; t < kLuPanel; ++t)
+    for (int q = 0; q < 4; ++q) {
+        const int idx = t + 256 * q;
+        const int kk = idx >> 6, r = idx & 63;
+        Ls[kk][r] = (r0 + r < rest) ? Fe[(int64_t)(j0 + kk) * m + b0 + r0 + r] : czero<float>();
+        const int c = idx >> 4, k2 = idx & 15;
+        Us[k2][c] = (c0 + c < rest) ? Fe[(int64_t)(b0 + c0 + c) * m + j0 + k2] : czero<float>();
+    }
+    __syncthreads();
+    cx<float> acc[4][4];
 #pragma unroll
-        for (int u = 0; u < kLuPanel; ++u)
-            if (u < t && t < nbw) val[t] = val[t] - L11[u * kLuPanel + t] * val[u];
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int i = 0; i < 2 * kLuPanel; ++i)
-        if (ds[i] >= 0) col[ds[i]] = val[i];
+        for (int j = 0; j < 4; ++j) acc[i][j] = czero<float>();
+#pragma unroll 2
+    for (int kk = 0; kk < kLuPanel; ++kk) {
+        cx<float> a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = Ls[kk][tr + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Us[kk][tc + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + a[i] * b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = c0 + tc + 16 * j;
+        if (c < rest) {
+            cx<float>* col = Fe + (int64_t)(b0 + c) * m + b0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = r0 + tr + 16 * i;
+                if (r < rest) col[r] = cv[i][j] - acc[i][j];
+            }
+        }
+    }
 }
 
 // R = [I F_IB] (k x m per node, column-major, leading dimension k)
@@ -1274,7 +1346,7 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             continue;
         }
         if (k > kLuMin && k <= kLuMaxRows) {
-            const unsigned g_rows = (unsigned)(((k + 255) / 256) * (int64_t)L.nb);
+            const unsigned g_rows = (unsigned)(((k + kRowsCols - 1) / kRowsCols) * (int64_t)L.nb);
             for (int j0 = 0; j0 < k; j0 += kLuPanel) {
                 const int nbw = std::min(kLuPanel, k - j0);
                 const int nr = k - j0;   // panel rows: 512 threads x 1 or 2 rows, 1024 x 2 above 1024 rows
@@ -1289,11 +1361,11 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
                 k_lu_rows<<<g_rows, 256, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv);
                 CV_LAUNCH_CHECK();
                 const int rest = k - j0 - nbw;
-                if (rest > 0)   // A22 -= L21 U12
-                    CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, rest, rest,
-                                                        nbw, &minus_one, F + (int64_t)j0 * m + j0 + nbw, m, sF,
-                                                        F + (int64_t)(j0 + nbw) * m + j0, m, sF, &one,
-                                                        F + (int64_t)(j0 + nbw) * m + j0 + nbw, m, sF, L.nb));
+                if (rest > 0) {   // A22 -= L21 U12 (rest > 0 only after a full panel)
+                    const unsigned tiles = (unsigned)((rest + 63) / 64);
+                    k_lu_update<<<dim3(tiles, tiles, (unsigned)L.nb), 256, 0, s>>>((cx<float>*)F, m, j0, rest);
+                    CV_LAUNCH_CHECK();
+                }
             }
         } else {
             CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
