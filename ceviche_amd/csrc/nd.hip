@@ -19,19 +19,19 @@
 // then forms Q = F_II^-1 and X = Q F_IB, passes S = F_BB - F_BI X up to its
 // parent and forms G = F_BI Q.  Fronts of at most 64 interior cells do this
 // by a Gauss-Jordan sweep (k_nd_gj, k_nd_gj_panel); larger interior blocks
-// are factorised F_II = P L U and give [Q X] by the triangular solves of
-// [I F_IB] (the LU of at most kNdSolveMin cells, on fronts the sweep kernels
-// do not hold, by inverting in place from the LU factors and one product).
+// up to kLuMaxRows cells by a blocked Gauss-Jordan on [F_II | I F_IB] with
+// 16-column LU panels, which leaves [Q X] directly; above that F_II = P L U
+// by rocsolver and [Q X] by the triangular solves of [I F_IB] (fronts of at
+// most 64 interior cells that the sweep kernels do not hold: the LU, then
+// the inverse in place and one product).
 // F keeps Q, X and G in place of its blocks:
 //   F = [[Q, X], [G, S]].
 // Nodes of equal height (distance to the leaves) are processed together as
 // strided batches padded to the level's largest |I| (kmax, identity padding)
 // and |B| (bmax, zero padding): dense LU, inversion or triangular solves
 // (rocsolver getrf, getri / getrs) and the products (rocblas gemm) run on the
-// matrix cores, one launch sequence per level; the Schur complements of a
-// level's large interior blocks (kLuMin < k <= kLuMaxRows) take an own
-// blocked LU.  The symbolic analysis (tree, index sets, maps) is host C++
-// done once per grid shape.
+// matrix cores, one launch sequence per level.  The symbolic analysis
+// (tree, index sets, maps) is host C++ done once per grid shape.
 //
 // Solve of A x = b (transpose 0), one matrix-vector product per node and sweep:
 //   up the tree:   [y_I; t] = [Q; G] b_I,  b_B -= t
@@ -689,12 +689,11 @@ __global__ void __launch_bounds__(1024) k_nd_gj_panel(cx<float>* __restrict__ F,
         }
 }
 
-// ---- blocked LU with partial pivoting of the large interior blocks
-// (kLuMin < k <= kLuMaxRows): per panel of kLuPanel columns one panel factorisation, one
-// interchange + triangular-solve launch and one rocBLAS GEMM, where
-// rocsolver's getrf spends several short launches on every column.  The
-// result is the LAPACK form getrs takes: L (unit, strictly lower) and U in
-// place of F_II, ipiv 1-based sequential row interchanges.
+// ---- panels of the large interior blocks (kLuMin < k <= kLuMaxRows): LU
+// with partial pivoting of kLuPanel columns at a time, L (unit, strictly
+// lower) and U in place of the panel, ipiv 1-based sequential row
+// interchanges (the LAPACK form); the blocked Gauss-Jordan below carries
+// each panel to the rest of the augmented matrix.
 constexpr int kLuPanel = 16;
 constexpr int kLuMin = 64;
 constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
@@ -819,34 +818,55 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
     }
 }
 
-// The panel's row interchanges in the other columns of F_II (j < j0 and
-// j >= j0 + nbw, rows j0..k-1), then U12 = L11^-1 A12 in the columns to the
-// right (L11 = the panel's unit lower nbw x nbw block).  64 columns per
-// workgroup, workgroups node-major (ceil(k / 64) per node).  The
-// interchanges touch at most 2 nbw positions: each lane traces entry
-// i = lane mod 32 -- position j0 + i (i < nbw) or the target of interchange
-// i - nbw (when below the panel and not an earlier target) -- back through
-// the interchanges to the position its final value comes from.  Lanes then
-// gather their entry of 8 columns (the 16 panel rows of a column are one
-// contiguous run), one thread per column solves with L11 in LDS, and the
-// values go back with the same mapping.
-constexpr int kRowsCols = 64;
+constexpr int kRowsCols = 64;   // augmented columns per workgroup of k_gj_rows
+
+// ---- blocked Gauss-Jordan on the augmented matrix [F_II | R], R = [I F_IB]
+// (k x m per node, leading dimension k, node stride k m): after each panel
+// LU, the panel's interchanges and L11^-1 go to every column right of the
+// panel (k_gj_rows), and its columns are eliminated from every other row
+// (k_gj_update: rows below with L21 as in the LU, rows above with
+// F[row, panel] U11^-1).  F_II then holds only the panels' upper blocks U11
+// on its diagonal, and one block-diagonal solve (k_gj_final) leaves
+// [Q X] = F_II^-1 [I F_IB] in R.  Columns left of a panel are never needed
+// again, so the interchanges skip them.
+
+// Column j of node e's augmented matrix (rows 0..k-1): F_II's columns j < k
+// (leading dimension m), then R's.
+__device__ __forceinline__ cx<float>* aug_col(cx<float>* Fe, cx<float>* Re, int m, int k, int j) {
+    return j < k ? Fe + (int64_t)j * m : Re + (int64_t)(j - k) * k;
+}
+
+// The interchanges of the panel j0..j0+nbw-1 and L11^-1 (its unit lower
+// block) applied to the augmented columns j0 + nbw .. k + m - 1, 64 columns
+// per workgroup, workgroups node-major.  The interchanges touch at most
+// 2 nbw positions: each lane traces entry i = lane mod 32 -- position
+// j0 + i (i < nbw) or the target of interchange i - nbw (when below the
+// panel and not an earlier target) -- back through the interchanges to the
+// position its final value comes from.  Lanes then gather their entry of 8
+// columns (the 16 panel rows of a column are one contiguous run), one
+// thread per column solves with L11 in LDS, and the values go back with the
+// same mapping: one round trip per column instead of nbw dependent
+// exchanges.
 __global__ void __launch_bounds__(256)
-k_lu_rows(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, const rocblas_int* __restrict__ ipiv) {
+k_gj_rows(cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k, int j0, int nbw,
+          const rocblas_int* __restrict__ ipiv) {
     __shared__ cx<float> L11[kLuPanel * kLuPanel];
     __shared__ cx<float> vals[2 * kLuPanel][kRowsCols + 1];
-    const int bpn = (k + kRowsCols - 1) / kRowsCols;
+    const int ncol = k + m;
+    const int cols = ncol - j0 - nbw;
+    const int bpn = (cols + kRowsCols - 1) / kRowsCols;
     const int e = blockIdx.x / bpn;
-    const int jb = (blockIdx.x - e * bpn) * kRowsCols;
+    const int jb = j0 + nbw + (blockIdx.x - e * bpn) * kRowsCols;
     const int lane = threadIdx.x & 63, li = lane & 31, cg = threadIdx.x >> 5;
     cx<float>* Fe = F + (int64_t)e * m * m;
+    cx<float>* Re = R + (int64_t)e * k * m;
     for (int i = threadIdx.x; i < nbw * nbw; i += 256) {
         const int c = i / nbw, r = i - c * nbw;   // L11[c * kLuPanel + r] = F[j0 + r, j0 + c]
         L11[c * kLuPanel + r] = Fe[(int64_t)(j0 + c) * m + j0 + r];
     }
     const int tl = li < kLuPanel ? li : li - kLuPanel;
     const int my_pv = (tl < nbw) ? ipiv[(int64_t)e * k + j0 + tl] - 1 : -1;
-    int pvu[kLuPanel];   // the interchange targets, uniform (lanes 0..15 hold them)
+    int pvu[kLuPanel];
 #pragma unroll
     for (int u = 0; u < kLuPanel; ++u) pvu[u] = __builtin_amdgcn_readlane(my_pv, u);
     int dst = -1, src = -1;
@@ -868,70 +888,88 @@ k_lu_rows(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, const rocbla
             src = y;
         }
     }
-    // gather: lane entry li of columns jb + cg + 8q
 #pragma unroll
     for (int q = 0; q < kRowsCols / 8; ++q) {
         const int j = jb + cg + 8 * q;
-        const bool act = dst >= 0 && j < k && (j < j0 || j >= j0 + nbw);
-        vals[li][cg + 8 * q] = act ? Fe[(int64_t)j * m + src] : czero<float>();
+        vals[li][cg + 8 * q] = (dst >= 0 && j < ncol) ? aug_col(Fe, Re, m, k, j)[src] : czero<float>();
     }
     __syncthreads();
-    if (threadIdx.x < kRowsCols) {
-        const int j = jb + threadIdx.x;
-        if (j >= j0 + nbw && j < k) {
-            cx<float> x[kLuPanel];
+    if (threadIdx.x < kRowsCols && jb + (int)threadIdx.x < ncol) {
+        cx<float> x[kLuPanel];
 #pragma unroll
-            for (int t = 0; t < kLuPanel; ++t) x[t] = vals[t][threadIdx.x];
+        for (int t = 0; t < kLuPanel; ++t) x[t] = vals[t][threadIdx.x];
 #pragma unroll
-            for (int t = 0; t < kLuPanel; ++t)
+        for (int t = 0; t < kLuPanel; ++t)
 #pragma unroll
-                for (int u = 0; u < kLuPanel; ++u)
-                    if (u < t && t < nbw) x[t] = x[t] - L11[u * kLuPanel + t] * x[u];
+            for (int u = 0; u < kLuPanel; ++u)
+                if (u < t && t < nbw) x[t] = x[t] - L11[u * kLuPanel + t] * x[u];
 #pragma unroll
-            for (int t = 0; t < kLuPanel; ++t) vals[t][threadIdx.x] = x[t];
-        }
+        for (int t = 0; t < kLuPanel; ++t) vals[t][threadIdx.x] = x[t];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kRowsCols / 8; ++q) {
         const int j = jb + cg + 8 * q;
-        if (dst >= 0 && j < k && ((j < j0 && dst != src) || j >= j0 + nbw))
-            Fe[(int64_t)j * m + dst] = vals[li][cg + 8 * q];
+        if (dst >= 0 && j < ncol) aug_col(Fe, Re, m, k, j)[dst] = vals[li][cg + 8 * q];
     }
 }
 
-// The trailing update of the blocked LU after a full 16-column panel,
-// A22 -= L21 U12 (A22 the rest x rest block below and right of the panel,
-// L21 rest x 16 under it, U12 16 x rest beside it; column-major, leading
-// dimension m, node e at F + e m^2).  One workgroup per 64 x 64 tile of A22
-// (grid.x row tiles, grid.y column tiles, grid.z nodes): L21's and U12's
-// slices staged in LDS, each thread accumulates a 4 x 4 block (rows tr + 16i,
-// columns tc + 16j) and reads and writes its A22 entries once, a wave
-// covering 4 columns x 16 consecutive rows per access.  Memory-bound.
+// Elimination of the panel's columns from every other row of the augmented
+// columns j0 + nbw .. k + m - 1: out[r, c] -= w_r . (panel rows of c), with
+// w_r = L21 row r below the panel and w_r = F[r, panel] U11^-1 above it
+// (solved by one thread per row of the tile, U11 and its reciprocal
+// diagonal in LDS).  One workgroup per 64 x 64 tile (grid.x row tiles over
+// 0..k-1, grid.y column tiles, grid.z nodes): the tile's outputs are
+// loaded first so that their latency overlaps the staging, the multipliers
+// and the panel rows' entries are staged in LDS, and each thread
+// accumulates a 4 x 4 block (rows tr + 16i, columns tc + 16j; a wave covers
+// 4 columns x 16 consecutive rows per access).  The panel rows are not
+// written.  Memory-bound: each output is read and written once.
 __global__ void __launch_bounds__(256)
-k_lu_update(cx<float>* __restrict__ F, int m, int j0, int rest) {
-    __shared__ cx<float> Ls[kLuPanel][64];   // Ls[kk][r] = L21[r0 + r, kk]
-    __shared__ cx<float> Us[kLuPanel][64];   // Us[kk][c] = U12[kk, c0 + c]
+k_gj_update(cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k, int j0, int nbw) {
+    __shared__ cx<float> Ls[kLuPanel][64];
+    __shared__ cx<float> Us[kLuPanel][64];
+    __shared__ cx<float> U11[kLuPanel][kLuPanel + 1];
+    __shared__ cx<float> rdiag[kLuPanel];
     const int t = threadIdx.x, tr = t & 15, tc = t >> 4;
-    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const int r0 = blockIdx.x * 64;
+    const int cb = j0 + nbw + blockIdx.y * 64;
+    const int ncol = k + m;
     cx<float>* Fe = F + (int64_t)blockIdx.z * m * m;
-    const int b0 = j0 + kLuPanel;   // first row / column of A22
-    // this thread's A22 entries first, so their loads overlap the staging and the products
+    cx<float>* Re = R + (int64_t)blockIdx.z * k * m;
     cx<float> cv[4][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int c = min(c0 + tc + 16 * j, rest - 1);
-        const cx<float>* col = Fe + (int64_t)(b0 + c) * m + b0;
+        const cx<float>* col = aug_col(Fe, Re, m, k, min(cb + tc + 16 * j, ncol - 1));
 #pragma unroll
-        for (int i = 0; i < 4; ++i) cv[i][j] = col[min(r0 + tr + 16 * i, rest - 1)];
+        for (int i = 0; i < 4; ++i) cv[i][j] = col[min(r0 + tr + 16 * i, k - 1)];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int idx = t + 256 * q;
         const int kk = idx >> 6, r = idx & 63;
-        Ls[kk][r] = (r0 + r < rest) ? Fe[(int64_t)(j0 + kk) * m + b0 + r0 + r] : czero<float>();
+        Ls[kk][r] = (r0 + r < k && kk < nbw) ? Fe[(int64_t)(j0 + kk) * m + r0 + r] : czero<float>();
         const int c = idx >> 4, k2 = idx & 15;
-        Us[k2][c] = (c0 + c < rest) ? Fe[(int64_t)(b0 + c0 + c) * m + j0 + k2] : czero<float>();
+        Us[k2][c] = (cb + c < ncol && k2 < nbw) ? aug_col(Fe, Re, m, k, cb + c)[j0 + k2] : czero<float>();
+    }
+    {
+        const int a = t & 15, bc = t >> 4;
+        U11[a][bc] = (a <= bc && bc < nbw) ? Fe[(int64_t)(j0 + bc) * m + j0 + a] : czero<float>();
+        if (t < kLuPanel) rdiag[t] = t < nbw ? crecip(Fe[(int64_t)(j0 + t) * m + j0 + t]) : czero<float>();
+    }
+    __syncthreads();
+    if (r0 < j0 && t < 64 && r0 + t < j0) {   // rows above the panel: w U11 = F[row, panel]
+        cx<float> w[kLuPanel];
+#pragma unroll
+        for (int a = 0; a < kLuPanel; ++a) {
+            cx<float> x = Ls[a][t];
+#pragma unroll
+            for (int u = 0; u < kLuPanel; ++u)
+                if (u < a) x = x - w[u] * U11[u][a];
+            w[a] = a < nbw ? x * rdiag[a] : czero<float>();
+        }
+#pragma unroll
+        for (int a = 0; a < kLuPanel; ++a) Ls[a][t] = w[a];
     }
     __syncthreads();
     cx<float> acc[4][4];
@@ -953,16 +991,53 @@ k_lu_update(cx<float>* __restrict__ F, int m, int j0, int rest) {
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int c = c0 + tc + 16 * j;
-        if (c < rest) {
-            cx<float>* col = Fe + (int64_t)(b0 + c) * m + b0;
+        const int c = cb + tc + 16 * j;
+        if (c < ncol) {
+            cx<float>* col = aug_col(Fe, Re, m, k, c);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = r0 + tr + 16 * i;
-                if (r < rest) col[r] = cv[i][j] - acc[i][j];
+                if (r < k && (r < j0 || r >= j0 + nbw)) col[r] = cv[i][j] - acc[i][j];
             }
         }
     }
+}
+
+// [Q X] = U^-1 Z per 16-row block of R once every panel is eliminated (F_II
+// is then block diagonal with the panels' U11): one thread per (block,
+// column of R), grid.x column groups, grid.y blocks, grid.z nodes.
+__global__ void __launch_bounds__(256)
+k_gj_final(const cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k) {
+    __shared__ cx<float> U[kLuPanel][kLuPanel + 1];
+    __shared__ cx<float> rd[kLuPanel];
+    const int j0 = blockIdx.y * kLuPanel, nbw = min(kLuPanel, k - j0);
+    const cx<float>* Fe = F + (int64_t)blockIdx.z * m * m;
+    cx<float>* Re = R + (int64_t)blockIdx.z * k * m;
+    const int t = threadIdx.x;
+    {
+        const int a = t & 15, bc = t >> 4;
+        U[a][bc] = (a <= bc && bc < nbw) ? Fe[(int64_t)(j0 + bc) * m + j0 + a] : czero<float>();
+        if (t < kLuPanel) rd[t] = t < nbw ? crecip(Fe[(int64_t)(j0 + t) * m + j0 + t]) : czero<float>();
+    }
+    __syncthreads();
+    const int c = blockIdx.x * 256 + t;
+    if (c >= m) return;
+    cx<float>* col = Re + (int64_t)c * k + j0;
+    cx<float> y[kLuPanel];
+#pragma unroll
+    for (int a = 0; a < kLuPanel; ++a) y[a] = a < nbw ? col[a] : czero<float>();
+#pragma unroll
+    for (int a = kLuPanel - 1; a >= 0; --a)
+        if (a < nbw) {
+            cx<float> x = y[a];
+#pragma unroll
+            for (int u = 0; u < kLuPanel; ++u)
+                if (u > a && u < nbw) x = x - U[a][u] * y[u];
+            y[a] = x * rd[a];
+        }
+#pragma unroll
+    for (int a = 0; a < kLuPanel; ++a)
+        if (a < nbw) col[a] = y[a];
 }
 
 // R = [I F_IB] (k x m per node, column-major, leading dimension k)
@@ -1345,28 +1420,35 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             }
             continue;
         }
+        bool gj = false;   // [Q X] left in R by the blocked Gauss-Jordan
         if (k > kLuMin && k <= kLuMaxRows) {
-            const unsigned g_rows = (unsigned)(((k + kRowsCols - 1) / kRowsCols) * (int64_t)L.nb);
+            gj = true;
+            cx<float>* Fc = (cx<float>*)F;
+            cx<float>* Rc = (cx<float>*)R;
+            k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>(Fc, Rc, L.nb, k, m);
+            CV_LAUNCH_CHECK();
+            const int ncol = k + m;
             for (int j0 = 0; j0 < k; j0 += kLuPanel) {
                 const int nbw = std::min(kLuPanel, k - j0);
                 const int nr = k - j0;   // panel rows: 512 threads x 1 or 2 rows, 1024 x 2 above 1024 rows
                 if (nr <= 512)
-                    k_lu_panel<512, 1><<<L.nb, 512, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv, info + L.info_first);
+                    k_lu_panel<512, 1><<<L.nb, 512, 0, s>>>(Fc, m, k, j0, nbw, piv, info + L.info_first);
                 else if (nr <= 1024)
-                    k_lu_panel<512, 2><<<L.nb, 512, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv, info + L.info_first);
+                    k_lu_panel<512, 2><<<L.nb, 512, 0, s>>>(Fc, m, k, j0, nbw, piv, info + L.info_first);
                 else
-                    k_lu_panel<1024, 2><<<L.nb, 1024, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv,
-                                                             info + L.info_first);
+                    k_lu_panel<1024, 2><<<L.nb, 1024, 0, s>>>(Fc, m, k, j0, nbw, piv, info + L.info_first);
                 CV_LAUNCH_CHECK();
-                k_lu_rows<<<g_rows, 256, 0, s>>>((cx<float>*)F, m, k, j0, nbw, piv);
+                const int cols = ncol - j0 - nbw;   // >= m > 0
+                k_gj_rows<<<(unsigned)(((cols + kRowsCols - 1) / kRowsCols) * (int64_t)L.nb), 256, 0, s>>>(
+                    Fc, Rc, m, k, j0, nbw, piv);
                 CV_LAUNCH_CHECK();
-                const int rest = k - j0 - nbw;
-                if (rest > 0) {   // A22 -= L21 U12 (rest > 0 only after a full panel)
-                    const unsigned tiles = (unsigned)((rest + 63) / 64);
-                    k_lu_update<<<dim3(tiles, tiles, (unsigned)L.nb), 256, 0, s>>>((cx<float>*)F, m, j0, rest);
-                    CV_LAUNCH_CHECK();
-                }
+                k_gj_update<<<dim3((unsigned)((k + 63) / 64), (unsigned)((cols + 63) / 64), (unsigned)L.nb), 256, 0,
+                              s>>>(Fc, Rc, m, k, j0, nbw);
+                CV_LAUNCH_CHECK();
             }
+            k_gj_final<<<dim3((unsigned)((m + 255) / 256), (unsigned)((k + kLuPanel - 1) / kLuPanel),
+                              (unsigned)L.nb), 256, 0, s>>>(Fc, Rc, m, k);
+            CV_LAUNCH_CHECK();
         } else {
             CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
         }
@@ -1375,10 +1457,13 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             // than the in-place inverse plus a product for these k), into the
             // top k rows of F; then S = F_BB - F_BI X and G = F_BI Q
             const rocblas_stride sR = (rocblas_stride)k * m;
-            k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, (cx<float>*)R, L.nb, k, m);
-            CV_LAUNCH_CHECK();
-            CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, k, m, F, m, sF, piv, k, R, k, sR,
-                                                   L.nb));
+            if (!gj) {
+                k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, (cx<float>*)R, L.nb, k,
+                                                                     m);
+                CV_LAUNCH_CHECK();
+                CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, k, m, F, m, sF, piv, k, R, k, sR,
+                                                       L.nb));
+            }
             if (b > 0)
                 CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
                                                     &minus_one, F + k, m, sF, R + (int64_t)k * k, k, sR, &one,

@@ -586,8 +586,9 @@ int cv_fdfd_fgmres_direct(const cv_fdfd_desc* desc, const cv_fdfd_desc* desc_hos
  * children share from a dense frontal matrix, nodes of equal height in
  * strided batches: fronts of <= 64 eliminated cells by one Gauss-Jordan
  * sweep with partial pivoting per node (whole front up to 112 rows, else the
- * pivot panel and two Schur-complement GEMMs), larger ones by a blocked LU
- * with partial pivoting and the same complement products.  Replaces
+ * pivot panel and two Schur-complement GEMMs), larger ones up to 2048 cells
+ * by a blocked Gauss-Jordan on [F_II | I F_IB] with 16-column LU panels
+ * (partial pivoting) and the same complement products.  Replaces
  * the sparse direct factorisation of solvers.py:50-62 (spsolve / pardiso)
  * behind primitives.py:103-119 for those grids.
  *   cv_nd_plan_create: symbolic analysis (host) of an nx x ny grid, NULL on
