@@ -100,12 +100,14 @@ def fdfd_main(steps, warmup, N=1024):
             "frac": round(tflops / 157.3, 4), "traffic": None,
             "kernel": "cv_nd_factor (per tree level: frontal assembly, then the interior elimination: a "
                       "Gauss-Jordan sweep in one kernel for fronts of <= 64 interior cells (the whole front up to "
-                      "112 rows, the left block up to 448 rows plus the X and S products), own panel LU + "
-                      "triangular solves of [I F_IB] above 64 cells; the X, S and G products)",
+                      "112 rows, the left block up to 448 rows plus the X and S products), above 64 cells a "
+                      "blocked Gauss-Jordan on [F_II | I F_IB] with 16-column LU panels that leaves [Q X]; the S "
+                      "and G products)",
             "avg_factor_ms": round(fact_ms, 2), "algorithmic_flops_per_factor": flops,
             "apply_ms": round(apply_ms, 3),
             "note": "real flops of one nested-dissection factorisation at unpadded sizes (per node with k interior "
-                    "and b boundary cells: LU k^3/3, inverse from the LU factors 2k^3/3, X = Q F_IB k^2 b, "
+                    "and b boundary cells: the inverse k^3 (LU k^3/3 + 2k^3/3, the same count as the Gauss-Jordan "
+                    "elimination), X = F_II^-1 F_IB k^2 b, "
                     "S = F_BB - F_BI X b^2 k, G = F_BI Q b k^2 complex multiply-adds of 8 real flops) over its "
                     "time (HIP events on "
                     "the launch stream, 3 factorisations); the factorisation is a sequence of launches per level, "
