@@ -9,7 +9,7 @@ name = "ceviche_amd"
 __version__ = '0.1.0'
 
 from .fdfd import fdfd_ez, fdfd_hz, fdfd_mf_ez  # noqa: E402,F401
-from ._fdfd_op import SolverOptions  # noqa: E402,F401
+from ._fdfd_op import SolverOptions, autotune_instance_workers  # noqa: E402,F401
 from .fdtd import fdtd  # noqa: E402,F401
 from .jacobians import jacobian  # noqa: E402,F401
 from . import primitives  # noqa: E402,F401

@@ -13,6 +13,7 @@ A^H = conj(A)^T where the reference (autograd convention) uses A^T.
 """
 import ctypes
 import threading
+import time
 import warnings
 from concurrent.futures import ThreadPoolExecutor
 
@@ -417,6 +418,60 @@ INSTANCE_WORKERS = 8
 _WORKER_STREAMS = {}
 
 
+_TUNED_WORKERS = {}
+
+
+def _device_key(device):
+    d = torch.device(device)
+    return d.index if d.index is not None else torch.cuda.current_device()
+
+
+def instance_workers(device):
+    """Instances of an uncoupled batch solved at once on a device: the count
+    autotune_instance_workers chose there, else INSTANCE_WORKERS."""
+    return _TUNED_WORKERS.get(_device_key(device), INSTANCE_WORKERS)
+
+
+def autotune_instance_workers(op, eps, b, candidates=(1, 2, 4, 8, 12), opts=None):
+    """Times the default forward solve (each instance by its direct
+    factorisation and FGMRES) of an uncoupled batch once per candidate count
+    of instances solved at once, after one untimed solve that builds the
+    plans and library handles, and keeps the fastest count for this device
+    (instance_workers); returns {count: seconds}.  The batch should look like
+    the ones to come (sizes, number of instances); its solutions are
+    discarded.  eps and b are the batch's packed permittivity and source
+    term (as FdfdSolve takes them)."""
+    if op.n_inst < 2:
+        raise ValueError('autotune_instance_workers needs a batch of at least two instances')
+    opts = opts or SolverOptions()
+    if opts.method not in ('auto', 'direct'):
+        raise ValueError("the instance count only matters for method 'auto' or 'direct'")
+    key = _device_key(b.device)
+    m0, m1 = op.materials(eps)
+
+    def one():
+        torch.cuda.synchronize(b.device)
+        t0 = time.perf_counter()
+        FdfdSolve.apply(m0, m1, b, op, None, opts)
+        torch.cuda.synchronize(b.device)
+        return time.perf_counter() - t0
+
+    saved = _TUNED_WORKERS.get(key)
+    times = {}
+    try:
+        one()
+        for c in candidates:
+            _TUNED_WORKERS[key] = max(1, int(c))
+            times[int(c)] = one()
+    finally:
+        if saved is None:
+            _TUNED_WORKERS.pop(key, None)
+        else:
+            _TUNED_WORKERS[key] = saved
+    _TUNED_WORKERS[key] = min(times, key=times.get)
+    return times
+
+
 def _worker_streams(device, n):
     """n long-lived streams of a device (default priority), created once."""
     d = torch.device(device)
@@ -764,8 +819,8 @@ class InstanceDirect:
             return None
 
     def solve(self, bs, m0_64, m1_64, transpose=False, opts=None):
-        """Every instance by its direct factorisation, up to INSTANCE_WORKERS
-        at once on their own streams (largest first); under method 'auto' an
+        """Every instance by its direct factorisation, up to
+        instance_workers(device) at once on their own streams (largest first); under method 'auto' an
         instance whose factorisation or FGMRES fails (and its later adjoint /
         tangent solves) falls back to Jacobi BiCGSTAB alone, on the caller's
         stream after the others."""
@@ -773,7 +828,7 @@ class InstanceDirect:
         x = torch.empty(bs.shape, dtype=torch.complex128, device=bs.device)
         outs = [None] * n
         order = sorted(range(n), key=lambda k: -self.sizes[k])
-        workers = min(INSTANCE_WORKERS, n)
+        workers = min(instance_workers(bs.device), n)
         if workers > 1:
             main = torch.cuda.current_stream(bs.device)
             ready = torch.cuda.Event()

@@ -321,3 +321,34 @@ def test_nested_dissection_pivots_on_blob_designs(N):
     if N == 200:
         ez = R.solve_ez(d['omega'], d['dL'], d['eps'], d['npml'], d['src'])[2]
         assert np.linalg.norm(Ez.cpu().numpy() - ez) / np.linalg.norm(ez) <= 1e-8
+
+
+def test_instance_worker_autotune_keeps_the_fields():
+    """autotune_instance_workers times the default solve of an uncoupled batch
+    (six instances, uneven shapes and frequencies) per candidate count of
+    instances solved at once and keeps the fastest for the device; every
+    instance is solved on its own, so the fields with the chosen count match
+    one instance at a time (to 1e-12)."""
+    import ceviche_amd._fdfd_op as fo
+    from ceviche_amd import autotune_instance_workers
+    from ceviche_amd._fdfd_op import FdfdOperator, FdfdSolve, SolverOptions
+    rng = np.random.default_rng(5)
+    shapes = [(40, 56), (64, 48), (52, 52), (36, 70), (60, 60), (48, 40)]
+    specs = [dict(omega=OMEGA * (1 + 0.01 * i), dL=50e-9, shape=sh, npml=(6, 6)) for i, sh in enumerate(shapes)]
+    op = FdfdOperator('ez', specs)
+    eps = torch.from_numpy(np.concatenate([rng.uniform(1, 6, sh).ravel() for sh in shapes])).cuda()
+    b = torch.from_numpy(np.concatenate([rng.standard_normal(sh).ravel() + 0j for sh in shapes])).cuda()
+    opts = SolverOptions(tol=1e-10)
+    try:
+        times = autotune_instance_workers(op, eps, b, candidates=(1, 2, 4), opts=opts)
+        print('autotune', times, flush=True)
+        assert set(times) == {1, 2, 4}
+        assert fo.instance_workers(b.device) == min(times, key=times.get)
+        m0, m1 = op.materials(eps)
+        x_tuned = FdfdSolve.apply(m0, m1, b, op, None, opts)
+        assert all(i['method'] == 'direct' for i in op.last_info)
+        fo._TUNED_WORKERS[fo._device_key(b.device)] = 1
+        x_one = FdfdSolve.apply(m0, m1, b, op, None, opts)
+        assert torch.linalg.vector_norm(x_tuned - x_one) <= 1e-12 * torch.linalg.vector_norm(x_one)
+    finally:
+        fo._TUNED_WORKERS.clear()
