@@ -61,6 +61,9 @@ namespace {
 constexpr int kLeafCells = 32;
 constexpr int kMvTile = 64;        // rows and columns of one tile of the application products
 constexpr int kNdSolveMin = 64;   // interior blocks above this size: triangular solves instead of the inverse
+constexpr int kLuPanel = 16;      // columns per panel of the blocked Gauss-Jordan
+constexpr int kLuMin = 64;        // interior blocks above this size take the blocked Gauss-Jordan ...
+constexpr int kLuMaxRows = 2048;  // ... up to this size (the panel kernels hold 1024 threads x 2 rows)
 
 struct NdGroup {               // (parent, child) pairs whose child sits at one level, one child slot
     int child_level = 0, slot = 0;
@@ -91,7 +94,7 @@ struct cv_nd_plan {
     int nx = 0, ny = 0;
     std::vector<cv::NdLevel> L;
     int64_t plan_bytes = 0, factor_bytes = 0;
-    int64_t off_w = 0, off_part = 0, off_tb = 0, off_info = 0, off_r = 0;
+    int64_t off_w = 0, off_part = 0, off_tb = 0, off_info = 0, off_r = 0, off_r2 = 0;
     int64_t n_nodes = 0;
     double flops = 0.0;   // real flops of one numeric factorisation (unpadded sizes)
 };
@@ -294,7 +297,7 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         if (!contrib.empty()) L.bs_ptr.push_back((int32_t)contrib.size());
     }
     // device layouts: index arrays in the plan buffer, numeric data in the factor buffer
-    int64_t po = 0, fo = 0, pmax = 0, rmax = 0, tbmax = 0, info_first = 0;
+    int64_t po = 0, fo = 0, pmax = 0, rmax = 0, r2max = 0, tbmax = 0, info_first = 0;
     auto take = [](int64_t& o, int64_t bytes) {
         const int64_t r = o;
         o += al256(bytes);
@@ -321,6 +324,8 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
         pmax = std::max<int64_t>(pmax, (int64_t)L.nb * std::max<int64_t>((int64_t)((L.kmax + kMvTile - 1) / kMvTile) * L.m,
                                                                         (int64_t)((L.bmax + kMvTile - 1) / kMvTile) * L.kmax));
         rmax = std::max<int64_t>(rmax, (int64_t)L.nb * L.kmax * L.m);
+        if (L.kmax > kLuMin && L.kmax <= kLuMaxRows)   // the blocked Gauss-Jordan's second buffer
+            r2max = std::max<int64_t>(r2max, (int64_t)L.nb * L.kmax * (L.kmax + L.m));
         tbmax = std::max<int64_t>(tbmax, (int64_t)L.nb * L.bmax);
     }
     P->n_nodes = info_first;
@@ -329,6 +334,7 @@ extern "C" cv_nd_plan* cv_nd_plan_create(int nx, int ny) {
     P->off_part = take(fo, 8 * std::max<int64_t>(1, pmax));
     P->off_tb = take(fo, 8 * std::max<int64_t>(1, tbmax));
     P->off_r = take(fo, 8 * std::max<int64_t>(1, rmax));
+    P->off_r2 = take(fo, 8 * std::max<int64_t>(1, r2max));
     P->plan_bytes = std::max<int64_t>(po, 256);
     P->factor_bytes = fo;
     return P;
@@ -694,12 +700,10 @@ __global__ void __launch_bounds__(1024) k_nd_gj_panel(cx<float>* __restrict__ F,
 // lower) and U in place of the panel, ipiv 1-based sequential row
 // interchanges (the LAPACK form); the blocked Gauss-Jordan below carries
 // each panel to the rest of the augmented matrix.
-constexpr int kLuPanel = 16;
-constexpr int kLuMin = 64;
-constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels hold (1024 threads x 2 rows)
 
 // LU of the panel rows j0..k-1, columns j0..j0+nbw-1 of every node
-// (blockIdx.x), NT threads holding RPT rows each in registers.  Rows are not
+// (blockIdx.x; node e's matrix at F + e nstride, leading dimension ld), NT
+// threads holding RPT rows each in registers.  Rows are not
 // moved while the panel is factorised: each thread keeps the current
 // (interchanged) position of its rows, and per column every wave publishes
 // its candidate pivot (largest |a|^2, ties to the lower position) with the
@@ -713,7 +717,7 @@ constexpr int kLuMaxRows = 2048;   // largest interior block the panel kernels h
 // operation for operation, as exchanging the rows at every column.
 template <int NT, int RPT>
 __global__ void __launch_bounds__(NT)
-k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int* __restrict__ ipiv,
+k_lu_panel(cx<float>* __restrict__ F, int ld, int64_t nstride, int k, int j0, int nbw, rocblas_int* __restrict__ ipiv,
            rocblas_int* __restrict__ info) {
     constexpr int NW = NT / 64;
     static_assert(NW <= 16, "the wave keys are reduced within one row of 16 lanes");
@@ -721,7 +725,7 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
     __shared__ cx<float> s_row[2][NW][kLuPanel];
     const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nr = k - j0;
-    cx<float>* Fe = F + (int64_t)e * m * m;
+    cx<float>* Fe = F + (int64_t)e * nstride;
     cx<float> a[RPT][kLuPanel];
     int pos[RPT];   // current position of each held row (-1: no row)
 #pragma unroll
@@ -731,7 +735,7 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
         pos[q] = r < nr ? r : -1;
 #pragma unroll
         for (int c = 0; c < kLuPanel; ++c) {
-            const cx<float> v = Fe[(int64_t)(j0 + min(c, nbw - 1)) * m + j0 + rr];
+            const cx<float> v = Fe[(int64_t)(j0 + min(c, nbw - 1)) * ld + j0 + rr];
             a[q][c] = (r < nr && c < nbw) ? v : czero<float>();
         }
     }
@@ -809,7 +813,7 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
         if (pos[q] >= 0)
 #pragma unroll
             for (int c = 0; c < kLuPanel; ++c)
-                if (c < nbw) Fe[(int64_t)(j0 + c) * m + j0 + pos[q]] = a[q][c];
+                if (c < nbw) Fe[(int64_t)(j0 + c) * ld + j0 + pos[q]] = a[q][c];
     if (tid == 0) {
 #pragma unroll
         for (int c = 0; c < kLuPanel; ++c)
@@ -818,52 +822,65 @@ k_lu_panel(cx<float>* __restrict__ F, int m, int k, int j0, int nbw, rocblas_int
     }
 }
 
-constexpr int kRowsCols = 64;   // augmented columns per workgroup of k_gj_rows
-
 // ---- blocked Gauss-Jordan on the augmented matrix [F_II | R], R = [I F_IB]
-// (k x m per node, leading dimension k, node stride k m): after each panel
-// LU, the panel's interchanges and L11^-1 go to every column right of the
-// panel (k_gj_rows), and its columns are eliminated from every other row
-// (k_gj_update: rows below with L21 as in the LU, rows above with
-// F[row, panel] U11^-1).  F_II then holds only the panels' upper blocks U11
-// on its diagonal, and one block-diagonal solve (k_gj_final) leaves
-// [Q X] = F_II^-1 [I F_IB] in R.  Columns left of a panel are never needed
-// again, so the interchanges skip them.
+// (k x m per node, leading dimension k, node stride k m).  Each step reads
+// the augmented matrix from one of two buffers and writes every column
+// right of its panel (all k rows) to the other: buffer 0 is F_II in the
+// front (leading dimension m, node stride m^2) beside R, buffer 1 one
+// k x (k + m) matrix per node (leading dimension k).  Step p factorises its
+// 16-column panel in buffer p mod 2 (k_lu_panel), then k_gj_step carries the
+// panel's interchanges and L11^-1 to the right columns and eliminates the
+// panel's columns from every other row (rows below with L21, rows above
+// with F[row, panel] U11^-1).  F_II ends block diagonal with the panels'
+// U11 (panel q's in buffer q mod 2), and k_gj_final solves each block,
+// leaving [Q X] = F_II^-1 [I F_IB] in the right part of the last buffer.
+// Columns left of a panel are not needed again and are not carried.
 
-// Column j of node e's augmented matrix (rows 0..k-1): F_II's columns j < k
-// (leading dimension m), then R's.
-__device__ __forceinline__ cx<float>* aug_col(cx<float>* Fe, cx<float>* Re, int m, int k, int j) {
-    return j < k ? Fe + (int64_t)j * m : Re + (int64_t)(j - k) * k;
+// Column j (0 <= j < k + m) of node e's augmented matrix in buffer b.
+__device__ __forceinline__ cx<float>* aug_col(cx<float>* F, cx<float>* R, cx<float>* B1, int m, int k, int e, int b,
+                                              int j) {
+    if (b) return B1 + (int64_t)e * k * (k + m) + (int64_t)j * k;
+    return j < k ? F + (int64_t)e * m * m + (int64_t)j * m : R + (int64_t)e * k * m + (int64_t)(j - k) * k;
 }
 
-// The interchanges of the panel j0..j0+nbw-1 and L11^-1 (its unit lower
-// block) applied to the augmented columns j0 + nbw .. k + m - 1, 64 columns
-// per workgroup, workgroups node-major.  The interchanges touch at most
-// 2 nbw positions: each lane traces entry i = lane mod 32 -- position
-// j0 + i (i < nbw) or the target of interchange i - nbw (when below the
-// panel and not an earlier target) -- back through the interchanges to the
-// position its final value comes from.  Lanes then gather their entry of 8
-// columns (the 16 panel rows of a column are one contiguous run), one
-// thread per column solves with L11 in LDS, and the values go back with the
-// same mapping: one round trip per column instead of nbw dependent
-// exchanges.
-__global__ void __launch_bounds__(256)
-k_gj_rows(cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k, int j0, int nbw,
-          const rocblas_int* __restrict__ ipiv) {
-    __shared__ cx<float> L11[kLuPanel * kLuPanel];
-    __shared__ cx<float> vals[2 * kLuPanel][kRowsCols + 1];
+// One step of the blocked Gauss-Jordan for the panel j0..j0+nbw-1 (factorised
+// in buffer sb), writing the augmented columns j0 + nbw .. k + m - 1 of rows
+// 0..k-1 to buffer 1 - sb.  One workgroup per 64 x 64 tile (grid.x row tiles,
+// grid.y column tiles, grid.z nodes).  Every workgroup derives the panel
+// rows' new values for its columns itself, from the unmodified source: the
+// interchanges touch at most 2 nbw positions, and lane i (mod 32) traces
+// entry i -- position j0 + i (i < nbw) or the target of interchange i - nbw
+// (when below the panel and not an earlier target) -- back through the
+// interchanges to the position its value comes from; lanes gather their
+// entry of 8 columns, one thread per column applies L11^-1 to the panel
+// rows (U12), and another wave forms the multipliers of the tile's rows
+// above the panel (w U11 = F[row, panel]).  Outputs: U12 in the panel rows;
+// elsewhere the row's value after the interchanges (the moved value for a
+// target row) minus its multiplier times U12.  The tile's outputs are loaded
+// first so that their latency overlaps the rest; each thread accumulates a
+// 4 x 4 block (rows tr + 16i, columns tc + 16j).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_gj_step(cx<float>* __restrict__ F, cx<float>* __restrict__ R, cx<float>* __restrict__ B1, int m, int k, int j0,
+          int nbw, int sb, const rocblas_int* __restrict__ ipiv) {
+    constexpr int NE = 2 * kLuPanel;
+    __shared__ cx<float> Ls[kLuPanel][64];        // multipliers of the tile's rows
+    __shared__ cx<float> G[NE][64 + 1];           // gathered entries of the tile's columns (U12, moved values)
+    __shared__ cx<float> PB[kLuPanel][kLuPanel + 1];   // the panel block: L11 below, U11 on and above the diagonal
+    __shared__ cx<float> rdiag[kLuPanel];
+    __shared__ int tmap[64];                      // tile row -> target entry, -1 if none
+    const int t = threadIdx.x, tr = t & 15, tc = t >> 4, lane = t & 63, li = lane & 31;
+    const int r0 = blockIdx.x * 64;
+    const int cb = j0 + nbw + blockIdx.y * 64;
+    const int e = blockIdx.z, db = 1 - sb;
     const int ncol = k + m;
-    const int cols = ncol - j0 - nbw;
-    const int bpn = (cols + kRowsCols - 1) / kRowsCols;
-    const int e = blockIdx.x / bpn;
-    const int jb = j0 + nbw + (blockIdx.x - e * bpn) * kRowsCols;
-    const int lane = threadIdx.x & 63, li = lane & 31, cg = threadIdx.x >> 5;
-    cx<float>* Fe = F + (int64_t)e * m * m;
-    cx<float>* Re = R + (int64_t)e * k * m;
-    for (int i = threadIdx.x; i < nbw * nbw; i += 256) {
-        const int c = i / nbw, r = i - c * nbw;   // L11[c * kLuPanel + r] = F[j0 + r, j0 + c]
-        L11[c * kLuPanel + r] = Fe[(int64_t)(j0 + c) * m + j0 + r];
+    cx<float> cv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const cx<float>* col = aug_col(F, R, B1, m, k, e, sb, min(cb + tc + 16 * j, ncol - 1));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cv[i][j] = col[min(r0 + tr + 16 * i, k - 1)];
     }
+    // the interchanges, traced per entry (every wave holds the same entries)
     const int tl = li < kLuPanel ? li : li - kLuPanel;
     const int my_pv = (tl < nbw) ? ipiv[(int64_t)e * k + j0 + tl] - 1 : -1;
     int pvu[kLuPanel];
@@ -888,88 +905,52 @@ k_gj_rows(cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k, in
             src = y;
         }
     }
+    int tdst[kLuPanel];   // target positions below the panel, uniform; -1: none
 #pragma unroll
-    for (int q = 0; q < kRowsCols / 8; ++q) {
-        const int j = jb + cg + 8 * q;
-        vals[li][cg + 8 * q] = (dst >= 0 && j < ncol) ? aug_col(Fe, Re, m, k, j)[src] : czero<float>();
-    }
-    __syncthreads();
-    if (threadIdx.x < kRowsCols && jb + (int)threadIdx.x < ncol) {
-        cx<float> x[kLuPanel];
+    for (int i = 0; i < kLuPanel; ++i) tdst[i] = __builtin_amdgcn_readlane(dst, kLuPanel + i);
+    const int cg = t >> 5;
 #pragma unroll
-        for (int t = 0; t < kLuPanel; ++t) x[t] = vals[t][threadIdx.x];
-#pragma unroll
-        for (int t = 0; t < kLuPanel; ++t)
-#pragma unroll
-            for (int u = 0; u < kLuPanel; ++u)
-                if (u < t && t < nbw) x[t] = x[t] - L11[u * kLuPanel + t] * x[u];
-#pragma unroll
-        for (int t = 0; t < kLuPanel; ++t) vals[t][threadIdx.x] = x[t];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kRowsCols / 8; ++q) {
-        const int j = jb + cg + 8 * q;
-        if (dst >= 0 && j < ncol) aug_col(Fe, Re, m, k, j)[dst] = vals[li][cg + 8 * q];
-    }
-}
-
-// Elimination of the panel's columns from every other row of the augmented
-// columns j0 + nbw .. k + m - 1: out[r, c] -= w_r . (panel rows of c), with
-// w_r = L21 row r below the panel and w_r = F[r, panel] U11^-1 above it
-// (solved by one thread per row of the tile, U11 and its reciprocal
-// diagonal in LDS).  One workgroup per 64 x 64 tile (grid.x row tiles over
-// 0..k-1, grid.y column tiles, grid.z nodes): the tile's outputs are
-// loaded first so that their latency overlaps the staging, the multipliers
-// and the panel rows' entries are staged in LDS, and each thread
-// accumulates a 4 x 4 block (rows tr + 16i, columns tc + 16j; a wave covers
-// 4 columns x 16 consecutive rows per access).  The panel rows are not
-// written.  Memory-bound: each output is read and written once.
-__global__ void __launch_bounds__(256)
-k_gj_update(cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k, int j0, int nbw) {
-    __shared__ cx<float> Ls[kLuPanel][64];
-    __shared__ cx<float> Us[kLuPanel][64];
-    __shared__ cx<float> U11[kLuPanel][kLuPanel + 1];
-    __shared__ cx<float> rdiag[kLuPanel];
-    const int t = threadIdx.x, tr = t & 15, tc = t >> 4;
-    const int r0 = blockIdx.x * 64;
-    const int cb = j0 + nbw + blockIdx.y * 64;
-    const int ncol = k + m;
-    cx<float>* Fe = F + (int64_t)blockIdx.z * m * m;
-    cx<float>* Re = R + (int64_t)blockIdx.z * k * m;
-    cx<float> cv[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const cx<float>* col = aug_col(Fe, Re, m, k, min(cb + tc + 16 * j, ncol - 1));
-#pragma unroll
-        for (int i = 0; i < 4; ++i) cv[i][j] = col[min(r0 + tr + 16 * i, k - 1)];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = t + 256 * q;
-        const int kk = idx >> 6, r = idx & 63;
-        Ls[kk][r] = (r0 + r < k && kk < nbw) ? Fe[(int64_t)(j0 + kk) * m + r0 + r] : czero<float>();
-        const int c = idx >> 4, k2 = idx & 15;
-        Us[k2][c] = (cb + c < ncol && k2 < nbw) ? aug_col(Fe, Re, m, k, cb + c)[j0 + k2] : czero<float>();
+    for (int q = 0; q < 8; ++q) {
+        const int c = cg + 8 * q, j = cb + c;
+        G[li][c] = (dst >= 0 && j < ncol) ? aug_col(F, R, B1, m, k, e, sb, j)[src] : czero<float>();
     }
     {
         const int a = t & 15, bc = t >> 4;
-        U11[a][bc] = (a <= bc && bc < nbw) ? Fe[(int64_t)(j0 + bc) * m + j0 + a] : czero<float>();
-        if (t < kLuPanel) rdiag[t] = t < nbw ? crecip(Fe[(int64_t)(j0 + t) * m + j0 + t]) : czero<float>();
+        PB[a][bc] = bc < nbw && a < nbw ? aug_col(F, R, B1, m, k, e, sb, j0 + bc)[j0 + a] : czero<float>();
     }
-    __syncthreads();
-    if (r0 < j0 && t < 64 && r0 + t < j0) {   // rows above the panel: w U11 = F[row, panel]
-        cx<float> w[kLuPanel];
 #pragma unroll
-        for (int a = 0; a < kLuPanel; ++a) {
-            cx<float> x = Ls[a][t];
+    for (int q = 0; q < kLuPanel * 64 / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int kk = idx >> 6, r = idx & 63;
+        Ls[kk][r] = (r0 + r < k && kk < nbw) ? aug_col(F, R, B1, m, k, e, sb, j0 + kk)[r0 + r] : czero<float>();
+    }
+    if (t < 64) {
+        int f = -1;
+#pragma unroll
+        for (int i = 0; i < kLuPanel; ++i)
+            if (tdst[i] == r0 + t) f = i;
+        tmap[t] = f;
+    }
+    if (t < kLuPanel) rdiag[t] = t < nbw ? crecip(aug_col(F, R, B1, m, k, e, sb, j0 + t)[j0 + t]) : czero<float>();
+    __syncthreads();
+    if (t < 64) {   // U12 of column t: L11^-1 on the gathered panel rows
+        cx<float> x[kLuPanel];
+#pragma unroll
+        for (int a = 0; a < kLuPanel; ++a) x[a] = G[a][t];
+#pragma unroll
+        for (int a = 0; a < kLuPanel; ++a)
 #pragma unroll
             for (int u = 0; u < kLuPanel; ++u)
-                if (u < a) x = x - w[u] * U11[u][a];
-            w[a] = a < nbw ? x * rdiag[a] : czero<float>();
-        }
+                if (u < a && a < nbw) x[a] = x[a] - PB[a][u] * x[u];
 #pragma unroll
-        for (int a = 0; a < kLuPanel; ++a) Ls[a][t] = w[a];
+        for (int a = 0; a < kLuPanel; ++a) G[a][t] = x[a];
+    } else if (t < 128 && r0 + (t - 64) < j0) {   // rows above the panel (second wave): w U11 = F[row, panel]
+        const int rr = t - 64;   // w replaces the row's entries in Ls as it is solved
+        for (int a = 0; a < kLuPanel; ++a) {
+            cx<float> x = Ls[a][rr];
+            for (int u = 0; u < a; ++u) x = x - Ls[u][rr] * PB[u][a];
+            Ls[a][rr] = a < nbw ? x * rdiag[a] : czero<float>();
+        }
     }
     __syncthreads();
     cx<float> acc[4][4];
@@ -983,7 +964,7 @@ k_gj_update(cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k, 
 #pragma unroll
         for (int i = 0; i < 4; ++i) a[i] = Ls[kk][tr + 16 * i];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = Us[kk][tc + 16 * j];
+        for (int j = 0; j < 4; ++j) b[j] = G[kk][tc + 16 * j];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -991,38 +972,49 @@ k_gj_update(cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k, 
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int c = cb + tc + 16 * j;
+        const int cl = tc + 16 * j, c = cb + cl;
         if (c < ncol) {
-            cx<float>* col = aug_col(Fe, Re, m, k, c);
+            cx<float>* col = aug_col(F, R, B1, m, k, e, db, c);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int r = r0 + tr + 16 * i;
-                if (r < k && (r < j0 || r >= j0 + nbw)) col[r] = cv[i][j] - acc[i][j];
+                const int rl = tr + 16 * i, r = r0 + rl;
+                if (r < k) {
+                    // the panel row's U12, or the (moved) value minus the update; both
+                    // LDS reads are unconditional and opaque, so that the choice stays a
+                    // choice of values and cv stays in registers
+                    const int f = tmap[rl];
+                    cx<float> u12 = G[min(max(r - j0, 0), kLuPanel - 1)][cl];
+                    cx<float> mv = G[kLuPanel + max(f, 0)][cl];
+                    asm volatile("" : "+v"(u12.re), "+v"(u12.im), "+v"(mv.re), "+v"(mv.im));
+                    const cx<float> base = f >= 0 ? mv : cv[i][j];
+                    col[r] = (r >= j0 && r < j0 + nbw) ? u12 : base - acc[i][j];
+                }
             }
         }
     }
 }
 
-// [Q X] = U^-1 Z per 16-row block of R once every panel is eliminated (F_II
-// is then block diagonal with the panels' U11): one thread per (block,
-// column of R), grid.x column groups, grid.y blocks, grid.z nodes.
+// [Q X] = U^-1 Z per panel's block of rows once every panel is eliminated
+// (F_II is then block diagonal with the panels' U11; panel q's in buffer
+// q mod 2, the right-hand sides in buffer fin): one thread per (block,
+// column of [Q X]), grid.x column groups, grid.y blocks, grid.z nodes.
 __global__ void __launch_bounds__(256)
-k_gj_final(const cx<float>* __restrict__ F, cx<float>* __restrict__ R, int m, int k) {
+k_gj_final(cx<float>* __restrict__ F, cx<float>* __restrict__ R, cx<float>* __restrict__ B1, int m, int k, int fin) {
     __shared__ cx<float> U[kLuPanel][kLuPanel + 1];
     __shared__ cx<float> rd[kLuPanel];
-    const int j0 = blockIdx.y * kLuPanel, nbw = min(kLuPanel, k - j0);
-    const cx<float>* Fe = F + (int64_t)blockIdx.z * m * m;
-    cx<float>* Re = R + (int64_t)blockIdx.z * k * m;
+    const int q = blockIdx.y, j0 = q * kLuPanel, nbw = min(kLuPanel, k - j0);
+    const int e = blockIdx.z;
     const int t = threadIdx.x;
     {
         const int a = t & 15, bc = t >> 4;
-        U[a][bc] = (a <= bc && bc < nbw) ? Fe[(int64_t)(j0 + bc) * m + j0 + a] : czero<float>();
-        if (t < kLuPanel) rd[t] = t < nbw ? crecip(Fe[(int64_t)(j0 + t) * m + j0 + t]) : czero<float>();
+        U[a][bc] = (a <= bc && bc < nbw) ? aug_col(F, R, B1, m, k, e, q & 1, j0 + bc)[j0 + a] : czero<float>();
     }
+    __syncthreads();
+    if (t < kLuPanel) rd[t] = t < nbw ? crecip(U[t][t]) : czero<float>();
     __syncthreads();
     const int c = blockIdx.x * 256 + t;
     if (c >= m) return;
-    cx<float>* col = Re + (int64_t)c * k + j0;
+    cx<float>* col = aug_col(F, R, B1, m, k, e, fin, k + c) + j0;
     cx<float> y[kLuPanel];
 #pragma unroll
     for (int a = 0; a < kLuPanel; ++a) y[a] = a < nbw ? col[a] : czero<float>();
@@ -1420,35 +1412,44 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             }
             continue;
         }
-        bool gj = false;   // [Q X] left in R by the blocked Gauss-Jordan
+        bool gj = false;   // [Q X] left by the blocked Gauss-Jordan in Rq (node stride sRq, leading dimension k)
+        rc_t* Rq = R;
+        rocblas_stride sRq = (rocblas_stride)k * m;
         if (k > kLuMin && k <= kLuMaxRows) {
             gj = true;
             cx<float>* Fc = (cx<float>*)F;
             cx<float>* Rc = (cx<float>*)R;
+            cx<float>* B1 = (cx<float>*)(fb + P->off_r2);
             k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>(Fc, Rc, L.nb, k, m);
             CV_LAUNCH_CHECK();
             const int ncol = k + m;
-            for (int j0 = 0; j0 < k; j0 += kLuPanel) {
+            int sb = 0;   // buffer of the current panel
+            for (int j0 = 0; j0 < k; j0 += kLuPanel, sb ^= 1) {
                 const int nbw = std::min(kLuPanel, k - j0);
                 const int nr = k - j0;   // panel rows: 512 threads x 1 or 2 rows, 1024 x 2 above 1024 rows
+                cx<float>* Fp = sb ? B1 : Fc;
+                const int ld = sb ? k : m;
+                const int64_t ns = sb ? (int64_t)k * ncol : (int64_t)m * m;
                 if (nr <= 512)
-                    k_lu_panel<512, 1><<<L.nb, 512, 0, s>>>(Fc, m, k, j0, nbw, piv, info + L.info_first);
+                    k_lu_panel<512, 1><<<L.nb, 512, 0, s>>>(Fp, ld, ns, k, j0, nbw, piv, info + L.info_first);
                 else if (nr <= 1024)
-                    k_lu_panel<512, 2><<<L.nb, 512, 0, s>>>(Fc, m, k, j0, nbw, piv, info + L.info_first);
+                    k_lu_panel<512, 2><<<L.nb, 512, 0, s>>>(Fp, ld, ns, k, j0, nbw, piv, info + L.info_first);
                 else
-                    k_lu_panel<1024, 2><<<L.nb, 1024, 0, s>>>(Fc, m, k, j0, nbw, piv, info + L.info_first);
+                    k_lu_panel<1024, 2><<<L.nb, 1024, 0, s>>>(Fp, ld, ns, k, j0, nbw, piv, info + L.info_first);
                 CV_LAUNCH_CHECK();
                 const int cols = ncol - j0 - nbw;   // >= m > 0
-                k_gj_rows<<<(unsigned)(((cols + kRowsCols - 1) / kRowsCols) * (int64_t)L.nb), 256, 0, s>>>(
-                    Fc, Rc, m, k, j0, nbw, piv);
-                CV_LAUNCH_CHECK();
-                k_gj_update<<<dim3((unsigned)((k + 63) / 64), (unsigned)((cols + 63) / 64), (unsigned)L.nb), 256, 0,
-                              s>>>(Fc, Rc, m, k, j0, nbw);
+                k_gj_step<<<dim3((unsigned)((k + 63) / 64), (unsigned)((cols + 63) / 64), (unsigned)L.nb), 256, 0, s>>>(
+                    Fc, Rc, B1, m, k, j0, nbw, sb, piv);
                 CV_LAUNCH_CHECK();
             }
-            k_gj_final<<<dim3((unsigned)((m + 255) / 256), (unsigned)((k + kLuPanel - 1) / kLuPanel),
-                              (unsigned)L.nb), 256, 0, s>>>(Fc, Rc, m, k);
+            // sb is now the buffer the last step wrote
+            k_gj_final<<<dim3((unsigned)((m + 255) / 256), (unsigned)((k + kLuPanel - 1) / kLuPanel), (unsigned)L.nb),
+                         256, 0, s>>>(Fc, Rc, B1, m, k, sb);
             CV_LAUNCH_CHECK();
+            if (sb) {
+                Rq = (rc_t*)(B1 + (int64_t)k * k);
+                sRq = (rocblas_stride)k * ncol;
+            }
         } else {
             CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
         }
@@ -1464,12 +1465,14 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
                 CV_RB(rocsolver_cgetrs_strided_batched(h, rocblas_operation_none, k, m, F, m, sF, piv, k, R, k, sR,
                                                        L.nb));
             }
+            const rc_t* Rx = gj ? Rq : R;
+            const rocblas_stride sRx = gj ? sRq : sR;
             if (b > 0)
                 CV_RB(rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b, k,
-                                                    &minus_one, F + k, m, sF, R + (int64_t)k * k, k, sR, &one,
+                                                    &minus_one, F + k, m, sF, Rx + (int64_t)k * k, k, sRx, &one,
                                                     F + (int64_t)k * m + k, m, sF, L.nb));
-            k_nd_block_copy<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)R, sR, k, (cx<float>*)F, sF,
-                                                                        m, L.nb, k, m);
+            k_nd_block_copy<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)Rx, sRx, k, (cx<float>*)F,
+                                                                        sF, m, L.nb, k, m);
             CV_LAUNCH_CHECK();
         } else {
             // Q = F_II^-1 in place: LU with partial pivoting, then the inverse
