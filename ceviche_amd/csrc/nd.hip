@@ -60,7 +60,7 @@ namespace {
 
 constexpr int kLeafCells = 32;
 constexpr int kMvTile = 64;        // rows and columns of one tile of the application products
-constexpr int kNdSolveMin = 64;   // interior blocks above this size: triangular solves instead of the inverse
+constexpr int kNdSolveMin = 64;   // interior blocks above this size: [Q X] directly, not the inverse and a product
 constexpr int kLuPanel = 16;      // columns per panel of the blocked Gauss-Jordan
 constexpr int kLuMin = 64;        // interior blocks above this size take the blocked Gauss-Jordan ...
 constexpr int kLuMaxRows = 2048;  // ... up to this size (the panel kernels hold 1024 threads x 2 rows)
@@ -1454,9 +1454,10 @@ extern "C" int cv_nd_factor(const cv_nd_plan* P, const void* plan_dev, const cv_
             CV_RB(rocsolver_cgetrf_strided_batched(h, k, k, F, m, sF, piv, k, info + L.info_first, L.nb));
         }
         if (k > kNdSolveMin) {
-            // [Q X] = F_II^-1 [I F_IB] by the LU triangular solves (faster
-            // than the in-place inverse plus a product for these k), into the
-            // top k rows of F; then S = F_BB - F_BI X and G = F_BI Q
+            // [Q X] = F_II^-1 [I F_IB] (from the blocked Gauss-Jordan, or by
+            // the LU triangular solves above kLuMaxRows; both faster than the
+            // in-place inverse plus a product for these k), into the top k
+            // rows of F; then S = F_BB - F_BI X and G = F_BI Q
             const rocblas_stride sR = (rocblas_stride)k * m;
             if (!gj) {
                 k_nd_rhs<<<nblk((int64_t)L.nb * k * m), 256, 0, s>>>((const cx<float>*)F, (cx<float>*)R, L.nb, k,
