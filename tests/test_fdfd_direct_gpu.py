@@ -352,3 +352,26 @@ def test_instance_worker_autotune_keeps_the_fields():
         assert torch.linalg.vector_norm(x_tuned - x_one) <= 1e-12 * torch.linalg.vector_norm(x_one)
     finally:
         fo._TUNED_WORKERS.clear()
+
+
+def test_nested_dissection_root_above_1024_cells():
+    """A 1100 x 1100 grid: the root separator's interior block holds more than
+    1024 cells, so its first panels take the 1024-thread panel kernel (two
+    rows per thread) and its last panel is 12 columns wide.  Under
+    method='direct' the factorisation must succeed and FGMRES reach 1e-10 in
+    at most 40 steps."""
+    import ceviche_amd as cv
+    N = 1100
+    rng = np.random.default_rng(21)
+    eps = np.ones((N, N))
+    c0, c1 = N // 3, 2 * N // 3
+    eps[c0:c1, c0:c1] = rng.uniform(1, 4, (c1 - c0, c1 - c0))
+    src = np.zeros((N, N))
+    src[N // 2, N // 4:N // 4 + 20] = 1.0
+    print('nd root 1100', flush=True)
+    F = cv.fdfd_ez(OMEGA, 40e-9, eps, [20, 20], solver_options=cv.SolverOptions(tol=1e-10, method='direct'))
+    F.solve(torch.from_numpy(src))
+    info = F.last_solve_info
+    print('  method', info['method'], 'iterations', info['iterations'], 'residual %.2e' % info['rel_residual'],
+          flush=True)
+    assert info.get('factor') == 'nd' and info['rel_residual'] <= 1e-10 and info['iterations'] <= 40
