@@ -4,9 +4,13 @@ Sparse-matrix helpers (utils.py:13-73): make_sparse, get_entries_indices,
 transpose_indices, block_4, make_IO_matrices; random test data
 (utils.py:78-103): make_rand, make_rand_complex, make_rand_indeces,
 make_rand_entries_indices, make_rand_sparse, make_rand_sparse_density;
-numerical derivatives (utils.py:108-125): der_num, grad_num.  These are host
+numerical derivatives (utils.py:108-150, 239-276): der_num, grad_num,
+jac_num, vjp_maker_num; value and shape helpers (utils.py:200-236):
+float_2_array, reshape_to_ND, get_value, get_shape.  These are host
 utilities like the reference's (scipy / numpy formats); torch inputs to the
-sparse helpers stay on their device.
+sparse helpers stay on their device.  Not carried over: the 3D FDFD grid
+helpers (grid_center_to_xyz, grid_xyz_to_center, vec_zz_to_xy, for the
+reference's 3D FDFD stub) and the matplotlib animation aniplot.
 
 Probe and spectrum post-processing (utils.py:316-332, 350-399):
 
@@ -146,6 +150,87 @@ def grad_num(fn, arg, delta=1e-6):
         grad[i] = der_num(fn, arg, i, delta)
         grad[i] += der_num(fn, arg, i, 1j * delta)
     return grad
+
+
+def jac_num(fn, arg, step_size=1e-7):
+    """Forward-difference Jacobian of fn at arg, laid out (input entries,
+    output entries) as in utils.py:127-150: row i is (fn(arg + h e_i) -
+    fn(arg)) / h over the flattened output (real float64).  Scalars count
+    as one-entry arrays."""
+    x0 = float_2_array(arg)
+    flat = np.array(x0, dtype=np.float64).ravel()
+    f0 = np.ravel(get_value(float_2_array(fn(arg))))
+    jac = np.zeros((flat.size, f0.size))
+    for i in range(flat.size):
+        xi = flat.copy()
+        xi[i] += step_size
+        fi = np.ravel(get_value(fn(xi.reshape(np.shape(x0)))))
+        jac[i, :] = np.real(get_value((fi - f0) / step_size))
+    return jac
+
+
+def float_2_array(x):
+    """x unchanged when it is an array (numpy or torch), else a one-entry
+    numpy array holding it (utils.py:200-204)."""
+    if isinstance(x, (np.ndarray, torch.Tensor)):
+        return x
+    return np.array([x])
+
+
+def reshape_to_ND(arr, N):
+    """arr with trailing axes of length 1 added up to N dimensions; ValueError
+    when it already has more (utils.py:206-215)."""
+    nd = len(arr.shape)
+    if nd > N:
+        raise ValueError("array is larger than {} dimensional, given shape {}".format(N, tuple(arr.shape)))
+    return arr.reshape(tuple(arr.shape) + (1,) * (N - nd))
+
+
+def get_value(x):
+    """The plain value of x: a torch tensor detached from its autograd graph
+    (the counterpart of unboxing an autograd ArrayBox, utils.py:220-224),
+    anything else unchanged."""
+    if isinstance(x, torch.Tensor):
+        return x.detach()
+    return x
+
+
+def get_shape(x):
+    """Shape of x also for non-arrays (utils.py:229-236): (1,) for a Python
+    int or float, (len(x),) for a tuple or list, else tuple(x.shape)."""
+    if isinstance(x, (float, int)):
+        return (1,)
+    if isinstance(x, (tuple, list)):
+        return (len(x),)
+    return tuple(np.shape(x)) if not isinstance(x, torch.Tensor) else tuple(x.shape)
+
+
+def vjp_maker_num(fn, arg_inds, steps):
+    """Numerical VJP makers for fn with respect to the arguments at positions
+    arg_inds, forward differences with the matching steps (utils.py:239-276):
+    one maker per argument, called as maker(fn_out, *args) and returning
+    vjp(v) = [sum(v * (fn(args with entry p stepped) - fn_out) / step) for
+    every entry p], real float64."""
+    def one(arg_ind, step):
+        def vjp_maker(fn_out, *args):
+            shape = np.shape(args[arg_ind])
+            size = int(np.prod(shape)) if shape else 1
+
+            def vjp(v):
+                out = np.zeros(size)
+                for ip in range(size):
+                    moved = list(args)
+                    flat = np.array(args[arg_ind], dtype=np.float64).ravel()
+                    flat[ip] += step
+                    moved[arg_ind] = flat.reshape(shape)
+                    out[ip] = np.real(np.sum(v * (fn(*moved) - fn_out) / step))
+                return out
+
+            return vjp
+
+        return vjp_maker
+
+    return tuple(one(arg_inds[ia], steps[ia]) for ia in range(len(arg_inds)))
 
 
 def measure_fields(F, source, steps, probes, component='Ez'):

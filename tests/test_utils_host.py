@@ -120,3 +120,48 @@ def test_primitives_module_exports_the_reference_helpers():
     for name in ('make_sparse', 'transpose_indices', 'make_rand', 'make_rand_complex', 'make_rand_indeces',
                  'make_rand_sparse', 'der_num', 'grad_num', 'get_entries_indices', 'make_IO_matrices'):
         assert name in P.__all__ and callable(getattr(P, name))
+
+
+def test_jac_num_of_a_linear_map_is_its_transpose_layout():
+    """jac_num lays the Jacobian out (input entries, output entries)
+    (utils.py:127-150): for f(x) = A x it is A^T up to the forward-difference
+    rounding; a quadratic's rows are its gradient at x (+ h/2 per entry)."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((4, 6))
+    x = rng.standard_normal(6)
+    J = U.jac_num(lambda v: A @ v, x, step_size=1e-6)
+    assert J.shape == (6, 4) and np.allclose(J, A.T, atol=1e-8)
+    Jq = U.jac_num(lambda v: np.sum(v ** 2), x, step_size=1e-7)
+    assert Jq.shape == (6, 1) and np.allclose(Jq[:, 0], 2 * x, atol=1e-6)
+    assert U.jac_num(lambda v: 3.0 * v, 2.0).shape == (1, 1)
+
+
+def test_vjp_maker_num_against_the_closed_form():
+    """vjp_maker_num (utils.py:239-276): one maker per listed argument; for
+    f(a, b) = a * b (elementwise) the VJP with cotangent v is v * b for a and
+    v * a for b, exactly for a bilinear map up to the step's rounding."""
+    rng = np.random.default_rng(4)
+    a, b, v = rng.standard_normal(5), rng.standard_normal(5), rng.standard_normal(5)
+    f = lambda p, q: p * q
+    mk_a, mk_b = U.vjp_maker_num(f, [0, 1], [1e-6, 1e-6])
+    out = f(a, b)
+    assert np.allclose(mk_a(out, a, b)(v), v * b, atol=1e-8)
+    assert np.allclose(mk_b(out, a, b)(v), v * a, atol=1e-8)
+
+
+def test_value_and_shape_helpers():
+    """float_2_array, reshape_to_ND, get_value, get_shape (utils.py:200-236)."""
+    assert np.array_equal(U.float_2_array(2.5), np.array([2.5]))
+    arr = np.zeros((2, 3))
+    assert U.float_2_array(arr) is arr
+    assert U.reshape_to_ND(np.zeros((4, 5)), 3).shape == (4, 5, 1)
+    assert U.reshape_to_ND(np.zeros(4), 3).shape == (4, 1, 1)
+    with pytest.raises(ValueError):
+        U.reshape_to_ND(np.zeros((2, 2, 2, 2)), 3)
+    t = torch.ones(3, requires_grad=True) * 2
+    g = U.get_value(t)
+    assert not g.requires_grad and torch.equal(g, torch.full((3,), 2.0))
+    assert U.get_value(arr) is arr
+    assert U.get_shape(1.0) == (1,) and U.get_shape(3) == (1,)
+    assert U.get_shape([1, 2, 3]) == (3,) and U.get_shape((1, 2)) == (2,)
+    assert U.get_shape(np.zeros((2, 7))) == (2, 7) and U.get_shape(torch.zeros(4, 1)) == (4, 1)
