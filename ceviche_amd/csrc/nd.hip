@@ -1173,6 +1173,54 @@ k_nd_mv_node(const cx<float>* __restrict__ F, int m, int row0, int col0, int M, 
     }
 }
 
+// The non-transposed product for nodes with at most 32 outputs (the down
+// sweep's x_I = y_I - X x_B on the small levels), one 64-thread workgroup per
+// node: output o = t mod Mp (Mp the power of two >= M) and slice t / Mp of
+// the inputs, so every lane has loads in flight; the slices are summed by
+// lane exchanges.  Outputs as in k_nd_mv_node (o < k: interior cells, else
+// the contribution buffer).
+template <int MP>
+__global__ void __launch_bounds__(64)
+k_nd_mv_node_split(const cx<float>* __restrict__ F, int m, int row0, int col0, int M, int N,
+                   const int32_t* __restrict__ xcells, int xn, const int32_t* __restrict__ icells, int k, int b,
+                   cx<float>* __restrict__ w, cx<float>* __restrict__ tb, int set_interior) {
+    constexpr int S = 64 / MP;
+    __shared__ cx<float> xs[kMvNode];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const int32_t* xc = xcells + (int64_t)e * xn;
+    for (int q = tid; q < xn; q += 64) {
+        const int32_t c = xc[q];
+        xs[q] = c >= 0 ? w[c] : czero<float>();
+    }
+    __syncthreads();
+    const int o = tid % MP, sl = tid / MP;
+    const cx<float>* A = F + (int64_t)e * m * m + row0 + (int64_t)col0 * m;
+    cx<float> acc = czero<float>();
+    if (o < M) {
+        int q = sl;
+        for (; q + 3 * S < N; q += 4 * S) {
+            cx<float> a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = A[o + (int64_t)(q + u * S) * m];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = acc + a[u] * xs[q + u * S];
+        }
+        for (; q < N; q += S) acc = acc + A[o + (int64_t)q * m] * xs[q];
+    }
+#pragma unroll
+    for (int off = MP; off < 64; off <<= 1) {
+        acc.re += __shfl_xor(acc.re, off);
+        acc.im += __shfl_xor(acc.im, off);
+    }
+    if (sl != 0 || o >= M) return;
+    if (o < k) {
+        const int32_t c = icells[(int64_t)e * k + o];
+        if (c >= 0) w[c] = set_interior ? acc : w[c] - acc;
+    } else {
+        tb[(int64_t)e * b + (o - k)] = acc;
+    }
+}
+
 // Sum of the nt partial-sum tiles of each output o (Mout per node): o < k
 // to the interior cells (w = sum when set_interior, else w -= sum), o >= k
 // to the contribution buffer tb (e * b + o - k).
@@ -1276,7 +1324,12 @@ int nd_apply(const cv_nd_plan* P, const void* plan_dev, const void* factor, cons
         const int32_t* ic = (const int32_t*)(pb + L.off_icells);
         const cx<float>* F = (const cx<float>*)(fb + L.off_F);
         if (L.m <= kMvNode) {   // small nodes: one fused launch, one workgroup per node
-            if (!trans)
+            if (!trans && M <= 32) {   // few outputs: the inputs split over the lanes
+                auto* kern = M <= 8 ? k_nd_mv_node_split<8> : M <= 16 ? k_nd_mv_node_split<16>
+                                                                     : k_nd_mv_node_split<32>;
+                kern<<<L.nb, 64, 0, s>>>(F, L.m, row0, col0, M, N, xcells, xn, ic, L.kmax, L.bmax, w, tb,
+                                         set_interior);
+            } else if (!trans)
                 k_nd_mv_node<false><<<L.nb, 64 * ((M + 63) / 64), 0, s>>>(F, L.m, row0, col0, M, N, xcells, xn, ic,
                                                                          L.kmax, L.bmax, w, tb, set_interior);
             else
